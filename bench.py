@@ -1,0 +1,148 @@
+"""Benchmark of the matrix-free Newton-Krylov hot path on one MI355X.
+
+Headline (the JSON ``value``): algorithmic HBM bandwidth of the FP64 tangent product A(v) on the 400x400
+unit-square Q4 mesh (160,000 elements, 321,602 dofs), state = 5 % uniaxial stretch + N(0,(1e-3 h)^2)
+noise, probe v ~ N(0,1), left edge clamped.  One step = one A(v) over the whole mesh, inputs resident in
+HBM.  Algorithmic bytes per A(v):
+
+    B_alg = N_qp (10 s_C + 2 nen s_G) + 4 n_el nen + 2 n_dof s_v
+
+(10 unique moduli and nen x 2 shape gradients per Gauss point, int32 connectivity, v read once, y written
+once).  Extra fields report the FP32-moduli product, the 200x200 Q9 product, and the CPU baseline: the
+oracle's op-for-op restatement of the reference's ``qp`` product on the host cores.
+
+    python bench.py [--gpus 1] [--steps 1000] [--warmup 20]
+"""
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path[:0] = [ROOT, os.path.join(ROOT, "differentiable-fea_amd")]
+
+import torch  # noqa: E402
+
+METRIC = "A(v) HBM GB/s at 160k Q4; solves/s + p50/p99 latency, mixed vs FP64"
+HBM_PEAK_GBS = 8000.0
+
+
+def b_alg(n_el, nen, nq, n_dof, s):
+    return nq * n_el * (10 * s + 2 * nen * s) + 4 * n_el * nen + 2 * n_dof * s
+
+
+def problem(et, n):
+    from difffea_amd.problem import build_problem
+    p = build_problem(dict(element_type=et, nx=n, ny=n, E=1.0, nu=0.3, load_mode="displacement",
+                           load_value=(0.1, None), fixed_side="left", load_side="right"))
+    g = torch.Generator().manual_seed(0)
+    X = p["nodes"]
+    u = torch.zeros_like(X)
+    u[:, 0] = 0.05 * X[:, 0]
+    u = u.reshape(-1) + 1e-3 * (1.0 / n) * torch.randn(u.numel(), dtype=torch.float64, generator=g)
+    v = torch.randn(u.numel(), dtype=torch.float64, generator=torch.Generator().manual_seed(1))
+    theta = torch.ones(p["conn"].shape[0], dtype=torch.float64)
+    return p, u, v, theta
+
+
+def time_apply(p, u, v, theta, precision, steps, warmup):
+    from difffea_amd.operators import make_tangent_operator
+    dev = torch.device("cuda")
+    A = make_tangent_operator("hip", u.to(dev), theta.to(dev), p, precision=precision)
+    vd = v.to(dev, torch.float64 if precision == "fp64" else torch.float32)
+    y = torch.empty_like(vd)
+    sysm = A.system
+    for _ in range(warmup):
+        sysm.apply(vd, y)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        sysm.apply(vd, y)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    m = sysm.mesh
+    s = 8 if precision == "fp64" else 4
+    B = b_alg(m.n_el, m.nen, m.nq, m.n_dof, s)
+    return dict(ms=ms, gbs=B / (ms * 1e-3) / 1e9, bytes=B, products_per_s=1e3 / ms, y=y)
+
+
+def cpu_baseline(p, u, v, theta, seconds=10.0):
+    """The oracle's 'qp' restatement (gather, einsum, batched 4x4 matmul, einsum, index_add_), FP64."""
+    from oracle import fem
+    nthr = os.cpu_count() or 1
+    torch.set_num_threads(nthr)
+    p = fem.precompute_geometry(p)
+    C, _ = fem.qp_setup(u, theta, p)
+    A = fem.make_A_operator_qp(u, theta, p, C=C)
+    for _ in range(2):
+        A(v)
+    times = []
+    t_end = time.perf_counter() + seconds
+    while len(times) < 7 or time.perf_counter() < t_end:
+        t0 = time.perf_counter()
+        A(v)
+        times.append(time.perf_counter() - t0)
+    med = statistics.median(times)
+    n_el, nen = p["conn"].shape
+    B = b_alg(n_el, nen, p["dN_dX"].shape[1], u.numel(), 8)
+    return dict(value=B / med / 1e9, unit="GB/s", cores=torch.get_num_threads(), kind="port",
+                sample=f"400x400 Q4 FP64 A(v), oracle qp restatement, median of {len(times)} products "
+                       f"({med * 1e3:.2f} ms each) after 2 warm-ups",
+                ms_per_product=med * 1e3)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (use --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+    if not torch.cuda.is_available():
+        print("bench.py needs a ROCm GPU", file=sys.stderr)
+        sys.exit(2)
+
+    p, u, v, th = problem("quad4", 400)
+    r64 = time_apply(p, u, v, th, "fp64", args.steps, args.warmup)
+    print(f"# Q4 400x400 fp64 A(v): {r64['ms'] * 1e3:.2f} us, {r64['gbs']:.0f} GB/s", file=sys.stderr, flush=True)
+    r32 = time_apply(p, u, v, th, "mixed", args.steps, args.warmup)
+    print(f"# Q4 400x400 fp32 A(v): {r32['ms'] * 1e3:.2f} us, {r32['gbs']:.0f} GB/s", file=sys.stderr, flush=True)
+    p9, u9, v9, th9 = problem("quad9", 200)
+    q9 = time_apply(p9, u9, v9, th9, "fp64", args.steps, args.warmup)
+    print(f"# Q9 200x200 fp64 A(v): {q9['ms'] * 1e3:.2f} us, {q9['gbs']:.0f} GB/s", file=sys.stderr, flush=True)
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(p, u, v, th, args.cpu_seconds)
+        print(f"# CPU baseline: {cpu['ms_per_product']:.2f} ms on {cpu['cores']} threads", file=sys.stderr, flush=True)
+
+    out = {
+        "metric": METRIC, "value": round(r64["gbs"], 2), "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": r64["ms"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "tangent product A(v), 400x400 unit-square Q4 (160,000 el, 321,602 dofs, 2x2 Gauss), "
+                               "neo-Hookean mu=0.3846 lambda=0.5769, 5% stretch + noise, left edge clamped, "
+                               "FP64 moduli", "n_elements": 160000, "n_dofs": 321602,
+                   "bytes_per_step": r64["bytes"]},
+        "products_per_s": round(r64["products_per_s"], 1),
+        "roofline": {"bound": "hbm", "achieved": round(r64["gbs"], 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(r64["gbs"] / HBM_PEAK_GBS, 4), "traffic": None},
+        "fp32_moduli": {"us_per_product": round(r32["ms"] * 1e3, 3), "gbs": round(r32["gbs"], 2),
+                        "bytes_per_step": r32["bytes"]},
+        "q9_200x200_fp64": {"us_per_product": round(q9["ms"] * 1e3, 3), "gbs": round(q9["gbs"], 2),
+                            "bytes_per_step": q9["bytes"]},
+        "cpu_baseline": None if cpu is None else {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")},
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,1041 @@
+// libdfea_hip: gfx950 kernels and the C ABI (include/dfea.h) of the matrix-free Newton-Krylov hot path.
+//
+// Execution model
+//   * Elements are grouped into chunks (<= CHUNK_Q4 / CHUNK_Q9 elements, <= MAXLN distinct nodes).  One
+//     workgroup handles one chunk: it gathers the chunk's nodal values into LDS, each thread evaluates one
+//     element from structure-of-arrays moduli / geometry planes (coalesced HBM reads), and the element
+//     vectors are summed into LDS node accumulators colour by colour (elements of one colour share no
+//     node), so the summation order is fixed: no atomics, bit-reproducible results.
+//   * Nodes used by one chunk only are written directly.  Nodes shared between chunks receive one partial
+//     per chunk; a second kernel sums the partials of each shared node in chunk order.
+//   * Every launch processes a list of work items {solve, block}: a batch of independent solves of the
+//     same element type and precision goes through one launch; per-solve reductions depend only on the
+//     solve's own blocks, so a solve gives the same bits alone and inside a batch.
+//   * Workgroup -> item mapping is XCD-aware: consecutive items (spatially adjacent chunks) land on the
+//     same XCD, so nodal values shared by neighbouring chunks are reused in that XCD's L2.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <cstddef>
+#include <type_traits>
+
+#include "dfea.h"
+
+namespace {
+
+constexpr int CHUNK_Q4 = 256;     // elements per chunk (= workgroup size) for quad4
+constexpr int CHUNK_Q9 = 64;      // elements per chunk (= workgroup size) for quad9
+constexpr int MAXLN = 640;        // local nodes per chunk (LDS capacity)
+constexpr int VBS = 256;          // workgroup size of the vector kernels
+constexpr int VPT = 4;            // dofs per thread in the vector kernels
+constexpr int VBLK = VBS * VPT;   // dofs per vector block
+constexpr int FBS = 256;          // workgroup size of the shared-node kernels
+constexpr int NXCD = 8;
+
+thread_local char g_err[256] = "";
+
+int fail(int code, const char* what) {
+    snprintf(g_err, sizeof g_err, "%s", what);
+    return code;
+}
+
+int hip_status(const char* where) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        snprintf(g_err, sizeof g_err, "%s: %s", where, hipGetErrorString(e));
+        return -1000 - static_cast<int>(e);
+    }
+    return 0;
+}
+
+template <typename T> struct V2 { T x, y; };
+
+__device__ __forceinline__ int xcd_remap(int b, int n) {
+    // hardware places workgroup b on XCD b % 8; give each XCD a contiguous range of items
+    if (n < NXCD) return b;
+    int x = b % NXCD, pos = b / NXCD;
+    int base = n / NXCD, rem = n % NXCD;
+    return x * base + (x < rem ? x : rem) + pos;
+}
+
+// deterministic workgroup sum; the result is valid on thread 0
+template <int BS>
+__device__ double block_sum(double v, double* s_red) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) s_red[w] = v;
+    __syncthreads();
+    double t = 0.0;
+    if (threadIdx.x == 0) {
+        for (int i = 0; i < BS / 64; ++i) t += s_red[i];
+    }
+    __syncthreads();
+    return t;
+}
+
+// sum of n partials in a fixed order, identical in every workgroup that calls it; valid on thread 0
+template <int BS>
+__device__ double sum_partials(const double* p, int n, double* s_red) {
+    double v = 0.0;
+    for (int i = threadIdx.x; i < n; i += BS) v += p[i];
+    return block_sum<BS>(v, s_red);
+}
+
+// broadcast a thread-0 value to the workgroup
+__device__ double bcast0(double v, double* s_slot) {
+    if (threadIdx.x == 0) *s_slot = v;
+    __syncthreads();
+    double r = *s_slot;
+    __syncthreads();
+    return r;
+}
+
+// ------------------------------------------------------------------------------------------------------
+// neo-Hookean closed forms (plane strain), see `src/difffea/materials.py:65-73`:
+//   P = mu F + (lam lnJ - mu) G,  G = F^-T
+//   d2psi/dF_ij dF_kl = mu d_ik d_jl + lam G_ij G_kl + (mu - lam lnJ) G_il G_kj
+// ------------------------------------------------------------------------------------------------------
+struct Kin {
+    double F[4];     // F00 F01 F10 F11
+    double G[4];     // F^-T
+    double lnJ;
+};
+
+__device__ __forceinline__ Kin kinematics(const double F00, const double F01, const double F10, const double F11) {
+    Kin k;
+    k.F[0] = F00; k.F[1] = F01; k.F[2] = F10; k.F[3] = F11;
+    const double J = F00 * F11 - F01 * F10;
+    const double iJ = 1.0 / J;
+    k.G[0] = F11 * iJ;  k.G[1] = -F10 * iJ;
+    k.G[2] = -F01 * iJ; k.G[3] = F00 * iJ;
+    k.lnJ = log(J);
+    return k;
+}
+
+// 10 unique moduli in the order (00,01,02,03,11,12,13,22,23,33) of the flat index m = 2i+j
+__device__ __forceinline__ void moduli10(const Kin& k, double mu, double lam, double w, double* c) {
+    const double a = mu - lam * k.lnJ;
+    int t = 0;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+#pragma unroll
+        for (int n = m; n < 4; ++n) {
+            const int i = m >> 1, j = m & 1, kk = n >> 1, l = n & 1;
+            double v = lam * k.G[m] * k.G[n] + a * k.G[2 * i + l] * k.G[2 * kk + j];
+            if (i == kk && j == l) v += mu;
+            c[t++] = w * v;
+        }
+    }
+}
+
+// index of unique modulus (m, n) in the 10-entry list
+__device__ __forceinline__ constexpr int uidx(int m, int n) {
+    return m <= n ? (m * 4 - (m * (m - 1)) / 2 + (n - m)) : (n * 4 - (n * (n - 1)) / 2 + (m - n));
+}
+
+struct Item { int s, b; };
+__device__ __forceinline__ Item load_item(const int* items) {
+    const int b = xcd_remap(blockIdx.x, gridDim.x);
+    Item it;
+    it.s = items[2 * b];
+    it.b = items[2 * b + 1];
+    return it;
+}
+
+__device__ __forceinline__ double pcg_status(const dfea_desc& D) { return D.sc[DFEA_SC_STATUS]; }
+
+// ======================================================================================================
+// geometry: dN_dX = dN_dxi J^-1, w = gw det J   (`src/difffea/operators.py:85-120`)
+// items: {s, chunk}; thread per element of the chunk
+// ======================================================================================================
+template <int NEN, int NQ, int BS>
+__global__ __launch_bounds__(BS) void k_geometry(const dfea_desc* __restrict__ tab, const int* __restrict__ items) {
+    const Item it = load_item(items);
+    const dfea_desc& D = tab[it.s];
+    const dfea_chunk ch = D.chunks[it.b];
+    const int t = threadIdx.x;
+    if (t >= ch.ne) return;
+    const int e = ch.e0 + t;
+    const int ne = D.n_el;
+    double X[NEN][2];
+#pragma unroll
+    for (int a = 0; a < NEN; ++a) {
+        const int n = D.conn[a * ne + e];
+        X[a][0] = D.nodes[2 * n];
+        X[a][1] = D.nodes[2 * n + 1];
+    }
+    for (int q = 0; q < NQ; ++q) {
+        double J00 = 0, J01 = 0, J10 = 0, J11 = 0;   // J[i][j] = dX_i / dxi_j
+#pragma unroll
+        for (int a = 0; a < NEN; ++a) {
+            const double g0 = D.dNdxi[(q * NEN + a) * 2], g1 = D.dNdxi[(q * NEN + a) * 2 + 1];
+            J00 += X[a][0] * g0; J01 += X[a][0] * g1;
+            J10 += X[a][1] * g0; J11 += X[a][1] * g1;
+        }
+        const double det = J00 * J11 - J01 * J10;
+        const double id = 1.0 / det;
+        const double I00 = J11 * id, I01 = -J01 * id, I10 = -J10 * id, I11 = J00 * id;   // J^-1
+#pragma unroll
+        for (int a = 0; a < NEN; ++a) {
+            const double g0 = D.dNdxi[(q * NEN + a) * 2], g1 = D.dNdxi[(q * NEN + a) * 2 + 1];
+            const double d0 = g0 * I00 + g1 * I10;
+            const double d1 = g0 * I01 + g1 * I11;
+            const size_t p0 = (size_t)((q * NEN + a) * 2) * ne + e;
+            D.dN64[p0] = d0;
+            D.dN64[p0 + ne] = d1;
+            if (D.dN32) {
+                D.dN32[p0] = (float)d0;
+                D.dN32[p0 + ne] = (float)d1;
+            }
+        }
+        D.w[(size_t)q * ne + e] = D.gw[q] * det;
+    }
+}
+
+// ======================================================================================================
+// chunk kernels: gather -> element -> colour accumulate -> owned write / shared partial
+// ======================================================================================================
+template <typename T>
+struct ChunkLDS {
+    V2<T> v[MAXLN];      // gathered nodal values (unmasked)
+    V2<T> acc[MAXLN];    // nodal accumulators
+    unsigned char bits[MAXLN];
+    double red[16];
+};
+
+template <typename T, int NEN>
+__device__ __forceinline__ void colour_accumulate(const dfea_chunk& ch, int t, const int* l, const V2<T>* fe,
+                                                  V2<T>* acc) {
+    for (int c = 0; c < ch.ncol; ++c) {
+        if (t >= ch.col[c] && t < ch.col[c + 1]) {
+#pragma unroll
+            for (int a = 0; a < NEN; ++a) {
+                V2<T> s = acc[l[a]];
+                s.x += fe[a].x;
+                s.y += fe[a].y;
+                acc[l[a]] = s;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---- tangent product  y = m J (m v) + (1 - m) v  (`operators.py:385-412`) ----------------------------------
+// PCG: the direction update p = z + beta p happens on the gather, and p.Ap partials are produced.
+template <int NEN, int NQ, int BS, typename T, bool PCG>
+__global__ __launch_bounds__(BS) void k_apply(const dfea_desc* __restrict__ tab, const int* __restrict__ items,
+                                             const T* __restrict__ vin, T* __restrict__ vout) {
+    __shared__ ChunkLDS<T> L;
+    const Item it = load_item(items);
+    const dfea_desc& D = tab[it.s];
+    if (PCG && pcg_status(D) != DFEA_PCG_RUNNING) return;
+    const dfea_chunk ch = D.chunks[it.b];
+    const int t = threadIdx.x;
+    const T* v = PCG ? nullptr : (vin ? vin : static_cast<const T*>(D.vec[DFEA_P]));
+    T* y = PCG ? static_cast<T*>(D.vec[DFEA_AP]) : (vout ? vout : static_cast<T*>(D.vec[DFEA_AP]));
+    T* p = static_cast<T*>(D.vec[DFEA_P]);
+    const T* z = static_cast<const T*>(D.vec[DFEA_Z]);
+    const double beta = PCG ? D.sc[DFEA_SC_BETA] : 0.0;
+
+    // 1. gather
+    for (int k = t; k < ch.nk; k += BS) {
+        const uint32_t g = D.lnode[ch.k0 + k];
+        const int n = (int)(g & DFEA_NODE_MASK);
+        V2<T> val;
+        if (PCG) {
+            V2<T> zz = reinterpret_cast<const V2<T>*>(z)[n];
+            V2<T> pp = reinterpret_cast<const V2<T>*>(p)[n];
+            val.x = (T)((double)zz.x + beta * (double)pp.x);
+            val.y = (T)((double)zz.y + beta * (double)pp.y);
+            if (k < ch.nown) reinterpret_cast<V2<T>*>(p)[n] = val;    // owned: only this chunk touches n
+        } else {
+            val = reinterpret_cast<const V2<T>*>(v)[n];
+        }
+        L.v[k] = val;
+        L.acc[k] = V2<T>{0, 0};
+        L.bits[k] = (unsigned char)(g >> 30);
+    }
+    __syncthreads();
+
+    // 2. element
+    int l[NEN];
+    V2<T> fe[NEN];
+    if (t < ch.ne) {
+        const int ne = D.n_el;
+        const int e = ch.e0 + t;
+        const T* dN = std::is_same<T, double>::value ? (const T*)D.dN64 : (const T*)D.dN32;
+        const T* C = std::is_same<T, double>::value ? (const T*)D.C64 : (const T*)D.C32;
+        V2<T> ve[NEN];
+#pragma unroll
+        for (int a = 0; a < NEN; ++a) {
+            l[a] = D.lelem[a * ne + e];
+            ve[a] = L.v[l[a]];
+            const unsigned char b = L.bits[l[a]];
+            if (b & 1) ve[a].x = 0;
+            if (b & 2) ve[a].y = 0;
+            fe[a] = V2<T>{0, 0};
+        }
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            T d[NEN][2];
+#pragma unroll
+            for (int a = 0; a < NEN; ++a) {
+                d[a][0] = dN[(size_t)((q * NEN + a) * 2) * ne + e];
+                d[a][1] = dN[(size_t)((q * NEN + a) * 2 + 1) * ne + e];
+            }
+            T c[10];
+#pragma unroll
+            for (int k = 0; k < 10; ++k) c[k] = C[(size_t)(k * NQ + q) * ne + e];
+            T g[4] = {0, 0, 0, 0};    // g[2i+j] = sum_a ve[a][i] dN[a][j]
+#pragma unroll
+            for (int a = 0; a < NEN; ++a) {
+                g[0] += ve[a].x * d[a][0];
+                g[1] += ve[a].x * d[a][1];
+                g[2] += ve[a].y * d[a][0];
+                g[3] += ve[a].y * d[a][1];
+            }
+            T s[4];
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                T acc = 0;
+#pragma unroll
+                for (int n = 0; n < 4; ++n) acc += c[uidx(m, n)] * g[n];
+                s[m] = acc;
+            }
+#pragma unroll
+            for (int a = 0; a < NEN; ++a) {
+                fe[a].x += s[0] * d[a][0] + s[1] * d[a][1];
+                fe[a].y += s[2] * d[a][0] + s[3] * d[a][1];
+            }
+        }
+    }
+    colour_accumulate<T, NEN>(ch, t, l, fe, L.acc);
+
+    // 3. owned nodes: final value; shared nodes: partial
+    double dot = 0.0;
+    V2<T>* part = reinterpret_cast<V2<T>*>(D.part);
+    for (int k = t; k < ch.nk; k += BS) {
+        const V2<T> acc = L.acc[k];
+        if (k < ch.nown) {
+            const V2<T> raw = L.v[k];
+            const unsigned char b = L.bits[k];
+            V2<T> out;
+            out.x = (b & 1) ? raw.x : acc.x;
+            out.y = (b & 2) ? raw.y : acc.y;
+            const int n = (int)(D.lnode[ch.k0 + k] & DFEA_NODE_MASK);
+            reinterpret_cast<V2<T>*>(y)[n] = out;
+            if (PCG) dot += (double)raw.x * (double)out.x + (double)raw.y * (double)out.y;
+        } else {
+            part[ch.pbase + (k - ch.nown)] = acc;
+        }
+    }
+    if (PCG) {
+        const double s = block_sum<BS>(dot, L.red);
+        if (t == 0) D.red[it.b] = s;
+    }
+}
+
+// shared nodes of the tangent product
+template <typename T, bool PCG>
+__global__ __launch_bounds__(FBS) void k_apply_fin(const dfea_desc* __restrict__ tab, const int* __restrict__ items,
+                                                  const T* __restrict__ vin, T* __restrict__ vout) {
+    __shared__ double s_red[16];
+    const Item it = load_item(items);
+    const dfea_desc& D = tab[it.s];
+    if (PCG && pcg_status(D) != DFEA_PCG_RUNNING) return;
+    const int i = it.b * FBS + threadIdx.x;
+    const T* v = PCG ? nullptr : (vin ? vin : static_cast<const T*>(D.vec[DFEA_P]));
+    T* y = PCG ? static_cast<T*>(D.vec[DFEA_AP]) : (vout ? vout : static_cast<T*>(D.vec[DFEA_AP]));
+    double dot = 0.0;
+    if (i < D.n_shared) {
+        const V2<T>* part = reinterpret_cast<const V2<T>*>(D.part);
+        V2<T> acc{0, 0};
+        for (int j = D.sh_off[i]; j < D.sh_off[i + 1]; ++j) {
+            const V2<T> pj = part[D.sh_slot[j]];
+            acc.x += pj.x;
+            acc.y += pj.y;
+        }
+        const uint32_t g = D.sh_node[i];
+        const int n = (int)(g & DFEA_NODE_MASK);
+        V2<T> raw;
+        if (PCG) {
+            V2<T>* p = static_cast<V2<T>*>(D.vec[DFEA_P]);
+            const V2<T> zz = static_cast<const V2<T>*>(D.vec[DFEA_Z])[n];
+            const V2<T> pp = p[n];
+            const double beta = D.sc[DFEA_SC_BETA];
+            raw.x = (T)((double)zz.x + beta * (double)pp.x);
+            raw.y = (T)((double)zz.y + beta * (double)pp.y);
+            p[n] = raw;
+        } else {
+            raw = reinterpret_cast<const V2<T>*>(v)[n];
+        }
+        V2<T> out;
+        out.x = (g & DFEA_FIXED_X) ? raw.x : acc.x;
+        out.y = (g & DFEA_FIXED_Y) ? raw.y : acc.y;
+        reinterpret_cast<V2<T>*>(y)[n] = out;
+        if (PCG) dot = (double)raw.x * (double)out.x + (double)raw.y * (double)out.y;
+    }
+    if (PCG) {
+        const double s = block_sum<FBS>(dot, s_red);
+        if (threadIdx.x == 0) D.red[D.n_chunks + it.b] = s;
+    }
+}
+
+// ---- residual  R = m (R_int(u) - lf f)  (`operators.py:131-220`) -------------------------------------------
+template <int NEN, int NQ, int BS>
+__global__ __launch_bounds__(BS) void k_residual(const dfea_desc* __restrict__ tab, const int* __restrict__ items,
+                                                const double* __restrict__ uin, double* __restrict__ rout,
+                                                double lf_arg) {
+    __shared__ ChunkLDS<double> L;
+    const Item it = load_item(items);
+    const dfea_desc& D = tab[it.s];
+    const dfea_chunk ch = D.chunks[it.b];
+    const int t = threadIdx.x;
+    const double* u = uin ? uin : static_cast<const double*>(D.vec[DFEA_U]);
+    double* R = rout ? rout : static_cast<double*>(D.vec[DFEA_RES]);
+    const double lf = lf_arg >= 0.0 ? lf_arg : D.sc[DFEA_SC_LF];
+    for (int k = t; k < ch.nk; k += BS) {
+        const uint32_t g = D.lnode[ch.k0 + k];
+        const int n = (int)(g & DFEA_NODE_MASK);
+        L.v[k] = reinterpret_cast<const V2<double>*>(u)[n];
+        L.acc[k] = V2<double>{0, 0};
+        L.bits[k] = (unsigned char)(g >> 30);
+    }
+    __syncthreads();
+    int l[NEN];
+    V2<double> fe[NEN];
+    if (t < ch.ne) {
+        const int ne = D.n_el;
+        const int e = ch.e0 + t;
+        const double th = D.theta[e];
+        const double mu = D.mu0 * th, lam = D.lam0 * th;
+        V2<double> ue[NEN];
+#pragma unroll
+        for (int a = 0; a < NEN; ++a) {
+            l[a] = D.lelem[a * ne + e];
+            ue[a] = L.v[l[a]];
+            fe[a] = V2<double>{0, 0};
+        }
+        for (int q = 0; q < NQ; ++q) {
+            double d[NEN][2];
+            double F00 = 1, F01 = 0, F10 = 0, F11 = 1;
+#pragma unroll
+            for (int a = 0; a < NEN; ++a) {
+                d[a][0] = D.dN64[(size_t)((q * NEN + a) * 2) * ne + e];
+                d[a][1] = D.dN64[(size_t)((q * NEN + a) * 2 + 1) * ne + e];
+                F00 += ue[a].x * d[a][0]; F01 += ue[a].x * d[a][1];
+                F10 += ue[a].y * d[a][0]; F11 += ue[a].y * d[a][1];
+            }
+            const Kin k = kinematics(F00, F01, F10, F11);
+            const double w = D.w[(size_t)q * ne + e];
+            const double b = lam * k.lnJ - mu;
+            double P[4];
+#pragma unroll
+            for (int m = 0; m < 4; ++m) P[m] = w * (mu * k.F[m] + b * k.G[m]);
+#pragma unroll
+            for (int a = 0; a < NEN; ++a) {
+                fe[a].x += P[0] * d[a][0] + P[1] * d[a][1];
+                fe[a].y += P[2] * d[a][0] + P[3] * d[a][1];
+            }
+        }
+    }
+    colour_accumulate<double, NEN>(ch, t, l, fe, L.acc);
+    V2<double>* part = reinterpret_cast<V2<double>*>(D.part);
+    for (int k = t; k < ch.nk; k += BS) {
+        const V2<double> acc = L.acc[k];
+        if (k < ch.nown) {
+            const int n = (int)(D.lnode[ch.k0 + k] & DFEA_NODE_MASK);
+            const unsigned char b = L.bits[k];
+            V2<double> out;
+            out.x = (b & 1) ? 0.0 : acc.x - lf * D.f[2 * n];
+            out.y = (b & 2) ? 0.0 : acc.y - lf * D.f[2 * n + 1];
+            reinterpret_cast<V2<double>*>(R)[n] = out;
+        } else {
+            part[ch.pbase + (k - ch.nown)] = acc;
+        }
+    }
+}
+
+__global__ __launch_bounds__(FBS) void k_residual_fin(const dfea_desc* __restrict__ tab, const int* __restrict__ items,
+                                                     double* __restrict__ rout, double lf_arg) {
+    const Item it = load_item(items);
+    const dfea_desc& D = tab[it.s];
+    const int i = it.b * FBS + threadIdx.x;
+    if (i >= D.n_shared) return;
+    double* R = rout ? rout : static_cast<double*>(D.vec[DFEA_RES]);
+    const double lf = lf_arg >= 0.0 ? lf_arg : D.sc[DFEA_SC_LF];
+    const V2<double>* part = reinterpret_cast<const V2<double>*>(D.part);
+    V2<double> acc{0, 0};
+    for (int j = D.sh_off[i]; j < D.sh_off[i + 1]; ++j) {
+        const V2<double> pj = part[D.sh_slot[j]];
+        acc.x += pj.x;
+        acc.y += pj.y;
+    }
+    const uint32_t g = D.sh_node[i];
+    const int n = (int)(g & DFEA_NODE_MASK);
+    V2<double> out;
+    out.x = (g & DFEA_FIXED_X) ? 0.0 : acc.x - lf * D.f[2 * n];
+    out.y = (g & DFEA_FIXED_Y) ? 0.0 : acc.y - lf * D.f[2 * n + 1];
+    reinterpret_cast<V2<double>*>(R)[n] = out;
+}
+
+// ---- moduli + Jacobi diagonal  (`operators.py:316-342`, `:477-556`) ------------------------------------------
+template <int NEN, int NQ, int BS, typename CT>
+__global__ __launch_bounds__(BS) void k_moduli(const dfea_desc* __restrict__ tab, const int* __restrict__ items,
+                                              const double* __restrict__ uin) {
+    __shared__ ChunkLDS<double> L;
+    const Item it = load_item(items);
+    const dfea_desc& D = tab[it.s];
+    const dfea_chunk ch = D.chunks[it.b];
+    const int t = threadIdx.x;
+    const double* u = uin ? uin : static_cast<const double*>(D.vec[DFEA_U]);
+    for (int k = t; k < ch.nk; k += BS) {
+        const uint32_t g = D.lnode[ch.k0 + k];
+        const int n = (int)(g & DFEA_NODE_MASK);
+        L.v[k] = reinterpret_cast<const V2<double>*>(u)[n];
+        L.acc[k] = V2<double>{0, 0};
+        L.bits[k] = (unsigned char)(g >> 30);
+    }
+    __syncthreads();
+    int l[NEN];
+    V2<double> fe[NEN];
+    if (t < ch.ne) {
+        const int ne = D.n_el;
+        const int e = ch.e0 + t;
+        const double th = D.theta[e];
+        const double mu = D.mu0 * th, lam = D.lam0 * th;
+        CT* Cout = std::is_same<CT, double>::value ? (CT*)D.C64 : (CT*)D.C32;
+        V2<double> ue[NEN];
+#pragma unroll
+        for (int a = 0; a < NEN; ++a) {
+            l[a] = D.lelem[a * ne + e];
+            ue[a] = L.v[l[a]];
+            fe[a] = V2<double>{0, 0};
+        }
+        for (int q = 0; q < NQ; ++q) {
+            double d[NEN][2];
+            double F00 = 1, F01 = 0, F10 = 0, F11 = 1;
+#pragma unroll
+            for (int a = 0; a < NEN; ++a) {
+                d[a][0] = D.dN64[(size_t)((q * NEN + a) * 2) * ne + e];
+                d[a][1] = D.dN64[(size_t)((q * NEN + a) * 2 + 1) * ne + e];
+                F00 += ue[a].x * d[a][0]; F01 += ue[a].x * d[a][1];
+                F10 += ue[a].y * d[a][0]; F11 += ue[a].y * d[a][1];
+            }
+            const Kin k = kinematics(F00, F01, F10, F11);
+            double c[10];
+            moduli10(k, mu, lam, D.w[(size_t)q * ne + e], c);
+#pragma unroll
+            for (int kk = 0; kk < 10; ++kk) Cout[(size_t)(kk * NQ + q) * ne + e] = (CT)c[kk];
+            // K^e_{(a,i),(a,i)} += sum_{j,l} C[2i+j][2i+l] dN[a][j] dN[a][l]
+#pragma unroll
+            for (int a = 0; a < NEN; ++a) {
+                const double d0 = d[a][0], d1 = d[a][1];
+                fe[a].x += c[uidx(0, 0)] * d0 * d0 + 2.0 * c[uidx(0, 1)] * d0 * d1 + c[uidx(1, 1)] * d1 * d1;
+                fe[a].y += c[uidx(2, 2)] * d0 * d0 + 2.0 * c[uidx(2, 3)] * d0 * d1 + c[uidx(3, 3)] * d1 * d1;
+            }
+        }
+    }
+    colour_accumulate<double, NEN>(ch, t, l, fe, L.acc);
+    V2<double>* part = reinterpret_cast<V2<double>*>(D.part);
+    for (int k = t; k < ch.nk; k += BS) {
+        const V2<double> acc = L.acc[k];
+        if (k < ch.nown) {
+            const int n = (int)(D.lnode[ch.k0 + k] & DFEA_NODE_MASK);
+            const unsigned char b = L.bits[k];
+            const double dx = (b & 1) ? 1.0 : fmax(acc.x, 1e-6);
+            const double dy = (b & 2) ? 1.0 : fmax(acc.y, 1e-6);
+            if (std::is_same<CT, double>::value) {
+                D.diag64[2 * n] = dx;
+                D.diag64[2 * n + 1] = dy;
+            } else {
+                D.diag32[2 * n] = (float)dx;
+                D.diag32[2 * n + 1] = (float)dy;
+            }
+        } else {
+            part[ch.pbase + (k - ch.nown)] = acc;
+        }
+    }
+}
+
+template <typename CT>
+__global__ __launch_bounds__(FBS) void k_moduli_fin(const dfea_desc* __restrict__ tab, const int* __restrict__ items) {
+    const Item it = load_item(items);
+    const dfea_desc& D = tab[it.s];
+    const int i = it.b * FBS + threadIdx.x;
+    if (i >= D.n_shared) return;
+    const V2<double>* part = reinterpret_cast<const V2<double>*>(D.part);
+    V2<double> acc{0, 0};
+    for (int j = D.sh_off[i]; j < D.sh_off[i + 1]; ++j) {
+        const V2<double> pj = part[D.sh_slot[j]];
+        acc.x += pj.x;
+        acc.y += pj.y;
+    }
+    const uint32_t g = D.sh_node[i];
+    const int n = (int)(g & DFEA_NODE_MASK);
+    const double dx = (g & DFEA_FIXED_X) ? 1.0 : fmax(acc.x, 1e-6);
+    const double dy = (g & DFEA_FIXED_Y) ? 1.0 : fmax(acc.y, 1e-6);
+    if (std::is_same<CT, double>::value) {
+        D.diag64[2 * n] = dx;
+        D.diag64[2 * n + 1] = dy;
+    } else {
+        D.diag32[2 * n] = (float)dx;
+        D.diag32[2 * n + 1] = (float)dy;
+    }
+}
+
+// ---- adjoint: g[e] = -sum (m lam)_e . R^e_int(u; mu0, lam0)  (`adjoint.py:75-106`) ----------------------------
+template <int NEN, int NQ, int BS>
+__global__ __launch_bounds__(BS) void k_adjoint(const dfea_desc* __restrict__ tab, const int* __restrict__ items,
+                                               const double* __restrict__ u, const double* __restrict__ lam_in,
+                                               double* __restrict__ gout) {
+    __shared__ ChunkLDS<double> L;    // v: u, acc: masked lambda
+    const Item it = load_item(items);
+    const dfea_desc& D = tab[it.s];
+    const dfea_chunk ch = D.chunks[it.b];
+    const int t = threadIdx.x;
+    for (int k = t; k < ch.nk; k += BS) {
+        const uint32_t g = D.lnode[ch.k0 + k];
+        const int n = (int)(g & DFEA_NODE_MASK);
+        L.v[k] = reinterpret_cast<const V2<double>*>(u)[n];
+        V2<double> lm = reinterpret_cast<const V2<double>*>(lam_in)[n];
+        if (g & DFEA_FIXED_X) lm.x = 0.0;
+        if (g & DFEA_FIXED_Y) lm.y = 0.0;
+        L.acc[k] = lm;
+    }
+    __syncthreads();
+    if (t >= ch.ne) return;
+    const int ne = D.n_el;
+    const int e = ch.e0 + t;
+    V2<double> ue[NEN], le[NEN];
+#pragma unroll
+    for (int a = 0; a < NEN; ++a) {
+        const int la = D.lelem[a * ne + e];
+        ue[a] = L.v[la];
+        le[a] = L.acc[la];
+    }
+    double s = 0.0;
+    for (int q = 0; q < NQ; ++q) {
+        double d[NEN][2];
+        double F00 = 1, F01 = 0, F10 = 0, F11 = 1;
+#pragma unroll
+        for (int a = 0; a < NEN; ++a) {
+            d[a][0] = D.dN64[(size_t)((q * NEN + a) * 2) * ne + e];
+            d[a][1] = D.dN64[(size_t)((q * NEN + a) * 2 + 1) * ne + e];
+            F00 += ue[a].x * d[a][0]; F01 += ue[a].x * d[a][1];
+            F10 += ue[a].y * d[a][0]; F11 += ue[a].y * d[a][1];
+        }
+        const Kin k = kinematics(F00, F01, F10, F11);
+        const double w = D.w[(size_t)q * ne + e];
+        const double b = D.lam0 * k.lnJ - D.mu0;
+        double P[4];
+#pragma unroll
+        for (int m = 0; m < 4; ++m) P[m] = w * (D.mu0 * k.F[m] + b * k.G[m]);
+#pragma unroll
+        for (int a = 0; a < NEN; ++a)
+            s += le[a].x * (P[0] * d[a][0] + P[1] * d[a][1]) + le[a].y * (P[2] * d[a][0] + P[3] * d[a][1]);
+    }
+    gout[D.perm[e]] = -s;
+}
+
+// ======================================================================================================
+// vector kernels (items {s, dof block})
+// ======================================================================================================
+__device__ __forceinline__ int ndof(const dfea_desc& D) { return 2 * D.n_nodes; }
+
+// partial dot products of FP64 vectors -> red[dot region][blk]
+__global__ __launch_bounds__(VBS) void k_dot_part(const dfea_desc* __restrict__ tab, const int* __restrict__ items,
+                                                 const double* __restrict__ xin, const double* __restrict__ yin) {
+    __shared__ double s_red[16];
+    const Item it = load_item(items);
+    const dfea_desc& D = tab[it.s];
+    const double* x = xin ? xin : static_cast<const double*>(D.vec[DFEA_RES]);
+    const double* y = yin ? yin : x;
+    const int n = ndof(D);
+    double s = 0.0;
+    const int base = it.b * VBLK;
+#pragma unroll
+    for (int j = 0; j < VPT; ++j) {
+        const int i = base + j * VBS + threadIdx.x;
+        if (i < n) s += x[i] * y[i];
+    }
+    s = block_sum<VBS>(s, s_red);
+    if (threadIdx.x == 0) D.red[D.n_chunks + D.n_finblk + 2 * D.n_vblk + it.b] = s;
+}
+
+__global__ __launch_bounds__(VBS) void k_dot_final(const dfea_desc* __restrict__ tab, const int* __restrict__ items) {
+    __shared__ double s_red[16];
+    const Item it = load_item(items);
+    const dfea_desc& D = tab[it.s];
+    const double s = sum_partials<VBS>(D.red + D.n_chunks + D.n_finblk + 2 * D.n_vblk, D.n_vblk, s_red);
+    if (threadIdx.x == 0) D.sc[DFEA_SC_NORM] = s;
+}
+
+// mode 0: out = where(fixed, lf uD, u);  mode 1: out = u + alpha dx
+template <typename T>
+__global__ __launch_bounds__(VBS) void k_axpy(const dfea_desc* __restrict__ tab, const int* __restrict__ items, int mode,
+                                             const double* __restrict__ uin, const T* __restrict__ dxin,
+                                             const double* __restrict__ uD, const double* __restrict__ sarg,
+                                             double* __restrict__ out) {
+    const Item it = load_item(items);
+    const dfea_desc& D = tab[it.s];
+    const int n = ndof(D);
+    const double* u = uin ? uin : static_cast<const double*>(D.vec[DFEA_U]);
+    const double a = sarg[it.s];
+    const T* dx = dxin ? dxin : static_cast<const T*>(D.vec[DFEA_X]);
+    const int base = it.b * VBLK;
+#pragma unroll
+    for (int j = 0; j < VPT; ++j) {
+        const int i = base + j * VBS + threadIdx.x;
+        if (i >= n) continue;
+        if (mode == 0) {
+            const bool fixed = (D.nfix[i >> 1] >> (i & 1)) & 1;
+            out[i] = fixed ? a * uD[i] : u[i];
+        } else {
+            out[i] = u[i] + a * (double)dx[i];
+        }
+    }
+}
+
+// PCG start: x = 0, r = b, z = r / diag, p = z; partials of r.z and b.b
+template <typename T>
+__global__ __launch_bounds__(VBS) void k_pcg_init(const dfea_desc* __restrict__ tab, const int* __restrict__ items,
+                                                 int neg_res) {
+    __shared__ double s_red[16];
+    const Item it = load_item(items);
+    const dfea_desc& D = tab[it.s];
+    const int n = ndof(D);
+    T* x = static_cast<T*>(D.vec[DFEA_X]);
+    T* r = static_cast<T*>(D.vec[DFEA_R]);
+    T* z = static_cast<T*>(D.vec[DFEA_Z]);
+    T* p = static_cast<T*>(D.vec[DFEA_P]);
+    const T* bT = static_cast<const T*>(D.vec[DFEA_B]);
+    const double* res = static_cast<const double*>(D.vec[DFEA_RES]);
+    const T* dg = std::is_same<T, double>::value ? (const T*)D.diag64 : (const T*)D.diag32;
+    double rz = 0.0, bb = 0.0;
+    const int base = it.b * VBLK;
+#pragma unroll
+    for (int j = 0; j < VPT; ++j) {
+        const int i = base + j * VBS + threadIdx.x;
+        if (i >= n) continue;
+        const T b = neg_res ? (T)(-res[i]) : bT[i];
+        const T zz = b / dg[i];
+        x[i] = 0;
+        r[i] = b;
+        z[i] = zz;
+        p[i] = zz;
+        rz += (double)b * (double)zz;
+        bb += (double)b * (double)b;
+    }
+    rz = block_sum<VBS>(rz, s_red);
+    bb = block_sum<VBS>(bb, s_red);
+    if (threadIdx.x == 0) {
+        double* ru = D.red + D.n_chunks + D.n_finblk;
+        ru[2 * it.b] = rz;
+        ru[2 * it.b + 1] = bb;
+    }
+}
+
+__global__ __launch_bounds__(VBS) void k_pcg_init_final(const dfea_desc* __restrict__ tab, const int* __restrict__ items) {
+    __shared__ double s_red[16];
+    const Item it = load_item(items);
+    const dfea_desc& D = tab[it.s];
+    const double* ru = D.red + D.n_chunks + D.n_finblk;
+    double rz = 0.0, bb = 0.0;
+    for (int i = threadIdx.x; i < D.n_vblk; i += VBS) {
+        rz += ru[2 * i];
+        bb += ru[2 * i + 1];
+    }
+    rz = block_sum<VBS>(rz, s_red);
+    bb = block_sum<VBS>(bb, s_red);
+    if (threadIdx.x == 0) {
+        double* sc = D.sc;
+        sc[DFEA_SC_RZ] = rz;
+        sc[DFEA_SC_BB] = bb;
+        sc[DFEA_SC_BETA] = 0.0;
+        sc[DFEA_SC_K] = 0.0;
+        sc[DFEA_SC_ALPHA] = 0.0;
+        sc[DFEA_SC_PAP] = 0.0;
+        sc[DFEA_SC_RR] = bb;
+        if (sqrt(bb) == 0.0) {            // nothing to do: the right-hand side is zero
+            sc[DFEA_SC_STATUS] = DFEA_PCG_CONVERGED;
+            sc[DFEA_SC_ITERS] = 0.0;
+        } else {
+            sc[DFEA_SC_STATUS] = DFEA_PCG_RUNNING;
+            sc[DFEA_SC_ITERS] = 0.0;
+        }
+    }
+}
+
+// x += alpha p, r -= alpha Ap, z = r / diag, partials r.z and r.r  (alpha = rz / p.Ap from the partials)
+template <typename T>
+__global__ __launch_bounds__(VBS) void k_pcg_update(const dfea_desc* __restrict__ tab, const int* __restrict__ items) {
+    __shared__ double s_red[16];
+    __shared__ double s_b;
+    const Item it = load_item(items);
+    const dfea_desc& D = tab[it.s];
+    if (pcg_status(D) != DFEA_PCG_RUNNING) return;
+    const double pAp = bcast0(sum_partials<VBS>(D.red, D.n_chunks + D.n_finblk, s_red), &s_b);
+    const int n = ndof(D);
+    T* x = static_cast<T*>(D.vec[DFEA_X]);
+    T* r = static_cast<T*>(D.vec[DFEA_R]);
+    T* z = static_cast<T*>(D.vec[DFEA_Z]);
+    const T* p = static_cast<const T*>(D.vec[DFEA_P]);
+    const T* Ap = static_cast<const T*>(D.vec[DFEA_AP]);
+    const T* dg = std::is_same<T, double>::value ? (const T*)D.diag64 : (const T*)D.diag32;
+    const int base = it.b * VBLK;
+    if (!(pAp > 0.0)) {                 // non-positive curvature: stop; at k = 0 the answer is z
+        if (D.sc[DFEA_SC_K] == 0.0) {
+#pragma unroll
+            for (int j = 0; j < VPT; ++j) {
+                const int i = base + j * VBS + threadIdx.x;
+                if (i < n) x[i] = z[i];
+            }
+        }
+        return;
+    }
+    const double alpha = D.sc[DFEA_SC_RZ] / pAp;
+    double rz = 0.0, rr = 0.0;
+#pragma unroll
+    for (int j = 0; j < VPT; ++j) {
+        const int i = base + j * VBS + threadIdx.x;
+        if (i >= n) continue;
+        const T xi = (T)((double)x[i] + alpha * (double)p[i]);
+        const T ri = (T)((double)r[i] - alpha * (double)Ap[i]);
+        const T zi = ri / dg[i];
+        x[i] = xi;
+        r[i] = ri;
+        z[i] = zi;
+        rz += (double)ri * (double)zi;
+        rr += (double)ri * (double)ri;
+    }
+    rz = block_sum<VBS>(rz, s_red);
+    rr = block_sum<VBS>(rr, s_red);
+    if (threadIdx.x == 0) {
+        double* ru = D.red + D.n_chunks + D.n_finblk;
+        ru[2 * it.b] = rz;
+        ru[2 * it.b + 1] = rr;
+    }
+}
+
+// end of one PCG iteration: convergence / curvature / iteration cap, beta   (one workgroup per solve)
+__global__ __launch_bounds__(VBS) void k_pcg_scalar(const dfea_desc* __restrict__ tab, const int* __restrict__ items) {
+    __shared__ double s_red[16];
+    __shared__ double s_b;
+    const Item it = load_item(items);
+    const dfea_desc& D = tab[it.s];
+    if (pcg_status(D) != DFEA_PCG_RUNNING) return;
+    const double pAp = bcast0(sum_partials<VBS>(D.red, D.n_chunks + D.n_finblk, s_red), &s_b);
+    double* sc = D.sc;
+    const double k = sc[DFEA_SC_K];
+    if (!(pAp > 0.0)) {
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            sc[DFEA_SC_PAP] = pAp;
+            sc[DFEA_SC_STATUS] = DFEA_PCG_CURVATURE;
+            sc[DFEA_SC_ITERS] = k;
+        }
+        return;
+    }
+    const double* ru = D.red + D.n_chunks + D.n_finblk;
+    double rz = 0.0, rr = 0.0;
+    for (int i = threadIdx.x; i < D.n_vblk; i += VBS) {
+        rz += ru[2 * i];
+        rr += ru[2 * i + 1];
+    }
+    rz = block_sum<VBS>(rz, s_red);
+    rr = block_sum<VBS>(rr, s_red);
+    if (threadIdx.x == 0) {
+        sc[DFEA_SC_PAP] = pAp;
+        sc[DFEA_SC_ALPHA] = sc[DFEA_SC_RZ] / pAp;
+        sc[DFEA_SC_RR] = rr;
+        if (sqrt(rr) <= sc[DFEA_SC_RTOL] * sqrt(sc[DFEA_SC_BB])) {
+            sc[DFEA_SC_STATUS] = DFEA_PCG_CONVERGED;
+            sc[DFEA_SC_ITERS] = k + 1.0;
+        } else {
+            sc[DFEA_SC_BETA] = rz / sc[DFEA_SC_RZ];
+            sc[DFEA_SC_RZ] = rz;
+            sc[DFEA_SC_K] = k + 1.0;
+            if (k + 1.0 >= sc[DFEA_SC_MAXIT]) {
+                sc[DFEA_SC_STATUS] = DFEA_PCG_MAXITER;
+                sc[DFEA_SC_ITERS] = sc[DFEA_SC_MAXIT];
+            }
+        }
+    }
+}
+
+// ======================================================================================================
+// host launch helpers
+// ======================================================================================================
+inline hipStream_t S(void* s) { return static_cast<hipStream_t>(s); }
+
+template <typename K, typename... A>
+int launch(K kern, int grid, int block, void* stream, const char* name, A... args) {
+    if (grid <= 0) return 0;
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, S(stream), args...);
+    return hip_status(name);
+}
+
+bool bad_nen(int nen) { return nen != 4 && nen != 9; }
+
+template <int NEN, int BS, typename T>
+int pcg_iter(const dfea_desc* table, const int32_t* items, int n_items, const int32_t* fin_items,
+                    int n_fin_items, const int32_t* vitems, int n_vitems, const int32_t* solve_items, int n_solves,
+                    int n_iter, void* stream) {
+    for (int i = 0; i < n_iter; ++i) {
+        int rc = launch(k_apply<NEN, NEN, BS, T, true>, n_items, BS, stream, "k_apply(pcg)", table, items,
+                        (const T*)nullptr, (T*)nullptr);
+        if (!rc) rc = launch(k_apply_fin<T, true>, n_fin_items, FBS, stream, "k_apply_fin(pcg)", table, fin_items,
+                             (const T*)nullptr, (T*)nullptr);
+        if (!rc) rc = launch(k_pcg_update<T>, n_vitems, VBS, stream, "k_pcg_update", table, vitems);
+        if (!rc) rc = launch(k_pcg_scalar, n_solves, VBS, stream, "k_pcg_scalar", table, solve_items);
+        if (rc) return rc;
+    }
+    return 0;
+}
+
+
+}  // namespace
+
+// ======================================================================================================
+// C ABI
+// ======================================================================================================
+extern "C" {
+
+int dfea_version(void) { return 100; }
+const char* dfea_last_error(void) { return g_err; }
+int dfea_desc_size(void) { return (int)sizeof(dfea_desc); }
+
+int dfea_desc_offset(int field) {
+    static const int off[] = {
+        (int)offsetof(dfea_desc, chunks), (int)offsetof(dfea_desc, lnode), (int)offsetof(dfea_desc, lelem),
+        (int)offsetof(dfea_desc, sh_node), (int)offsetof(dfea_desc, sh_off), (int)offsetof(dfea_desc, sh_slot),
+        (int)offsetof(dfea_desc, conn), (int)offsetof(dfea_desc, perm), (int)offsetof(dfea_desc, nodes),
+        (int)offsetof(dfea_desc, dNdxi), (int)offsetof(dfea_desc, gw), (int)offsetof(dfea_desc, nfix),
+        (int)offsetof(dfea_desc, dN64), (int)offsetof(dfea_desc, dN32), (int)offsetof(dfea_desc, w),
+        (int)offsetof(dfea_desc, theta), (int)offsetof(dfea_desc, C64), (int)offsetof(dfea_desc, C32),
+        (int)offsetof(dfea_desc, diag64), (int)offsetof(dfea_desc, diag32), (int)offsetof(dfea_desc, f),
+        (int)offsetof(dfea_desc, part), (int)offsetof(dfea_desc, red), (int)offsetof(dfea_desc, sc),
+        (int)offsetof(dfea_desc, vec), (int)offsetof(dfea_desc, mu0), (int)offsetof(dfea_desc, lam0),
+        (int)offsetof(dfea_desc, n_chunks), (int)offsetof(dfea_desc, n_lnodes), (int)offsetof(dfea_desc, n_shared),
+        (int)offsetof(dfea_desc, n_part), (int)offsetof(dfea_desc, n_el), (int)offsetof(dfea_desc, n_nodes),
+        (int)offsetof(dfea_desc, nen), (int)offsetof(dfea_desc, nq), (int)offsetof(dfea_desc, precision),
+        (int)offsetof(dfea_desc, n_finblk), (int)offsetof(dfea_desc, n_vblk), (int)offsetof(dfea_desc, pad0)};
+    if (field < 0 || field >= (int)(sizeof off / sizeof off[0])) return -1;
+    return off[field];
+}
+
+int dfea_red_size(int n_chunks, int n_finblk, int n_vblk) { return n_chunks + n_finblk + 3 * n_vblk; }
+int dfea_vblk(int n_dof) { return (n_dof + VBLK - 1) / VBLK; }
+int dfea_finblk(int n_shared) { return (n_shared + FBS - 1) / FBS; }
+int dfea_chunk_elems(int nen) { return nen == 4 ? CHUNK_Q4 : nen == 9 ? CHUNK_Q9 : -1; }
+int dfea_chunk_nodes(void) { return MAXLN; }
+
+int dfea_geometry(const dfea_desc* table, const int32_t* items, int n_items, int nen, void* stream) {
+    if (!table || !items || bad_nen(nen)) return fail(DFEA_EINVAL, "dfea_geometry: bad argument");
+    if (nen == 4) return launch(k_geometry<4, 4, CHUNK_Q4>, n_items, CHUNK_Q4, stream, "k_geometry", table, items);
+    return launch(k_geometry<9, 9, CHUNK_Q9>, n_items, CHUNK_Q9, stream, "k_geometry", table, items);
+}
+
+int dfea_moduli(const dfea_desc* table, const int32_t* items, int n_items, const int32_t* fin_items,
+                int n_fin_items, const double* u, int nen, int precision, void* stream) {
+    if (!table || !items || bad_nen(nen) || (precision != DFEA_FP64 && precision != DFEA_FP32))
+        return fail(DFEA_EINVAL, "dfea_moduli: bad argument");
+    int rc;
+    if (precision == DFEA_FP64) {
+        rc = nen == 4 ? launch(k_moduli<4, 4, CHUNK_Q4, double>, n_items, CHUNK_Q4, stream, "k_moduli", table, items, u)
+                      : launch(k_moduli<9, 9, CHUNK_Q9, double>, n_items, CHUNK_Q9, stream, "k_moduli", table, items, u);
+        if (rc) return rc;
+        return launch(k_moduli_fin<double>, n_fin_items, FBS, stream, "k_moduli_fin", table, fin_items);
+    }
+    rc = nen == 4 ? launch(k_moduli<4, 4, CHUNK_Q4, float>, n_items, CHUNK_Q4, stream, "k_moduli", table, items, u)
+                  : launch(k_moduli<9, 9, CHUNK_Q9, float>, n_items, CHUNK_Q9, stream, "k_moduli", table, items, u);
+    if (rc) return rc;
+    return launch(k_moduli_fin<float>, n_fin_items, FBS, stream, "k_moduli_fin", table, fin_items);
+}
+
+int dfea_residual(const dfea_desc* table, const int32_t* items, int n_items, const int32_t* fin_items,
+                  int n_fin_items, const double* u, double* R, double lf, int nen, void* stream) {
+    if (!table || !items || bad_nen(nen)) return fail(DFEA_EINVAL, "dfea_residual: bad argument");
+    int rc = nen == 4 ? launch(k_residual<4, 4, CHUNK_Q4>, n_items, CHUNK_Q4, stream, "k_residual", table, items, u, R, lf)
+                      : launch(k_residual<9, 9, CHUNK_Q9>, n_items, CHUNK_Q9, stream, "k_residual", table, items, u, R, lf);
+    if (rc) return rc;
+    return launch(k_residual_fin, n_fin_items, FBS, stream, "k_residual_fin", table, fin_items, R, lf);
+}
+
+int dfea_apply(const dfea_desc* table, const int32_t* items, int n_items, const int32_t* fin_items,
+               int n_fin_items, const void* v, void* y, int nen, int precision, void* stream) {
+    if (!table || !items || bad_nen(nen) || (precision != DFEA_FP64 && precision != DFEA_FP32))
+        return fail(DFEA_EINVAL, "dfea_apply: bad argument");
+    int rc;
+    if (precision == DFEA_FP64) {
+        const double* vv = static_cast<const double*>(v);
+        double* yy = static_cast<double*>(y);
+        rc = nen == 4 ? launch(k_apply<4, 4, CHUNK_Q4, double, false>, n_items, CHUNK_Q4, stream, "k_apply", table, items, vv, yy)
+                      : launch(k_apply<9, 9, CHUNK_Q9, double, false>, n_items, CHUNK_Q9, stream, "k_apply", table, items, vv, yy);
+        if (rc) return rc;
+        return launch(k_apply_fin<double, false>, n_fin_items, FBS, stream, "k_apply_fin", table, fin_items, vv, yy);
+    }
+    const float* vv = static_cast<const float*>(v);
+    float* yy = static_cast<float*>(y);
+    rc = nen == 4 ? launch(k_apply<4, 4, CHUNK_Q4, float, false>, n_items, CHUNK_Q4, stream, "k_apply", table, items, vv, yy)
+                  : launch(k_apply<9, 9, CHUNK_Q9, float, false>, n_items, CHUNK_Q9, stream, "k_apply", table, items, vv, yy);
+    if (rc) return rc;
+    return launch(k_apply_fin<float, false>, n_fin_items, FBS, stream, "k_apply_fin", table, fin_items, vv, yy);
+}
+
+int dfea_dot(const dfea_desc* table, const int32_t* items, int n_items, const int32_t* solve_items, int n_solves,
+             const double* x, const double* y, void* stream) {
+    if (!table || !items || !solve_items) return fail(DFEA_EINVAL, "dfea_dot: bad argument");
+    int rc = launch(k_dot_part, n_items, VBS, stream, "k_dot_part", table, items, x, y);
+    if (rc) return rc;
+    return launch(k_dot_final, n_solves, VBS, stream, "k_dot_final", table, solve_items);
+}
+
+int dfea_axpy(const dfea_desc* table, const int32_t* items, int n_items, int mode, const double* u, const void* dx,
+              const double* uD, const double* sarg, double* out, int precision, void* stream) {
+    if (!table || !items || !sarg || !out || (mode != 0 && mode != 1) || (mode == 0 && !uD))
+        return fail(DFEA_EINVAL, "dfea_axpy: bad argument");
+    if (precision == DFEA_FP64)
+        return launch(k_axpy<double>, n_items, VBS, stream, "k_axpy", table, items, mode, u,
+                      static_cast<const double*>(dx), uD, sarg, out);
+    return launch(k_axpy<float>, n_items, VBS, stream, "k_axpy", table, items, mode, u,
+                  static_cast<const float*>(dx), uD, sarg, out);
+}
+
+int dfea_pcg_init(const dfea_desc* table, const int32_t* vitems, int n_vitems, const int32_t* solve_items,
+                  int n_solves, int neg_res, int precision, void* stream) {
+    if (!table || !vitems || !solve_items) return fail(DFEA_EINVAL, "dfea_pcg_init: bad argument");
+    int rc = precision == DFEA_FP64
+                 ? launch(k_pcg_init<double>, n_vitems, VBS, stream, "k_pcg_init", table, vitems, neg_res)
+                 : launch(k_pcg_init<float>, n_vitems, VBS, stream, "k_pcg_init", table, vitems, neg_res);
+    if (rc) return rc;
+    return launch(k_pcg_init_final, n_solves, VBS, stream, "k_pcg_init_final", table, solve_items);
+}
+
+int dfea_pcg_iterate(const dfea_desc* table, const int32_t* items, int n_items, const int32_t* fin_items,
+                     int n_fin_items, const int32_t* vitems, int n_vitems, const int32_t* solve_items, int n_solves,
+                     int n_iter, int nen, int precision, void* stream) {
+    if (!table || !items || !vitems || !solve_items || bad_nen(nen) || n_iter < 0)
+        return fail(DFEA_EINVAL, "dfea_pcg_iterate: bad argument");
+    if (precision == DFEA_FP64)
+        return nen == 4 ? pcg_iter<4, CHUNK_Q4, double>(table, items, n_items, fin_items, n_fin_items, vitems, n_vitems,
+                                                        solve_items, n_solves, n_iter, stream)
+                        : pcg_iter<9, CHUNK_Q9, double>(table, items, n_items, fin_items, n_fin_items, vitems, n_vitems,
+                                                        solve_items, n_solves, n_iter, stream);
+    return nen == 4 ? pcg_iter<4, CHUNK_Q4, float>(table, items, n_items, fin_items, n_fin_items, vitems, n_vitems,
+                                                   solve_items, n_solves, n_iter, stream)
+                    : pcg_iter<9, CHUNK_Q9, float>(table, items, n_items, fin_items, n_fin_items, vitems, n_vitems,
+                                                   solve_items, n_solves, n_iter, stream);
+}
+
+int dfea_adjoint_grad(const dfea_desc* table, const int32_t* items, int n_items, const double* u, const double* lam,
+                      double* g, int nen, void* stream) {
+    if (!table || !items || !u || !lam || !g || bad_nen(nen)) return fail(DFEA_EINVAL, "dfea_adjoint_grad: bad argument");
+    if (nen == 4) return launch(k_adjoint<4, 4, CHUNK_Q4>, n_items, CHUNK_Q4, stream, "k_adjoint", table, items, u, lam, g);
+    return launch(k_adjoint<9, 9, CHUNK_Q9>, n_items, CHUNK_Q9, stream, "k_adjoint", table, items, u, lam, g);
+}
+
+}  // extern "C"

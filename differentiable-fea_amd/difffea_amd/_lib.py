@@ -1,0 +1,110 @@
+"""ctypes binding of libdfea_hip.so (the C ABI declared in include/dfea.h).
+
+The library is loaded after ``import torch`` so that it binds to the HIP runtime torch already loaded
+(both carry the soname ``libamdhip64.so.7``): one runtime, shared streams and device pointers.
+Loading fails loudly when the library has not been built -- there is no fallback path.
+"""
+import ctypes
+import os
+
+import torch  # noqa: F401  (must be imported first: it loads the HIP runtime the library binds to)
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_NAME = "libdfea_hip.so"
+LIB_PATH = os.path.join(_HERE, LIB_NAME)
+
+FP64, FP32 = 0, 1
+MAXCOL = 16
+FIXED_X, FIXED_Y, NODE_MASK = 0x40000000, 0x80000000, 0x3FFFFFFF
+VEC = dict(x=0, r=1, z=2, p=3, Ap=4, b=5, u=6, R=7)
+NVEC = 8
+SC = dict(rz=0, pAp=1, alpha=2, beta=3, rr=4, bb=5, rtol=6, status=7, k=8, maxit=9, iters=10, norm=11, lf=12,
+          step=13)
+NSC = 16
+PCG_RUNNING, PCG_CONVERGED, PCG_CURVATURE, PCG_MAXITER = 0, 1, 2, 3
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_D = ctypes.c_double
+
+
+class Desc(ctypes.Structure):
+    """Mirror of ``dfea_desc`` (field order = declaration order in include/dfea.h)."""
+    _fields_ = [
+        ("chunks", _P), ("lnode", _P), ("lelem", _P), ("sh_node", _P), ("sh_off", _P), ("sh_slot", _P),
+        ("conn", _P), ("perm", _P), ("nodes", _P), ("dNdxi", _P), ("gw", _P), ("nfix", _P),
+        ("dN64", _P), ("dN32", _P), ("w", _P), ("theta", _P), ("C64", _P), ("C32", _P),
+        ("diag64", _P), ("diag32", _P), ("f", _P), ("part", _P), ("red", _P), ("sc", _P),
+        ("vec", _P * NVEC), ("mu0", _D), ("lam0", _D),
+        ("n_chunks", ctypes.c_int32), ("n_lnodes", ctypes.c_int32), ("n_shared", ctypes.c_int32),
+        ("n_part", ctypes.c_int32), ("n_el", ctypes.c_int32), ("n_nodes", ctypes.c_int32),
+        ("nen", ctypes.c_int32), ("nq", ctypes.c_int32), ("precision", ctypes.c_int32),
+        ("n_finblk", ctypes.c_int32), ("n_vblk", ctypes.c_int32), ("pad0", ctypes.c_int32),
+    ]
+
+
+CHUNK_INTS = 32     # sizeof(dfea_chunk) / 4
+
+_SIGNATURES = {
+    "dfea_version": (_I, []),
+    "dfea_last_error": (ctypes.c_char_p, []),
+    "dfea_desc_size": (_I, []),
+    "dfea_desc_offset": (_I, [_I]),
+    "dfea_red_size": (_I, [_I, _I, _I]),
+    "dfea_vblk": (_I, [_I]),
+    "dfea_finblk": (_I, [_I]),
+    "dfea_chunk_elems": (_I, [_I]),
+    "dfea_chunk_nodes": (_I, []),
+    "dfea_geometry": (_I, [_P, _P, _I, _I, _P]),
+    "dfea_moduli": (_I, [_P, _P, _I, _P, _I, _P, _I, _I, _P]),
+    "dfea_residual": (_I, [_P, _P, _I, _P, _I, _P, _P, _D, _I, _P]),
+    "dfea_apply": (_I, [_P, _P, _I, _P, _I, _P, _P, _I, _I, _P]),
+    "dfea_dot": (_I, [_P, _P, _I, _P, _I, _P, _P, _P]),
+    "dfea_axpy": (_I, [_P, _P, _I, _I, _P, _P, _P, _P, _P, _I, _P]),
+    "dfea_pcg_init": (_I, [_P, _P, _I, _P, _I, _I, _I, _P]),
+    "dfea_pcg_iterate": (_I, [_P, _P, _I, _P, _I, _P, _I, _P, _I, _I, _I, _I, _P]),
+    "dfea_adjoint_grad": (_I, [_P, _P, _I, _P, _P, _P, _I, _P]),
+}
+
+_lib = None
+
+
+class DfeaError(RuntimeError):
+    pass
+
+
+def load(path=None):
+    """Load (once) and return the ctypes handle; raises if the library is missing."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = path or os.environ.get("DFEA_LIB", LIB_PATH)
+    if not os.path.exists(path):
+        raise DfeaError(f"{LIB_NAME} is not built (looked for {path}); run __graft_entry__.build() first")
+    lib = ctypes.CDLL(path)
+    for name, (res, args) in _SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if lib.dfea_desc_size() != ctypes.sizeof(Desc):
+        raise DfeaError(f"dfea_desc layout mismatch: C {lib.dfea_desc_size()} B, ctypes {ctypes.sizeof(Desc)} B")
+    for i, (fname, _) in enumerate(Desc._fields_):
+        if lib.dfea_desc_offset(i) != getattr(Desc, fname).offset:
+            raise DfeaError(f"dfea_desc field {fname}: C offset {lib.dfea_desc_offset(i)}, "
+                            f"ctypes {getattr(Desc, fname).offset}")
+    _lib = lib
+    return lib
+
+
+def check(rc, what):
+    """Map a C return code to a Python exception (ValueError for invalid arguments)."""
+    if rc == 0:
+        return
+    msg = _lib.dfea_last_error().decode() if _lib is not None else ""
+    if rc == -1:
+        raise ValueError(f"{what}: {msg}")
+    raise DfeaError(f"{what} failed ({rc}): {msg}")
+
+
+def exported_symbols():
+    return sorted(_SIGNATURES)

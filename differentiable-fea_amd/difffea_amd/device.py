@@ -1,0 +1,260 @@
+"""Device-resident state of the hot path and thin wrappers around the C ABI.
+
+``HipMesh``    plan tables + geometry cache (dN_dX planes in FP64 and FP32, w_detJ) of one mesh + mask.
+``HipSystem``  one solve on a mesh: theta, precision, moduli, Jacobi diagonal, PCG / Newton vectors and
+               its scratch (shared-node partials, reduction partials, scalar block); owns a descriptor.
+``Batch``      a descriptor table + work-item tables over several systems of one element type and
+               precision, so that one launch advances all of them.
+
+All memory is allocated through torch (the caching allocator) and lives on the current CUDA(HIP) device;
+kernels are enqueued on torch's current stream.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from .elements import element_type_of, reference_gradient_table
+from .plan import MeshPlan
+
+F64, F32 = torch.float64, torch.float32
+
+
+def _stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _ptr(t):
+    return ctypes.c_void_p(0 if t is None else t.data_ptr())
+
+
+def _dev_int32(a, device):
+    return torch.from_numpy(np.ascontiguousarray(a).view(np.int32)).to(device)
+
+
+def _items(s, n, device):
+    it = np.zeros((max(n, 0), 2), np.int32)
+    it[:, 0] = s
+    it[:, 1] = np.arange(n)
+    return torch.from_numpy(it).to(device)
+
+
+class HipMesh:
+    """Mesh plan + geometry on the device.  ``params``: the reference's params contract."""
+
+    def __init__(self, params, device=None):
+        self.lib = _lib.load()
+        device = torch.device(device or "cuda")
+        if device.type != "cuda":
+            raise ValueError("HipMesh lives on a GPU device")
+        self.device = device
+        nodes = params["nodes"].detach().to("cpu", F64)
+        conn = params["conn"].detach().to("cpu", torch.int64)
+        fixed = params["fixed"].detach().to("cpu", torch.bool)
+        self.element_type = params.get("element_type") or element_type_of(conn.shape[1])
+        self.nen = conn.shape[1]
+        GP = params["GP"].detach().to("cpu", F64).numpy()
+        GW = params["GW"].detach().to("cpu", F64).numpy()
+        self.nq = GP.shape[0]
+        if (self.nen, self.nq) not in ((4, 4), (9, 9)):
+            raise ValueError(f"the hip kernels support quad4 with 2x2 and quad9 with 3x3 Gauss points, "
+                             f"got nen={self.nen}, n_q={self.nq}")
+        cij = params.get("cell_ij")
+        self.plan = p = MeshPlan(nodes.numpy(), conn.numpy(), fixed.numpy(),
+                                 cell_ij=None if cij is None else cij.detach().cpu().numpy())
+        self.n_el, self.n_nodes = p.n_el, p.n_nodes
+        self.n_dof = 2 * p.n_nodes
+        dv = device
+        self.t_chunks = _dev_int32(p.chunks, dv)
+        self.t_lnode = _dev_int32(p.lnode, dv)
+        self.t_lelem = torch.from_numpy(p.lelem.view(np.int16)).to(dv)
+        self.t_sh_node = _dev_int32(p.sh_node, dv)
+        self.t_sh_off = _dev_int32(p.sh_off, dv)
+        self.t_sh_slot = _dev_int32(p.sh_slot if len(p.sh_slot) else np.zeros(1, np.int32), dv)
+        self.t_conn = _dev_int32(p.conn_int, dv)
+        self.t_perm = _dev_int32(p.perm, dv)
+        self.perm = torch.from_numpy(p.perm.astype(np.int64)).to(dv)
+        self.t_nodes = nodes.contiguous().to(dv)
+        self.t_dNdxi = torch.from_numpy(reference_gradient_table(self.element_type, GP)).contiguous().to(dv)
+        self.t_gw = torch.from_numpy(GW).to(dv)
+        self.t_nfix = torch.from_numpy(p.nfix).to(dv)
+        self.dN64 = torch.empty(self.nq * self.nen * 2, self.n_el, dtype=F64, device=dv)
+        self.dN32 = torch.empty(self.nq * self.nen * 2, self.n_el, dtype=F32, device=dv)
+        self.w = torch.empty(self.nq, self.n_el, dtype=F64, device=dv)
+        self.n_vblk = self.lib.dfea_vblk(self.n_dof)
+        self.n_finblk = self.lib.dfea_finblk(p.n_shared)
+        self.red_size = self.lib.dfea_red_size(p.n_chunks, self.n_finblk, self.n_vblk)
+        # single-solve item tables (solve index 0)
+        self.items = _items(0, p.n_chunks, dv)
+        self.fin_items = _items(0, self.n_finblk, dv) if self.n_finblk else torch.zeros(1, 2, dtype=torch.int32, device=dv)
+        self.vitems = _items(0, self.n_vblk, dv)
+        self.solve_items = _items(0, 1, dv)
+        # geometry, once
+        d = self.fill_desc(_lib.Desc())
+        tab = self.upload(d)
+        _lib.check(self.lib.dfea_geometry(_ptr(tab), _ptr(self.items), p.n_chunks, self.nen, _stream()),
+                   "dfea_geometry")
+        self._geom_table = tab
+
+    def fill_desc(self, d):
+        p = self.plan
+        d.chunks, d.lnode, d.lelem = self.t_chunks.data_ptr(), self.t_lnode.data_ptr(), self.t_lelem.data_ptr()
+        d.sh_node, d.sh_off, d.sh_slot = self.t_sh_node.data_ptr(), self.t_sh_off.data_ptr(), self.t_sh_slot.data_ptr()
+        d.conn, d.perm, d.nodes = self.t_conn.data_ptr(), self.t_perm.data_ptr(), self.t_nodes.data_ptr()
+        d.dNdxi, d.gw, d.nfix = self.t_dNdxi.data_ptr(), self.t_gw.data_ptr(), self.t_nfix.data_ptr()
+        d.dN64, d.dN32, d.w = self.dN64.data_ptr(), self.dN32.data_ptr(), self.w.data_ptr()
+        d.n_chunks, d.n_lnodes, d.n_shared, d.n_part = p.n_chunks, p.n_lnodes, p.n_shared, p.n_part
+        d.n_el, d.n_nodes, d.nen, d.nq = p.n_el, p.n_nodes, self.nen, self.nq
+        d.n_finblk, d.n_vblk = self.n_finblk, self.n_vblk
+        return d
+
+    def upload(self, d):
+        raw = torch.frombuffer(bytearray(bytes(d)), dtype=torch.uint8)
+        return raw.to(self.device)
+
+    # -- element-order helpers ---------------------------------------------------------------------------
+    def to_internal(self, per_element):
+        """Per-element values (original order) -> internal chunk order."""
+        return per_element.to(self.device, F64)[self.perm].contiguous()
+
+
+class HipSystem:
+    """One solve (mesh + theta + material + precision) with all its device buffers."""
+
+    def __init__(self, mesh, theta, mat_p, f, precision="fp64"):
+        self.mesh = m = mesh
+        self.lib = mesh.lib
+        dv = mesh.device
+        if precision not in ("fp64", "mixed"):
+            raise ValueError(f"precision must be 'fp64' or 'mixed', got {precision!r}")
+        self.precision = precision
+        self.prec_id = _lib.FP64 if precision == "fp64" else _lib.FP32
+        self.vdt = F64 if precision == "fp64" else F32
+        self.theta = mesh.to_internal(theta)
+        mp = torch.as_tensor(mat_p, dtype=F64).cpu()
+        self.mu0, self.lam0 = float(mp[0]), float(mp[1])
+        self.f = f.to(dv, F64).contiguous()
+        nq, n_el, nd = m.nq, m.n_el, m.n_dof
+        self.C = None                    # moduli and diagonal: allocated by the first moduli() call
+        self.diag = None
+        self.part = torch.empty(max(1, m.plan.n_part) * 2, dtype=F64, device=dv)
+        self.red = torch.zeros(max(1, m.red_size), dtype=F64, device=dv)
+        self.sc = torch.zeros(_lib.NSC, dtype=F64, device=dv)
+        self.vec = {k: torch.zeros(nd, dtype=self.vdt, device=dv) for k in ("x", "r", "z", "p", "Ap", "b")}
+        self.vec["u"] = torch.zeros(nd, dtype=F64, device=dv)
+        self.vec["R"] = torch.zeros(nd, dtype=F64, device=dv)
+        self.desc = self._make_desc()
+        self.table = m.upload(self.desc)
+
+    def _make_desc(self):
+        d = self.mesh.fill_desc(_lib.Desc())
+        d.theta = self.theta.data_ptr()
+        if self.C is not None:
+            if self.prec_id == _lib.FP64:
+                d.C64, d.diag64 = self.C.data_ptr(), self.diag.data_ptr()
+            else:
+                d.C32, d.diag32 = self.C.data_ptr(), self.diag.data_ptr()
+        d.f, d.part, d.red, d.sc = self.f.data_ptr(), self.part.data_ptr(), self.red.data_ptr(), self.sc.data_ptr()
+        for k, i in _lib.VEC.items():
+            d.vec[i] = self.vec[k].data_ptr()
+        d.mu0, d.lam0 = self.mu0, self.lam0
+        d.precision = self.prec_id
+        return d
+
+    # -- kernels on this single system -------------------------------------------------------------------
+    def _ensure_moduli_buffers(self):
+        if self.C is None:
+            m = self.mesh
+            self.C = torch.empty(10 * m.nq, m.n_el, dtype=self.vdt, device=m.device)
+            self.diag = torch.empty(m.n_dof, dtype=self.vdt, device=m.device)
+            self.desc = self._make_desc()
+            self.table = m.upload(self.desc)
+
+    def moduli(self, u):
+        self._ensure_moduli_buffers()
+        m = self.mesh
+        u = u.to(F64).contiguous()
+        _lib.check(self.lib.dfea_moduli(_ptr(self.table), _ptr(m.items), m.plan.n_chunks, _ptr(m.fin_items),
+                                        m.n_finblk, _ptr(u), m.nen, self.prec_id, _stream()), "dfea_moduli")
+
+    def residual(self, u, out=None, lf=1.0):
+        m = self.mesh
+        u = u.to(F64).contiguous()
+        out = torch.empty_like(u) if out is None else out
+        _lib.check(self.lib.dfea_residual(_ptr(self.table), _ptr(m.items), m.plan.n_chunks, _ptr(m.fin_items),
+                                          m.n_finblk, _ptr(u), _ptr(out), float(lf), m.nen, _stream()),
+                   "dfea_residual")
+        return out
+
+    def apply(self, v, out=None):
+        m = self.mesh
+        if v.dtype != self.vdt or not v.is_contiguous():
+            v = v.to(self.vdt).contiguous()
+        out = torch.empty_like(v) if out is None else out
+        _lib.check(self.lib.dfea_apply(_ptr(self.table), _ptr(m.items), m.plan.n_chunks, _ptr(m.fin_items),
+                                       m.n_finblk, _ptr(v), _ptr(out), m.nen, self.prec_id, _stream()),
+                   "dfea_apply")
+        return out
+
+    def dot(self, x, y=None):
+        """Deterministic FP64 dot product on the device; returns a 0-d device tensor (no host sync)."""
+        m = self.mesh
+        x = x.to(F64).contiguous()
+        y = x if y is None else y.to(F64).contiguous()
+        _lib.check(self.lib.dfea_dot(_ptr(self.table), _ptr(m.vitems), m.n_vblk, _ptr(m.solve_items), 1,
+                                     _ptr(x), _ptr(y), _stream()), "dfea_dot")
+        return self.sc[_lib.SC["norm"]].clone()
+
+    def norm(self, x):
+        return torch.sqrt(self.dot(x))
+
+    def lift(self, u0, uD, lf):
+        """where(fixed, lf * u_D, u0) on the device."""
+        m = self.mesh
+        out = torch.empty(m.n_dof, dtype=F64, device=m.device)
+        a = torch.full((1,), float(lf), dtype=F64, device=m.device)
+        _lib.check(self.lib.dfea_axpy(_ptr(self.table), _ptr(m.vitems), m.n_vblk, 0, _ptr(u0.to(F64).contiguous()),
+                                      None, _ptr(uD.to(F64).contiguous()), _ptr(a), _ptr(out), self.prec_id,
+                                      _stream()), "dfea_axpy")
+        return out
+
+    def adjoint_grad(self, u, lam):
+        m = self.mesh
+        g = torch.empty(m.n_el, dtype=F64, device=m.device)
+        _lib.check(self.lib.dfea_adjoint_grad(_ptr(self.table), _ptr(m.items), m.plan.n_chunks,
+                                              _ptr(u.to(F64).contiguous()), _ptr(lam.to(F64).contiguous()), _ptr(g),
+                                              m.nen, _stream()), "dfea_adjoint_grad")
+        return g
+
+    # -- fused PCG ---------------------------------------------------------------------------------------------
+    def pcg_start(self, b=None, rtol=1e-5, max_iter=500, neg_res=False):
+        """Set up PCG for A x = b (b None & neg_res: b = -vec['R']); needs moduli() first."""
+        if self.C is None:
+            raise RuntimeError("pcg_start needs the moduli of the linearisation point (call moduli(u) first)")
+        m = self.mesh
+        if b is not None:
+            self.vec["b"].copy_(b)
+        self.sc[_lib.SC["rtol"]] = float(rtol)
+        self.sc[_lib.SC["maxit"]] = float(max_iter)
+        _lib.check(self.lib.dfea_pcg_init(_ptr(self.table), _ptr(m.vitems), m.n_vblk, _ptr(m.solve_items), 1,
+                                          int(neg_res), self.prec_id, _stream()), "dfea_pcg_init")
+
+    def pcg_iterate(self, n_iter):
+        m = self.mesh
+        _lib.check(self.lib.dfea_pcg_iterate(_ptr(self.table), _ptr(m.items), m.plan.n_chunks, _ptr(m.fin_items),
+                                             m.n_finblk, _ptr(m.vitems), m.n_vblk, _ptr(m.solve_items), 1,
+                                             int(n_iter), m.nen, self.prec_id, _stream()), "dfea_pcg_iterate")
+
+    def pcg_status(self):
+        s = self.sc[[_lib.SC["status"], _lib.SC["iters"], _lib.SC["k"]]].cpu()
+        return int(s[0]), int(s[1]), int(s[2])
+
+    def pcg_solve(self, b=None, rtol=1e-5, max_iter=500, neg_res=False, chunk=32):
+        """Run the fused PCG to completion; returns (x view, iterations, status)."""
+        self.pcg_start(b, rtol, max_iter, neg_res)
+        st, its, k = self.pcg_status()
+        while st == _lib.PCG_RUNNING:
+            self.pcg_iterate(min(chunk, max_iter - k))
+            st, its, k = self.pcg_status()
+        return self.vec["x"], its, st

@@ -1,0 +1,141 @@
+"""The ``hip`` tangent / preconditioner / residual variants behind the reference's operator API.
+
+Mirrors `src/difffea/operators.py`:
+
+* ``make_tangent_operator(kind, u, theta, params) -> A``      (`operators.py:415-435`)
+  ``A(v) = m * J(u)(m v) + (1 - m) v``; kinds ``TANGENT_OPERATORS = ("auto", "hip")``; unknown kind ->
+  ``ValueError``.  ``"auto"`` is ``"hip"`` (this package only ships the hip variant).
+* ``make_AT_operator(u, theta, params)``                       (`operators.py:441-471`)
+  the hyperelastic tangent is symmetric, so the transpose product is the same kernel.
+* ``make_preconditioner(kind, u, theta, params)``              (`operators.py:585, 702-710`)
+  ``"jacobi"``: ``r -> r / diag`` with the diagonal from the fused moduli kernel (clamp 1e-6, pivot 1).
+* ``compute_jacobi_diagonal(u, theta, params)``                (`operators.py:477-556`)
+* ``global_residual(u, theta, params, load_factor)``           (`operators.py:159-220`)
+
+Every closure keeps its own device buffers (moduli, diagonal), so closures built at different states stay
+valid side by side, as the reference's functional closures do.
+"""
+import weakref
+
+import torch
+
+from . import _lib
+from .device import HipMesh, HipSystem
+
+TANGENT_OPERATORS = ("auto", "hip")
+PRECONDITIONERS = ("jacobi",)
+PRECISIONS = ("fp64", "mixed")
+
+_MESHES = {}
+
+
+def _check_material(params):
+    """Neo-Hookean with theta scaling both Lame parameters -> (mu0, lam0) tensor."""
+    if "material" in params:
+        if params["material"] != "neo_hookean":
+            raise ValueError(f"the hip kernels implement the neo_hookean material, not {params['material']!r}")
+        sc = params.get("mat_scalable")
+        if sc is not None and not bool(torch.as_tensor(sc).all()):
+            raise ValueError("the hip kernels scale both neo-Hookean parameters by theta")
+        return torch.as_tensor(params["mat_p"], dtype=torch.float64).cpu()
+    return torch.tensor([float(params["mu0"]), float(params["lmbda0"])], dtype=torch.float64)
+
+
+def get_mesh(params, device=None):
+    """Device plan + geometry of ``params`` (built once, cached while the mesh tensors are alive)."""
+    device = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
+    if device.type != "cuda":
+        raise ValueError("the hip variant runs on a GPU: move the state to a cuda (ROCm) device")
+    key = (id(params["nodes"]), id(params["conn"]), id(params["fixed"]), str(device))
+    ent = _MESHES.get(key)
+    if ent is not None:
+        refs, mesh = ent
+        if all(r() is not None for r in refs):
+            return mesh
+    with torch.cuda.device(device):
+        mesh = HipMesh(params, device)
+    _MESHES[key] = ([weakref.ref(params["nodes"]), weakref.ref(params["conn"]), weakref.ref(params["fixed"])], mesh)
+    return mesh
+
+
+def make_system(u, theta, params, precision="fp64"):
+    if precision not in PRECISIONS:
+        raise ValueError(f"Unknown precision '{precision}'. Choose one of {PRECISIONS}.")
+    mesh = get_mesh(params, u.device)
+    return HipSystem(mesh, theta.detach(), _check_material(params), params["f"], precision)
+
+
+class HipTangent:
+    """Callable closure ``v -> A(v)`` holding the moduli and diagonal at its linearisation point."""
+
+    def __init__(self, u, theta, params, precision="fp64"):
+        self.system = make_system(u, theta, params, precision)
+        self.system.moduli(u.detach())
+        self._u = u
+
+    def __call__(self, v):
+        y = self.system.apply(v)
+        return y if y.dtype == v.dtype else y.to(v.dtype)
+
+    @property
+    def diag(self):
+        return self.system.diag
+
+
+_LAST = {}
+
+
+def make_A_operator_hip(u, theta, params, precision="fp64"):
+    A = HipTangent(u, theta, params, precision)
+    _LAST["op"] = (weakref.ref(u), u._version, id(theta), id(params), A)
+    return A
+
+
+def make_tangent_operator(kind, u, theta, params, precision="fp64"):
+    """Build the forward tangent operator by name (``"auto"`` or ``"hip"``)."""
+    if kind in ("auto", "hip"):
+        return make_A_operator_hip(u, theta, params, precision)
+    raise ValueError(f"Unknown tangent operator '{kind}'. Choose one of {TANGENT_OPERATORS}.")
+
+
+def make_AT_operator(u, theta, params, precision="fp64"):
+    """Transpose product; the hyperelastic tangent with the symmetric pivot is symmetric, so A^T = A."""
+    return make_A_operator_hip(u, theta, params, precision)
+
+
+def _diag_at(u, theta, params, precision="fp64"):
+    ent = _LAST.get("op")
+    if ent is not None:
+        ref, ver, tid, pid, A = ent
+        if ref() is u and u._version == ver and tid == id(theta) and pid == id(params) and \
+                A.system.precision == precision:
+            return A.diag
+    return HipTangent(u, theta, params, precision).diag
+
+
+def compute_jacobi_diagonal(u, theta, params):
+    """diag(A): free dofs clamped to >= 1e-6, constrained dofs 1 (FP64)."""
+    return _diag_at(u, theta, params).clone()
+
+
+def make_jacobi_preconditioner(u, theta, params, precision="fp64"):
+    diag = _diag_at(u, theta, params, precision)
+    return lambda r: r / (diag if diag.dtype == r.dtype else diag.to(r.dtype))
+
+
+def make_preconditioner(kind, u, theta, params, precision="fp64"):
+    """Build a preconditioner by name: ``"jacobi"``."""
+    if kind == "jacobi":
+        return make_jacobi_preconditioner(u, theta, params, precision)
+    raise ValueError(f"Unknown preconditioner '{kind}'. Choose one of {PRECONDITIONERS}.")
+
+
+def global_residual(u, theta, params, load_factor=1.0):
+    """R = m * (R_int(u) - lf f) with the analytic neo-Hookean stress (FP64)."""
+    sys_ = make_system(u, theta, params, "fp64")
+    return sys_.residual(u.detach(), lf=float(load_factor))
+
+
+__all__ = ["TANGENT_OPERATORS", "PRECONDITIONERS", "make_tangent_operator", "make_AT_operator",
+           "make_preconditioner", "make_jacobi_preconditioner", "compute_jacobi_diagonal", "global_residual",
+           "get_mesh", "make_system", "HipTangent", "_lib"]

@@ -1,0 +1,241 @@
+"""Mesh plan: the integer tables that let one workgroup assemble one chunk of elements deterministically.
+
+Built once per (mesh, Dirichlet mask) on the host with vectorised numpy; uploaded to HBM once.
+
+1. Element order.  Elements are sorted along a Morton (Z) curve of their cell coordinates, then cut into
+   chunks of ``chunk_elems`` consecutive elements.  On structured grids the cell coordinates are the exact
+   integer cell indices (detected by rank), so chunks are square tiles (16x16 quad4, 8x8 quad9); otherwise
+   element centroids are quantised onto a power-of-two grid.
+2. Colours.  Elements are coloured so that two elements of one colour never share a node: the parity of
+   the cell indices on structured grids (4 colours), else Jones-Plassmann on the node-sharing graph.  Inside
+   a chunk the elements are sorted by colour, so each colour is a contiguous range of threads.
+3. Local nodes.  Each chunk lists its distinct nodes, owned nodes first (used by no other chunk), then
+   shared nodes.  ``lelem[a][e]`` is the local index of element node a.  Fixed dofs are folded into the
+   node ids as bits 30/31.
+4. Shared nodes.  Every (chunk, shared node) pair gets a partial slot; ``sh_off``/``sh_slot`` list the slots
+   of each shared node in chunk order, which fixes the order of the final summation.
+
+Nothing here depends on which other solves run in the same batch, so results are batch-independent.
+"""
+import numpy as np
+
+from . import _lib
+
+
+def _part1by1(x):
+    x = x.astype(np.uint64) & np.uint64(0xFFFFFFFF)
+    x = (x | (x << np.uint64(16))) & np.uint64(0x0000FFFF0000FFFF)
+    x = (x | (x << np.uint64(8))) & np.uint64(0x00FF00FF00FF00FF)
+    x = (x | (x << np.uint64(4))) & np.uint64(0x0F0F0F0F0F0F0F0F)
+    x = (x | (x << np.uint64(2))) & np.uint64(0x3333333333333333)
+    x = (x | (x << np.uint64(1))) & np.uint64(0x5555555555555555)
+    return x
+
+
+def morton(ix, iy):
+    return _part1by1(ix) | (_part1by1(iy) << np.uint64(1))
+
+
+def _rank(vals, tol):
+    """Rank of every value among the distinct values (values closer than tol are equal)."""
+    q = np.round(vals / tol).astype(np.int64)
+    u, inv = np.unique(q, return_inverse=True)
+    return inv, len(u)
+
+
+def cell_coordinates(nodes, conn):
+    """Integer cell coordinates of the elements and whether they form an exact structured grid."""
+    c = nodes[conn].mean(axis=1)
+    n_el = conn.shape[0]
+    ext = max(float(np.ptp(c[:, 0])), float(np.ptp(c[:, 1])), 1e-300)
+    ix, nx = _rank(c[:, 0], 1e-9 * ext)
+    iy, ny = _rank(c[:, 1], 1e-9 * ext)
+    if nx * ny == n_el and len(np.unique(ix * ny + iy)) == n_el:
+        return ix, iy, True
+    bits = max(1, int(np.ceil(np.log2(max(2.0, 2.0 * np.sqrt(n_el))))))
+    g = (1 << bits) - 1
+    lo = c.min(axis=0)
+    sc = np.ptp(c, axis=0)
+    sc[sc == 0] = 1.0
+    q = np.floor((c - lo) / sc * g).astype(np.int64)
+    return q[:, 0], q[:, 1], False
+
+
+def _parity_colouring_valid(conn, ix, iy):
+    """True when every two node-sharing elements differ in the parity of ix or of iy, so that the
+    4-colour parity colouring is a valid colouring (fails e.g. for periodic grids of odd period)."""
+    a, b = _node_sharing_pairs(conn)
+    return bool(np.all((ix[a] % 2 != ix[b] % 2) | (iy[a] % 2 != iy[b] % 2)))
+
+
+def _node_sharing_pairs(conn):
+    """All (e1, e2), e1 != e2, of elements that share at least one node."""
+    n_el, nen = conn.shape
+    e = np.repeat(np.arange(n_el), nen)
+    n = conn.reshape(-1)
+    order = np.argsort(n, kind="stable")
+    n_s, e_s = n[order], e[order]
+    starts = np.flatnonzero(np.r_[True, n_s[1:] != n_s[:-1]])
+    counts = np.diff(np.r_[starts, len(n_s)])
+    a_list, b_list = [], []
+    for k in range(1, counts.max()):
+        sel = starts[counts > k]
+        # pair element at position start+j with element at start+k, for all j < k
+        for j in range(k):
+            a_list.append(e_s[sel + j])
+            b_list.append(e_s[sel + k])
+    if not a_list:
+        return np.zeros(0, np.int64), np.zeros(0, np.int64)
+    a = np.concatenate(a_list)
+    b = np.concatenate(b_list)
+    keep = a != b
+    a, b = a[keep], b[keep]
+    return np.r_[a, b], np.r_[b, a]
+
+
+def jones_plassmann(conn, seed=0):
+    """Distance-1 colouring of the node-sharing graph (elements sharing a node get different colours).
+
+    Each round selects the uncoloured elements whose random priority beats every uncoloured neighbour
+    (an independent set) and gives each the smallest colour none of its coloured neighbours uses.
+    """
+    n_el = conn.shape[0]
+    a, b = _node_sharing_pairs(conn)
+    rng = np.random.default_rng(seed)
+    prio = rng.permutation(n_el)
+    colour = np.full(n_el, -1, np.int64)
+    used = np.zeros(n_el, np.int64)              # bit c set: a neighbour has colour c
+    while (colour < 0).any():
+        un = colour < 0
+        m = un[a] & un[b] & (prio[b] > prio[a])
+        blocked = np.zeros(n_el, bool)
+        blocked[a[m]] = True
+        sel = np.flatnonzero(un & ~blocked)
+        u = used[sel]
+        c = np.zeros(len(sel), np.int64)
+        while True:
+            taken = (u >> c) & 1
+            if not taken.any():
+                break
+            c += taken
+        colour[sel] = c
+        # tell the neighbours of the newly coloured elements
+        nb = np.isin(b, sel) if len(sel) < n_el else np.ones(len(b), bool)
+        np.bitwise_or.at(used, a[nb], np.int64(1) << colour[b[nb]])
+    return colour
+
+
+class MeshPlan:
+    """Integer tables of one mesh + Dirichlet mask (numpy arrays, see module docstring)."""
+
+    def __init__(self, nodes, conn, fixed, chunk_elems=None, max_nodes=None, cell_ij=None):
+        nodes = np.asarray(nodes, np.float64)
+        conn = np.asarray(conn, np.int64)
+        fixed = np.asarray(fixed, bool).reshape(-1, 2)
+        n_el, nen = conn.shape
+        N = nodes.shape[0]
+        lib = _lib.load()
+        E = chunk_elems or lib.dfea_chunk_elems(nen)
+        MAXLN = max_nodes or lib.dfea_chunk_nodes()
+        if E <= 0:
+            raise ValueError(f"no chunk size for elements with {nen} nodes")
+        if N >= (1 << 30):
+            raise ValueError("meshes with 2^30 or more nodes are not supported")
+        self.n_el, self.nen, self.n_nodes = n_el, nen, N
+
+        # 1. order
+        if cell_ij is not None:                 # logical cell indices supplied by the mesh generator
+            cij = np.asarray(cell_ij, np.int64)
+            ix, iy, structured = cij[:, 0], cij[:, 1], True
+        else:
+            ix, iy, structured = cell_coordinates(nodes, conn)
+        key = morton(ix, iy)
+        order = np.argsort(key, kind="stable")
+        # 2. colours
+        if structured and _parity_colouring_valid(conn, ix, iy):
+            colour = (ix % 2) * 2 + (iy % 2)
+        else:
+            colour = jones_plassmann(conn)
+        if colour.max() >= _lib.MAXCOL:
+            raise ValueError(f"mesh needs {colour.max() + 1} colours, more than {_lib.MAXCOL}")
+        n_chunks = (n_el + E - 1) // E
+        chunk_of_pos = np.arange(n_el) // E
+        # within a chunk: sort by colour, keep Morton order inside a colour
+        perm = order[np.lexsort((np.arange(n_el), colour[order], chunk_of_pos))]
+        self.perm = perm.astype(np.int32)                          # internal e -> original element
+        ce = conn[perm]                                            # (n_el, nen), internal order
+        ch_e = chunk_of_pos                                        # chunk of internal element
+        col_sorted = colour[perm]
+
+        # colour ranges per chunk
+        chunks = np.zeros((n_chunks, _lib.CHUNK_INTS), np.int32)
+        e0 = np.arange(n_chunks) * E
+        ne = np.minimum(E, n_el - e0)
+        chunks[:, 0] = e0
+        chunks[:, 1] = ne
+        ncol = np.zeros(n_chunks, np.int32)
+        cnt = np.zeros((n_chunks, _lib.MAXCOL), np.int64)
+        np.add.at(cnt, (ch_e, col_sorted), 1)
+        # compact the colours present in each chunk (empty colours are skipped)
+        for c in range(n_chunks):
+            present = cnt[c][cnt[c] > 0]
+            ncol[c] = len(present)
+            chunks[c, 8:8 + len(present) + 1] = np.r_[0, np.cumsum(present)]
+        chunks[:, 6] = ncol
+
+        # 3. local nodes: distinct (chunk, node) pairs, owned first
+        pc = np.repeat(ch_e, nen)
+        pn = ce.reshape(-1)
+        pk = pc.astype(np.int64) * N + pn
+        upk = np.unique(pk)
+        uc, un = upk // N, upk % N
+        touch = np.bincount(un, minlength=N)                       # chunks touching each node
+        shared = touch[un] > 1
+        k2 = uc * (2 * N) + shared * N + un                        # owned first inside each chunk
+        o2 = np.argsort(k2, kind="stable")
+        uc, un, shared, k2 = uc[o2], un[o2], shared[o2], k2[o2]
+        k0 = np.searchsorted(uc, np.arange(n_chunks))
+        nk = np.diff(np.r_[k0, len(uc)])
+        if nk.max() > MAXLN:
+            raise ValueError(f"a chunk touches {nk.max()} nodes, more than {MAXLN}")
+        nown = np.bincount(uc[~shared], minlength=n_chunks)
+        chunks[:, 2] = k0
+        chunks[:, 3] = nk
+        chunks[:, 4] = nown
+        bits = fixed[:, 0].astype(np.uint64) * np.uint64(_lib.FIXED_X) + \
+            fixed[:, 1].astype(np.uint64) * np.uint64(_lib.FIXED_Y)
+        self.lnode = (un.astype(np.uint64) | bits[un]).astype(np.uint32)
+        # local index of every element node
+        key_e = pc.astype(np.int64) * (2 * N) + (touch[pn] > 1) * N + pn
+        pos = np.searchsorted(k2, key_e)
+        assert np.all(k2[pos] == key_e)
+        loc = (pos - k0[pc]).reshape(n_el, nen)
+        self.lelem = np.ascontiguousarray(loc.T).astype(np.uint16)      # [nen][n_el]
+
+        # 4. shared nodes and partial slots
+        sh_pos = np.flatnonzero(shared)                            # in (chunk, owned-first) order
+        slot_of = np.full(len(uc), -1, np.int64)
+        slot_of[sh_pos] = np.arange(len(sh_pos))
+        pbase = np.zeros(n_chunks, np.int64)
+        first_sh = k0 + nown
+        has = nk > nown
+        pbase[has] = slot_of[first_sh[has]]
+        chunks[:, 5] = pbase
+        so = np.lexsort((uc[sh_pos], un[sh_pos]))                  # by node, then chunk
+        sh_nodes_all = un[sh_pos][so]
+        self.sh_slot = slot_of[sh_pos][so].astype(np.int32)
+        snodes, sstart = np.unique(sh_nodes_all, return_index=True)
+        self.sh_off = np.r_[sstart, len(sh_nodes_all)].astype(np.int32)
+        self.sh_node = (snodes.astype(np.uint64) | bits[snodes]).astype(np.uint32)
+        self.chunks = chunks
+        self.conn_int = np.ascontiguousarray(ce.T).astype(np.int32)     # [nen][n_el]
+        self.n_chunks = n_chunks
+        self.n_lnodes = len(un)
+        self.n_shared = len(snodes)
+        self.n_part = len(sh_pos)
+        self.structured = structured
+        self.nfix = (fixed[:, 0].astype(np.uint8) | (fixed[:, 1].astype(np.uint8) << 1))
+
+    def stats(self):
+        return dict(n_chunks=self.n_chunks, n_lnodes=self.n_lnodes, n_shared=self.n_shared, n_part=self.n_part,
+                    max_chunk_nodes=int(self.chunks[:, 3].max()), max_colours=int(self.chunks[:, 6].max()))

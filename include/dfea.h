@@ -1,0 +1,170 @@
+/*
+ * dfea.h -- C ABI of libdfea_hip.so, the MI355X (gfx950) matrix-free Newton-Krylov hot path of
+ * Differentiable-FEA (plane-strain compressible neo-Hookean, quad4 / quad9 elements).
+ *
+ * Conventions
+ *   - Every entry point returns 0 on success, DFEA_EINVAL (-1) for an invalid argument, or
+ *     -(1000 + hipError_t) when a HIP call failed.  dfea_last_error() gives a message for the last failure.
+ *   - All work is enqueued on the caller's stream (hipStream_t passed as void*); no entry point
+ *     synchronises the stream except where stated (dfea_pcg_run reads a status block at chunk ends).
+ *   - The caller allocates and owns every buffer.  Descriptors (dfea_desc) hold device pointers into those
+ *     buffers; a descriptor table in device memory describes a batch of independent solves.
+ *   - Work items: an int32 pair {solve index, block index} per workgroup.  For one solve, the items are
+ *     simply {0, 0}, {0, 1}, ... ; for a batch, the concatenation over solves.
+ *
+ * Data layout in HBM (internal element order = chunk order, see dfea_chunk)
+ *   dN[(q*nen + a)*2 + j][e]   shape-function gradients dN_a/dX_j at Gauss point q (planes of n_el)
+ *   w[q][e]                    Gauss weight * det(dX/dxi)
+ *   C[k*nq + q][e]             the 10 unique symmetric tangent moduli, k over
+ *                              (00,01,02,03,11,12,13,22,23,33) of the flat index m = 2i+j,
+ *                              already multiplied by w (`src/difffea/operators.py:316-342`)
+ *   vectors                    node-major dofs 2a+i, as the reference (`boundary_conditions.py:171-179`)
+ */
+#ifndef DFEA_H
+#define DFEA_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DFEA_EINVAL (-1)
+#define DFEA_MAXCOL 16         /* element colours per chunk */
+#define DFEA_FIXED_X 0x40000000u
+#define DFEA_FIXED_Y 0x80000000u
+#define DFEA_NODE_MASK 0x3fffffffu
+
+enum { DFEA_FP64 = 0, DFEA_FP32 = 1 };
+
+/* vector slots of a descriptor */
+enum { DFEA_X = 0, DFEA_R = 1, DFEA_Z = 2, DFEA_P = 3, DFEA_AP = 4, DFEA_B = 5, DFEA_U = 6, DFEA_RES = 7,
+       DFEA_NVEC = 8 };
+
+/* scalar block of a solve (doubles) */
+enum { DFEA_SC_RZ = 0, DFEA_SC_PAP = 1, DFEA_SC_ALPHA = 2, DFEA_SC_BETA = 3, DFEA_SC_RR = 4, DFEA_SC_BB = 5,
+       DFEA_SC_RTOL = 6, DFEA_SC_STATUS = 7, DFEA_SC_K = 8, DFEA_SC_MAXIT = 9, DFEA_SC_ITERS = 10,
+       DFEA_SC_NORM = 11, DFEA_SC_LF = 12, DFEA_SC_STEP = 13, DFEA_NSC = 16 };
+
+/* PCG status values stored in sc[DFEA_SC_STATUS] */
+enum { DFEA_PCG_RUNNING = 0, DFEA_PCG_CONVERGED = 1, DFEA_PCG_CURVATURE = 2, DFEA_PCG_MAXITER = 3 };
+
+/* A chunk: the elements handled by one workgroup (dfea_chunk_elems / dfea_chunk_nodes bound its size). */
+typedef struct dfea_chunk {
+    int32_t e0, ne;        /* internal element range [e0, e0+ne) */
+    int32_t k0, nk;        /* local node table range [k0, k0+nk) in lnode */
+    int32_t nown;          /* local nodes k0..k0+nown-1 belong to this chunk only; the rest are shared */
+    int32_t pbase;         /* partial slot of the first shared local node */
+    int32_t ncol;          /* element colours; colour c = local elements [col[c], col[c+1]) */
+    int32_t pad;
+    int32_t col[DFEA_MAXCOL + 1];
+    int32_t pad2[7];       /* 128 bytes */
+} dfea_chunk;
+
+/* Everything the kernels need about one solve (device pointers). */
+typedef struct dfea_desc {
+    const dfea_chunk* chunks;   /* [n_chunks] */
+    const uint32_t* lnode;      /* [n_lnodes] node id | DFEA_FIXED_X | DFEA_FIXED_Y */
+    const uint16_t* lelem;      /* [nen][n_el] local index (within the chunk) of element node a */
+    const uint32_t* sh_node;    /* [n_shared] node id | fixed bits */
+    const int32_t* sh_off;      /* [n_shared+1] */
+    const int32_t* sh_slot;     /* partial slots of each shared node, in chunk order */
+    const int32_t* conn;        /* [nen][n_el] node ids, internal element order */
+    const int32_t* perm;        /* [n_el] original element id of internal element e */
+    const double* nodes;        /* [N][2] reference coordinates */
+    const double* dNdxi;        /* [nq][nen][2] reference-element gradients at the Gauss points */
+    const double* gw;           /* [nq] Gauss weights */
+    const uint8_t* nfix;        /* [N] bit 0: x fixed, bit 1: y fixed */
+    double* dN64;
+    float* dN32;
+    double* w;
+    double* theta;              /* [n_el] internal order */
+    double* C64;
+    float* C32;
+    double* diag64;
+    float* diag32;
+    double* f;                  /* [2N] external force at load factor 1 */
+    double* part;               /* [n_part] pairs (double2 or float2) */
+    double* red;                /* reduction partials, dfea_red_size() doubles */
+    double* sc;                 /* [DFEA_NSC] */
+    void* vec[DFEA_NVEC];       /* PCG / Newton vectors; x,r,z,p,Ap,b in the solve's precision, u,R fp64 */
+    double mu0, lam0;
+    int32_t n_chunks, n_lnodes, n_shared, n_part;
+    int32_t n_el, n_nodes, nen, nq;
+    int32_t precision, n_finblk, n_vblk, pad0;
+} dfea_desc;
+
+/* ---- library ---------------------------------------------------------------------------------------- */
+int dfea_version(void);                                 /* 100*major + minor */
+const char* dfea_last_error(void);
+int dfea_desc_size(void);                               /* sizeof(dfea_desc), for binding checks */
+int dfea_desc_offset(int field);                        /* byte offset of field #field (declaration order) */
+int dfea_red_size(int n_chunks, int n_finblk, int n_vblk);   /* doubles needed for dfea_desc.red */
+int dfea_vblk(int n_dof);                               /* dof blocks of the vector kernels */
+int dfea_finblk(int n_shared);                          /* blocks of the shared-node kernels */
+int dfea_chunk_elems(int nen);                          /* maximum elements per chunk for this element */
+int dfea_chunk_nodes(void);                             /* maximum local nodes per chunk */
+
+/* ---- once per mesh ---------------------------------------------------------------------------------- */
+/* Geometry cache dN_dX, w_detJ (dN32 too when non-null).
+ * Replaces `src/difffea/operators.py:85-128` (precompute_geometry / get_geometry). */
+int dfea_geometry(const dfea_desc* table, const int32_t* items, int n_items, int nen, void* stream);
+
+/* ---- once per Newton step --------------------------------------------------------------------------- */
+/* Tangent moduli C (10 unique entries, weight folded in) at state u, stored FP64 (precision FP64) or FP32,
+ * fused with the Jacobi diagonal (clamp 1e-6 on free dofs, 1 on fixed dofs).
+ * u == NULL uses vec[DFEA_U]; diag_out == NULL uses diag64/diag32 of the solve.
+ * Replaces `operators.py:316-342` (_qp_setup) and `operators.py:477-570` (compute_jacobi_diagonal).
+ * items: chunk items; fin_items: shared-node items (dfea_finblk blocks per solve). */
+int dfea_moduli(const dfea_desc* table, const int32_t* items, int n_items, const int32_t* fin_items,
+                int n_fin_items, const double* u, int nen, int precision, void* stream);
+
+/* ---- residual --------------------------------------------------------------------------------------- */
+/* R = m * (R_int(u) - lf * f) with the analytic Piola stress; lf from sc[DFEA_SC_LF] when lf < 0.
+ * u/R NULL -> vec[DFEA_U] / vec[DFEA_RES].  Replaces `operators.py:131-220` (global_residual). */
+int dfea_residual(const dfea_desc* table, const int32_t* items, int n_items, const int32_t* fin_items,
+                  int n_fin_items, const double* u, double* R, double lf, int nen, void* stream);
+
+/* ---- tangent product -------------------------------------------------------------------------------- */
+/* y = m * J(u)(m v) + (1-m) v from the stored moduli (FP64 or FP32 per descriptor precision).
+ * v/y NULL -> vec[DFEA_P] / vec[DFEA_AP].  Deterministic (no atomics).
+ * Replaces the closure of `operators.py:345-412` (make_A_operator_qp / _qp_ew). */
+int dfea_apply(const dfea_desc* table, const int32_t* items, int n_items, const int32_t* fin_items,
+               int n_fin_items, const void* v, void* y, int nen, int precision, void* stream);
+
+/* ---- deterministic reductions ----------------------------------------------------------------------- */
+/* sc[DFEA_SC_NORM] = sum(x*y) over the solve's dofs (x==y gives the squared norm); FP64 vectors,
+ * x/y NULL -> vec[DFEA_RES].  items: dof-block items (dfea_vblk per solve); solve_items: {s, 0} per solve.
+ * Replaces torch.dot / torch.norm at `newton_krylov.py:56,65,78,116,142`. */
+int dfea_dot(const dfea_desc* table, const int32_t* items, int n_items, const int32_t* solve_items,
+             int n_solves, const double* x, const double* y, void* stream);
+
+/* ---- Newton helpers ----------------------------------------------------------------------------------- */
+/* out = where(fixed, lf*uD, u) (the load-level lift, `newton_krylov.py:139`) when mode == 0;
+ * out = u + alpha * dx (the line-search trial, `newton_krylov.py:114`) when mode == 1, dx in the
+ * solve's precision (vec[DFEA_X] when dx == NULL).  alpha / lf come from per-solve doubles `sarg`. */
+int dfea_axpy(const dfea_desc* table, const int32_t* items, int n_items, int mode, const double* u,
+              const void* dx, const double* uD, const double* sarg, double* out, int precision, void* stream);
+
+/* ---- PCG ---------------------------------------------------------------------------------------------- */
+/* Initialise x=0, r=b, z=r/diag, p=z and the scalar block (rtol, max_iter from sc).  b: vec[DFEA_B]
+ * (solve precision) or, when b64 != NULL (one solve) / flag neg_res (batch), b = -R from vec[DFEA_RES].
+ * Replaces `newton_krylov.py:51-60`. */
+int dfea_pcg_init(const dfea_desc* table, const int32_t* vitems, int n_vitems, const int32_t* solve_items,
+                  int n_solves, int neg_res, int precision, void* stream);
+/* n_iter PCG iterations for every running solve of the batch (fused: apply + dot, shared finalize,
+ * axpy + dots, scalar update); finished solves skip their work.  Semantics of `newton_krylov.py:62-88`. */
+int dfea_pcg_iterate(const dfea_desc* table, const int32_t* items, int n_items, const int32_t* fin_items,
+                     int n_fin_items, const int32_t* vitems, int n_vitems, const int32_t* solve_items,
+                     int n_solves, int n_iter, int nen, int precision, void* stream);
+
+/* ---- adjoint -------------------------------------------------------------------------------------------- */
+/* g[e] = -sum_{a,i} (m*lam)[2 conn(e,a)+i] * R^e_int[a,i](u; mu0, lam0), original element order.
+ * dL/dtheta for neo-Hookean theta-scaling.  Replaces `adjoint.py:75-106` (compute_implicit_vjp). */
+int dfea_adjoint_grad(const dfea_desc* table, const int32_t* items, int n_items, const double* u,
+                      const double* lam, double* g, int nen, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,381 @@
+"""CPU oracle (FP64, PyTorch-CPU + torch.func) for the matrix-free Newton-Krylov hot path.
+
+TEST INFRASTRUCTURE ONLY.  This module is imported by ``tests/``, by ``__graft_entry__.smoke()`` and by the
+``cpu_baseline`` leg of ``bench.py`` -- as the checker, never as the thing measured or shipped.  The product
+path (``difffea_amd``) never imports it and fails loudly when its HIP library is missing.
+
+It restates, in its own code, the reference's algorithms step for step (file:line of the reference's
+``src/difffea`` is cited on every function), including the reference's use of automatic differentiation
+(stress = grad psi, moduli = hessian psi, tangent = jvp, transpose = vjp, diagonal = jvp on unit vectors).
+
+How each function is pinned (tests/test_oracle_*.py):
+  * geometry               -- affine/rectangle closed forms, shape-gradient finite differences, mesh area
+  * psi / P / C            -- closed forms P = mu F + (lam lnJ - mu) F^-T and
+                              C = mu d_ik d_jl + lam G_ij G_kl + (mu - lam lnJ) G_il G_kj, linear-elastic
+                              tensor at F = I, objectivity, stress-free reference state
+  * global_residual        -- rigid-body invariance, R_int = grad(total energy), finite differences
+  * qp / qp_ew / jvp A(v)  -- mutual agreement < 1e-11, symmetry, central finite differences of R
+  * jacobi diagonal        -- diagonal of the dense operator
+  * pcg                    -- dense solve
+  * newton_krylov_solve    -- equilibrium, reactions balance the applied load, golden trajectory (G2)
+  * adjoint                -- central finite differences; golden reference adjoint (G3)
+"""
+import torch
+from torch.func import grad, hessian, jvp, vjp, vmap
+
+DT = torch.float64
+
+
+# ---------------------------------------------------------------------------------------------------------
+# geometry cache                                                    (operators.py:85-120, kinematics.py:78-104)
+# ---------------------------------------------------------------------------------------------------------
+def shape_fn_quad4(xi, eta):
+    """Bilinear Q4, CCW corners (shape_functions_2d.py:77-97)."""
+    return torch.stack([0.25 * (1 - xi) * (1 - eta), 0.25 * (1 + xi) * (1 - eta),
+                        0.25 * (1 + xi) * (1 + eta), 0.25 * (1 - xi) * (1 + eta)])
+
+
+def shape_fn_quad9(xi, eta):
+    """Biquadratic Lagrange Q9: corners, mid-sides (bottom, right, top, left), centre (:132-169)."""
+    a = [0.5 * xi * (xi - 1), 1 - xi ** 2, 0.5 * xi * (xi + 1)]
+    b = [0.5 * eta * (eta - 1), 1 - eta ** 2, 0.5 * eta * (eta + 1)]
+    return torch.stack([a[0] * b[0], a[2] * b[0], a[2] * b[2], a[0] * b[2],
+                        a[1] * b[0], a[2] * b[1], a[1] * b[2], a[0] * b[1], a[1] * b[1]])
+
+
+SHAPE_FN = {"quad4": shape_fn_quad4, "quad9": shape_fn_quad9}
+
+
+def gauss_rule_2d(n):
+    """Tensor Gauss-Legendre rule, eta-major / xi-minor ordering (gauss_quadratures.py:30-94)."""
+    import numpy as np
+    p, w = np.polynomial.legendre.leggauss(n)
+    p, w = torch.tensor(p, dtype=DT), torch.tensor(w, dtype=DT)
+    GP = torch.stack([p.repeat(n), p.repeat_interleave(n)], dim=1)
+    return GP, torch.outer(w, w).reshape(-1)
+
+
+def reference_gradients(element_type, GP):
+    """dN_a/dxi_j at the Gauss points by forward-mode AD of the shape functions (grad_shape_functions_2d.py)."""
+    from torch.func import jacfwd
+    fn = SHAPE_FN[element_type]
+    return vmap(lambda x, e: torch.stack(jacfwd(fn, argnums=(0, 1))(x, e), dim=-1))(GP[:, 0], GP[:, 1])
+
+
+def precompute_geometry(params):
+    """dN_dX[e,q,a,j] = sum_k dN_a/dxi_k (J^-1)_{kj},  w_detJ[e,q] = w_q det J, J = dX/dxi (columns)."""
+    et = params["element_type"]
+    dN_dxi = reference_gradients(et, params["GP"].to(DT))                       # (n_q, nen, 2)
+    Xe = params["nodes"][params["conn"]].to(DT)                                  # (n_el, nen, 2)
+    J = torch.einsum("eai,qaj->eqij", Xe, dN_dxi)                                # J[i,j] = dX_i/dxi_j
+    detJ = torch.linalg.det(J)
+    invJ = torch.linalg.inv(J)
+    dN_dX = torch.einsum("qaj,eqji->eqai", dN_dxi, invJ)
+    return {**params, "dN_dX": dN_dX, "w_detJ": params["GW"].to(DT)[None, :] * detJ}
+
+
+def get_geometry(params):
+    if "dN_dX" in params:
+        return params["dN_dX"], params["w_detJ"]
+    g = precompute_geometry(params)
+    return g["dN_dX"], g["w_detJ"]
+
+
+# ---------------------------------------------------------------------------------------------------------
+# material                                                          (materials.py:37-73, 239-246)
+# ---------------------------------------------------------------------------------------------------------
+def psi_neo_hookean(F, p):
+    """Plane-strain compressible neo-Hookean energy through the 3-D embedding F3 = [[F,0],[0,1]].
+
+    psi = mu/2 (I1 - 3 - 2 ln J) + lam/2 (ln J)^2 with I1 = tr(F3^T F3) = F:F + 1 and J = det F (explicit
+    2x2 determinant so that second derivatives stay finite at F = I).
+    """
+    mu, lam = p[0], p[1]
+    I1 = torch.sum(F * F) + 1.0
+    logJ = torch.log(F[0, 0] * F[1, 1] - F[0, 1] * F[1, 0])
+    return 0.5 * mu * (I1 - 3.0 - 2.0 * logJ) + 0.5 * lam * logJ ** 2
+
+
+def element_material_parameters(theta, params):
+    """(n_el, 2): nominal [mu, lam] with the scalable entries (both, for neo-Hookean) times theta_e."""
+    p0 = params["mat_p"].to(theta.dtype)
+    sc = params["mat_scalable"]
+    factor = torch.where(sc[None, :], theta[:, None], torch.ones_like(theta)[:, None])
+    return p0[None, :] * factor
+
+
+# ---------------------------------------------------------------------------------------------------------
+# residual                                                          (operators.py:131-220)
+# ---------------------------------------------------------------------------------------------------------
+def elem_residual_precomputed(ue, dN_dX, w_detJ, p):
+    """R^e[a,i] = sum_q w_q detJ_q P_ij(F_q) dN_a/dX_j with P = grad psi (automatic differentiation)."""
+    F = torch.eye(2, dtype=ue.dtype) + torch.einsum("ai,qaj->qij", ue, dN_dX)
+    P = vmap(grad(psi_neo_hookean), in_dims=(0, None))(F, p)
+    return torch.einsum("q,qij,qaj->ai", w_detJ, P, dN_dX)
+
+
+def global_residual(u, theta, params, load_factor=1.0):
+    """R = m * (scatter_e R^e(u) - lf f)."""
+    conn = params["conn"]
+    N = params["nodes"].shape[0]
+    ue = u.reshape(N, 2)[conn]
+    dN_dX, w_detJ = get_geometry(params)
+    pe = element_material_parameters(theta, params)
+    Re = vmap(elem_residual_precomputed)(ue, dN_dX, w_detJ, pe)
+    Rint = torch.zeros(N, 2, dtype=u.dtype).index_add_(0, conn.reshape(-1), Re.reshape(-1, 2))
+    return params["m"] * (Rint.reshape(-1) - load_factor * params["f"])
+
+
+# ---------------------------------------------------------------------------------------------------------
+# tangent operators                                                 (operators.py:231-435)
+# ---------------------------------------------------------------------------------------------------------
+def make_A_operator(u, theta, params):
+    """'jvp': A(v) = m J(u)(m v) + (1-m) v with J v by forward-mode AD of the residual."""
+    m = params["m"]
+
+    def A(v):
+        _, Jv = jvp(lambda w: global_residual(w, theta, params), (u,), (m * v,))
+        return m * Jv + (1.0 - m) * v
+    return A
+
+
+def qp_setup(u, theta, params):
+    """C[e,q,2i+j,2k+l] = w_detJ[e,q] * d2psi/dF_ij dF_kl (hessian by AD), and dN_dX."""
+    conn = params["conn"]
+    N = params["nodes"].shape[0]
+    n_el = conn.shape[0]
+    dN_dX, w_detJ = get_geometry(params)
+    n_q = dN_dX.shape[1]
+    pe = element_material_parameters(theta, params)
+    ue = u.reshape(N, 2)[conn]
+    F = torch.eye(2, dtype=u.dtype) + torch.einsum("eai,eqaj->eqij", ue, dN_dX)
+    H = vmap(vmap(hessian(psi_neo_hookean), in_dims=(0, None)), in_dims=(0, 0))(F, pe)
+    return (H * w_detJ[:, :, None, None, None, None]).reshape(n_el, n_q, 4, 4), dN_dX
+
+
+def make_A_operator_qp(u, theta, params, C=None):
+    """'qp': gather -> einsum -> batched 4x4 matmul -> einsum -> index_add_ -> identity pivot.
+
+    This op-for-op restatement is also the timed CPU baseline of the tangent product.
+    """
+    conn, m = params["conn"], params["m"]
+    N = params["nodes"].shape[0]
+    n_el = conn.shape[0]
+    if C is None:
+        C, dN_dX = qp_setup(u, theta, params)
+    else:
+        dN_dX, _ = get_geometry(params)
+    n_q = dN_dX.shape[1]
+    flat = conn.reshape(-1)
+
+    def A(v):
+        ve = (m * v).reshape(N, 2)[conn]
+        g = torch.einsum("eai,eqaj->eqij", ve, dN_dX).reshape(n_el, n_q, 4, 1)
+        s = torch.matmul(C, g).reshape(n_el, n_q, 2, 2)
+        fe = torch.einsum("eqij,eqaj->eai", s, dN_dX)
+        Jv = torch.zeros(N, 2, dtype=v.dtype).index_add_(0, flat, fe.reshape(-1, 2))
+        return m * Jv.reshape(-1) + (1.0 - m) * v
+    return A
+
+
+def make_A_operator_qp_ew(u, theta, params):
+    """'qp_ew': the same product with broadcast multiply + sum instead of matmul."""
+    conn, m = params["conn"], params["m"]
+    N = params["nodes"].shape[0]
+    n_el = conn.shape[0]
+    C, dN_dX = qp_setup(u, theta, params)
+    n_q = dN_dX.shape[1]
+    flat = conn.reshape(-1)
+
+    def A(v):
+        ve = (m * v).reshape(N, 2)[conn]
+        g = (ve[:, None, :, :, None] * dN_dX[:, :, :, None, :]).sum(2)
+        s = (C * g.reshape(n_el, n_q, 1, 4)).sum(-1).reshape(n_el, n_q, 2, 2)
+        fe = (s[:, :, None, :, :] * dN_dX[:, :, :, None, :]).sum(dim=(1, 4))
+        Jv = torch.zeros(N, 2, dtype=v.dtype).index_add_(0, flat, fe.reshape(-1, 2))
+        return m * Jv.reshape(-1) + (1.0 - m) * v
+    return A
+
+
+ORACLE_TANGENTS = ("jvp", "qp", "qp_ew")
+
+
+def make_tangent_operator(kind, u, theta, params):
+    if kind in ("auto", "qp"):
+        return make_A_operator_qp(u, theta, params)
+    if kind == "qp_ew":
+        return make_A_operator_qp_ew(u, theta, params)
+    if kind == "jvp":
+        return make_A_operator(u, theta, params)
+    raise ValueError(f"Unknown tangent operator '{kind}'. Choose one of {ORACLE_TANGENTS}.")
+
+
+def make_AT_operator(u, theta, params):
+    """AT(v) = m J^T(m v) + (1-m) v by reverse-mode AD (operators.py:441-471)."""
+    m = params["m"]
+    _, f_vjp = vjp(lambda w: global_residual(w, theta, params), u)
+
+    def AT(v):
+        (Jt,) = f_vjp(m * v)
+        return m * Jt + (1.0 - m) * v
+    return AT
+
+
+# ---------------------------------------------------------------------------------------------------------
+# Jacobi diagonal                                                   (operators.py:477-570)
+# ---------------------------------------------------------------------------------------------------------
+def compute_jacobi_diagonal(u, theta, params):
+    """Per element: e_k . (J^e e_k) for the 2 nen local unit vectors (jvp), scatter, clamp / pivot."""
+    conn = params["conn"]
+    N = params["nodes"].shape[0]
+    n_el, nen = conn.shape
+    ue = u.reshape(N, 2)[conn]
+    pe = element_material_parameters(theta, params)
+    dN_dX, w_detJ = get_geometry(params)
+    basis = torch.eye(2 * nen, dtype=u.dtype).reshape(-1, nen, 2)
+
+    def entry(u_e, dN_e, w_e, p_e, e_vec):
+        _, Je = jvp(lambda w: elem_residual_precomputed(w, dN_e, w_e, p_e), (u_e,), (e_vec,))
+        return (Je * e_vec).sum()
+
+    def one(u_e, dN_e, w_e, p_e):
+        return vmap(entry, in_dims=(None, None, None, None, 0))(u_e, dN_e, w_e, p_e, basis)
+
+    de = vmap(one)(ue, dN_dX, w_detJ, pe).reshape(n_el, nen, 2)
+    d = torch.zeros(N, 2, dtype=u.dtype).index_add_(0, conn.reshape(-1), de.reshape(-1, 2)).reshape(-1)
+    return torch.where(params["m"] > 0.0, torch.clamp(d, min=1e-6), torch.ones_like(d))
+
+
+def make_jacobi_preconditioner(u, theta, params):
+    diag = compute_jacobi_diagonal(u, theta, params)
+    return lambda r: r / diag
+
+
+# ---------------------------------------------------------------------------------------------------------
+# Krylov + Newton                                                   (newton_krylov.py:28-264)
+# ---------------------------------------------------------------------------------------------------------
+def pcg(A_fn, b, Minv_fn, rtol=1e-5, max_iter=500):
+    """Textbook Jacobi-PCG with x0 = 0; curvature guard; stop when ||r|| <= rtol ||b||."""
+    x = torch.zeros_like(b)
+    r = b.clone()
+    z = Minv_fn(r)
+    p = z.clone()
+    rz = torch.dot(r, z)
+    nb = torch.norm(b)
+    if nb == 0.0:
+        return x, 0
+    for k in range(max_iter):
+        Ap = A_fn(p)
+        curv = torch.dot(p, Ap)
+        if curv <= 0.0:
+            return (z if k == 0 else x), k
+        alpha = rz / curv
+        x = x + alpha * p
+        r = r - alpha * Ap
+        if torch.norm(r) <= rtol * nb:
+            return x, k + 1
+        z = Minv_fn(r)
+        rz_new = torch.dot(r, z)
+        p = z + (rz_new / rz) * p
+        rz = rz_new
+    return x, max_iter
+
+
+def armijo_line_search(u, du, rn, theta, params, load_factor, c1=1e-4, min_alpha=2.0 ** -5):
+    alpha = 1.0
+    while alpha >= min_alpha:
+        uc = u + alpha * du
+        Rc = global_residual(uc, theta, params, load_factor)
+        rc = torch.norm(Rc)
+        if rc < (1.0 - c1 * alpha) * rn:
+            return uc, Rc, rc, alpha, True
+        alpha *= 0.5
+    return u, None, rn, alpha, False
+
+
+def newton_at_load(u0, theta, params, lf, tol, max_newton, step_id, tangent="qp",
+                   cg_rtol=None, cg_max_iter=500, callback=None):
+    u = torch.where(params["fixed"], lf * params["u_D"], u0)
+    R = global_residual(u, theta, params, lf)
+    r0 = torch.norm(R)
+    rn = r0
+    records = []
+    for it in range(max_newton + 1):
+        if rn < tol:
+            return u, True, records
+        if it == max_newton:
+            break
+        eta = min(0.5, float((rn / (r0 + 1e-14)) ** 0.5)) if cg_rtol is None else cg_rtol
+        A = make_tangent_operator(tangent, u, theta, params)
+        Minv = make_jacobi_preconditioner(u, theta, params)
+        du, k = pcg(A, -R, Minv, rtol=eta, max_iter=cg_max_iter)
+        u, Rn, rn, alpha, ok = armijo_line_search(u, du, rn, theta, params, lf)
+        rec = dict(step=step_id, load_factor=lf, iteration=it, residual=float(rn), cg_iters=int(k), alpha=alpha)
+        records.append(rec)
+        if callback is not None:
+            callback(rec)
+        if not ok:
+            return u, False, records
+        R = Rn
+    return u, False, records
+
+
+@torch.no_grad()
+def newton_krylov_solve(params, theta, n_load_steps=5, tol=1e-8, max_newton=25, max_cutbacks=8,
+                        tangent="qp", cg_rtol=None, cg_max_iter=500, callback=None):
+    """Adaptive load continuation around ``newton_at_load``; returns (u, info) like the reference."""
+    n = 2 * params["nodes"].shape[0]
+    u = torch.zeros(n, dtype=DT)
+    info = {"converged": False, "load_factor": 0.0, "history": [], "total_newton": 0, "total_cg": 0,
+            "n_cutbacks": 0, "snapshots": [(0.0, u.clone())]}
+    d_nom = 1.0 / n_load_steps
+    d = d_nom
+    lf = 0.0
+    step, in_row = 0, 0
+    while lf < 1.0 - 1e-12:
+        lf_try = min(1.0, lf + d)
+        step += 1
+        u_try, ok, recs = newton_at_load(u, theta, params, lf_try, tol, max_newton, step, tangent,
+                                         cg_rtol, cg_max_iter, callback)
+        info["history"] += recs
+        info["total_newton"] += len(recs)
+        info["total_cg"] += sum(r["cg_iters"] for r in recs)
+        if ok:
+            u, lf = u_try, lf_try
+            info["snapshots"].append((lf, u.clone()))
+            in_row = 0
+            d = min(d_nom, 1.5 * d)
+        else:
+            in_row += 1
+            info["n_cutbacks"] += 1
+            if in_row > max_cutbacks:
+                info["load_factor"] = lf
+                return u, info
+            d *= 0.5
+    info["converged"] = True
+    info["load_factor"] = 1.0
+    return u, info
+
+
+# ---------------------------------------------------------------------------------------------------------
+# adjoint sensitivities                                             (adjoint.py:31-106, sensitivity.py:7-38)
+# ---------------------------------------------------------------------------------------------------------
+def solve_adjoint(u_star, theta, params, dL_du, rtol=1e-6, max_iter=500):
+    AT = make_AT_operator(u_star, theta, params)
+    Minv = make_jacobi_preconditioner(u_star, theta, params)
+    return pcg(AT, params["m"] * dL_du, Minv, rtol=rtol, max_iter=max_iter)
+
+
+def compute_implicit_vjp(u_star, theta, params, lam):
+    _, f_vjp = vjp(lambda th: global_residual(u_star, th, params, 1.0), theta)
+    (g,) = f_vjp(-lam)
+    return g
+
+
+def compute_sensitivity(params, theta, loss_fn, n_load_steps=5, tol=1e-8, max_newton=25,
+                        adjoint_rtol=1e-10, adjoint_max_iter=500):
+    theta = theta.detach()
+    u, _ = newton_krylov_solve(params, theta, n_load_steps, tol, max_newton)
+    dL_du = grad(loss_fn)(u)
+    lam, _ = solve_adjoint(u, theta, params, dL_du, rtol=adjoint_rtol, max_iter=adjoint_max_iter)
+    return u, compute_implicit_vjp(u, theta, params, lam)

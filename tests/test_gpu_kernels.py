@@ -1,0 +1,122 @@
+"""HIP kernels vs the CPU oracle on the same inputs (FP64 bar: 1e-11 relative, the reference's own
+cross-variant tolerance, test_tangent_operators.py:31-36)."""
+import pytest
+import torch
+
+from helpers import cantilever, rel, state, stretch_state, unit_square
+from oracle import fem
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda"
+CASES = [("quad4", 8), ("quad9", 4), ("quad4", 37), ("quad9", 19)]
+
+
+def _setup(et, n, seed=0, nonuniform=False):
+    p = fem.precompute_geometry(unit_square(et, n))
+    u, v = state(p, seed)
+    th = torch.ones(p["conn"].shape[0], dtype=torch.float64)
+    if nonuniform:
+        th = 0.5 + torch.rand(th.shape[0], dtype=torch.float64, generator=torch.Generator().manual_seed(seed + 7))
+    return p, u, v, th
+
+
+@pytest.mark.parametrize("et,n", CASES)
+def test_apply_fp64_matches_oracle(et, n):
+    from difffea_amd.operators import make_tangent_operator
+    p, u, v, th = _setup(et, n, nonuniform=True)
+    ref_qp = fem.make_A_operator_qp(u, th, p)(v)
+    ref_jvp = fem.make_A_operator(u, th, p)(v)
+    A = make_tangent_operator("hip", u.to(DEV), th.to(DEV), p)
+    y = A(v.to(DEV))
+    assert rel(y, ref_qp) < 1e-11
+    assert rel(y, ref_jvp) < 1e-11
+
+
+@pytest.mark.parametrize("et,n", CASES)
+def test_diagonal_and_residual_match_oracle(et, n):
+    from difffea_amd.operators import compute_jacobi_diagonal, global_residual
+    p, u, v, th = _setup(et, n, seed=3, nonuniform=True)
+    p = dict(p, f=torch.randn(p["f"].shape[0], dtype=torch.float64, generator=torch.Generator().manual_seed(5)))
+    d = compute_jacobi_diagonal(u.to(DEV), th.to(DEV), p)
+    assert rel(d, fem.compute_jacobi_diagonal(u, th, p)) < 1e-11
+    R = global_residual(u.to(DEV), th.to(DEV), p, 0.7)
+    assert rel(R, fem.global_residual(u, th, p, 0.7)) < 1e-11
+
+
+@pytest.mark.parametrize("et,n", [("quad4", 8), ("quad9", 4)])
+def test_apply_is_bitwise_repeatable_and_symmetric(et, n):
+    from difffea_amd.operators import make_tangent_operator
+    p, u, v, th = _setup(et, n, seed=4)
+    A = make_tangent_operator("hip", u.to(DEV), th.to(DEV), p)
+    y1, y2 = A(v.to(DEV)), A(v.to(DEV))
+    assert torch.equal(y1, y2)
+    w = torch.randn_like(v)
+    a, b = float(w.to(DEV) @ A(v.to(DEV))), float(v.to(DEV) @ A(w.to(DEV)))
+    assert a == pytest.approx(b, rel=1e-10)
+
+
+@pytest.mark.parametrize("et,n", [("quad4", 8), ("quad9", 4)])
+def test_apply_mixed_precision(et, n):
+    from difffea_amd.operators import make_tangent_operator
+    p, u, v, th = _setup(et, n, seed=6)
+    ref = fem.make_A_operator_qp(u, th, p)(v)
+    y = make_tangent_operator("hip", u.to(DEV), th.to(DEV), p, precision="mixed")(v.to(DEV).float())
+    assert y.dtype == torch.float32
+    assert rel(y, ref) < 1e-5
+
+
+@pytest.mark.parametrize("et,n", [("quad4", 12), ("quad9", 6)])
+def test_fused_pcg_matches_oracle_pcg(et, n):
+    from difffea_amd.operators import make_system
+    p, u, v, th = _setup(et, n, seed=8)
+    b = v * p["m"]
+    A = fem.make_A_operator_qp(u, th, p)
+    M = fem.make_jacobi_preconditioner(u, th, p)
+    x_ref, k_ref = fem.pcg(A, b, M, rtol=1e-10, max_iter=2000)
+    s = make_system(u.to(DEV), th.to(DEV), p)
+    s.moduli(u.to(DEV))
+    x, k, st = s.pcg_solve(b.to(DEV), rtol=1e-10, max_iter=2000)
+    assert st == 1
+    assert abs(k - k_ref) <= 1
+    assert rel(x, x_ref) < 1e-9
+
+
+def test_pcg_zero_rhs_and_iteration_cap():
+    from difffea_amd.operators import make_system
+    p, u, v, th = _setup("quad4", 8, seed=9)
+    s = make_system(u.to(DEV), th.to(DEV), p)
+    s.moduli(u.to(DEV))
+    x, k, st = s.pcg_solve(torch.zeros_like(v).to(DEV), rtol=1e-8)
+    assert (st, k) == (1, 0) and float(x.abs().max()) == 0.0
+    b = v * p["m"]
+    x, k, st = s.pcg_solve(b.to(DEV), rtol=1e-14, max_iter=5)
+    x_ref, k_ref = fem.pcg(fem.make_A_operator_qp(u, th, p), b, fem.make_jacobi_preconditioner(u, th, p),
+                           rtol=1e-14, max_iter=5)
+    assert (st, k, k_ref) == (3, 5, 5)
+    assert rel(x, x_ref) < 1e-11
+
+
+@pytest.mark.parametrize("et,n", [("quad4", 8), ("quad9", 4)])
+def test_adjoint_gradient_kernel_matches_oracle_vjp(et, n):
+    from difffea_amd.operators import make_system
+    p, u, v, th = _setup(et, n, seed=10, nonuniform=True)
+    lam = v
+    ref = fem.compute_implicit_vjp(u, th, p, lam)
+    s = make_system(u.to(DEV), th.to(DEV), p)
+    g = s.adjoint_grad(u.to(DEV), lam.to(DEV))
+    assert rel(g, ref) < 1e-11
+
+
+@pytest.mark.parametrize("et,n", [("quad4", 400), ("quad9", 200)])
+def test_apply_at_full_size(et, n):
+    """The bench workloads: 400x400 Q4 and 200x200 Q9 with the stretched, noisy state."""
+    from difffea_amd.operators import make_tangent_operator
+    p = fem.precompute_geometry(unit_square(et, n))
+    u = stretch_state(p, 1.0 / n, seed=0)
+    v = torch.randn(u.shape[0], dtype=torch.float64, generator=torch.Generator().manual_seed(1))
+    th = torch.ones(p["conn"].shape[0], dtype=torch.float64)
+    ref = fem.make_A_operator_qp(u, th, p)(v)
+    y = make_tangent_operator("hip", u.to(DEV), th.to(DEV), p)(v.to(DEV))
+    assert rel(y, ref) < 1e-11
+    y32 = make_tangent_operator("hip", u.to(DEV), th.to(DEV), p, precision="mixed")(v.to(DEV).float())
+    assert rel(y32, ref) < 1e-5
