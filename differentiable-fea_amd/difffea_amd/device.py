@@ -63,6 +63,8 @@ class HipMesh:
         cij = params.get("cell_ij")
         self.plan = p = MeshPlan(nodes.numpy(), conn.numpy(), fixed.numpy(),
                                  cell_ij=None if cij is None else cij.detach().cpu().numpy())
+        if p.validate() > self.lib.dfea_chunk_elems(self.nen):
+            raise ValueError("chunk larger than the kernel's workgroup")
         self.n_el, self.n_nodes = p.n_el, p.n_nodes
         self.n_dof = 2 * p.n_nodes
         dv = device

@@ -236,6 +236,28 @@ class MeshPlan:
         self.structured = structured
         self.nfix = (fixed[:, 0].astype(np.uint8) | (fixed[:, 1].astype(np.uint8) << 1))
 
+    def validate(self):
+        """Host-side bounds checks of every index the kernels follow (run before any launch)."""
+        ch = self.chunks
+        E = ch[:, 1].max()
+        assert ch[:, 0].min() >= 0 and (ch[:, 0] + ch[:, 1]).max() <= self.n_el
+        assert (ch[:, 2] + ch[:, 3]).max() <= self.n_lnodes and (ch[:, 4] <= ch[:, 3]).all()
+        assert (ch[:, 5] + ch[:, 3] - ch[:, 4]).max() <= max(self.n_part, 0)
+        # every element's local indices fall inside its chunk's node range
+        chunk_of_e = np.repeat(np.arange(self.n_chunks), ch[:, 1])
+        assert (self.lelem.astype(np.int64) < ch[chunk_of_e, 3][None, :]).all()
+        # colour ranges tile the chunk
+        for c in range(self.n_chunks):
+            nc = ch[c, 6]
+            assert 1 <= nc <= _lib.MAXCOL and ch[c, 8] == 0 and ch[c, 8 + nc] == ch[c, 1]
+        assert (self.lnode & _lib.NODE_MASK).max() < self.n_nodes
+        if self.n_shared:
+            assert self.sh_off[0] == 0 and self.sh_off[-1] == len(self.sh_slot) == self.n_part
+            assert self.sh_slot.min() >= 0 and self.sh_slot.max() < self.n_part
+            assert (self.sh_node & _lib.NODE_MASK).max() < self.n_nodes
+        assert self.conn_int.min() >= 0 and self.conn_int.max() < self.n_nodes
+        return E
+
     def stats(self):
         return dict(n_chunks=self.n_chunks, n_lnodes=self.n_lnodes, n_shared=self.n_shared, n_part=self.n_part,
                     max_chunk_nodes=int(self.chunks[:, 3].max()), max_colours=int(self.chunks[:, 6].max()))
