@@ -1,0 +1,217 @@
+"""Newton-Krylov solve on device-resident state, mirroring `src/difffea/newton_krylov.py`.
+
+* ``pcg(A_fn, b, Minv_fn, rtol, max_iter)``  the closure-based PCG (`newton_krylov.py:28-88`), for any
+  operator closures on any device (same algorithm, same early exits, same return values).
+* ``newton_krylov_solve(params, theta, ..., tangent="hip", precision="fp64")`` (`newton_krylov.py:186-264`):
+  load continuation with cut-backs, Eisenstat-Walker forcing, Armijo line search -- the same state machine
+  and ``info`` dictionary -- where every vector operation runs in the HIP library: residual, moduli +
+  Jacobi diagonal (one fused kernel per Newton step), the fused PCG (four kernels per iteration, device
+  scalars, status read once per chunk of iterations), line-search trials.
+* ``precision="mixed"``: moduli, A(v) and the inner CG vectors in FP32 (dot products accumulated in FP64),
+  while the Newton residual, the state u and the line search stay FP64 -- Newton itself is the outer
+  refinement loop, so the converged residual meets the same FP64 tolerance.  The inner relative tolerance
+  is floored at ``MIXED_RTOL_FLOOR`` (FP32 CG cannot resolve smaller relative residuals).
+"""
+import time
+
+import torch
+
+from . import _lib
+from .operators import PRECONDITIONERS, PRECISIONS, TANGENT_OPERATORS, get_mesh, _check_material
+from .device import HipSystem
+
+MIXED_RTOL_FLOOR = 1e-6
+
+
+def pcg(A_fn, b, Minv_fn, rtol=1e-5, max_iter=500):
+    """Textbook Jacobi-PCG with x0 = 0 (returns (x, n_iters))."""
+    x = torch.zeros_like(b)
+    r = b.clone()
+    z = Minv_fn(r)
+    p = z.clone()
+    rz = torch.dot(r, z)
+    nb = torch.norm(b)
+    if nb == 0.0:
+        return x, 0
+    for k in range(max_iter):
+        Ap = A_fn(p)
+        curv = torch.dot(p, Ap)
+        if curv <= 0.0:
+            return (z if k == 0 else x), k
+        alpha = rz / curv
+        x = x + alpha * p
+        r = r - alpha * Ap
+        if torch.norm(r) <= rtol * nb:
+            return x, k + 1
+        z = Minv_fn(r)
+        rz_new = torch.dot(r, z)
+        p = z + (rz_new / rz) * p
+        rz = rz_new
+    return x, max_iter
+
+
+class HipNewton:
+    """The Newton-Krylov state machine of one solve on one ``HipSystem`` (device-resident)."""
+
+    def __init__(self, params, theta, precision="fp64", device=None):
+        if precision not in PRECISIONS:
+            raise ValueError(f"Unknown precision '{precision}'. Choose one of {PRECISIONS}.")
+        device = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
+        self.params = params
+        self.mesh = get_mesh(params, device)
+        self.sys = HipSystem(self.mesh, theta.detach(), _check_material(params), params["f"], precision)
+        self.device = self.mesh.device
+        self.uD = params["u_D"].to(self.device, torch.float64).contiguous()
+        nd = self.mesh.n_dof
+        self.ucand = torch.empty(nd, dtype=torch.float64, device=self.device)
+        self._alpha = torch.empty(1, dtype=torch.float64, device=self.device)
+        self.pcg_chunk = 16
+
+    # -- building blocks -------------------------------------------------------------------------------------
+    def residual(self, u, lf, out=None):
+        return self.sys.residual(u, out=self.sys.vec["R"] if out is None else out, lf=lf)
+
+    def norm(self, R):
+        return float(torch.sqrt(self.sys.dot(R)))
+
+    def linear_solve(self, rtol, max_iter):
+        """PCG on J du = -R with R in sys.vec['R']; returns (du, iterations)."""
+        s = self.sys
+        if s.precision == "mixed":
+            rtol = max(rtol, MIXED_RTOL_FLOOR)
+        s.pcg_start(rtol=rtol, max_iter=max_iter, neg_res=True)
+        st, its, k = s.pcg_status()
+        chunk = self.pcg_chunk
+        while st == _lib.PCG_RUNNING:
+            s.pcg_iterate(min(chunk, max_iter - k))
+            st, its, k = s.pcg_status()
+            chunk = min(2 * chunk, 256)
+        return s.vec["x"], its
+
+    def trial(self, u, du, alpha, out):
+        self._alpha.fill_(alpha)
+        m = self.mesh
+        _lib.check(m.lib.dfea_axpy(ctypes_ptr(self.sys.table), ctypes_ptr(m.vitems), m.n_vblk, 1, ctypes_ptr(u),
+                                   ctypes_ptr(du), None, ctypes_ptr(self._alpha), ctypes_ptr(out),
+                                   self.sys.prec_id, stream()), "dfea_axpy")
+        return out
+
+    # -- reference state machine ---------------------------------------------------------------------------
+    def armijo(self, u, du, rn, lf, c1=1e-4, min_alpha=2.0 ** -5):
+        alpha = 1.0
+        while alpha >= min_alpha:
+            uc = self.trial(u, du, alpha, self.ucand)
+            R = self.residual(uc, lf)
+            rc = self.norm(R)
+            if rc < (1.0 - c1 * alpha) * rn:          # False for NaN (inverted element)
+                self.ucand = u
+                return uc, rc, alpha, True
+            alpha *= 0.5
+        return u, rn, alpha, False
+
+    def at_load(self, u0, lf, tol, max_newton, step_id, cg_rtol, cg_max_iter, callback, verbose):
+        u = self.sys.lift(u0, self.uD, lf)
+        R = self.residual(u, lf)
+        r0 = self.norm(R)
+        rn = r0
+        records = []
+        for it in range(max_newton + 1):
+            if rn < tol:
+                if verbose:
+                    print(f"  converged in {it} Newton iterations (||R|| = {rn:.3e})")
+                return u, True, records
+            if it == max_newton:
+                break
+            eta = min(0.5, (rn / (r0 + 1e-14)) ** 0.5) if cg_rtol is None else cg_rtol
+            self.sys.moduli(u)
+            du, k = self.linear_solve(eta, cg_max_iter)
+            if self.ucand is u:
+                self.ucand = torch.empty_like(u)
+            u, rn, alpha, ok = self.armijo(u, du, rn, lf)
+            rec = dict(step=step_id, load_factor=lf, iteration=it, residual=float(rn), cg_iters=int(k), alpha=alpha)
+            records.append(rec)
+            if callback is not None:
+                callback(rec)
+            if verbose:
+                print(f"  iter {it:2d} | ||R|| = {rn:.4e} | CG = {k:3d} | alpha = {alpha:.4f}")
+            if not ok:
+                return u, False, records
+        return u, False, records
+
+    def solve(self, n_load_steps=5, tol=1e-8, max_newton=25, verbose=False, callback=None, max_cutbacks=8,
+              cg_rtol=None, cg_max_iter=500):
+        n = self.mesh.n_dof
+        u = torch.zeros(n, dtype=torch.float64, device=self.device)
+        info = {"converged": False, "load_factor": 0.0, "history": [], "total_newton": 0, "total_cg": 0,
+                "n_cutbacks": 0, "snapshots": [(0.0, u.clone())], "precision": self.sys.precision}
+        d_nom = 1.0 / n_load_steps
+        d = d_nom
+        lf = 0.0
+        step, in_row = 0, 0
+        while lf < 1.0 - 1e-12:
+            lf_try = min(1.0, lf + d)
+            step += 1
+            if verbose:
+                print(f"\n=== Load step {step}: load factor {lf:.3f} -> {lf_try:.3f} ===")
+            u_try, ok, recs = self.at_load(u, lf_try, tol, max_newton, step, cg_rtol, cg_max_iter, callback, verbose)
+            info["history"] += recs
+            info["total_newton"] += len(recs)
+            info["total_cg"] += sum(r["cg_iters"] for r in recs)
+            if ok:
+                u, lf = u_try.clone(), lf_try
+                info["snapshots"].append((lf, u.clone()))
+                in_row = 0
+                d = min(d_nom, 1.5 * d)
+            else:
+                in_row += 1
+                info["n_cutbacks"] += 1
+                if in_row > max_cutbacks:
+                    info["load_factor"] = lf
+                    return u, info
+                d *= 0.5
+                if verbose:
+                    print(f"  Newton failed -> load increment reduced to {d:.4f}")
+        info["converged"] = True
+        info["load_factor"] = 1.0
+        return u, info
+
+
+def ctypes_ptr(t):
+    import ctypes
+    return ctypes.c_void_p(0 if t is None else t.data_ptr())
+
+
+def stream():
+    import ctypes
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+@torch.no_grad()
+def newton_krylov_solve(params, theta, n_load_steps=5, tol=1e-8, max_newton=25, verbose=True, callback=None,
+                        max_cutbacks=8, preconditioner="jacobi", tangent="auto", precision="fp64",
+                        cg_rtol=None, cg_max_iter=500, device=None):
+    """Solve R(u, theta) = 0 by load continuation + inexact Newton-Krylov; returns (u, info).
+
+    Same signature and ``info`` keys as the reference (``tol`` is absolute on ||R||_2), plus
+    ``precision`` ("fp64" | "mixed"), ``cg_rtol`` (fixed inner tolerance instead of Eisenstat-Walker),
+    ``cg_max_iter`` (the reference's PCG cap, 500) and ``device``.
+    """
+    if tangent not in TANGENT_OPERATORS:
+        raise ValueError(f"Unknown tangent operator '{tangent}'. Choose one of {TANGENT_OPERATORS}.")
+    if preconditioner not in PRECONDITIONERS:
+        raise ValueError(f"Unknown preconditioner '{preconditioner}'. Choose one of {PRECONDITIONERS}.")
+    if device is None:
+        device = theta.device if theta.is_cuda else torch.device("cuda", torch.cuda.current_device())
+    solver = HipNewton(params, theta.to(device), precision, device)
+    t0 = time.perf_counter()
+    u, info = solver.solve(n_load_steps, tol, max_newton, verbose, callback, max_cutbacks, cg_rtol, cg_max_iter)
+    torch.cuda.synchronize(solver.device)
+    info["seconds"] = time.perf_counter() - t0
+    return u, info
+
+
+def solve_newton_krylov(params, theta, n_load_steps=5, tol=1e-8, max_newton=25, verbose=True, **kw):
+    """Convenience wrapper returning only u (`newton_krylov.py:267-274`)."""
+    u, _ = newton_krylov_solve(params, theta, n_load_steps=n_load_steps, tol=tol, max_newton=max_newton,
+                               verbose=verbose, **kw)
+    return u

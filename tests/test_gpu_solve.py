@@ -1,0 +1,107 @@
+"""Complete solves and sensitivities on the GPU vs the oracle and the reference's golden vectors.
+Bars: solution within 1e-10 relative (FP64), Newton counts identical; mixed precision converges to the
+same FP64 residual tolerance; adjoint gradient within 1e-9 of the oracle / reference adjoint and 1e-6 of
+central finite differences."""
+import json
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from helpers import cantilever, rel, unit_square
+from oracle import fem
+
+pytestmark = pytest.mark.gpu
+DEV = torch.device("cuda")
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _solve(p, th, steps, tol, **kw):
+    from difffea_amd.newton_krylov import newton_krylov_solve
+    return newton_krylov_solve(p, th.to(DEV), steps, tol, verbose=False, **kw)
+
+
+@pytest.mark.parametrize("et", ["quad4", "quad9"])
+def test_solve_matches_oracle(et):
+    p = fem.precompute_geometry(cantilever(et, h=0.25, load=(0.0, -20.0)))
+    th = 0.5 + torch.rand(p["conn"].shape[0], dtype=torch.float64, generator=torch.Generator().manual_seed(1))
+    tol = 1e-9 * float(torch.norm(p["f"]))
+    u_ref, info_ref = fem.newton_krylov_solve(p, th, 3, tol)
+    u, info = _solve(p, th, 3, tol)
+    assert info["converged"] and info_ref["converged"]
+    assert info["total_newton"] == info_ref["total_newton"]
+    assert abs(info["total_cg"] - info_ref["total_cg"]) <= max(3, 0.03 * info_ref["total_cg"])
+    assert rel(u, u_ref) < 1e-10
+    assert float(torch.norm(fem.global_residual(u.cpu(), th, p))) < tol
+
+
+@pytest.mark.parametrize("name", ["g2_q4_disp_20", "g2_q9_force_8", "g2_q4_force_16"])
+def test_solve_matches_reference_golden(name):
+    from difffea_amd.problem import build_problem
+    z = np.load(os.path.join(GOLD, name + ".npz"))
+    spec = json.loads(str(z["spec"]))
+    spec.pop("mesher", None)
+    p = build_problem(spec)
+    th = torch.ones(p["conn"].shape[0], dtype=torch.float64)
+    u, info = _solve(p, th, int(z["n_load_steps"]), float(z["tol"]))
+    assert info["converged"]
+    assert info["total_newton"] == int(z["total_newton"])
+    assert rel(u, torch.from_numpy(z["u"])) < 1e-10
+
+
+@pytest.mark.parametrize("et", ["quad4", "quad9"])
+def test_mixed_precision_solve_reaches_fp64_tolerance(et):
+    p = fem.precompute_geometry(cantilever(et, h=0.25, load=(0.0, -20.0)))
+    th = torch.ones(p["conn"].shape[0], dtype=torch.float64)
+    tol = 1e-10 * float(torch.norm(p["f"]))
+    u64, i64 = _solve(p, th, 3, tol)
+    u32, i32 = _solve(p, th, 3, tol, precision="mixed")
+    assert i64["converged"] and i32["converged"]
+    assert float(torch.norm(fem.global_residual(u32.cpu(), th, p))) < tol
+    assert rel(u32, u64) < 1e-8
+
+
+def test_displacement_controlled_unit_square():
+    p = unit_square("quad4", 20, value=(0.1, None))
+    th = torch.ones(p["conn"].shape[0], dtype=torch.float64)
+    u, info = _solve(p, th, 3, 1e-11)
+    assert info["converged"]
+    right = p["nodes"][:, 0] > 1 - 1e-9
+    assert torch.allclose(u.cpu().reshape(-1, 2)[right, 0], torch.full((int(right.sum()),), 0.1, dtype=torch.float64))
+
+
+def test_adjoint_matches_reference_golden_and_oracle():
+    from difffea_amd.adjoint import compute_sensitivity
+    from difffea_amd.problem import build_problem
+    z = np.load(os.path.join(GOLD, "g3_q4_compliance_16.npz"))
+    spec = json.loads(str(z["spec"]))
+    spec.pop("mesher", None)
+    p = build_problem(spec)
+    th = torch.from_numpy(z["theta"])
+    f = p["f"].to(DEV)
+    u, g = compute_sensitivity(p, th.to(DEV), lambda w: torch.dot(f, w), n_load_steps=2, tol=1e-12)
+    assert rel(u, torch.from_numpy(z["u"])) < 1e-10
+    assert rel(g, torch.from_numpy(z["grad"])) < 1e-9
+
+
+def test_autograd_function_gradient_matches_finite_differences():
+    from difffea_amd.adjoint import equilibrium
+    p = cantilever("quad4", h=1.0, load=(50.0, 0.0))
+    n_el = p["conn"].shape[0]
+    th0 = 1.0 + 0.2 * torch.sin(torch.arange(n_el, dtype=torch.float64))
+    f = p["f"].to(DEV)
+    th = th0.to(DEV).requires_grad_(True)
+    u = equilibrium(th, p, n_load_steps=2, tol=1e-12)
+    L = torch.dot(f, u)
+    L.backward()
+    g = th.grad.detach().cpu()
+    eps, gfd = 1e-6, torch.zeros(n_el, dtype=torch.float64)
+    for i in range(n_el):
+        tp, tm = th0.clone(), th0.clone()
+        tp[i] += eps
+        tm[i] -= eps
+        up, _ = _solve(p, tp, 2, 1e-12)
+        um, _ = _solve(p, tm, 2, 1e-12)
+        gfd[i] = (float(torch.dot(f, up)) - float(torch.dot(f, um))) / (2 * eps)
+    assert float(torch.norm(g - gfd) / torch.norm(gfd)) < 1e-6

@@ -1,0 +1,256 @@
+"""Pins of the CPU oracle against mathematics that does not go through the oracle's own formulas:
+closed forms, textbook limits, invariants, finite differences and dense linear algebra."""
+import math
+
+import numpy as np
+import pytest
+import torch
+from torch.func import grad, hessian
+
+from helpers import DTYPE, cantilever, dense, state, unit_square
+from oracle import fem
+from difffea_amd import elements
+
+F_SAMPLE = torch.tensor([[1.12, 0.07], [-0.04, 0.93]], dtype=DTYPE)
+P0 = torch.tensor([1.3, 0.7], dtype=DTYPE)
+
+
+def rot(a):
+    return torch.tensor([[math.cos(a), -math.sin(a)], [math.sin(a), math.cos(a)]], dtype=DTYPE)
+
+
+# ---- shape functions and quadrature --------------------------------------------------------------------
+@pytest.mark.parametrize("et", ["quad4", "quad9"])
+def test_shape_functions_kronecker_and_partition(et):
+    X = elements.REFERENCE_NODES[et]
+    N = elements.shape_values(et, X[:, 0], X[:, 1])
+    assert np.allclose(N, np.eye(len(X)), atol=1e-15)
+    xi, eta = np.random.default_rng(0).uniform(-1, 1, (2, 20))
+    assert np.allclose(elements.shape_values(et, xi, eta).sum(-1), 1.0, atol=1e-14)
+    assert np.allclose(elements.shape_gradients(et, xi, eta).sum(-2), 0.0, atol=1e-13)
+
+
+@pytest.mark.parametrize("et", ["quad4", "quad9"])
+def test_shape_gradients_closed_form_vs_fd_and_oracle_ad(et):
+    xi, eta = np.random.default_rng(1).uniform(-1, 1, (2, 7))
+    h = 1e-6
+    fd_x = (elements.shape_values(et, xi + h, eta) - elements.shape_values(et, xi - h, eta)) / (2 * h)
+    fd_e = (elements.shape_values(et, xi, eta + h) - elements.shape_values(et, xi, eta - h)) / (2 * h)
+    g = elements.shape_gradients(et, xi, eta)
+    assert np.allclose(g[..., 0], fd_x, atol=1e-8) and np.allclose(g[..., 1], fd_e, atol=1e-8)
+    GP = torch.tensor(np.stack([xi, eta], 1))
+    assert np.allclose(fem.reference_gradients(et, GP).numpy(), g, atol=1e-14)
+    # oracle shape functions = product-side shape functions
+    assert np.allclose(torch.stack([fem.SHAPE_FN[et](torch.tensor(a), torch.tensor(b)) for a, b in zip(xi, eta)]).numpy(),
+                       elements.shape_values(et, xi, eta), atol=1e-15)
+
+
+def test_q4_gradient_hand_derivative():
+    """dN1/dxi = -(1-eta)/4, dN1/deta = -(1-xi)/4 (reference test_shape_functions.py:45-51)."""
+    g = elements.shape_gradients("quad4", 0.3, -0.2)
+    assert g[0, 0] == pytest.approx(-(1 + 0.2) / 4) and g[0, 1] == pytest.approx(-(1 - 0.3) / 4)
+
+
+@pytest.mark.parametrize("n", [2, 3])
+def test_gauss_rule_exactness_and_order(n):
+    GP, GW = elements.gauss_2d(n)
+    assert GW.sum() == pytest.approx(4.0)
+    # exact for x^a y^b with a, b <= 2n-1
+    for a in range(2 * n):
+        for b in range(2 * n):
+            exact = (0 if a % 2 else 2 / (a + 1)) * (0 if b % 2 else 2 / (b + 1))
+            assert (GW * GP[:, 0] ** a * GP[:, 1] ** b).sum() == pytest.approx(exact, abs=1e-14)
+    # eta-major, xi-minor: the first n points share eta
+    assert np.allclose(GP[:n, 1], GP[0, 1]) and not np.allclose(GP[:n, 0], GP[0, 0])
+    GPo, GWo = fem.gauss_rule_2d(n)
+    assert np.allclose(GPo.numpy(), GP) and np.allclose(GWo.numpy(), GW)
+
+
+# ---- geometry ------------------------------------------------------------------------------------------
+@pytest.mark.parametrize("et", ["quad4", "quad9"])
+def test_geometry_of_a_rectangle(et):
+    p = fem.precompute_geometry(unit_square(et, 3, ny=2))
+    dN, w = p["dN_dX"], p["w_detJ"]
+    assert float(w.sum()) == pytest.approx(1.0, abs=1e-14)             # area of the unit square
+    hx, hy = 1 / 3, 1 / 2
+    GP = p["GP"].numpy()
+    ref = elements.shape_gradients(et, GP[:, 0], GP[:, 1])             # (n_q, nen, 2)
+    expect = torch.tensor(ref) * torch.tensor([2 / hx, 2 / hy], dtype=DTYPE)   # affine map: J = diag(h/2)
+    assert torch.allclose(dN, expect.expand_as(dN), atol=1e-12)
+
+
+@pytest.mark.parametrize("et", ["quad4", "quad9"])
+def test_affine_displacement_gives_exact_deformation_gradient(et):
+    p = fem.precompute_geometry(unit_square(et, 3))
+    # distort the mesh a little (interior nodes only keep it valid)
+    F0 = torch.tensor([[1.2, 0.1], [-0.05, 0.9]], dtype=DTYPE)
+    u = (p["nodes"] @ (F0 - torch.eye(2, dtype=DTYPE)).T).reshape(-1)
+    ue = u.reshape(-1, 2)[p["conn"]]
+    F = torch.eye(2, dtype=DTYPE) + torch.einsum("eai,eqaj->eqij", ue, p["dN_dX"])
+    assert torch.allclose(F, F0.expand_as(F), atol=1e-12)
+
+
+# ---- material ------------------------------------------------------------------------------------------
+def _P_closed(F, mu, lam):
+    Fit = torch.linalg.inv(F).T
+    return mu * (F - Fit) + lam * torch.log(torch.linalg.det(F)) * Fit
+
+
+def test_piola_matches_closed_form():
+    P = grad(fem.psi_neo_hookean)(F_SAMPLE, P0)
+    assert torch.allclose(P, _P_closed(F_SAMPLE, 1.3, 0.7), atol=1e-13)
+
+
+def test_tangent_matches_finite_difference_of_closed_form_stress():
+    H = hessian(fem.psi_neo_hookean)(F_SAMPLE, P0)                      # (2,2,2,2)
+    h = 1e-6
+    for k in range(2):
+        for l in range(2):
+            E = torch.zeros(2, 2, dtype=DTYPE)
+            E[k, l] = h
+            fd = (_P_closed(F_SAMPLE + E, 1.3, 0.7) - _P_closed(F_SAMPLE - E, 1.3, 0.7)) / (2 * h)
+            assert torch.allclose(H[:, :, k, l], fd, atol=1e-8)
+
+
+def test_tangent_at_identity_is_isotropic_linear_elasticity():
+    mu, lam = 1.3, 0.7
+    H = hessian(fem.psi_neo_hookean)(torch.eye(2, dtype=DTYPE), P0)
+    d = torch.eye(2, dtype=DTYPE)
+    iso = lam * torch.einsum("ij,kl->ijkl", d, d) + mu * (torch.einsum("ik,jl->ijkl", d, d)
+                                                          + torch.einsum("il,jk->ijkl", d, d))
+    assert torch.allclose(H, iso, atol=1e-14)
+
+
+def test_energy_stress_free_reference_and_objectivity():
+    I = torch.eye(2, dtype=DTYPE)
+    assert abs(float(fem.psi_neo_hookean(I, P0))) < 1e-15
+    assert float(grad(fem.psi_neo_hookean)(I, P0).abs().max()) < 1e-15
+    assert float(fem.psi_neo_hookean(rot(0.9) @ F_SAMPLE, P0)) == pytest.approx(float(fem.psi_neo_hookean(F_SAMPLE, P0)), abs=1e-13)
+    # the 2-D energy mu/2 (F:F - 2 - 2 lnJ) + lam/2 lnJ^2 (reference constitutive_relations.py:43-48)
+    J = torch.linalg.det(F_SAMPLE)
+    psi2 = 0.65 * (float((F_SAMPLE * F_SAMPLE).sum()) - 2 - 2 * math.log(J)) + 0.35 * math.log(J) ** 2
+    assert float(fem.psi_neo_hookean(F_SAMPLE, P0)) == pytest.approx(psi2, abs=1e-14)
+
+
+# ---- residual ------------------------------------------------------------------------------------------
+@pytest.mark.parametrize("et", ["quad4", "quad9"])
+def test_rigid_body_invariance(et):
+    p = fem.precompute_geometry(cantilever(et))
+    p0 = dict(p, f=torch.zeros_like(p["f"]))
+    th = torch.ones(p["conn"].shape[0], dtype=DTYPE)
+    n = p["nodes"].shape[0]
+    t = torch.zeros(2 * n, dtype=DTYPE)
+    t[0::2], t[1::2] = 0.3, -0.2
+    assert float(fem.global_residual(t, th, p0).abs().max()) < 1e-9
+    X = p["nodes"]
+    u_rot = (X @ rot(0.7).T - X).reshape(-1)
+    assert float(fem.global_residual(u_rot, th, p0).abs().max()) < 1e-8
+
+
+@pytest.mark.parametrize("et", ["quad4", "quad9"])
+def test_internal_force_is_gradient_of_total_energy(et):
+    p = fem.precompute_geometry(unit_square(et, 3))
+    p = dict(p, m=torch.ones_like(p["m"]), f=torch.zeros_like(p["f"]))
+    th = 0.5 + torch.rand(p["conn"].shape[0], dtype=DTYPE, generator=torch.Generator().manual_seed(2))
+    N = p["nodes"].shape[0]
+
+    def energy(u):
+        ue = u.reshape(N, 2)[p["conn"]]
+        F = torch.eye(2, dtype=DTYPE) + torch.einsum("eai,eqaj->eqij", ue, p["dN_dX"])
+        J = F[..., 0, 0] * F[..., 1, 1] - F[..., 0, 1] * F[..., 1, 0]
+        lnJ = torch.log(J)
+        mu, lam = p["mat_p"][0] * th[:, None], p["mat_p"][1] * th[:, None]
+        psi = 0.5 * mu * ((F * F).sum((-1, -2)) - 2 - 2 * lnJ) + 0.5 * lam * lnJ ** 2
+        return (p["w_detJ"] * psi).sum()
+
+    u = 0.04 * torch.randn(2 * N, dtype=DTYPE, generator=torch.Generator().manual_seed(3))
+    assert torch.allclose(grad(energy)(u), fem.global_residual(u, th, p), atol=1e-12, rtol=1e-10)
+
+
+# ---- tangent operators, diagonal, PCG ------------------------------------------------------------------
+@pytest.mark.parametrize("et", ["quad4", "quad9"])
+def test_tangent_variants_agree_and_match_finite_differences(et):
+    p = fem.precompute_geometry(cantilever(et))
+    th = 0.5 + torch.rand(p["conn"].shape[0], dtype=DTYPE, generator=torch.Generator().manual_seed(4))
+    u, v = state(p, 0)
+    ref = fem.make_A_operator(u, th, p)(v)
+    for k in ("qp", "qp_ew"):
+        y = fem.make_tangent_operator(k, u, th, p)(v)
+        assert float((y - ref).abs().max() / ref.abs().max()) < 1e-12
+    AT = fem.make_AT_operator(u, th, p)(v)
+    assert float((AT - ref).abs().max() / ref.abs().max()) < 1e-12
+    vm = v * p["m"]
+    eps = 1e-6
+    fd = (fem.global_residual(u + eps * vm, th, p) - fem.global_residual(u - eps * vm, th, p)) / (2 * eps)
+    y = fem.make_A_operator_qp(u, th, p)(vm)
+    assert float((y * p["m"] - fd).abs().max()) < 1e-5 * float(fd.abs().max())
+
+
+@pytest.mark.parametrize("et", ["quad4", "quad9"])
+def test_diagonal_pcg_and_symmetry_against_dense(et):
+    p = fem.precompute_geometry(cantilever(et, h=0.5))
+    th = torch.ones(p["conn"].shape[0], dtype=DTYPE)
+    u, _ = state(p, 5, scale=0.002)
+    n = u.numel()
+    A = fem.make_A_operator_qp(u, th, p)
+    K = dense(A, n)
+    assert float((K - K.T).abs().max()) < 1e-9 * float(K.abs().max())
+    assert float(torch.linalg.eigvalsh(K).min()) > 0
+    d = fem.compute_jacobi_diagonal(u, th, p)
+    dd = torch.where(p["m"] > 0, torch.clamp(torch.diag(K), min=1e-6), torch.ones(n, dtype=DTYPE))
+    assert float((d - dd).abs().max()) < 1e-9 * float(dd.abs().max())
+    b = torch.randn(n, dtype=DTYPE, generator=torch.Generator().manual_seed(6)) * p["m"]
+    x, k = fem.pcg(A, b, fem.make_jacobi_preconditioner(u, th, p), rtol=1e-13, max_iter=5000)
+    xs = torch.linalg.solve(K, b)
+    assert float((x - xs).abs().max()) < 1e-9 * float(xs.abs().max())
+
+
+def test_pcg_edge_cases():
+    A = lambda x: x * torch.arange(1, 6, dtype=DTYPE)
+    M = lambda r: r
+    x, k = fem.pcg(A, torch.zeros(5, dtype=DTYPE), M)
+    assert k == 0 and float(x.abs().max()) == 0
+    neg = lambda x: -x
+    b = torch.ones(5, dtype=DTYPE)
+    x, k = fem.pcg(neg, b, M)                       # negative curvature at k = 0 returns z = M^-1 b
+    assert k == 0 and torch.equal(x, b)
+
+
+# ---- solve and adjoint ---------------------------------------------------------------------------------
+def test_solve_equilibrium_and_reactions_balance_load():
+    p = fem.precompute_geometry(cantilever("quad4", load=(0.0, -20.0)))
+    th = torch.ones(p["conn"].shape[0], dtype=DTYPE)
+    u, info = fem.newton_krylov_solve(p, th, 3, 1e-9)
+    assert info["converged"] and info["load_factor"] == 1.0
+    assert float(torch.norm(fem.global_residual(u, th, p))) < 1e-9
+    # reaction = internal force on the fixed dofs; it balances the applied load
+    p_all = dict(p, m=torch.ones_like(p["m"]), f=torch.zeros_like(p["f"]))
+    Rint = fem.global_residual(u, th, p_all).reshape(-1, 2)
+    reaction = (Rint * p["fixed"].reshape(-1, 2)).sum(0)
+    assert torch.allclose(reaction + p["f"].reshape(-1, 2).sum(0), torch.zeros(2, dtype=DTYPE), atol=1e-7)
+
+
+def test_displacement_controlled_solve_honours_the_prescribed_motion():
+    p = fem.precompute_geometry(unit_square("quad4", 6, value=(0.1, None)))
+    th = torch.ones(p["conn"].shape[0], dtype=DTYPE)
+    u, info = fem.newton_krylov_solve(p, th, 2, 1e-11)
+    assert info["converged"]
+    right = p["nodes"][:, 0] > 1 - 1e-9
+    assert torch.allclose(u.reshape(-1, 2)[right, 0], torch.full((int(right.sum()),), 0.1, dtype=DTYPE))
+
+
+def test_adjoint_gradient_matches_central_finite_differences():
+    p = fem.precompute_geometry(cantilever("quad4", h=1.0, load=(50.0, 0.0)))
+    n_el = p["conn"].shape[0]
+    th = 1.0 + 0.2 * torch.sin(torch.arange(n_el, dtype=DTYPE))
+    loss = lambda u: torch.dot(p["f"], u)
+    _, g = fem.compute_sensitivity(p, th, loss, n_load_steps=2, tol=1e-12)
+    eps, gfd = 1e-6, torch.zeros_like(th)
+    for i in range(n_el):
+        tp, tm = th.clone(), th.clone()
+        tp[i] += eps
+        tm[i] -= eps
+        Lp = loss(fem.newton_krylov_solve(p, tp, 2, 1e-12)[0])
+        Lm = loss(fem.newton_krylov_solve(p, tm, 2, 1e-12)[0])
+        gfd[i] = (Lp - Lm) / (2 * eps)
+    assert float(torch.norm(g - gfd) / torch.norm(gfd)) < 1e-6

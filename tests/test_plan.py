@@ -1,0 +1,116 @@
+"""Mesh plans, checked on the CPU: table consistency, colouring, and a numpy emulation of the chunked,
+colour-ordered assembly the kernels perform, compared with the oracle's scatter-add product."""
+import numpy as np
+import pytest
+import torch
+
+from helpers import cantilever, state, unit_square
+from oracle import fem
+from difffea_amd import _lib
+from difffea_amd.plan import MeshPlan, jones_plassmann
+from difffea_amd.problem import build_problem
+
+
+def _plan(p, **kw):
+    return MeshPlan(p["nodes"].numpy(), p["conn"].numpy(), p["fixed"].numpy(),
+                    cell_ij=None if "cell_ij" not in p else p["cell_ij"].numpy(), **kw)
+
+
+def _ogrid(n_rad=6, n_circ=16):
+    return build_problem(dict(element_type="quad9", geometry="plate_hole_ogrid",
+                              geom_params=dict(width=1.0, radius=0.2, n_rad=n_rad, n_circ=n_circ),
+                              E=1.0, nu=0.3, load_mode="force", load_value=(0.1, 0.0)))
+
+
+MESHES = {
+    "q4_37": lambda: unit_square("quad4", 37),
+    "q9_19": lambda: unit_square("quad9", 19),
+    "q4_cant": lambda: cantilever("quad4", h=0.05),
+    "q9_ogrid": lambda: _ogrid(),
+    "q9_ogrid_generic": lambda: {k: v for k, v in _ogrid().items() if k != "cell_ij"},
+}
+
+
+def _emulate_apply(plan, C, dN, v):
+    """numpy emulation of k_apply + k_apply_fin: per chunk gather -> element -> colour-ordered LDS
+    accumulation -> owned write / shared partial; then shared nodes sum their partials in slot order."""
+    nen = plan.nen
+    N = plan.n_nodes
+    y = np.full((N, 2), np.nan)
+    part = np.zeros((plan.n_part, 2))
+    vv = v.reshape(N, 2)
+    for c in range(plan.n_chunks):
+        e0, ne, k0, nk, nown, pbase, ncol = plan.chunks[c, :7]
+        ln = plan.lnode[k0:k0 + nk]
+        nodes = (ln & _lib.NODE_MASK).astype(np.int64)
+        bits = ln >> 30
+        raw = vv[nodes]
+        masked = raw.copy()
+        masked[bits & 1 == 1, 0] = 0
+        masked[bits & 2 == 2, 1] = 0
+        loc = plan.lelem[:, e0:e0 + ne].T.astype(np.int64)                  # (ne, nen)
+        ve = masked[loc]                                                    # (ne, nen, 2)
+        d = dN[e0:e0 + ne]                                                  # (ne, nq, nen, 2)
+        g = np.einsum("eai,eqaj->eqij", ve, d).reshape(ne, -1, 4)
+        s = np.einsum("eqmn,eqn->eqm", C[e0:e0 + ne], g).reshape(ne, -1, 2, 2)
+        fe = np.einsum("eqij,eqaj->eai", s, d)
+        acc = np.zeros((nk, 2))
+        col = plan.chunks[c, 8:8 + ncol + 1]
+        for k in range(ncol):
+            idx = loc[col[k]:col[k + 1]].reshape(-1)
+            assert len(np.unique(idx)) == len(idx), "two elements of one colour share a node"
+            acc[idx] += fe[col[k]:col[k + 1]].reshape(-1, 2)
+        out = np.where(np.stack([bits & 1, bits & 2], 1) > 0, raw, acc)
+        y[nodes[:nown]] = out[:nown]
+        part[pbase:pbase + nk - nown] = acc[nown:]
+    for i in range(plan.n_shared):
+        g_ = int(plan.sh_node[i])
+        n = g_ & _lib.NODE_MASK
+        acc = part[plan.sh_slot[plan.sh_off[i]:plan.sh_off[i + 1]]].sum(0)
+        y[n, 0] = vv[n, 0] if g_ & _lib.FIXED_X else acc[0]
+        y[n, 1] = vv[n, 1] if g_ & _lib.FIXED_Y else acc[1]
+    return y.reshape(-1)
+
+
+@pytest.mark.parametrize("name", list(MESHES))
+def test_plan_tables_and_emulated_assembly_match_oracle(name):
+    p = fem.precompute_geometry(MESHES[name]())
+    plan = _plan(p)
+    plan.validate()
+    # local tables reconstruct the connectivity
+    nodes_of = (plan.lnode & _lib.NODE_MASK).astype(np.int64)
+    ch_of_e = np.repeat(np.arange(plan.n_chunks), plan.chunks[:, 1])
+    glob = nodes_of[plan.chunks[ch_of_e, 2][None, :] + plan.lelem.astype(np.int64)]
+    assert np.array_equal(glob, plan.conn_int)
+    assert np.array_equal(plan.conn_int.T, p["conn"].numpy()[plan.perm])
+    # every node is owned by exactly one chunk or shared with a partial in each touching chunk
+    assert plan.n_lnodes - plan.n_part + plan.n_shared == plan.n_nodes
+    th = torch.ones(p["conn"].shape[0], dtype=torch.float64)
+    u, v = state(p, 2, scale=1e-3)
+    C, dN = fem.qp_setup(u, th, p)
+    y = _emulate_apply(plan, C.numpy()[plan.perm], dN.numpy()[plan.perm], v.numpy())
+    ref = fem.make_A_operator_qp(u, th, p, C=C)(v).numpy()
+    assert np.abs(y - ref).max() < 1e-12 * np.abs(ref).max()
+
+
+def test_structured_meshes_get_square_tiles_and_four_colours():
+    plan = _plan(unit_square("quad4", 400))
+    st = plan.stats()
+    assert st["n_chunks"] == 625 and st["max_chunk_nodes"] == 289 and st["max_colours"] == 4
+    plan9 = _plan(unit_square("quad9", 200))
+    assert plan9.stats()["max_chunk_nodes"] == 289
+
+
+def test_jones_plassmann_is_a_valid_colouring():
+    conn = _ogrid(4, 12)["conn"].numpy()
+    col = jones_plassmann(conn)
+    for n in range(conn.max() + 1):
+        els = np.flatnonzero((conn == n).any(1))
+        assert len(np.unique(col[els])) == len(els)
+    assert col.max() < _lib.MAXCOL
+
+
+def test_plan_rejects_oversized_chunks():
+    p = unit_square("quad4", 40)
+    with pytest.raises(ValueError):
+        _plan(p, max_nodes=100)
