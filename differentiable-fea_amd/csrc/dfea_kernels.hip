@@ -1031,6 +1031,51 @@ int dfea_pcg_iterate(const dfea_desc* table, const int32_t* items, int n_items, 
                                                    solve_items, n_solves, n_iter, stream);
 }
 
+int dfea_pcg_graph_create(const dfea_desc* table, const int32_t* items, int n_items, const int32_t* fin_items,
+                          int n_fin_items, const int32_t* vitems, int n_vitems, const int32_t* solve_items,
+                          int n_solves, int n_iter, int nen, int precision, void** graph) {
+    if (!graph || n_iter <= 0) return fail(DFEA_EINVAL, "dfea_pcg_graph_create: bad argument");
+    *graph = nullptr;
+    hipStream_t s = nullptr;
+    hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    if (e != hipSuccess) return fail(-1000 - (int)e, "dfea_pcg_graph_create: hipStreamCreate failed");
+    e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) {
+        hipStreamDestroy(s);
+        return fail(-1000 - (int)e, "dfea_pcg_graph_create: hipStreamBeginCapture failed");
+    }
+    int rc = dfea_pcg_iterate(table, items, n_items, fin_items, n_fin_items, vitems, n_vitems, solve_items,
+                              n_solves, n_iter, nen, precision, s);
+    hipGraph_t g = nullptr;
+    e = hipStreamEndCapture(s, &g);
+    hipStreamDestroy(s);
+    if (rc) {
+        if (g) hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess || !g) return fail(-1000 - (int)e, "dfea_pcg_graph_create: hipStreamEndCapture failed");
+    hipGraphExec_t x = nullptr;
+    e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (e != hipSuccess) return fail(-1000 - (int)e, "dfea_pcg_graph_create: hipGraphInstantiate failed");
+    *graph = (void*)x;
+    return 0;
+}
+
+int dfea_graph_launch(void* graph, void* stream) {
+    if (!graph) return fail(DFEA_EINVAL, "dfea_graph_launch: null graph");
+    hipError_t e = hipGraphLaunch((hipGraphExec_t)graph, S(stream));
+    if (e != hipSuccess) return fail(-1000 - (int)e, "dfea_graph_launch: hipGraphLaunch failed");
+    return 0;
+}
+
+int dfea_graph_destroy(void* graph) {
+    if (!graph) return 0;
+    hipError_t e = hipGraphExecDestroy((hipGraphExec_t)graph);
+    if (e != hipSuccess) return fail(-1000 - (int)e, "dfea_graph_destroy failed");
+    return 0;
+}
+
 int dfea_adjoint_grad(const dfea_desc* table, const int32_t* items, int n_items, const double* u, const double* lam,
                       double* g, int nen, void* stream) {
     if (!table || !items || !u || !lam || !g || bad_nen(nen)) return fail(DFEA_EINVAL, "dfea_adjoint_grad: bad argument");

@@ -64,6 +64,9 @@ _SIGNATURES = {
     "dfea_pcg_init": (_I, [_P, _P, _I, _P, _I, _I, _I, _P]),
     "dfea_pcg_iterate": (_I, [_P, _P, _I, _P, _I, _P, _I, _P, _I, _I, _I, _I, _P]),
     "dfea_adjoint_grad": (_I, [_P, _P, _I, _P, _P, _P, _I, _P]),
+    "dfea_pcg_graph_create": (_I, [_P, _P, _I, _P, _I, _P, _I, _P, _I, _I, _I, _I, ctypes.POINTER(_P)]),
+    "dfea_graph_launch": (_I, [_P, _P]),
+    "dfea_graph_destroy": (_I, [_P]),
 }
 
 _lib = None

@@ -33,7 +33,7 @@ def solve_adjoint(solver, u_star, dL_du, rtol=1e-6, max_iter=500):
     while st == _lib.PCG_RUNNING:
         s.pcg_iterate(min(chunk, max_iter - k))
         st, its, k = s.pcg_status()
-        chunk = min(2 * chunk, 256)
+        chunk = min(2 * chunk, 128)
     return s.vec["x"].to(torch.float64).clone(), its
 
 

@@ -146,6 +146,7 @@ class HipSystem:
         self.vec = {k: torch.zeros(nd, dtype=self.vdt, device=dv) for k in ("x", "r", "z", "p", "Ap", "b")}
         self.vec["u"] = torch.zeros(nd, dtype=F64, device=dv)
         self.vec["R"] = torch.zeros(nd, dtype=F64, device=dv)
+        self._graphs = {}
         self.desc = self._make_desc()
         self.table = m.upload(self.desc)
 
@@ -170,6 +171,7 @@ class HipSystem:
             m = self.mesh
             self.C = torch.empty(10 * m.nq, m.n_el, dtype=self.vdt, device=m.device)
             self.diag = torch.empty(m.n_dof, dtype=self.vdt, device=m.device)
+            self.release_graphs()             # graphs hold the previous table's address
             self.desc = self._make_desc()
             self.table = m.upload(self.desc)
 
@@ -242,21 +244,56 @@ class HipSystem:
         _lib.check(self.lib.dfea_pcg_init(_ptr(self.table), _ptr(m.vitems), m.n_vblk, _ptr(m.solve_items), 1,
                                           int(neg_res), self.prec_id, _stream()), "dfea_pcg_init")
 
-    def pcg_iterate(self, n_iter):
+    use_graphs = True
+
+    def _graph(self, n_iter):
+        """Executable HIP graph of n_iter fused PCG iterations (captured once per chunk length)."""
+        g = self._graphs.get(n_iter)
+        if g is None:
+            m = self.mesh
+            h = ctypes.c_void_p()
+            _lib.check(self.lib.dfea_pcg_graph_create(
+                _ptr(self.table), _ptr(m.items), m.plan.n_chunks, _ptr(m.fin_items), m.n_finblk, _ptr(m.vitems),
+                m.n_vblk, _ptr(m.solve_items), 1, int(n_iter), m.nen, self.prec_id, ctypes.byref(h)),
+                "dfea_pcg_graph_create")
+            self._graphs[n_iter] = g = h
+        return g
+
+    def pcg_iterate(self, n_iter, graph=None):
         m = self.mesh
+        if n_iter <= 0:
+            return
+        if self.use_graphs if graph is None else graph:
+            _lib.check(self.lib.dfea_graph_launch(self._graph(int(n_iter)), _stream()), "dfea_graph_launch")
+            return
         _lib.check(self.lib.dfea_pcg_iterate(_ptr(self.table), _ptr(m.items), m.plan.n_chunks, _ptr(m.fin_items),
                                              m.n_finblk, _ptr(m.vitems), m.n_vblk, _ptr(m.solve_items), 1,
                                              int(n_iter), m.nen, self.prec_id, _stream()), "dfea_pcg_iterate")
+
+    def release_graphs(self):
+        for g in self._graphs.values():
+            self.lib.dfea_graph_destroy(g)
+        self._graphs = {}
+
+    def __del__(self):
+        try:
+            self.release_graphs()
+        except Exception:
+            pass
 
     def pcg_status(self):
         s = self.sc[[_lib.SC["status"], _lib.SC["iters"], _lib.SC["k"]]].cpu()
         return int(s[0]), int(s[1]), int(s[2])
 
-    def pcg_solve(self, b=None, rtol=1e-5, max_iter=500, neg_res=False, chunk=32):
-        """Run the fused PCG to completion; returns (x view, iterations, status)."""
+    def pcg_solve(self, b=None, rtol=1e-5, max_iter=500, neg_res=False, chunk=16, graph=None):
+        """Run the fused PCG to completion; returns (x view, iterations, status).
+
+        Iterations run in chunks (graph replays); solves that finish inside a chunk skip the remaining
+        iterations on the device.  The status block is read once per chunk."""
         self.pcg_start(b, rtol, max_iter, neg_res)
         st, its, k = self.pcg_status()
         while st == _lib.PCG_RUNNING:
-            self.pcg_iterate(min(chunk, max_iter - k))
+            self.pcg_iterate(min(chunk, max_iter - k), graph)
             st, its, k = self.pcg_status()
+            chunk = min(2 * chunk, 128)
         return self.vec["x"], its, st

@@ -85,7 +85,7 @@ class HipNewton:
         while st == _lib.PCG_RUNNING:
             s.pcg_iterate(min(chunk, max_iter - k))
             st, its, k = s.pcg_status()
-            chunk = min(2 * chunk, 256)
+            chunk = min(2 * chunk, 128)
         return s.vec["x"], its
 
     def trial(self, u, du, alpha, out):

@@ -158,6 +158,16 @@ int dfea_pcg_iterate(const dfea_desc* table, const int32_t* items, int n_items, 
                      int n_fin_items, const int32_t* vitems, int n_vitems, const int32_t* solve_items,
                      int n_solves, int n_iter, int nen, int precision, void* stream);
 
+/* Capture n_iter iterations of dfea_pcg_iterate (same arguments) into an executable HIP graph; replay it
+ * with dfea_graph_launch on any stream.  The tables and buffers the descriptors point to must outlive the
+ * graph.  No memset nodes are recorded.  Graph capture of the iteration loop replaces the reference's
+ * per-iteration Python loop with two host syncs (`newton_krylov.py:62-86`). */
+int dfea_pcg_graph_create(const dfea_desc* table, const int32_t* items, int n_items, const int32_t* fin_items,
+                          int n_fin_items, const int32_t* vitems, int n_vitems, const int32_t* solve_items,
+                          int n_solves, int n_iter, int nen, int precision, void** graph);
+int dfea_graph_launch(void* graph, void* stream);
+int dfea_graph_destroy(void* graph);
+
 /* ---- adjoint -------------------------------------------------------------------------------------------- */
 /* g[e] = -sum_{a,i} (m*lam)[2 conn(e,a)+i] * R^e_int[a,i](u; mu0, lam0), original element order.
  * dL/dtheta for neo-Hookean theta-scaling.  Replaces `adjoint.py:75-106` (compute_implicit_vjp). */
