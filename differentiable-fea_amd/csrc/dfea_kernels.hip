@@ -673,17 +673,19 @@ __global__ __launch_bounds__(VBS) void k_dot_final(const dfea_desc* __restrict__
     if (threadIdx.x == 0) D.sc[DFEA_SC_NORM] = s;
 }
 
-// mode 0: out = where(fixed, lf uD, u);  mode 1: out = u + alpha dx
+// mode 0: out = where(fixed, lf uD, u);  mode 1: out = u + alpha dx;  mode 2: out = u (accept)
 template <typename T>
 __global__ __launch_bounds__(VBS) void k_axpy(const dfea_desc* __restrict__ tab, const int* __restrict__ items, int mode,
                                              const double* __restrict__ uin, const T* __restrict__ dxin,
-                                             const double* __restrict__ uD, const double* __restrict__ sarg,
-                                             double* __restrict__ out) {
+                                             const double* __restrict__ uDin, const double* __restrict__ sarg,
+                                             double* __restrict__ outp) {
     const Item it = load_item(items);
     const dfea_desc& D = tab[it.s];
     const int n = ndof(D);
-    const double* u = uin ? uin : static_cast<const double*>(D.vec[DFEA_U]);
-    const double a = sarg[it.s];
+    const double* u = uin ? uin : static_cast<const double*>(D.vec[mode == 2 ? DFEA_U : DFEA_UACC]);
+    double* out = outp ? outp : static_cast<double*>(D.vec[mode == 2 ? DFEA_UACC : DFEA_U]);
+    const double* uD = uDin ? uDin : static_cast<const double*>(D.vec[DFEA_UD]);
+    const double a = mode == 2 ? 0.0 : sarg[it.s];
     const T* dx = dxin ? dxin : static_cast<const T*>(D.vec[DFEA_X]);
     const int base = it.b * VBLK;
 #pragma unroll
@@ -693,8 +695,10 @@ __global__ __launch_bounds__(VBS) void k_axpy(const dfea_desc* __restrict__ tab,
         if (mode == 0) {
             const bool fixed = (D.nfix[i >> 1] >> (i & 1)) & 1;
             out[i] = fixed ? a * uD[i] : u[i];
-        } else {
+        } else if (mode == 1) {
             out[i] = u[i] + a * (double)dx[i];
+        } else {
+            out[i] = u[i];
         }
     }
 }
@@ -996,7 +1000,7 @@ int dfea_dot(const dfea_desc* table, const int32_t* items, int n_items, const in
 
 int dfea_axpy(const dfea_desc* table, const int32_t* items, int n_items, int mode, const double* u, const void* dx,
               const double* uD, const double* sarg, double* out, int precision, void* stream) {
-    if (!table || !items || !sarg || !out || (mode != 0 && mode != 1) || (mode == 0 && !uD))
+    if (!table || !items || (mode != 0 && mode != 1 && mode != 2) || (mode != 2 && !sarg))
         return fail(DFEA_EINVAL, "dfea_axpy: bad argument");
     if (precision == DFEA_FP64)
         return launch(k_axpy<double>, n_items, VBS, stream, "k_axpy", table, items, mode, u,

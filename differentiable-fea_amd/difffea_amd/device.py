@@ -124,7 +124,7 @@ class HipMesh:
 class HipSystem:
     """One solve (mesh + theta + material + precision) with all its device buffers."""
 
-    def __init__(self, mesh, theta, mat_p, f, precision="fp64"):
+    def __init__(self, mesh, theta, mat_p, f, precision="fp64", uD=None, sc=None):
         self.mesh = m = mesh
         self.lib = mesh.lib
         dv = mesh.device
@@ -142,10 +142,13 @@ class HipSystem:
         self.diag = None
         self.part = torch.empty(max(1, m.plan.n_part) * 2, dtype=F64, device=dv)
         self.red = torch.zeros(max(1, m.red_size), dtype=F64, device=dv)
-        self.sc = torch.zeros(_lib.NSC, dtype=F64, device=dv)
+        # the scalar block may be a row of a pool shared by a batch of solves
+        self.sc = torch.zeros(_lib.NSC, dtype=F64, device=dv) if sc is None else sc
         self.vec = {k: torch.zeros(nd, dtype=self.vdt, device=dv) for k in ("x", "r", "z", "p", "Ap", "b")}
         self.vec["u"] = torch.zeros(nd, dtype=F64, device=dv)
         self.vec["R"] = torch.zeros(nd, dtype=F64, device=dv)
+        self.vec["uacc"] = torch.zeros(nd, dtype=F64, device=dv)
+        self.vec["uD"] = torch.zeros(nd, dtype=F64, device=dv) if uD is None else uD.to(dv, F64).contiguous()
         self._graphs = {}
         self.desc = self._make_desc()
         self.table = m.upload(self.desc)

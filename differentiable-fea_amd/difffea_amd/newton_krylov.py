@@ -12,6 +12,7 @@
   refinement loop, so the converged residual meets the same FP64 tolerance.  The inner relative tolerance
   is floored at ``MIXED_RTOL_FLOOR`` (FP32 CG cannot resolve smaller relative residuals).
 """
+import math
 import time
 
 import torch
@@ -122,7 +123,7 @@ class HipNewton:
                 return u, True, records
             if it == max_newton:
                 break
-            eta = min(0.5, (rn / (r0 + 1e-14)) ** 0.5) if cg_rtol is None else cg_rtol
+            eta = min(0.5, math.sqrt(rn / (r0 + 1e-14))) if cg_rtol is None else cg_rtol
             self.sys.moduli(u)
             du, k = self.linear_solve(eta, cg_max_iter)
             if self.ucand is u:

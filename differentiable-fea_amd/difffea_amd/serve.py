@@ -1,0 +1,411 @@
+"""Served solve stream with continuous batching on one GPU.
+
+Requests (meshes of uneven size, quad4 / quad9, own material and load, optionally a compliance gradient)
+arrive over time.  The server admits them as slots and HBM budget free up, keeps every solve's state on
+the device, and advances all admitted solves together in *rounds*: in one round every solve performs its
+next device operation (lift, residual + norm, moduli + diagonal, PCG start, a chunk of PCG iterations,
+line-search trial, accept, adjoint gradient), and operations of the same kind, element type and precision
+from all solves go through ONE launch of the batched kernels (descriptor table + work items).  Requests that
+arrive while others run are admitted between rounds, so their work overlaps on the device.
+
+Each solve follows the reference's state machine (`src/difffea/newton_krylov.py:126-264`): load continuation
+with cut-backs, Eisenstat-Walker forcing, Armijo line search; the adjoint follows `src/difffea/adjoint.py`.
+Per-solve reductions use only the solve's own blocks, and no kernel's arithmetic depends on the batch, so a
+solve's result is bitwise the same alone and in any mix.
+
+Latency = completion time - arrival time; the report holds solves/s, p50 / p99 latency and per-request
+Newton / CG counts and final residuals.
+"""
+import ctypes
+import heapq
+import math
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import _lib
+from .device import HipMesh, HipSystem, _ptr, _stream
+from .problem import build_problem
+
+MIXED_RTOL_FLOOR = 1e-6
+
+
+# ------------------------------------------------------------------------------------------------------------
+# workload
+# ------------------------------------------------------------------------------------------------------------
+@dataclass
+class Request:
+    rid: int
+    arrival: float                 # seconds from the start of the stream
+    element_type: str
+    n: int                         # cells per side of the unit square
+    mu: float
+    lam: float
+    stretch: float                 # prescribed x-displacement of the right edge (displacement requests)
+    n_load_steps: int = 3
+    adjoint: bool = False          # compliance gradient w.r.t. per-element rho (force-loaded request)
+    seed: int = 0
+
+
+def make_stream(n_requests=2000, rate=50.0, seed=5, side=(32, 512), q4_fraction=0.7, adjoint_fraction=0.2):
+    """The served-stream workload: Poisson arrivals, side log-uniform, 70 % Q4 / 30 % Q9,
+    mu ~ U(0.2, 0.6), lambda/mu ~ U(1, 3), stretch ~ U(1 %, 15 %), 3 load steps, 20 % with adjoint."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, n_requests))
+    out = []
+    for i in range(n_requests):
+        et = "quad4" if rng.random() < q4_fraction else "quad9"
+        n = int(round(math.exp(rng.uniform(math.log(side[0]), math.log(side[1])))))
+        if et == "quad9":
+            n = max(side[0] // 2, n // 2)          # same node count per side as a Q4 mesh of side n
+        mu = rng.uniform(0.2, 0.6)
+        lam = mu * rng.uniform(1.0, 3.0)
+        out.append(Request(i, float(t[i]), et, n, mu, lam, rng.uniform(0.01, 0.15),
+                           adjoint=bool(rng.random() < adjoint_fraction), seed=int(rng.integers(1 << 30))))
+    return out
+
+
+def request_problem(req):
+    """params + theta of a request (unit square, left edge clamped, right edge loaded)."""
+    if req.adjoint:
+        # dead traction on the right edge: total force = stretch * E (uniaxial estimate), so strains match
+        E = req.mu * (3 * req.lam + 2 * req.mu) / (req.lam + req.mu)
+        spec = dict(element_type=req.element_type, nx=req.n, ny=req.n, material_params=dict(mu=req.mu, lmbda=req.lam),
+                    load_mode="force", load_value=(req.stretch * E * 0.5, 0.0))
+    else:
+        spec = dict(element_type=req.element_type, nx=req.n, ny=req.n, material_params=dict(mu=req.mu, lmbda=req.lam),
+                    load_mode="displacement", load_value=(req.stretch, None))
+    p = build_problem(spec)
+    n_el = p["conn"].shape[0]
+    if req.adjoint:
+        g = torch.Generator().manual_seed(req.seed)
+        theta = 0.1 + 0.9 * torch.rand(n_el, dtype=torch.float64, generator=g)
+    else:
+        theta = torch.ones(n_el, dtype=torch.float64)
+    return p, theta
+
+
+def footprint_bytes(element_type, n, precision):
+    """HBM bytes a request holds while admitted (geometry, moduli, vectors, plan), an upper estimate."""
+    nen, nq = (4, 4) if element_type == "quad4" else (9, 9)
+    n_el = n * n
+    side = n + 1 if nen == 4 else 2 * n + 1
+    n_dof = 2 * side * side
+    s = 8 if precision == "fp64" else 4
+    per_qp = 2 * nen * 12 + 8 + 10 * s
+    return n_el * nq * per_qp + n_dof * (6 * s + 6 * 8) + n_el * nen * 8 + 64 * n_el + (1 << 20)
+
+
+# ------------------------------------------------------------------------------------------------------------
+# one solve as a generator of device operations
+# ------------------------------------------------------------------------------------------------------------
+@dataclass
+class Job:
+    req: Request
+    slot: int
+    mesh: HipMesh
+    sys: HipSystem
+    params: dict
+    theta: torch.Tensor
+    tol: float = 0.0
+    gen: object = None
+    op: tuple = None
+    result: dict = field(default_factory=dict)
+    items: dict = field(default_factory=dict)
+
+
+def solve_ops(job, tol, max_newton=25, max_cutbacks=8, cg_max_iter=500):
+    """The reference's Newton-Krylov state machine, as a generator of device operations.
+
+    yields ("lift", lf) -> None, ("resnorm", lf) -> ||R||, ("moduli",) -> None,
+    ("pcg", rtol, max_iter, neg_res) -> iterations, ("trial", alpha) -> None, ("accept",) -> None
+    """
+    info = {"total_newton": 0, "total_cg": 0, "n_cutbacks": 0, "converged": False, "residual": float("nan")}
+    mixed = job.sys.precision == "mixed"
+    d_nom = 1.0 / job.req.n_load_steps
+    d, lf, in_row = d_nom, 0.0, 0
+    while lf < 1.0 - 1e-12:
+        lf_try = min(1.0, lf + d)
+        # ---- newton_at_load ----
+        yield ("lift", lf_try)                    # vec[u] = where(fixed, lf u_D, u_acc)
+        yield ("accept",)                         # the lifted state is the Newton start (u_acc = vec[u])
+        rn = yield ("resnorm", lf_try)
+        r0 = rn
+        ok = False
+        for it in range(max_newton + 1):
+            if rn < tol:
+                ok = True
+                break
+            if it == max_newton:
+                break
+            eta = min(0.5, math.sqrt(rn / (r0 + 1e-14)))
+            if mixed:
+                eta = max(eta, MIXED_RTOL_FLOOR)
+            yield ("moduli",)
+            k = yield ("pcg", eta, cg_max_iter, True)
+            info["total_newton"] += 1
+            info["total_cg"] += k
+            alpha, accepted = 1.0, False
+            while alpha >= 2.0 ** -5:
+                yield ("trial", alpha)            # vec[u] = u_acc + alpha x
+                rc = yield ("resnorm", lf_try)
+                if rc < (1.0 - 1e-4 * alpha) * rn:
+                    yield ("accept",)
+                    rn, accepted = rc, True
+                    break
+                alpha *= 0.5
+            if not accepted:
+                break
+        if ok:
+            lf = lf_try
+            info["last_good"] = lf
+            in_row = 0
+            d = min(d_nom, 1.5 * d)
+            # keep the converged state as the base of the next load step (u_acc already holds it)
+        else:
+            in_row += 1
+            info["n_cutbacks"] += 1
+            if in_row > max_cutbacks:
+                info["residual"] = rn
+                return info
+            d *= 0.5
+            yield ("restore",)                    # u_acc = last converged state
+    info["converged"] = True
+    info["residual"] = rn
+    if job.req.adjoint:
+        yield ("moduli",)                         # tangent at u* (vec[u] holds the accepted state)
+        k = yield ("adjoint",)                    # A lam = m f (compliance: dL/du = f); then the gradient kernel
+        info["adjoint_cg"] = k
+    return info
+
+
+# ------------------------------------------------------------------------------------------------------------
+# the server
+# ------------------------------------------------------------------------------------------------------------
+class Server:
+    """Continuous-batching solver for a stream of requests on one GPU (see module docstring)."""
+
+    def __init__(self, precision="fp64", max_slots=64, hbm_budget=64 << 30, pcg_chunk=8, rel_tol=1e-8,
+                 device=None, adjoint_rtol=1e-8):
+        self.lib = _lib.load()
+        self.device = torch.device(device or "cuda")
+        self.precision = precision
+        self.prec_id = _lib.FP64 if precision == "fp64" else _lib.FP32
+        self.max_slots = max_slots
+        self.hbm_budget = hbm_budget
+        self.pcg_chunk = pcg_chunk
+        self.rel_tol = rel_tol
+        self.adjoint_rtol = adjoint_rtol
+        dsz = self.lib.dfea_desc_size()
+        self.dsz = dsz
+        self.table = torch.zeros(max_slots * dsz, dtype=torch.uint8, device=self.device)
+        self.sc = torch.zeros(max_slots, _lib.NSC, dtype=torch.float64, device=self.device)
+        self.sarg = torch.zeros(max_slots, dtype=torch.float64, device=self.device)
+        self.free = list(range(max_slots))[::-1]
+        self.jobs = {}
+        self.used_bytes = 0
+
+    # -- admission -----------------------------------------------------------------------------------------
+    def _admit(self, req):
+        slot = self.free.pop()
+        params, theta = request_problem(req)
+        mesh = HipMesh(params, self.device)
+        mp = params["mat_p"]
+        sys_ = HipSystem(mesh, theta, mp, params["f"], self.precision, uD=params["u_D"], sc=self.sc[slot])
+        sys_._ensure_moduli_buffers()
+        self.table[slot * self.dsz:(slot + 1) * self.dsz].copy_(
+            torch.frombuffer(bytearray(bytes(sys_.desc)), dtype=torch.uint8))
+        job = Job(req, slot, mesh, sys_, params, theta)
+        job.bytes = footprint_bytes(req.element_type, req.n, self.precision)
+        self.used_bytes += job.bytes
+        pl = mesh.plan
+        job.items = {"chunk": self._items(slot, pl.n_chunks), "fin": self._items(slot, mesh.n_finblk),
+                     "vec": self._items(slot, mesh.n_vblk), "solve": self._items(slot, 1)}
+        # tolerance: relative to the residual of the state lifted to the full load (f-loaded: ||m f||)
+        if req.adjoint:
+            ref = float(torch.norm(params["f"] * params["m"]))
+        else:
+            uD = params["u_D"].to(self.device)
+            R = sys_.residual(sys_.lift(torch.zeros_like(uD), uD, 1.0), lf=1.0)
+            ref = float(torch.norm(R))
+        job.tol = self.rel_tol * ref
+        job.saved = torch.zeros_like(sys_.vec["uacc"])
+        job.gen = solve_ops(job, job.tol)
+        job.op = next(job.gen)
+        job.t_admit = time.perf_counter()
+        self.jobs[slot] = job
+        return job
+
+    def _retire(self, job, info):
+        self.free.append(job.slot)
+        self.used_bytes -= job.bytes
+        del self.jobs[job.slot]
+        job.sys.release_graphs()
+
+    @staticmethod
+    def _items(slot, n):
+        it = np.zeros((n, 2), np.int32)
+        it[:, 0] = slot
+        it[:, 1] = np.arange(n)
+        return it
+
+    def _dev_items(self, jobs, kind):
+        arrs = [j.items[kind] for j in jobs if len(j.items[kind])]
+        if not arrs:
+            return None, 0
+        a = np.concatenate(arrs)
+        return torch.from_numpy(a).to(self.device, non_blocking=False), len(a)
+
+    # -- one round -------------------------------------------------------------------------------------------
+    def _round(self):
+        lib, T, st = self.lib, _ptr(self.table), _stream()
+        by = {}
+        for j in self.jobs.values():
+            by.setdefault(j.op[0], []).append(j)
+        sarg_host = None
+        # lift / trial: per-solve scalar (lf or alpha) in sarg
+        for kind, mode in (("lift", 0), ("trial", 1)):
+            js = by.get(kind, [])
+            if not js:
+                continue
+            if sarg_host is None:
+                sarg_host = np.zeros(self.max_slots)
+            for j in js:
+                sarg_host[j.slot] = j.op[1]
+        if sarg_host is not None:
+            self.sarg.copy_(torch.from_numpy(sarg_host))
+        for kind, mode in (("lift", 0), ("trial", 1), ("accept", 2)):
+            js = by.get(kind, [])
+            for prec in (_lib.FP64, _lib.FP32):
+                sub = [j for j in js if j.sys.prec_id == prec]
+                it, n = self._dev_items(sub, "vec")
+                if n:
+                    _lib.check(lib.dfea_axpy(T, _ptr(it), n, mode, None, None, None, _ptr(self.sarg), None, prec, st),
+                               "dfea_axpy")
+        # restore: u_acc = saved converged state (rare, per solve)
+        for j in by.get("restore", []):
+            j.sys.vec["uacc"].copy_(j.saved)
+        # residual + norm (lf per solve in sc[LF])
+        js = by.get("resnorm", [])
+        if js:
+            idx = torch.tensor([j.slot for j in js])
+            self.sc[idx, _lib.SC["lf"]] = torch.tensor([float(j.op[1]) for j in js], dtype=torch.float64).to(self.device)
+            for nen in (4, 9):
+                sub = [j for j in js if j.mesh.nen == nen]
+                it, n = self._dev_items(sub, "chunk")
+                if not n:
+                    continue
+                fi, nf = self._dev_items(sub, "fin")
+                _lib.check(lib.dfea_residual(T, _ptr(it), n, _ptr(fi), nf, None, None, -1.0, nen, st), "dfea_residual")
+            it, n = self._dev_items(js, "vec")
+            si, ns = self._dev_items(js, "solve")
+            _lib.check(lib.dfea_dot(T, _ptr(it), n, _ptr(si), ns, None, None, st), "dfea_dot")
+        # moduli + diagonal at vec[u]
+        js = by.get("moduli", [])
+        for nen in (4, 9):
+            sub = [j for j in js if j.mesh.nen == nen]
+            it, n = self._dev_items(sub, "chunk")
+            if n:
+                fi, nf = self._dev_items(sub, "fin")
+                _lib.check(lib.dfea_moduli(T, _ptr(it), n, _ptr(fi), nf, None, nen, self.prec_id, st), "dfea_moduli")
+        # PCG start (new solves) then a chunk of iterations for every running solve
+        js = by.get("pcg", []) + by.get("adjoint", [])
+        new = [j for j in js if not getattr(j, "pcg_running", False)]
+        if new:
+            idx = torch.tensor([j.slot for j in new])
+            adj_rtol = max(self.adjoint_rtol, MIXED_RTOL_FLOOR) if self.precision == "mixed" else self.adjoint_rtol
+            vals = torch.tensor([[float(j.op[1]) if j.op[0] == "pcg" else adj_rtol,
+                                  float(j.op[2]) if j.op[0] == "pcg" else 5000.0] for j in new], dtype=torch.float64)
+            self.sc[idx, _lib.SC["rtol"]] = vals[:, 0].to(self.device)
+            self.sc[idx, _lib.SC["maxit"]] = vals[:, 1].to(self.device)
+            for neg in (True, False):
+                sub = [j for j in new if (j.op[0] == "pcg") == neg]
+                if not sub:
+                    continue
+                if not neg:                      # adjoint right-hand side m f (compliance), and the moduli at u*
+                    for j in sub:
+                        j.sys.vec["b"].copy_((j.params["m"] * j.params["f"]).to(self.device, j.sys.vdt))
+                it, n = self._dev_items(sub, "vec")
+                si, ns = self._dev_items(sub, "solve")
+                _lib.check(lib.dfea_pcg_init(T, _ptr(it), n, _ptr(si), ns, int(neg), self.prec_id, st), "dfea_pcg_init")
+            for j in new:
+                j.pcg_running = True
+        if js:
+            for nen in (4, 9):
+                sub = [j for j in js if j.mesh.nen == nen]
+                if not sub:
+                    continue
+                it, n = self._dev_items(sub, "chunk")
+                fi, nf = self._dev_items(sub, "fin")
+                vi, nv = self._dev_items(sub, "vec")
+                si, ns = self._dev_items(sub, "solve")
+                _lib.check(lib.dfea_pcg_iterate(T, _ptr(it), n, _ptr(fi), nf, _ptr(vi), nv, _ptr(si), ns,
+                                                self.pcg_chunk, nen, self.prec_id, st), "dfea_pcg_iterate")
+        # one host sync: status, iterations, norms
+        sc = self.sc.cpu().numpy()
+        finished = []
+        for j in list(self.jobs.values()):
+            kind = j.op[0]
+            if kind in ("pcg", "adjoint"):
+                status = int(sc[j.slot, _lib.SC["status"]])
+                if status == _lib.PCG_RUNNING:
+                    continue
+                j.pcg_running = False
+                val = int(sc[j.slot, _lib.SC["iters"]])
+                if kind == "adjoint":
+                    lam = j.sys.vec["x"].to(torch.float64)
+                    j.result["grad"] = j.sys.adjoint_grad(j.sys.vec["uacc"], lam)
+            elif kind == "resnorm":
+                val = math.sqrt(sc[j.slot, _lib.SC["norm"]])
+            else:
+                val = None
+            try:
+                j.op = j.gen.send(val)
+                if j.op[0] == "lift":
+                    # a new load level starts from the last converged state: remember it for cut-backs
+                    j.saved.copy_(j.sys.vec["uacc"])
+            except StopIteration as stop:
+                finished.append((j, stop.value))
+        return finished
+
+    # -- driver ------------------------------------------------------------------------------------------------
+    def run(self, requests, realtime=True, keep_solution=False):
+        """Serve ``requests`` (sorted by arrival).  Returns the per-request records and a summary."""
+        pending = sorted(requests, key=lambda r: r.arrival)
+        records = []
+        t0 = time.perf_counter()
+        i = 0
+        while i < len(pending) or self.jobs:
+            now = time.perf_counter() - t0
+            while i < len(pending) and (not realtime or pending[i].arrival <= now) and self.free:
+                r = pending[i]
+                fb = footprint_bytes(r.element_type, r.n, self.precision)
+                if self.jobs and self.used_bytes + fb > self.hbm_budget:
+                    break
+                job = self._admit(r)
+                job.t_arrival = t0 + r.arrival if realtime else job.t_admit
+                i += 1
+            if not self.jobs:
+                if i < len(pending):
+                    time.sleep(max(0.0, min(0.01, pending[i].arrival - (time.perf_counter() - t0))))
+                continue
+            for job, info in self._round():
+                t_done = time.perf_counter()
+                rec = dict(rid=job.req.rid, element_type=job.req.element_type, n=job.req.n, adjoint=job.req.adjoint,
+                           n_dof=job.mesh.n_dof, tol_abs=job.tol, latency=t_done - job.t_arrival,
+                           queue=job.t_admit - job.t_arrival, **info)
+                if keep_solution:
+                    rec["u"] = job.sys.vec["uacc"].detach().cpu().clone()
+                    if "grad" in job.result:
+                        rec["grad"] = job.result["grad"].detach().cpu().clone()
+                records.append(rec)
+                self._retire(job, info)
+        wall = time.perf_counter() - t0
+        lat = np.array([r["latency"] for r in records])
+        summary = dict(precision=self.precision, n_requests=len(records), wall_s=wall,
+                       solves_per_s=len(records) / wall, p50_latency_s=float(np.percentile(lat, 50)),
+                       p99_latency_s=float(np.percentile(lat, 99)),
+                       converged=int(sum(r["converged"] for r in records)))
+        return records, summary

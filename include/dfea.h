@@ -39,7 +39,7 @@ enum { DFEA_FP64 = 0, DFEA_FP32 = 1 };
 
 /* vector slots of a descriptor */
 enum { DFEA_X = 0, DFEA_R = 1, DFEA_Z = 2, DFEA_P = 3, DFEA_AP = 4, DFEA_B = 5, DFEA_U = 6, DFEA_RES = 7,
-       DFEA_NVEC = 8 };
+       DFEA_UACC = 8, DFEA_UD = 9, DFEA_NVEC = 10 };
 
 /* scalar block of a solve (doubles) */
 enum { DFEA_SC_RZ = 0, DFEA_SC_PAP = 1, DFEA_SC_ALPHA = 2, DFEA_SC_BETA = 3, DFEA_SC_RR = 4, DFEA_SC_BB = 5,
@@ -87,7 +87,8 @@ typedef struct dfea_desc {
     double* part;               /* [n_part] pairs (double2 or float2) */
     double* red;                /* reduction partials, dfea_red_size() doubles */
     double* sc;                 /* [DFEA_NSC] */
-    void* vec[DFEA_NVEC];       /* PCG / Newton vectors; x,r,z,p,Ap,b in the solve's precision, u,R fp64 */
+    void* vec[DFEA_NVEC];       /* PCG / Newton vectors; x,r,z,p,Ap,b in the solve's precision; u (evaluation
+                                   point), R, u_acc (accepted state), u_D (prescribed values) FP64 */
     double mu0, lam0;
     int32_t n_chunks, n_lnodes, n_shared, n_part;
     int32_t n_el, n_nodes, nen, nq;
@@ -140,9 +141,13 @@ int dfea_dot(const dfea_desc* table, const int32_t* items, int n_items, const in
              int n_solves, const double* x, const double* y, void* stream);
 
 /* ---- Newton helpers ----------------------------------------------------------------------------------- */
-/* out = where(fixed, lf*uD, u) (the load-level lift, `newton_krylov.py:139`) when mode == 0;
- * out = u + alpha * dx (the line-search trial, `newton_krylov.py:114`) when mode == 1, dx in the
- * solve's precision (vec[DFEA_X] when dx == NULL).  alpha / lf come from per-solve doubles `sarg`. */
+/* mode 0: out = where(fixed, lf*uD, u)   (the load-level lift, `newton_krylov.py:139`);
+ * mode 1: out = u + alpha * dx           (the line-search trial, `newton_krylov.py:114`), dx in the solve's
+ *                                         precision;
+ * mode 2: out = u                        (accept a trial state).
+ * NULL pointers select the descriptor's vectors: u -> vec[DFEA_UACC], dx -> vec[DFEA_X], uD -> vec[DFEA_UD],
+ * out -> vec[DFEA_U] (mode 2: u -> vec[DFEA_U], out -> vec[DFEA_UACC]).  alpha / lf come from the per-solve
+ * doubles sarg[solve]. */
 int dfea_axpy(const dfea_desc* table, const int32_t* items, int n_items, int mode, const double* u,
               const void* dx, const double* uD, const double* sarg, double* out, int precision, void* stream);
 
