@@ -1045,22 +1045,22 @@ int dfea_pcg_graph_create(const dfea_desc* table, const int32_t* items, int n_it
     if (e != hipSuccess) return fail(-1000 - (int)e, "dfea_pcg_graph_create: hipStreamCreate failed");
     e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) {
-        hipStreamDestroy(s);
+        (void)hipStreamDestroy(s);
         return fail(-1000 - (int)e, "dfea_pcg_graph_create: hipStreamBeginCapture failed");
     }
     int rc = dfea_pcg_iterate(table, items, n_items, fin_items, n_fin_items, vitems, n_vitems, solve_items,
                               n_solves, n_iter, nen, precision, s);
     hipGraph_t g = nullptr;
     e = hipStreamEndCapture(s, &g);
-    hipStreamDestroy(s);
+    (void)hipStreamDestroy(s);
     if (rc) {
-        if (g) hipGraphDestroy(g);
+        if (g) (void)hipGraphDestroy(g);
         return rc;
     }
     if (e != hipSuccess || !g) return fail(-1000 - (int)e, "dfea_pcg_graph_create: hipStreamEndCapture failed");
     hipGraphExec_t x = nullptr;
     e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
-    hipGraphDestroy(g);
+    (void)hipGraphDestroy(g);
     if (e != hipSuccess) return fail(-1000 - (int)e, "dfea_pcg_graph_create: hipGraphInstantiate failed");
     *graph = (void*)x;
     return 0;
