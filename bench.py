@@ -95,6 +95,83 @@ def cpu_baseline(p, u, v, theta, seconds=10.0):
                 ms_per_product=med * 1e3)
 
 
+def solve_config1(precision, cg_rtol=None, cg_max_iter=500):
+    """Unit-square 100x100 Q4, E=1 nu=0.3, left edge clamped, right edge u_x = 0.1 in 3 load steps; Newton
+    tolerance 1e-10 relative to ||R|| of the state lifted to the full load."""
+    from difffea_amd.problem import build_problem
+    from difffea_amd.newton_krylov import newton_krylov_solve, HipNewton
+    p = build_problem(dict(element_type="quad4", nx=100, ny=100, E=1.0, nu=0.3, load_mode="displacement",
+                           load_value=(0.1, None)))
+    th = torch.ones(p["conn"].shape[0], dtype=torch.float64, device="cuda")
+    probe = HipNewton(p, th)
+    uD = p["u_D"].cuda()
+    tol = 1e-10 * float(torch.norm(probe.sys.residual(probe.sys.lift(torch.zeros_like(uD), uD, 1.0), lf=1.0)))
+    newton_krylov_solve(p, th, 3, tol, verbose=False, precision=precision, cg_rtol=cg_rtol, cg_max_iter=cg_max_iter)
+    u, info = newton_krylov_solve(p, th, 3, tol, verbose=False, precision=precision, cg_rtol=cg_rtol,
+                                  cg_max_iter=cg_max_iter)
+    return u, dict(seconds=round(info["seconds"], 4), solves_per_s=round(1.0 / info["seconds"], 3),
+                   newton=info["total_newton"], cg=info["total_cg"], converged=info["converged"], tol_abs=tol)
+
+
+def solve_config3(precision):
+    """O-grid plate with a hole r=0.2, 64 x 256 Q9, E_e lognormal (mean 1, sigma 0.3, seed 2), nu=0.3,
+    left edge clamped, dead traction on the right edge (total 0.1) in 5 load steps; Newton tolerance
+    1e-10 relative to ||m f||."""
+    import math
+    from difffea_amd.problem import build_problem
+    from difffea_amd.newton_krylov import newton_krylov_solve
+    p = build_problem(dict(element_type="quad9", geometry="plate_hole_ogrid",
+                           geom_params=dict(width=1.0, radius=0.2, n_rad=64, n_circ=256), E=1.0, nu=0.3,
+                           load_mode="force", load_value=(0.1, 0.0)))
+    g = torch.Generator().manual_seed(2)
+    n_el = p["conn"].shape[0]
+    th = torch.exp(torch.randn(n_el, dtype=torch.float64, generator=g) * 0.3 - 0.045).cuda()
+    tol = 1e-10 * float(torch.norm(p["f"] * p["m"]))
+    u, info = newton_krylov_solve(p, th, 5, tol, verbose=False, precision=precision)
+    return u, dict(seconds=round(info["seconds"], 4), newton=info["total_newton"], cg=info["total_cg"],
+                   converged=info["converged"], tol_abs=tol)
+
+
+def sequential_fp64(n_requests, seed=5, side=(32, 512)):
+    """The plain FP64 GPU solver: the same requests solved one after another with newton_krylov_solve."""
+    import numpy as np
+    from difffea_amd.serve import make_stream, request_problem
+    from difffea_amd.newton_krylov import HipNewton
+    reqs = make_stream(n_requests, rate=1e9, seed=seed, side=side)
+    lat = []
+    t0 = time.perf_counter()
+    for r in reqs:
+        t1 = time.perf_counter()
+        p, th = request_problem(r)
+        solver = HipNewton(p, th.cuda())
+        if r.adjoint:
+            ref = float(torch.norm(p["f"] * p["m"]))
+        else:
+            uD = p["u_D"].cuda()
+            ref = float(torch.norm(solver.sys.residual(solver.sys.lift(torch.zeros_like(uD), uD, 1.0), lf=1.0)))
+        u, info = solver.solve(r.n_load_steps, 1e-8 * ref)
+        if r.adjoint:
+            from difffea_amd.adjoint import solve_adjoint, compute_implicit_vjp
+            lam, _ = solve_adjoint(solver, u, p["f"].cuda(), rtol=1e-8)
+            compute_implicit_vjp(solver, u, lam)
+        torch.cuda.synchronize()
+        lat.append(time.perf_counter() - t1)
+    wall = time.perf_counter() - t0
+    return dict(precision="fp64", mode="sequential newton_krylov_solve", n_requests=len(reqs), wall_s=wall,
+                solves_per_s=len(reqs) / wall, p50_latency_s=float(np.percentile(lat, 50)),
+                p99_latency_s=float(np.percentile(lat, 99)))
+
+
+def serve_stream(n_requests, precision, rate=None, seed=5, side=(32, 512)):
+    from difffea_amd.serve import Server, make_stream
+    reqs = make_stream(n_requests, rate=rate or 1e9, seed=seed, side=side)
+    srv = Server(precision, max_slots=64)
+    recs, summ = srv.run(reqs, realtime=rate is not None)
+    summ["n_newton"] = int(sum(r["total_newton"] for r in recs))
+    summ["n_cg"] = int(sum(r["total_cg"] for r in recs))
+    return summ
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -102,6 +179,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-solve", action="store_true", help="skip the complete-solve workloads")
+    ap.add_argument("--serve-requests", type=int, default=64,
+                    help="requests of the served stream (the full workload is 2000)")
+    ap.add_argument("--serve-side", type=int, nargs=2, default=(32, 256),
+                    help="side range of the served meshes (the full workload is 32 512)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (use --gpus 1)", file=sys.stderr)
@@ -118,6 +200,31 @@ def main():
     p9, u9, v9, th9 = problem("quad9", 200)
     q9 = time_apply(p9, u9, v9, th9, "fp64", args.steps, args.warmup)
     print(f"# Q9 200x200 fp64 A(v): {q9['ms'] * 1e3:.2f} us, {q9['gbs']:.0f} GB/s", file=sys.stderr, flush=True)
+
+    solves = None
+    if not args.no_solve:
+        solves = {}
+        u64, solves["c1_fp64_ew"] = solve_config1("fp64")
+        u32, solves["c1_mixed_ew"] = solve_config1("mixed")
+        solves["c1_mixed_vs_fp64_rel"] = float((u32 - u64).abs().max() / u64.abs().max())
+        _, solves["c1_fp64_cg1e-12"] = solve_config1("fp64", cg_rtol=1e-12, cg_max_iter=5000)
+        print(f"# config-1 solves: {solves}", file=sys.stderr, flush=True)
+        v64, solves["c3_fp64"] = solve_config3("fp64")
+        v32, solves["c3_mixed"] = solve_config3("mixed")
+        solves["c3_mixed_vs_fp64_rel"] = float((v32 - v64).abs().max() / v64.abs().max())
+        print(f"# config-3 solves: {solves['c3_fp64']} {solves['c3_mixed']}", file=sys.stderr, flush=True)
+        if args.serve_requests > 0:
+            sat64 = serve_stream(args.serve_requests, "fp64", side=tuple(args.serve_side))
+            print(f"# served fp64 (saturated): {sat64}", file=sys.stderr, flush=True)
+            rate = 0.8 * sat64["solves_per_s"]
+            seq = sequential_fp64(args.serve_requests, side=tuple(args.serve_side))
+            print(f"# plain FP64 GPU solver (sequential): {seq}", file=sys.stderr, flush=True)
+            srv = {"saturated_fp64": sat64, "plain_fp64_sequential": seq,
+                   "fp64": serve_stream(args.serve_requests, "fp64", rate, side=tuple(args.serve_side)),
+                   "mixed": serve_stream(args.serve_requests, "mixed", rate, side=tuple(args.serve_side)),
+                   "arrival_rate_per_s": rate}
+            print(f"# served: {srv}", file=sys.stderr, flush=True)
+            solves["served"] = srv
 
     cpu = None
     if not args.no_cpu:
@@ -140,6 +247,7 @@ def main():
         "q9_200x200_fp64": {"us_per_product": round(q9["ms"] * 1e3, 3), "gbs": round(q9["gbs"], 2),
                             "bytes_per_step": q9["bytes"]},
         "cpu_baseline": None if cpu is None else {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")},
+        "solves": solves,
     }
     print(json.dumps(out))
 

@@ -7,7 +7,7 @@
 //     vectors are summed into LDS node accumulators colour by colour (elements of one colour share no
 //     node), so the summation order is fixed: no atomics, bit-reproducible results.
 //   * Nodes used by one chunk only are written directly.  Nodes shared between chunks receive one partial
-//     per chunk; a second kernel sums the partials of each shared node in chunk order.
+//     per chunk, stored contiguously per node in chunk order; a second kernel sums them in that order.
 //   * Every launch processes a list of work items {solve, block}: a batch of independent solves of the
 //     same element type and precision goes through one launch; per-solve reductions depend only on the
 //     solve's own blocks, so a solve gives the same bits alone and inside a batch.
@@ -135,6 +135,20 @@ __device__ __forceinline__ constexpr int uidx(int m, int n) {
     return m <= n ? (m * 4 - (m * (m - 1)) / 2 + (n - m)) : (n * 4 - (n * (n - 1)) / 2 + (m - n));
 }
 
+// the scalar fields of a chunk record; the colour table is read through a pointer (a local copy of the
+// record indexed by a loop variable would be placed in scratch memory)
+struct ChunkView {
+    int e0, ne, k0, nk, nown, pbase, ncol;
+    const int32_t* col;
+};
+
+__device__ __forceinline__ ChunkView chunk_view(const dfea_chunk* c) {
+    ChunkView v;
+    v.e0 = c->e0; v.ne = c->ne; v.k0 = c->k0; v.nk = c->nk; v.nown = c->nown; v.pbase = c->pbase; v.ncol = c->ncol;
+    v.col = c->col;
+    return v;
+}
+
 struct Item { int s, b; };
 __device__ __forceinline__ Item load_item(const int* items) {
     const int b = xcd_remap(blockIdx.x, gridDim.x);
@@ -154,7 +168,7 @@ template <int NEN, int NQ, int BS>
 __global__ __launch_bounds__(BS) void k_geometry(const dfea_desc* __restrict__ tab, const int* __restrict__ items) {
     const Item it = load_item(items);
     const dfea_desc& D = tab[it.s];
-    const dfea_chunk ch = D.chunks[it.b];
+    const ChunkView ch = chunk_view(D.chunks + it.b);
     const int t = threadIdx.x;
     if (t >= ch.ne) return;
     const int e = ch.e0 + t;
@@ -206,7 +220,7 @@ struct ChunkLDS {
 };
 
 template <typename T, int NEN>
-__device__ __forceinline__ void colour_accumulate(const dfea_chunk& ch, int t, const int* l, const V2<T>* fe,
+__device__ __forceinline__ void colour_accumulate(const ChunkView& ch, int t, const int* l, const V2<T>* fe,
                                                   V2<T>* acc) {
     for (int c = 0; c < ch.ncol; ++c) {
         if (t >= ch.col[c] && t < ch.col[c + 1]) {
@@ -224,20 +238,47 @@ __device__ __forceinline__ void colour_accumulate(const dfea_chunk& ch, int t, c
 
 // ---- tangent product  y = m J (m v) + (1 - m) v  (`operators.py:385-412`) ----------------------------------
 // PCG: the direction update p = z + beta p happens on the gather, and p.Ap partials are produced.
+// The element data of the first PF Gauss points is loaded before the gather barrier, so the HBM stream
+// starts at once instead of after the gather's two dependent round trips.
 template <int NEN, int NQ, int BS, typename T, bool PCG>
 __global__ __launch_bounds__(BS) void k_apply(const dfea_desc* __restrict__ tab, const int* __restrict__ items,
                                              const T* __restrict__ vin, T* __restrict__ vout) {
+    constexpr int PF = NEN == 4 ? NQ : 2;
     __shared__ ChunkLDS<T> L;
     const Item it = load_item(items);
     const dfea_desc& D = tab[it.s];
     if (PCG && pcg_status(D) != DFEA_PCG_RUNNING) return;
-    const dfea_chunk ch = D.chunks[it.b];
+    const ChunkView ch = chunk_view(D.chunks + it.b);
     const int t = threadIdx.x;
     const T* v = PCG ? nullptr : (vin ? vin : static_cast<const T*>(D.vec[DFEA_P]));
     T* y = PCG ? static_cast<T*>(D.vec[DFEA_AP]) : (vout ? vout : static_cast<T*>(D.vec[DFEA_AP]));
     T* p = static_cast<T*>(D.vec[DFEA_P]);
     const T* z = static_cast<const T*>(D.vec[DFEA_Z]);
     const double beta = PCG ? D.sc[DFEA_SC_BETA] : 0.0;
+    const int ne = D.n_el;
+    const int e = ch.e0 + t;
+    const bool has_el = t < ch.ne;
+    const T* dN = std::is_same<T, double>::value ? (const T*)D.dN64 : (const T*)D.dN32;
+    const T* C = std::is_same<T, double>::value ? (const T*)D.C64 : (const T*)D.C32;
+
+    // 0. element data of the first PF Gauss points, local node indices
+    int l[NEN];
+    T d[NQ][NEN][2];
+    T c[NQ][10];
+    if (has_el) {
+#pragma unroll
+        for (int a = 0; a < NEN; ++a) l[a] = D.lelem[a * ne + e];
+#pragma unroll
+        for (int q = 0; q < PF; ++q) {
+#pragma unroll
+            for (int a = 0; a < NEN; ++a) {
+                d[q][a][0] = dN[(size_t)((q * NEN + a) * 2) * ne + e];
+                d[q][a][1] = dN[(size_t)((q * NEN + a) * 2 + 1) * ne + e];
+            }
+#pragma unroll
+            for (int k = 0; k < 10; ++k) c[q][k] = C[(size_t)(k * NQ + q) * ne + e];
+        }
+    }
 
     // 1. gather
     for (int k = t; k < ch.nk; k += BS) {
@@ -260,17 +301,11 @@ __global__ __launch_bounds__(BS) void k_apply(const dfea_desc* __restrict__ tab,
     __syncthreads();
 
     // 2. element
-    int l[NEN];
     V2<T> fe[NEN];
-    if (t < ch.ne) {
-        const int ne = D.n_el;
-        const int e = ch.e0 + t;
-        const T* dN = std::is_same<T, double>::value ? (const T*)D.dN64 : (const T*)D.dN32;
-        const T* C = std::is_same<T, double>::value ? (const T*)D.C64 : (const T*)D.C32;
+    if (has_el) {
         V2<T> ve[NEN];
 #pragma unroll
         for (int a = 0; a < NEN; ++a) {
-            l[a] = D.lelem[a * ne + e];
             ve[a] = L.v[l[a]];
             const unsigned char b = L.bits[l[a]];
             if (b & 1) ve[a].x = 0;
@@ -279,35 +314,35 @@ __global__ __launch_bounds__(BS) void k_apply(const dfea_desc* __restrict__ tab,
         }
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
-            T d[NEN][2];
+            if (q >= PF) {
 #pragma unroll
-            for (int a = 0; a < NEN; ++a) {
-                d[a][0] = dN[(size_t)((q * NEN + a) * 2) * ne + e];
-                d[a][1] = dN[(size_t)((q * NEN + a) * 2 + 1) * ne + e];
+                for (int a = 0; a < NEN; ++a) {
+                    d[q][a][0] = dN[(size_t)((q * NEN + a) * 2) * ne + e];
+                    d[q][a][1] = dN[(size_t)((q * NEN + a) * 2 + 1) * ne + e];
+                }
+#pragma unroll
+                for (int k = 0; k < 10; ++k) c[q][k] = C[(size_t)(k * NQ + q) * ne + e];
             }
-            T c[10];
-#pragma unroll
-            for (int k = 0; k < 10; ++k) c[k] = C[(size_t)(k * NQ + q) * ne + e];
             T g[4] = {0, 0, 0, 0};    // g[2i+j] = sum_a ve[a][i] dN[a][j]
 #pragma unroll
             for (int a = 0; a < NEN; ++a) {
-                g[0] += ve[a].x * d[a][0];
-                g[1] += ve[a].x * d[a][1];
-                g[2] += ve[a].y * d[a][0];
-                g[3] += ve[a].y * d[a][1];
+                g[0] += ve[a].x * d[q][a][0];
+                g[1] += ve[a].x * d[q][a][1];
+                g[2] += ve[a].y * d[q][a][0];
+                g[3] += ve[a].y * d[q][a][1];
             }
-            T s[4];
+            T sq[4];
 #pragma unroll
             for (int m = 0; m < 4; ++m) {
                 T acc = 0;
 #pragma unroll
-                for (int n = 0; n < 4; ++n) acc += c[uidx(m, n)] * g[n];
-                s[m] = acc;
+                for (int n = 0; n < 4; ++n) acc += c[q][uidx(m, n)] * g[n];
+                sq[m] = acc;
             }
 #pragma unroll
             for (int a = 0; a < NEN; ++a) {
-                fe[a].x += s[0] * d[a][0] + s[1] * d[a][1];
-                fe[a].y += s[2] * d[a][0] + s[3] * d[a][1];
+                fe[a].x += sq[0] * d[q][a][0] + sq[1] * d[q][a][1];
+                fe[a].y += sq[2] * d[q][a][0] + sq[3] * d[q][a][1];
             }
         }
     }
@@ -328,7 +363,7 @@ __global__ __launch_bounds__(BS) void k_apply(const dfea_desc* __restrict__ tab,
             reinterpret_cast<V2<T>*>(y)[n] = out;
             if (PCG) dot += (double)raw.x * (double)out.x + (double)raw.y * (double)out.y;
         } else {
-            part[ch.pbase + (k - ch.nown)] = acc;
+            part[D.pslot[ch.pbase + (k - ch.nown)]] = acc;
         }
     }
     if (PCG) {
@@ -353,7 +388,7 @@ __global__ __launch_bounds__(FBS) void k_apply_fin(const dfea_desc* __restrict__
         const V2<T>* part = reinterpret_cast<const V2<T>*>(D.part);
         V2<T> acc{0, 0};
         for (int j = D.sh_off[i]; j < D.sh_off[i + 1]; ++j) {
-            const V2<T> pj = part[D.sh_slot[j]];
+            const V2<T> pj = part[j];
             acc.x += pj.x;
             acc.y += pj.y;
         }
@@ -391,7 +426,7 @@ __global__ __launch_bounds__(BS) void k_residual(const dfea_desc* __restrict__ t
     __shared__ ChunkLDS<double> L;
     const Item it = load_item(items);
     const dfea_desc& D = tab[it.s];
-    const dfea_chunk ch = D.chunks[it.b];
+    const ChunkView ch = chunk_view(D.chunks + it.b);
     const int t = threadIdx.x;
     const double* u = uin ? uin : static_cast<const double*>(D.vec[DFEA_U]);
     double* R = rout ? rout : static_cast<double*>(D.vec[DFEA_RES]);
@@ -453,7 +488,7 @@ __global__ __launch_bounds__(BS) void k_residual(const dfea_desc* __restrict__ t
             out.y = (b & 2) ? 0.0 : acc.y - lf * D.f[2 * n + 1];
             reinterpret_cast<V2<double>*>(R)[n] = out;
         } else {
-            part[ch.pbase + (k - ch.nown)] = acc;
+            part[D.pslot[ch.pbase + (k - ch.nown)]] = acc;
         }
     }
 }
@@ -469,7 +504,7 @@ __global__ __launch_bounds__(FBS) void k_residual_fin(const dfea_desc* __restric
     const V2<double>* part = reinterpret_cast<const V2<double>*>(D.part);
     V2<double> acc{0, 0};
     for (int j = D.sh_off[i]; j < D.sh_off[i + 1]; ++j) {
-        const V2<double> pj = part[D.sh_slot[j]];
+        const V2<double> pj = part[j];
         acc.x += pj.x;
         acc.y += pj.y;
     }
@@ -488,7 +523,7 @@ __global__ __launch_bounds__(BS) void k_moduli(const dfea_desc* __restrict__ tab
     __shared__ ChunkLDS<double> L;
     const Item it = load_item(items);
     const dfea_desc& D = tab[it.s];
-    const dfea_chunk ch = D.chunks[it.b];
+    const ChunkView ch = chunk_view(D.chunks + it.b);
     const int t = threadIdx.x;
     const double* u = uin ? uin : static_cast<const double*>(D.vec[DFEA_U]);
     for (int k = t; k < ch.nk; k += BS) {
@@ -555,7 +590,7 @@ __global__ __launch_bounds__(BS) void k_moduli(const dfea_desc* __restrict__ tab
                 D.diag32[2 * n + 1] = (float)dy;
             }
         } else {
-            part[ch.pbase + (k - ch.nown)] = acc;
+            part[D.pslot[ch.pbase + (k - ch.nown)]] = acc;
         }
     }
 }
@@ -569,7 +604,7 @@ __global__ __launch_bounds__(FBS) void k_moduli_fin(const dfea_desc* __restrict_
     const V2<double>* part = reinterpret_cast<const V2<double>*>(D.part);
     V2<double> acc{0, 0};
     for (int j = D.sh_off[i]; j < D.sh_off[i + 1]; ++j) {
-        const V2<double> pj = part[D.sh_slot[j]];
+        const V2<double> pj = part[j];
         acc.x += pj.x;
         acc.y += pj.y;
     }
@@ -594,7 +629,7 @@ __global__ __launch_bounds__(BS) void k_adjoint(const dfea_desc* __restrict__ ta
     __shared__ ChunkLDS<double> L;    // v: u, acc: masked lambda
     const Item it = load_item(items);
     const dfea_desc& D = tab[it.s];
-    const dfea_chunk ch = D.chunks[it.b];
+    const ChunkView ch = chunk_view(D.chunks + it.b);
     const int t = threadIdx.x;
     for (int k = t; k < ch.nk; k += BS) {
         const uint32_t g = D.lnode[ch.k0 + k];
@@ -915,7 +950,7 @@ int dfea_desc_size(void) { return (int)sizeof(dfea_desc); }
 int dfea_desc_offset(int field) {
     static const int off[] = {
         (int)offsetof(dfea_desc, chunks), (int)offsetof(dfea_desc, lnode), (int)offsetof(dfea_desc, lelem),
-        (int)offsetof(dfea_desc, sh_node), (int)offsetof(dfea_desc, sh_off), (int)offsetof(dfea_desc, sh_slot),
+        (int)offsetof(dfea_desc, sh_node), (int)offsetof(dfea_desc, sh_off), (int)offsetof(dfea_desc, pslot),
         (int)offsetof(dfea_desc, conn), (int)offsetof(dfea_desc, perm), (int)offsetof(dfea_desc, nodes),
         (int)offsetof(dfea_desc, dNdxi), (int)offsetof(dfea_desc, gw), (int)offsetof(dfea_desc, nfix),
         (int)offsetof(dfea_desc, dN64), (int)offsetof(dfea_desc, dN32), (int)offsetof(dfea_desc, w),

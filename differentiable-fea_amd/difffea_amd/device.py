@@ -73,7 +73,7 @@ class HipMesh:
         self.t_lelem = torch.from_numpy(p.lelem.view(np.int16)).to(dv)
         self.t_sh_node = _dev_int32(p.sh_node, dv)
         self.t_sh_off = _dev_int32(p.sh_off, dv)
-        self.t_sh_slot = _dev_int32(p.sh_slot if len(p.sh_slot) else np.zeros(1, np.int32), dv)
+        self.t_pslot = _dev_int32(p.pslot if len(p.pslot) else np.zeros(1, np.int32), dv)
         self.t_conn = _dev_int32(p.conn_int, dv)
         self.t_perm = _dev_int32(p.perm, dv)
         self.perm = torch.from_numpy(p.perm.astype(np.int64)).to(dv)
@@ -102,7 +102,7 @@ class HipMesh:
     def fill_desc(self, d):
         p = self.plan
         d.chunks, d.lnode, d.lelem = self.t_chunks.data_ptr(), self.t_lnode.data_ptr(), self.t_lelem.data_ptr()
-        d.sh_node, d.sh_off, d.sh_slot = self.t_sh_node.data_ptr(), self.t_sh_off.data_ptr(), self.t_sh_slot.data_ptr()
+        d.sh_node, d.sh_off, d.pslot = self.t_sh_node.data_ptr(), self.t_sh_off.data_ptr(), self.t_pslot.data_ptr()
         d.conn, d.perm, d.nodes = self.t_conn.data_ptr(), self.t_perm.data_ptr(), self.t_nodes.data_ptr()
         d.dNdxi, d.gw, d.nfix = self.t_dNdxi.data_ptr(), self.t_gw.data_ptr(), self.t_nfix.data_ptr()
         d.dN64, d.dN32, d.w = self.dN64.data_ptr(), self.dN32.data_ptr(), self.w.data_ptr()

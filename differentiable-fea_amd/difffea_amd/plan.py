@@ -12,8 +12,9 @@ Built once per (mesh, Dirichlet mask) on the host with vectorised numpy; uploade
 3. Local nodes.  Each chunk lists its distinct nodes, owned nodes first (used by no other chunk), then
    shared nodes.  ``lelem[a][e]`` is the local index of element node a.  Fixed dofs are folded into the
    node ids as bits 30/31.
-4. Shared nodes.  Every (chunk, shared node) pair gets a partial slot; ``sh_off``/``sh_slot`` list the slots
-   of each shared node in chunk order, which fixes the order of the final summation.
+4. Shared nodes.  Every (chunk, shared node) pair gets a partial slot.  Slots are ordered by shared node,
+   then chunk: the partials of shared node i are slots ``sh_off[i]..sh_off[i+1]-1``, summed in that order.
+   ``pslot`` maps a chunk's shared entries (chunk order) to their slots.
 
 Nothing here depends on which other solves run in the same batch, so results are batch-independent.
 """
@@ -223,7 +224,9 @@ class MeshPlan:
         chunks[:, 5] = pbase
         so = np.lexsort((uc[sh_pos], un[sh_pos]))                  # by node, then chunk
         sh_nodes_all = un[sh_pos][so]
-        self.sh_slot = slot_of[sh_pos][so].astype(np.int32)
+        chunk_major = slot_of[sh_pos][so]                            # node-major position -> chunk-major entry
+        self.pslot = np.empty(len(chunk_major), np.int32)
+        self.pslot[chunk_major] = np.arange(len(chunk_major), dtype=np.int32)
         snodes, sstart = np.unique(sh_nodes_all, return_index=True)
         self.sh_off = np.r_[sstart, len(sh_nodes_all)].astype(np.int32)
         self.sh_node = (snodes.astype(np.uint64) | bits[snodes]).astype(np.uint32)
@@ -252,8 +255,8 @@ class MeshPlan:
             assert 1 <= nc <= _lib.MAXCOL and ch[c, 8] == 0 and ch[c, 8 + nc] == ch[c, 1]
         assert (self.lnode & _lib.NODE_MASK).max() < self.n_nodes
         if self.n_shared:
-            assert self.sh_off[0] == 0 and self.sh_off[-1] == len(self.sh_slot) == self.n_part
-            assert self.sh_slot.min() >= 0 and self.sh_slot.max() < self.n_part
+            assert self.sh_off[0] == 0 and self.sh_off[-1] == len(self.pslot) == self.n_part
+            assert np.array_equal(np.sort(self.pslot), np.arange(self.n_part))
             assert (self.sh_node & _lib.NODE_MASK).max() < self.n_nodes
         assert self.conn_int.min() >= 0 and self.conn_int.max() < self.n_nodes
         return E

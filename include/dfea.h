@@ -68,7 +68,8 @@ typedef struct dfea_desc {
     const uint16_t* lelem;      /* [nen][n_el] local index (within the chunk) of element node a */
     const uint32_t* sh_node;    /* [n_shared] node id | fixed bits */
     const int32_t* sh_off;      /* [n_shared+1] */
-    const int32_t* sh_slot;     /* partial slots of each shared node, in chunk order */
+    const int32_t* pslot;       /* [n_part] partial slot of each (chunk, shared node) entry, chunk order;
+                                   the partials of shared node i are slots sh_off[i]..sh_off[i+1]-1 */
     const int32_t* conn;        /* [nen][n_el] node ids, internal element order */
     const int32_t* perm;        /* [n_el] original element id of internal element e */
     const double* nodes;        /* [N][2] reference coordinates */

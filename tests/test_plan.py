@@ -62,11 +62,13 @@ def _emulate_apply(plan, C, dN, v):
             acc[idx] += fe[col[k]:col[k + 1]].reshape(-1, 2)
         out = np.where(np.stack([bits & 1, bits & 2], 1) > 0, raw, acc)
         y[nodes[:nown]] = out[:nown]
-        part[pbase:pbase + nk - nown] = acc[nown:]
+        part[plan.pslot[pbase:pbase + nk - nown]] = acc[nown:]
     for i in range(plan.n_shared):
         g_ = int(plan.sh_node[i])
         n = g_ & _lib.NODE_MASK
-        acc = part[plan.sh_slot[plan.sh_off[i]:plan.sh_off[i + 1]]].sum(0)
+        acc = np.zeros(2)
+        for j in range(plan.sh_off[i], plan.sh_off[i + 1]):   # slot order = chunk order
+            acc += part[j]
         y[n, 0] = vv[n, 0] if g_ & _lib.FIXED_X else acc[0]
         y[n, 1] = vv[n, 1] if g_ & _lib.FIXED_Y else acc[1]
     return y.reshape(-1)
