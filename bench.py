@@ -127,7 +127,7 @@ def solve_config3(precision):
     n_el = p["conn"].shape[0]
     th = torch.exp(torch.randn(n_el, dtype=torch.float64, generator=g) * 0.3 - 0.045).cuda()
     tol = 1e-10 * float(torch.norm(p["f"] * p["m"]))
-    u, info = newton_krylov_solve(p, th, 5, tol, verbose=False, precision=precision)
+    u, info = newton_krylov_solve(p, th, 5, tol, verbose=False, precision=precision, cg_max_iter=20000)
     return u, dict(seconds=round(info["seconds"], 4), newton=info["total_newton"], cg=info["total_cg"],
                    converged=info["converged"], tol_abs=tol)
 
@@ -149,10 +149,10 @@ def sequential_fp64(n_requests, seed=5, side=(32, 512)):
         else:
             uD = p["u_D"].cuda()
             ref = float(torch.norm(solver.sys.residual(solver.sys.lift(torch.zeros_like(uD), uD, 1.0), lf=1.0)))
-        u, info = solver.solve(r.n_load_steps, 1e-8 * ref)
+        u, info = solver.solve(r.n_load_steps, 1e-8 * ref, cg_max_iter=20000)
         if r.adjoint:
             from difffea_amd.adjoint import solve_adjoint, compute_implicit_vjp
-            lam, _ = solve_adjoint(solver, u, p["f"].cuda(), rtol=1e-8)
+            lam, _ = solve_adjoint(solver, u, p["f"].cuda(), rtol=1e-8, max_iter=20000)
             compute_implicit_vjp(solver, u, lam)
         torch.cuda.synchronize()
         lat.append(time.perf_counter() - t1)

@@ -188,7 +188,7 @@ class Server:
     """Continuous-batching solver for a stream of requests on one GPU (see module docstring)."""
 
     def __init__(self, precision="fp64", max_slots=64, hbm_budget=64 << 30, pcg_chunk=8, rel_tol=1e-8,
-                 device=None, adjoint_rtol=1e-8):
+                 device=None, adjoint_rtol=1e-8, cg_max_iter=20000):
         self.lib = _lib.load()
         self.device = torch.device(device or "cuda")
         self.precision = precision
@@ -198,6 +198,7 @@ class Server:
         self.pcg_chunk = pcg_chunk
         self.rel_tol = rel_tol
         self.adjoint_rtol = adjoint_rtol
+        self.cg_max_iter = cg_max_iter
         dsz = self.lib.dfea_desc_size()
         self.dsz = dsz
         self.table = torch.zeros(max_slots * dsz, dtype=torch.uint8, device=self.device)
@@ -232,7 +233,7 @@ class Server:
             ref = float(torch.norm(R))
         job.tol = self.rel_tol * ref
         job.saved = torch.zeros_like(sys_.vec["uacc"])
-        job.gen = solve_ops(job, job.tol)
+        job.gen = solve_ops(job, job.tol, cg_max_iter=self.cg_max_iter)
         job.op = next(job.gen)
         job.t_admit = time.perf_counter()
         self.jobs[slot] = job
@@ -317,7 +318,8 @@ class Server:
             idx = torch.tensor([j.slot for j in new])
             adj_rtol = max(self.adjoint_rtol, MIXED_RTOL_FLOOR) if self.precision == "mixed" else self.adjoint_rtol
             vals = torch.tensor([[float(j.op[1]) if j.op[0] == "pcg" else adj_rtol,
-                                  float(j.op[2]) if j.op[0] == "pcg" else 5000.0] for j in new], dtype=torch.float64)
+                                  float(j.op[2]) if j.op[0] == "pcg" else float(self.cg_max_iter)] for j in new],
+                                dtype=torch.float64)
             self.sc[idx, _lib.SC["rtol"]] = vals[:, 0].to(self.device)
             self.sc[idx, _lib.SC["maxit"]] = vals[:, 1].to(self.device)
             for neg in (True, False):
