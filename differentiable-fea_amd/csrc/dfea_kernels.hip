@@ -6,8 +6,10 @@
 //     element from structure-of-arrays moduli / geometry planes (coalesced HBM reads), and the element
 //     vectors are summed into LDS node accumulators colour by colour (elements of one colour share no
 //     node), so the summation order is fixed: no atomics, bit-reproducible results.
-//   * Nodes used by one chunk only are written directly.  Nodes shared between chunks receive one partial
-//     per chunk, stored contiguously per node in chunk order; a second kernel sums them in that order.
+//   * Halo plans (quad4): every node is owned by one chunk, and a chunk also evaluates the neighbouring
+//     ("halo") elements that touch its owned nodes, so every owned node is completed inside its workgroup.
+//     Partials plans (quad9): nodes used by several chunks receive one partial per chunk, stored
+//     contiguously per node in chunk order, and a second kernel sums them in that order.
 //   * Every launch processes a list of work items {solve, block}: a batch of independent solves of the
 //     same element type and precision goes through one launch; per-solve reductions depend only on the
 //     solve's own blocks, so a solve gives the same bits alone and inside a batch.
@@ -25,8 +27,10 @@
 
 namespace {
 
-constexpr int CHUNK_Q4 = 256;     // elements per chunk (= workgroup size) for quad4
-constexpr int CHUNK_Q9 = 64;      // elements per chunk (= workgroup size) for quad9
+constexpr int CHUNK_Q4 = 256;     // core elements per chunk for quad4
+constexpr int CHUNK_Q9 = 64;      // core elements per chunk for quad9
+constexpr int BS_Q4 = 320;        // workgroup size for quad4: core + halo elements (16x16 + 33 on a grid)
+constexpr int BS_Q9 = 64;         // workgroup size for quad9
 constexpr int MAXLN = 640;        // local nodes per chunk (LDS capacity)
 constexpr int VBS = 256;          // workgroup size of the vector kernels
 constexpr int VPT = 4;            // dofs per thread in the vector kernels
@@ -138,15 +142,46 @@ __device__ __forceinline__ constexpr int uidx(int m, int n) {
 // the scalar fields of a chunk record; the colour table is read through a pointer (a local copy of the
 // record indexed by a loop variable would be placed in scratch memory)
 struct ChunkView {
-    int e0, ne, k0, nk, nown, pbase, ncol;
+    int e0, ne, k0, nk, nown, pbase, ncol, h0, nh;
     const int32_t* col;
+    const int32_t* hcol;
 };
 
 __device__ __forceinline__ ChunkView chunk_view(const dfea_chunk* c) {
     ChunkView v;
     v.e0 = c->e0; v.ne = c->ne; v.k0 = c->k0; v.nk = c->nk; v.nown = c->nown; v.pbase = c->pbase; v.ncol = c->ncol;
+    v.h0 = c->h0; v.nh = c->nh;
     v.col = c->col;
+    v.hcol = c->hcol;
     return v;
+}
+
+// element of thread t: core threads 0..ne-1 take the chunk's contiguous elements, halo threads ne..ne+nh-1
+// the halo list; returns false for idle threads.  l[] receives the element's local node indices.
+template <int NEN>
+__device__ __forceinline__ bool thread_element(const dfea_desc& D, const ChunkView& ch, int t, int& e, int* l) {
+    if (t < ch.ne) {
+        e = ch.e0 + t;
+#pragma unroll
+        for (int a = 0; a < NEN; ++a) l[a] = D.lelem[a * D.n_el + e];
+        return true;
+    }
+    const int h = t - ch.ne;
+    if (h < ch.nh) {
+        const int hh = ch.h0 + h;
+        e = D.hel[hh];
+#pragma unroll
+        for (int a = 0; a < NEN; ++a) l[a] = D.hlelem[a * D.n_halo + hh];
+        return true;
+    }
+    e = 0;
+    return false;
+}
+
+__device__ __forceinline__ bool in_colour(const ChunkView& ch, int t, int c) {
+    if (t < ch.ne) return t >= ch.col[c] && t < ch.col[c + 1];
+    const int h = t - ch.ne;
+    return h >= ch.hcol[c] && h < ch.hcol[c + 1];
 }
 
 struct Item { int s, b; };
@@ -223,7 +258,7 @@ template <typename T, int NEN>
 __device__ __forceinline__ void colour_accumulate(const ChunkView& ch, int t, const int* l, const V2<T>* fe,
                                                   V2<T>* acc) {
     for (int c = 0; c < ch.ncol; ++c) {
-        if (t >= ch.col[c] && t < ch.col[c + 1]) {
+        if (in_colour(ch, t, c)) {
 #pragma unroll
             for (int a = 0; a < NEN; ++a) {
                 V2<T> s = acc[l[a]];
@@ -237,9 +272,17 @@ __device__ __forceinline__ void colour_accumulate(const ChunkView& ch, int t, co
 }
 
 // ---- tangent product  y = m J (m v) + (1 - m) v  (`operators.py:385-412`) ----------------------------------
-// PCG: the direction update p = z + beta p happens on the gather, and p.Ap partials are produced.
+// PCG: the direction update p' = z + beta p happens on the gather (p read from the buffer of this iteration's
+// parity, p' written to the other one), and partials of p'.Ap' are produced.
 // The element data of the first PF Gauss points is loaded before the gather barrier, so the HBM stream
 // starts at once instead of after the gather's two dependent round trips.
+template <typename T>
+__device__ __forceinline__ void pcg_dir_buffers(const dfea_desc& D, const T*& p_rd, T*& p_wr) {
+    const bool odd = ((int)D.sc[DFEA_SC_K]) & 1;
+    p_rd = static_cast<const T*>(D.vec[odd ? DFEA_P2 : DFEA_P]);
+    p_wr = static_cast<T*>(D.vec[odd ? DFEA_P : DFEA_P2]);
+}
+
 template <int NEN, int NQ, int BS, typename T, bool PCG>
 __global__ __launch_bounds__(BS) void k_apply(const dfea_desc* __restrict__ tab, const int* __restrict__ items,
                                              const T* __restrict__ vin, T* __restrict__ vout) {
@@ -252,22 +295,22 @@ __global__ __launch_bounds__(BS) void k_apply(const dfea_desc* __restrict__ tab,
     const int t = threadIdx.x;
     const T* v = PCG ? nullptr : (vin ? vin : static_cast<const T*>(D.vec[DFEA_P]));
     T* y = PCG ? static_cast<T*>(D.vec[DFEA_AP]) : (vout ? vout : static_cast<T*>(D.vec[DFEA_AP]));
-    T* p = static_cast<T*>(D.vec[DFEA_P]);
+    const T* p_rd = nullptr;
+    T* p_wr = nullptr;
+    if (PCG) pcg_dir_buffers<T>(D, p_rd, p_wr);
     const T* z = static_cast<const T*>(D.vec[DFEA_Z]);
     const double beta = PCG ? D.sc[DFEA_SC_BETA] : 0.0;
     const int ne = D.n_el;
-    const int e = ch.e0 + t;
-    const bool has_el = t < ch.ne;
     const T* dN = std::is_same<T, double>::value ? (const T*)D.dN64 : (const T*)D.dN32;
     const T* C = std::is_same<T, double>::value ? (const T*)D.C64 : (const T*)D.C32;
 
     // 0. element data of the first PF Gauss points, local node indices
     int l[NEN];
+    int e;
+    const bool has_el = thread_element<NEN>(D, ch, t, e, l);
     T d[NQ][NEN][2];
     T c[NQ][10];
     if (has_el) {
-#pragma unroll
-        for (int a = 0; a < NEN; ++a) l[a] = D.lelem[a * ne + e];
 #pragma unroll
         for (int q = 0; q < PF; ++q) {
 #pragma unroll
@@ -286,11 +329,11 @@ __global__ __launch_bounds__(BS) void k_apply(const dfea_desc* __restrict__ tab,
         const int n = (int)(g & DFEA_NODE_MASK);
         V2<T> val;
         if (PCG) {
-            V2<T> zz = reinterpret_cast<const V2<T>*>(z)[n];
-            V2<T> pp = reinterpret_cast<const V2<T>*>(p)[n];
+            const V2<T> zz = reinterpret_cast<const V2<T>*>(z)[n];
+            const V2<T> pp = reinterpret_cast<const V2<T>*>(p_rd)[n];
             val.x = (T)((double)zz.x + beta * (double)pp.x);
             val.y = (T)((double)zz.y + beta * (double)pp.y);
-            if (k < ch.nown) reinterpret_cast<V2<T>*>(p)[n] = val;    // owned: only this chunk touches n
+            if (k < ch.nown) reinterpret_cast<V2<T>*>(p_wr)[n] = val;   // owned: written by this chunk only
         } else {
             val = reinterpret_cast<const V2<T>*>(v)[n];
         }
@@ -348,12 +391,12 @@ __global__ __launch_bounds__(BS) void k_apply(const dfea_desc* __restrict__ tab,
     }
     colour_accumulate<T, NEN>(ch, t, l, fe, L.acc);
 
-    // 3. owned nodes: final value; shared nodes: partial
+    // 3. owned nodes: final value; shared nodes (partials plans): partial
     double dot = 0.0;
     V2<T>* part = reinterpret_cast<V2<T>*>(D.part);
     for (int k = t; k < ch.nk; k += BS) {
-        const V2<T> acc = L.acc[k];
         if (k < ch.nown) {
+            const V2<T> acc = L.acc[k];
             const V2<T> raw = L.v[k];
             const unsigned char b = L.bits[k];
             V2<T> out;
@@ -362,8 +405,8 @@ __global__ __launch_bounds__(BS) void k_apply(const dfea_desc* __restrict__ tab,
             const int n = (int)(D.lnode[ch.k0 + k] & DFEA_NODE_MASK);
             reinterpret_cast<V2<T>*>(y)[n] = out;
             if (PCG) dot += (double)raw.x * (double)out.x + (double)raw.y * (double)out.y;
-        } else {
-            part[D.pslot[ch.pbase + (k - ch.nown)]] = acc;
+        } else if (D.n_part) {
+            part[D.pslot[ch.pbase + (k - ch.nown)]] = L.acc[k];
         }
     }
     if (PCG) {
@@ -372,7 +415,7 @@ __global__ __launch_bounds__(BS) void k_apply(const dfea_desc* __restrict__ tab,
     }
 }
 
-// shared nodes of the tangent product
+// shared nodes of the tangent product (partials plans)
 template <typename T, bool PCG>
 __global__ __launch_bounds__(FBS) void k_apply_fin(const dfea_desc* __restrict__ tab, const int* __restrict__ items,
                                                   const T* __restrict__ vin, T* __restrict__ vout) {
@@ -396,13 +439,15 @@ __global__ __launch_bounds__(FBS) void k_apply_fin(const dfea_desc* __restrict__
         const int n = (int)(g & DFEA_NODE_MASK);
         V2<T> raw;
         if (PCG) {
-            V2<T>* p = static_cast<V2<T>*>(D.vec[DFEA_P]);
+            const T* p_rd;
+            T* p_wr;
+            pcg_dir_buffers<T>(D, p_rd, p_wr);
             const V2<T> zz = static_cast<const V2<T>*>(D.vec[DFEA_Z])[n];
-            const V2<T> pp = p[n];
+            const V2<T> pp = reinterpret_cast<const V2<T>*>(p_rd)[n];
             const double beta = D.sc[DFEA_SC_BETA];
             raw.x = (T)((double)zz.x + beta * (double)pp.x);
             raw.y = (T)((double)zz.y + beta * (double)pp.y);
-            p[n] = raw;
+            reinterpret_cast<V2<T>*>(p_wr)[n] = raw;
         } else {
             raw = reinterpret_cast<const V2<T>*>(v)[n];
         }
@@ -440,16 +485,16 @@ __global__ __launch_bounds__(BS) void k_residual(const dfea_desc* __restrict__ t
     }
     __syncthreads();
     int l[NEN];
+    int e;
+    const bool has_el = thread_element<NEN>(D, ch, t, e, l);
     V2<double> fe[NEN];
-    if (t < ch.ne) {
+    if (has_el) {
         const int ne = D.n_el;
-        const int e = ch.e0 + t;
         const double th = D.theta[e];
         const double mu = D.mu0 * th, lam = D.lam0 * th;
         V2<double> ue[NEN];
 #pragma unroll
         for (int a = 0; a < NEN; ++a) {
-            l[a] = D.lelem[a * ne + e];
             ue[a] = L.v[l[a]];
             fe[a] = V2<double>{0, 0};
         }
@@ -487,7 +532,7 @@ __global__ __launch_bounds__(BS) void k_residual(const dfea_desc* __restrict__ t
             out.x = (b & 1) ? 0.0 : acc.x - lf * D.f[2 * n];
             out.y = (b & 2) ? 0.0 : acc.y - lf * D.f[2 * n + 1];
             reinterpret_cast<V2<double>*>(R)[n] = out;
-        } else {
+        } else if (D.n_part) {
             part[D.pslot[ch.pbase + (k - ch.nown)]] = acc;
         }
     }
@@ -535,17 +580,18 @@ __global__ __launch_bounds__(BS) void k_moduli(const dfea_desc* __restrict__ tab
     }
     __syncthreads();
     int l[NEN];
+    int e;
+    const bool has_el = thread_element<NEN>(D, ch, t, e, l);
+    const bool core = t < ch.ne;                  // halo threads add to the diagonal but store no moduli
     V2<double> fe[NEN];
-    if (t < ch.ne) {
+    if (has_el) {
         const int ne = D.n_el;
-        const int e = ch.e0 + t;
         const double th = D.theta[e];
         const double mu = D.mu0 * th, lam = D.lam0 * th;
         CT* Cout = std::is_same<CT, double>::value ? (CT*)D.C64 : (CT*)D.C32;
         V2<double> ue[NEN];
 #pragma unroll
         for (int a = 0; a < NEN; ++a) {
-            l[a] = D.lelem[a * ne + e];
             ue[a] = L.v[l[a]];
             fe[a] = V2<double>{0, 0};
         }
@@ -562,8 +608,10 @@ __global__ __launch_bounds__(BS) void k_moduli(const dfea_desc* __restrict__ tab
             const Kin k = kinematics(F00, F01, F10, F11);
             double c[10];
             moduli10(k, mu, lam, D.w[(size_t)q * ne + e], c);
+            if (core) {
 #pragma unroll
-            for (int kk = 0; kk < 10; ++kk) Cout[(size_t)(kk * NQ + q) * ne + e] = (CT)c[kk];
+                for (int kk = 0; kk < 10; ++kk) Cout[(size_t)(kk * NQ + q) * ne + e] = (CT)c[kk];
+            }
             // K^e_{(a,i),(a,i)} += sum_{j,l} C[2i+j][2i+l] dN[a][j] dN[a][l]
 #pragma unroll
             for (int a = 0; a < NEN; ++a) {
@@ -589,7 +637,7 @@ __global__ __launch_bounds__(BS) void k_moduli(const dfea_desc* __restrict__ tab
                 D.diag32[2 * n] = (float)dx;
                 D.diag32[2 * n + 1] = (float)dy;
             }
-        } else {
+        } else if (D.n_part) {
             part[D.pslot[ch.pbase + (k - ch.nown)]] = acc;
         }
     }
@@ -641,7 +689,7 @@ __global__ __launch_bounds__(BS) void k_adjoint(const dfea_desc* __restrict__ ta
         L.acc[k] = lm;
     }
     __syncthreads();
-    if (t >= ch.ne) return;
+    if (t >= ch.ne) return;                       // per-element output: core elements only
     const int ne = D.n_el;
     const int e = ch.e0 + t;
     V2<double> ue[NEN], le[NEN];
@@ -821,7 +869,10 @@ __global__ __launch_bounds__(VBS) void k_pcg_update(const dfea_desc* __restrict_
     T* x = static_cast<T*>(D.vec[DFEA_X]);
     T* r = static_cast<T*>(D.vec[DFEA_R]);
     T* z = static_cast<T*>(D.vec[DFEA_Z]);
-    const T* p = static_cast<const T*>(D.vec[DFEA_P]);
+    const T* p_rd;
+    T* p_wr;
+    pcg_dir_buffers<T>(D, p_rd, p_wr);
+    const T* p = p_wr;                            // this iteration's direction
     const T* Ap = static_cast<const T*>(D.vec[DFEA_AP]);
     const T* dg = std::is_same<T, double>::value ? (const T*)D.diag64 : (const T*)D.diag32;
     const int base = it.b * VBLK;
@@ -950,7 +1001,7 @@ int dfea_desc_size(void) { return (int)sizeof(dfea_desc); }
 int dfea_desc_offset(int field) {
     static const int off[] = {
         (int)offsetof(dfea_desc, chunks), (int)offsetof(dfea_desc, lnode), (int)offsetof(dfea_desc, lelem),
-        (int)offsetof(dfea_desc, sh_node), (int)offsetof(dfea_desc, sh_off), (int)offsetof(dfea_desc, pslot),
+        (int)offsetof(dfea_desc, sh_node), (int)offsetof(dfea_desc, sh_off), (int)offsetof(dfea_desc, pslot), (int)offsetof(dfea_desc, hel), (int)offsetof(dfea_desc, hlelem),
         (int)offsetof(dfea_desc, conn), (int)offsetof(dfea_desc, perm), (int)offsetof(dfea_desc, nodes),
         (int)offsetof(dfea_desc, dNdxi), (int)offsetof(dfea_desc, gw), (int)offsetof(dfea_desc, nfix),
         (int)offsetof(dfea_desc, dN64), (int)offsetof(dfea_desc, dN32), (int)offsetof(dfea_desc, w),
@@ -961,7 +1012,7 @@ int dfea_desc_offset(int field) {
         (int)offsetof(dfea_desc, n_chunks), (int)offsetof(dfea_desc, n_lnodes), (int)offsetof(dfea_desc, n_shared),
         (int)offsetof(dfea_desc, n_part), (int)offsetof(dfea_desc, n_el), (int)offsetof(dfea_desc, n_nodes),
         (int)offsetof(dfea_desc, nen), (int)offsetof(dfea_desc, nq), (int)offsetof(dfea_desc, precision),
-        (int)offsetof(dfea_desc, n_finblk), (int)offsetof(dfea_desc, n_vblk), (int)offsetof(dfea_desc, pad0)};
+        (int)offsetof(dfea_desc, n_finblk), (int)offsetof(dfea_desc, n_vblk), (int)offsetof(dfea_desc, n_halo)};
     if (field < 0 || field >= (int)(sizeof off / sizeof off[0])) return -1;
     return off[field];
 }
@@ -971,11 +1022,12 @@ int dfea_vblk(int n_dof) { return (n_dof + VBLK - 1) / VBLK; }
 int dfea_finblk(int n_shared) { return (n_shared + FBS - 1) / FBS; }
 int dfea_chunk_elems(int nen) { return nen == 4 ? CHUNK_Q4 : nen == 9 ? CHUNK_Q9 : -1; }
 int dfea_chunk_nodes(void) { return MAXLN; }
+int dfea_chunk_threads(int nen) { return nen == 4 ? BS_Q4 : nen == 9 ? BS_Q9 : -1; }
 
 int dfea_geometry(const dfea_desc* table, const int32_t* items, int n_items, int nen, void* stream) {
     if (!table || !items || bad_nen(nen)) return fail(DFEA_EINVAL, "dfea_geometry: bad argument");
-    if (nen == 4) return launch(k_geometry<4, 4, CHUNK_Q4>, n_items, CHUNK_Q4, stream, "k_geometry", table, items);
-    return launch(k_geometry<9, 9, CHUNK_Q9>, n_items, CHUNK_Q9, stream, "k_geometry", table, items);
+    if (nen == 4) return launch(k_geometry<4, 4, BS_Q4>, n_items, BS_Q4, stream, "k_geometry", table, items);
+    return launch(k_geometry<9, 9, BS_Q9>, n_items, BS_Q9, stream, "k_geometry", table, items);
 }
 
 int dfea_moduli(const dfea_desc* table, const int32_t* items, int n_items, const int32_t* fin_items,
@@ -984,13 +1036,13 @@ int dfea_moduli(const dfea_desc* table, const int32_t* items, int n_items, const
         return fail(DFEA_EINVAL, "dfea_moduli: bad argument");
     int rc;
     if (precision == DFEA_FP64) {
-        rc = nen == 4 ? launch(k_moduli<4, 4, CHUNK_Q4, double>, n_items, CHUNK_Q4, stream, "k_moduli", table, items, u)
-                      : launch(k_moduli<9, 9, CHUNK_Q9, double>, n_items, CHUNK_Q9, stream, "k_moduli", table, items, u);
+        rc = nen == 4 ? launch(k_moduli<4, 4, BS_Q4, double>, n_items, BS_Q4, stream, "k_moduli", table, items, u)
+                      : launch(k_moduli<9, 9, BS_Q9, double>, n_items, BS_Q9, stream, "k_moduli", table, items, u);
         if (rc) return rc;
         return launch(k_moduli_fin<double>, n_fin_items, FBS, stream, "k_moduli_fin", table, fin_items);
     }
-    rc = nen == 4 ? launch(k_moduli<4, 4, CHUNK_Q4, float>, n_items, CHUNK_Q4, stream, "k_moduli", table, items, u)
-                  : launch(k_moduli<9, 9, CHUNK_Q9, float>, n_items, CHUNK_Q9, stream, "k_moduli", table, items, u);
+    rc = nen == 4 ? launch(k_moduli<4, 4, BS_Q4, float>, n_items, BS_Q4, stream, "k_moduli", table, items, u)
+                  : launch(k_moduli<9, 9, BS_Q9, float>, n_items, BS_Q9, stream, "k_moduli", table, items, u);
     if (rc) return rc;
     return launch(k_moduli_fin<float>, n_fin_items, FBS, stream, "k_moduli_fin", table, fin_items);
 }
@@ -998,8 +1050,8 @@ int dfea_moduli(const dfea_desc* table, const int32_t* items, int n_items, const
 int dfea_residual(const dfea_desc* table, const int32_t* items, int n_items, const int32_t* fin_items,
                   int n_fin_items, const double* u, double* R, double lf, int nen, void* stream) {
     if (!table || !items || bad_nen(nen)) return fail(DFEA_EINVAL, "dfea_residual: bad argument");
-    int rc = nen == 4 ? launch(k_residual<4, 4, CHUNK_Q4>, n_items, CHUNK_Q4, stream, "k_residual", table, items, u, R, lf)
-                      : launch(k_residual<9, 9, CHUNK_Q9>, n_items, CHUNK_Q9, stream, "k_residual", table, items, u, R, lf);
+    int rc = nen == 4 ? launch(k_residual<4, 4, BS_Q4>, n_items, BS_Q4, stream, "k_residual", table, items, u, R, lf)
+                      : launch(k_residual<9, 9, BS_Q9>, n_items, BS_Q9, stream, "k_residual", table, items, u, R, lf);
     if (rc) return rc;
     return launch(k_residual_fin, n_fin_items, FBS, stream, "k_residual_fin", table, fin_items, R, lf);
 }
@@ -1012,15 +1064,15 @@ int dfea_apply(const dfea_desc* table, const int32_t* items, int n_items, const 
     if (precision == DFEA_FP64) {
         const double* vv = static_cast<const double*>(v);
         double* yy = static_cast<double*>(y);
-        rc = nen == 4 ? launch(k_apply<4, 4, CHUNK_Q4, double, false>, n_items, CHUNK_Q4, stream, "k_apply", table, items, vv, yy)
-                      : launch(k_apply<9, 9, CHUNK_Q9, double, false>, n_items, CHUNK_Q9, stream, "k_apply", table, items, vv, yy);
+        rc = nen == 4 ? launch(k_apply<4, 4, BS_Q4, double, false>, n_items, BS_Q4, stream, "k_apply", table, items, vv, yy)
+                      : launch(k_apply<9, 9, BS_Q9, double, false>, n_items, BS_Q9, stream, "k_apply", table, items, vv, yy);
         if (rc) return rc;
         return launch(k_apply_fin<double, false>, n_fin_items, FBS, stream, "k_apply_fin", table, fin_items, vv, yy);
     }
     const float* vv = static_cast<const float*>(v);
     float* yy = static_cast<float*>(y);
-    rc = nen == 4 ? launch(k_apply<4, 4, CHUNK_Q4, float, false>, n_items, CHUNK_Q4, stream, "k_apply", table, items, vv, yy)
-                  : launch(k_apply<9, 9, CHUNK_Q9, float, false>, n_items, CHUNK_Q9, stream, "k_apply", table, items, vv, yy);
+    rc = nen == 4 ? launch(k_apply<4, 4, BS_Q4, float, false>, n_items, BS_Q4, stream, "k_apply", table, items, vv, yy)
+                  : launch(k_apply<9, 9, BS_Q9, float, false>, n_items, BS_Q9, stream, "k_apply", table, items, vv, yy);
     if (rc) return rc;
     return launch(k_apply_fin<float, false>, n_fin_items, FBS, stream, "k_apply_fin", table, fin_items, vv, yy);
 }
@@ -1060,13 +1112,13 @@ int dfea_pcg_iterate(const dfea_desc* table, const int32_t* items, int n_items, 
     if (!table || !items || !vitems || !solve_items || bad_nen(nen) || n_iter < 0)
         return fail(DFEA_EINVAL, "dfea_pcg_iterate: bad argument");
     if (precision == DFEA_FP64)
-        return nen == 4 ? pcg_iter<4, CHUNK_Q4, double>(table, items, n_items, fin_items, n_fin_items, vitems, n_vitems,
+        return nen == 4 ? pcg_iter<4, BS_Q4, double>(table, items, n_items, fin_items, n_fin_items, vitems, n_vitems,
                                                         solve_items, n_solves, n_iter, stream)
-                        : pcg_iter<9, CHUNK_Q9, double>(table, items, n_items, fin_items, n_fin_items, vitems, n_vitems,
+                        : pcg_iter<9, BS_Q9, double>(table, items, n_items, fin_items, n_fin_items, vitems, n_vitems,
                                                         solve_items, n_solves, n_iter, stream);
-    return nen == 4 ? pcg_iter<4, CHUNK_Q4, float>(table, items, n_items, fin_items, n_fin_items, vitems, n_vitems,
+    return nen == 4 ? pcg_iter<4, BS_Q4, float>(table, items, n_items, fin_items, n_fin_items, vitems, n_vitems,
                                                    solve_items, n_solves, n_iter, stream)
-                    : pcg_iter<9, CHUNK_Q9, float>(table, items, n_items, fin_items, n_fin_items, vitems, n_vitems,
+                    : pcg_iter<9, BS_Q9, float>(table, items, n_items, fin_items, n_fin_items, vitems, n_vitems,
                                                    solve_items, n_solves, n_iter, stream);
 }
 
@@ -1118,8 +1170,8 @@ int dfea_graph_destroy(void* graph) {
 int dfea_adjoint_grad(const dfea_desc* table, const int32_t* items, int n_items, const double* u, const double* lam,
                       double* g, int nen, void* stream) {
     if (!table || !items || !u || !lam || !g || bad_nen(nen)) return fail(DFEA_EINVAL, "dfea_adjoint_grad: bad argument");
-    if (nen == 4) return launch(k_adjoint<4, 4, CHUNK_Q4>, n_items, CHUNK_Q4, stream, "k_adjoint", table, items, u, lam, g);
-    return launch(k_adjoint<9, 9, CHUNK_Q9>, n_items, CHUNK_Q9, stream, "k_adjoint", table, items, u, lam, g);
+    if (nen == 4) return launch(k_adjoint<4, 4, BS_Q4>, n_items, BS_Q4, stream, "k_adjoint", table, items, u, lam, g);
+    return launch(k_adjoint<9, 9, BS_Q9>, n_items, BS_Q9, stream, "k_adjoint", table, items, u, lam, g);
 }
 
 }  // extern "C"

@@ -16,8 +16,8 @@ LIB_PATH = os.path.join(_HERE, LIB_NAME)
 FP64, FP32 = 0, 1
 MAXCOL = 16
 FIXED_X, FIXED_Y, NODE_MASK = 0x40000000, 0x80000000, 0x3FFFFFFF
-VEC = dict(x=0, r=1, z=2, p=3, Ap=4, b=5, u=6, R=7, uacc=8, uD=9)
-NVEC = 10
+VEC = dict(x=0, r=1, z=2, p=3, Ap=4, b=5, u=6, R=7, uacc=8, uD=9, p2=10)
+NVEC = 11
 SC = dict(rz=0, pAp=1, alpha=2, beta=3, rr=4, bb=5, rtol=6, status=7, k=8, maxit=9, iters=10, norm=11, lf=12,
           step=13)
 NSC = 16
@@ -31,7 +31,7 @@ _D = ctypes.c_double
 class Desc(ctypes.Structure):
     """Mirror of ``dfea_desc`` (field order = declaration order in include/dfea.h)."""
     _fields_ = [
-        ("chunks", _P), ("lnode", _P), ("lelem", _P), ("sh_node", _P), ("sh_off", _P), ("pslot", _P),
+        ("chunks", _P), ("lnode", _P), ("lelem", _P), ("sh_node", _P), ("sh_off", _P), ("pslot", _P), ("hel", _P), ("hlelem", _P),
         ("conn", _P), ("perm", _P), ("nodes", _P), ("dNdxi", _P), ("gw", _P), ("nfix", _P),
         ("dN64", _P), ("dN32", _P), ("w", _P), ("theta", _P), ("C64", _P), ("C32", _P),
         ("diag64", _P), ("diag32", _P), ("f", _P), ("part", _P), ("red", _P), ("sc", _P),
@@ -39,11 +39,12 @@ class Desc(ctypes.Structure):
         ("n_chunks", ctypes.c_int32), ("n_lnodes", ctypes.c_int32), ("n_shared", ctypes.c_int32),
         ("n_part", ctypes.c_int32), ("n_el", ctypes.c_int32), ("n_nodes", ctypes.c_int32),
         ("nen", ctypes.c_int32), ("nq", ctypes.c_int32), ("precision", ctypes.c_int32),
-        ("n_finblk", ctypes.c_int32), ("n_vblk", ctypes.c_int32), ("pad0", ctypes.c_int32),
+        ("n_finblk", ctypes.c_int32), ("n_vblk", ctypes.c_int32), ("n_halo", ctypes.c_int32),
     ]
 
 
-CHUNK_INTS = 32     # sizeof(dfea_chunk) / 4
+CHUNK_INTS = 64     # sizeof(dfea_chunk) / 4
+COL_OFF, HCOL_OFF = 12, 12 + MAXCOL + 1   # int offsets of col[] and hcol[] in a chunk record
 
 _SIGNATURES = {
     "dfea_version": (_I, []),
@@ -55,6 +56,7 @@ _SIGNATURES = {
     "dfea_finblk": (_I, [_I]),
     "dfea_chunk_elems": (_I, [_I]),
     "dfea_chunk_nodes": (_I, []),
+    "dfea_chunk_threads": (_I, [_I]),
     "dfea_geometry": (_I, [_P, _P, _I, _I, _P]),
     "dfea_moduli": (_I, [_P, _P, _I, _P, _I, _P, _I, _I, _P]),
     "dfea_residual": (_I, [_P, _P, _I, _P, _I, _P, _P, _D, _I, _P]),

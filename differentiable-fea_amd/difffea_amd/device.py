@@ -63,7 +63,7 @@ class HipMesh:
         cij = params.get("cell_ij")
         self.plan = p = MeshPlan(nodes.numpy(), conn.numpy(), fixed.numpy(),
                                  cell_ij=None if cij is None else cij.detach().cpu().numpy())
-        if p.validate() > self.lib.dfea_chunk_elems(self.nen):
+        if p.validate() > self.lib.dfea_chunk_threads(self.nen):
             raise ValueError("chunk larger than the kernel's workgroup")
         self.n_el, self.n_nodes = p.n_el, p.n_nodes
         self.n_dof = 2 * p.n_nodes
@@ -74,6 +74,9 @@ class HipMesh:
         self.t_sh_node = _dev_int32(p.sh_node, dv)
         self.t_sh_off = _dev_int32(p.sh_off, dv)
         self.t_pslot = _dev_int32(p.pslot if len(p.pslot) else np.zeros(1, np.int32), dv)
+        self.t_hel = _dev_int32(p.hel if len(p.hel) else np.zeros(1, np.int32), dv)
+        self.t_hlelem = torch.from_numpy(np.ascontiguousarray(p.hlelem).view(np.int16).reshape(-1) if p.hlelem.size
+                                         else np.zeros(1, np.int16)).to(dv)
         self.t_conn = _dev_int32(p.conn_int, dv)
         self.t_perm = _dev_int32(p.perm, dv)
         self.perm = torch.from_numpy(p.perm.astype(np.int64)).to(dv)
@@ -103,6 +106,7 @@ class HipMesh:
         p = self.plan
         d.chunks, d.lnode, d.lelem = self.t_chunks.data_ptr(), self.t_lnode.data_ptr(), self.t_lelem.data_ptr()
         d.sh_node, d.sh_off, d.pslot = self.t_sh_node.data_ptr(), self.t_sh_off.data_ptr(), self.t_pslot.data_ptr()
+        d.hel, d.hlelem, d.n_halo = self.t_hel.data_ptr(), self.t_hlelem.data_ptr(), len(p.hel)
         d.conn, d.perm, d.nodes = self.t_conn.data_ptr(), self.t_perm.data_ptr(), self.t_nodes.data_ptr()
         d.dNdxi, d.gw, d.nfix = self.t_dNdxi.data_ptr(), self.t_gw.data_ptr(), self.t_nfix.data_ptr()
         d.dN64, d.dN32, d.w = self.dN64.data_ptr(), self.dN32.data_ptr(), self.w.data_ptr()
@@ -144,7 +148,7 @@ class HipSystem:
         self.red = torch.zeros(max(1, m.red_size), dtype=F64, device=dv)
         # the scalar block may be a row of a pool shared by a batch of solves
         self.sc = torch.zeros(_lib.NSC, dtype=F64, device=dv) if sc is None else sc
-        self.vec = {k: torch.zeros(nd, dtype=self.vdt, device=dv) for k in ("x", "r", "z", "p", "Ap", "b")}
+        self.vec = {k: torch.zeros(nd, dtype=self.vdt, device=dv) for k in ("x", "r", "z", "p", "Ap", "b", "p2")}
         self.vec["u"] = torch.zeros(nd, dtype=F64, device=dv)
         self.vec["R"] = torch.zeros(nd, dtype=F64, device=dv)
         self.vec["uacc"] = torch.zeros(nd, dtype=F64, device=dv)

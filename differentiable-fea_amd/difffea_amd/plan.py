@@ -16,6 +16,14 @@ Built once per (mesh, Dirichlet mask) on the host with vectorised numpy; uploade
    then chunk: the partials of shared node i are slots ``sh_off[i]..sh_off[i+1]-1``, summed in that order.
    ``pslot`` maps a chunk's shared entries (chunk order) to their slots.
 
+Halo mode (default for quad4).  Every node is owned by exactly one chunk: the chunk of its adjacent element
+latest in Morton order (on a grid: the element up and to the right, so a 16x16 tile owns a 16x16 block of
+nodes).  A chunk evaluates its core elements plus the "halo" elements of other chunks that touch its owned
+nodes (33 per interior tile), so each owned node is completed inside one workgroup: no shared nodes, no
+partials, no second kernel, at the price of ~13 % more element reads.  Halo elements are listed per chunk
+(``hel``, with their local indices in ``hlelem``), sorted by colour like the core elements.  Partials mode
+(default for quad9, and the fallback when a chunk would exceed the workgroup) is described in 4.
+
 Nothing here depends on which other solves run in the same batch, so results are batch-independent.
 """
 import numpy as np
@@ -129,7 +137,7 @@ def jones_plassmann(conn, seed=0):
 class MeshPlan:
     """Integer tables of one mesh + Dirichlet mask (numpy arrays, see module docstring)."""
 
-    def __init__(self, nodes, conn, fixed, chunk_elems=None, max_nodes=None, cell_ij=None):
+    def __init__(self, nodes, conn, fixed, chunk_elems=None, max_nodes=None, cell_ij=None, mode="auto"):
         nodes = np.asarray(nodes, np.float64)
         conn = np.asarray(conn, np.int64)
         fixed = np.asarray(fixed, bool).reshape(-1, 2)
@@ -138,6 +146,11 @@ class MeshPlan:
         lib = _lib.load()
         E = chunk_elems or lib.dfea_chunk_elems(nen)
         MAXLN = max_nodes or lib.dfea_chunk_nodes()
+        BS = lib.dfea_chunk_threads(nen)
+        if mode == "auto":
+            mode = "halo" if nen == 4 else "partials"
+        if mode not in ("halo", "partials"):
+            raise ValueError(f"plan mode must be 'auto', 'halo' or 'partials', got {mode!r}")
         if E <= 0:
             raise ValueError(f"no chunk size for elements with {nen} nodes")
         if N >= (1 << 30):
@@ -168,6 +181,19 @@ class MeshPlan:
         ch_e = chunk_of_pos                                        # chunk of internal element
         col_sorted = colour[perm]
 
+        bits = fixed[:, 0].astype(np.uint64) * np.uint64(_lib.FIXED_X) + \
+            fixed[:, 1].astype(np.uint64) * np.uint64(_lib.FIXED_Y)
+        self.nfix = (fixed[:, 0].astype(np.uint8) | (fixed[:, 1].astype(np.uint8) << 1))
+        self.conn_int = np.ascontiguousarray(conn[perm].T).astype(np.int32)     # [nen][n_el]
+        self.n_chunks = n_chunks
+        self.structured = structured
+        if mode == "halo" and self._build_halo(conn, perm, colour, E, BS, MAXLN, bits):
+            self.mode = "halo"
+            return
+        self.mode = "partials"
+        self.hel = np.zeros(0, np.int32)
+        self.hlelem = np.zeros((nen, 0), np.uint16)
+
         # colour ranges per chunk
         chunks = np.zeros((n_chunks, _lib.CHUNK_INTS), np.int32)
         e0 = np.arange(n_chunks) * E
@@ -181,7 +207,7 @@ class MeshPlan:
         for c in range(n_chunks):
             present = cnt[c][cnt[c] > 0]
             ncol[c] = len(present)
-            chunks[c, 8:8 + len(present) + 1] = np.r_[0, np.cumsum(present)]
+            chunks[c, _lib.COL_OFF:_lib.COL_OFF + len(present) + 1] = np.r_[0, np.cumsum(present)]
         chunks[:, 6] = ncol
 
         # 3. local nodes: distinct (chunk, node) pairs, owned first
@@ -239,28 +265,125 @@ class MeshPlan:
         self.structured = structured
         self.nfix = (fixed[:, 0].astype(np.uint8) | (fixed[:, 1].astype(np.uint8) << 1))
 
+    def _build_halo(self, conn, perm, colour, E, BS, MAXLN, bits):
+        """Halo-mode tables; returns False (and builds nothing) when a chunk would not fit the workgroup."""
+        n_el, nen = conn.shape
+        N = self.n_nodes
+        n_chunks = self.n_chunks
+        pos = np.empty(n_el, np.int64)
+        pos[perm] = np.arange(n_el)                                # internal index of every element
+        chunk_of = pos // E                                        # core chunk (by internal position)
+        # owner of a node: chunk of its adjacent element latest in the internal order
+        last = np.full(N, -1, np.int64)
+        np.maximum.at(last, conn.reshape(-1), np.repeat(pos, nen))
+        owner = last // E
+        # (chunk, element) pairs: every chunk that owns a node of the element evaluates it
+        pc = owner[conn].reshape(-1)
+        pe = np.repeat(np.arange(n_el), nen)
+        key = np.unique(pc * n_el + pe)
+        hc, he = key // n_el, key % n_el
+        halo = hc != chunk_of[he]
+        hc, he = hc[halo], he[halo]
+        ncol_g = int(colour.max()) + 1
+        # halo entries sorted by (chunk, colour, internal index)
+        o = np.lexsort((pos[he], colour[he], hc))
+        hc, he = hc[o], he[o]
+        h0 = np.searchsorted(hc, np.arange(n_chunks))
+        nh = np.diff(np.r_[h0, len(hc)])
+        ne = np.minimum(E, n_el - np.arange(n_chunks) * E)
+        if (ne + nh).max() > BS:
+            return False
+        # local nodes: owned first, then the other nodes of the chunk's core and halo elements
+        ce = conn[perm]
+        core_pairs_c = np.repeat(np.arange(n_el) // E, nen)
+        core_pairs_n = ce.reshape(-1)
+        halo_pairs_c = np.repeat(hc, nen)
+        halo_pairs_n = conn[he].reshape(-1)
+        allc = np.r_[core_pairs_c, halo_pairs_c]
+        alln = np.r_[core_pairs_n, halo_pairs_n]
+        k = np.unique(allc * N + alln)
+        uc, un = k // N, k % N
+        own = owner[un] == uc
+        k2 = uc * (2 * N) + (~own) * N + un                        # owned first inside each chunk
+        o2 = np.argsort(k2, kind="stable")
+        uc, un, own, k2 = uc[o2], un[o2], own[o2], k2[o2]
+        k0 = np.searchsorted(uc, np.arange(n_chunks))
+        nk = np.diff(np.r_[k0, len(uc)])
+        if nk.max() > MAXLN:
+            return False
+        nown = np.bincount(uc[own], minlength=n_chunks)
+        if nown.sum() != N:
+            return False
+
+        def local(chunks, nodes_):
+            kk = chunks * (2 * N) + (owner[nodes_] != chunks) * N + nodes_
+            ps = np.searchsorted(k2, kk)
+            assert np.all(k2[ps] == kk)
+            return ps - k0[chunks]
+
+        self.lelem = np.ascontiguousarray(local(core_pairs_c, core_pairs_n).reshape(n_el, nen).T).astype(np.uint16)
+        self.hlelem = np.ascontiguousarray(local(halo_pairs_c, halo_pairs_n).reshape(-1, nen).T).astype(np.uint16)
+        self.hel = pos[he].astype(np.int32)                        # internal element ids
+        self.lnode = (un.astype(np.uint64) | bits[un]).astype(np.uint32)
+        chunks = np.zeros((n_chunks, _lib.CHUNK_INTS), np.int32)
+        chunks[:, 0] = np.arange(n_chunks) * E
+        chunks[:, 1] = ne
+        chunks[:, 2] = k0
+        chunks[:, 3] = nk
+        chunks[:, 4] = nown
+        chunks[:, 6] = ncol_g
+        chunks[:, 7] = h0
+        chunks[:, 8] = nh
+        col_sorted = colour[perm]
+        cc = np.zeros((n_chunks, ncol_g), np.int64)
+        np.add.at(cc, (np.arange(n_el) // E, col_sorted), 1)
+        hcc = np.zeros((n_chunks, ncol_g), np.int64)
+        np.add.at(hcc, (hc, colour[he]), 1)
+        chunks[:, _lib.COL_OFF:_lib.COL_OFF + ncol_g + 1] = np.c_[np.zeros(n_chunks, np.int64), np.cumsum(cc, 1)]
+        chunks[:, _lib.HCOL_OFF:_lib.HCOL_OFF + ncol_g + 1] = np.c_[np.zeros(n_chunks, np.int64), np.cumsum(hcc, 1)]
+        self.chunks = chunks
+        self.n_lnodes = len(un)
+        self.n_shared = 0
+        self.n_part = 0
+        self.pslot = np.zeros(0, np.int32)
+        self.sh_node = np.zeros(0, np.uint32)
+        self.sh_off = np.zeros(1, np.int32)
+        return True
+
     def validate(self):
-        """Host-side bounds checks of every index the kernels follow (run before any launch)."""
+        """Host-side bounds checks of every index the kernels follow (run before any launch).
+        Returns the largest number of threads a chunk needs (core + halo elements)."""
         ch = self.chunks
-        E = ch[:, 1].max()
         assert ch[:, 0].min() >= 0 and (ch[:, 0] + ch[:, 1]).max() <= self.n_el
         assert (ch[:, 2] + ch[:, 3]).max() <= self.n_lnodes and (ch[:, 4] <= ch[:, 3]).all()
-        assert (ch[:, 5] + ch[:, 3] - ch[:, 4]).max() <= max(self.n_part, 0)
+        if self.mode == "partials":
+            assert (ch[:, 5] + ch[:, 3] - ch[:, 4]).max() <= max(self.n_part, 0)
+            assert (ch[:, 8] == 0).all()
+        else:
+            assert self.n_part == 0 and ch[:, 4].sum() == self.n_nodes
+            assert (ch[:, 7] + ch[:, 8]).max() <= len(self.hel)
+            if len(self.hel):
+                assert self.hel.min() >= 0 and self.hel.max() < self.n_el
+                ch_of_h = np.repeat(np.arange(self.n_chunks), ch[:, 8])
+                assert (self.hlelem.astype(np.int64) < ch[ch_of_h, 3][None, :]).all()
         # every element's local indices fall inside its chunk's node range
         chunk_of_e = np.repeat(np.arange(self.n_chunks), ch[:, 1])
         assert (self.lelem.astype(np.int64) < ch[chunk_of_e, 3][None, :]).all()
-        # colour ranges tile the chunk
+        # colour ranges tile the core and halo threads
         for c in range(self.n_chunks):
             nc = ch[c, 6]
-            assert 1 <= nc <= _lib.MAXCOL and ch[c, 8] == 0 and ch[c, 8 + nc] == ch[c, 1]
+            assert 1 <= nc <= _lib.MAXCOL
+            assert ch[c, _lib.COL_OFF] == 0 and ch[c, _lib.COL_OFF + nc] == ch[c, 1]
+            assert ch[c, _lib.HCOL_OFF] == 0 and ch[c, _lib.HCOL_OFF + nc] == ch[c, 8]
         assert (self.lnode & _lib.NODE_MASK).max() < self.n_nodes
         if self.n_shared:
             assert self.sh_off[0] == 0 and self.sh_off[-1] == len(self.pslot) == self.n_part
             assert np.array_equal(np.sort(self.pslot), np.arange(self.n_part))
             assert (self.sh_node & _lib.NODE_MASK).max() < self.n_nodes
         assert self.conn_int.min() >= 0 and self.conn_int.max() < self.n_nodes
-        return E
+        return int((ch[:, 1] + ch[:, 8]).max())
 
     def stats(self):
-        return dict(n_chunks=self.n_chunks, n_lnodes=self.n_lnodes, n_shared=self.n_shared, n_part=self.n_part,
-                    max_chunk_nodes=int(self.chunks[:, 3].max()), max_colours=int(self.chunks[:, 6].max()))
+        return dict(mode=self.mode, n_chunks=self.n_chunks, n_lnodes=self.n_lnodes, n_shared=self.n_shared,
+                    n_part=self.n_part, n_halo=len(self.hel), max_chunk_nodes=int(self.chunks[:, 3].max()),
+                    max_colours=int(self.chunks[:, 6].max()), max_threads=int((self.chunks[:, 1] + self.chunks[:, 8]).max()))

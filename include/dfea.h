@@ -39,7 +39,10 @@ enum { DFEA_FP64 = 0, DFEA_FP32 = 1 };
 
 /* vector slots of a descriptor */
 enum { DFEA_X = 0, DFEA_R = 1, DFEA_Z = 2, DFEA_P = 3, DFEA_AP = 4, DFEA_B = 5, DFEA_U = 6, DFEA_RES = 7,
-       DFEA_UACC = 8, DFEA_UD = 9, DFEA_NVEC = 10 };
+       DFEA_UACC = 8, DFEA_UD = 9, DFEA_P2 = 10, DFEA_NVEC = 11 };
+/* PCG keeps the search direction in two buffers, p (vec[DFEA_P]) and p' (vec[DFEA_P2]): iteration k reads
+ * the old direction from buffer k mod 2 and writes the new one to the other, so no workgroup overwrites a
+ * value another workgroup still has to read. */
 
 /* scalar block of a solve (doubles) */
 enum { DFEA_SC_RZ = 0, DFEA_SC_PAP = 1, DFEA_SC_ALPHA = 2, DFEA_SC_BETA = 3, DFEA_SC_RR = 4, DFEA_SC_BB = 5,
@@ -49,16 +52,25 @@ enum { DFEA_SC_RZ = 0, DFEA_SC_PAP = 1, DFEA_SC_ALPHA = 2, DFEA_SC_BETA = 3, DFE
 /* PCG status values stored in sc[DFEA_SC_STATUS] */
 enum { DFEA_PCG_RUNNING = 0, DFEA_PCG_CONVERGED = 1, DFEA_PCG_CURVATURE = 2, DFEA_PCG_MAXITER = 3 };
 
-/* A chunk: the elements handled by one workgroup (dfea_chunk_elems / dfea_chunk_nodes bound its size). */
+/* A chunk: the elements handled by one workgroup (dfea_chunk_elems / dfea_chunk_nodes bound its size).
+ * Two plan modes:
+ *   partials  nodes used by one chunk are owned by it; nodes used by several chunks get one partial per
+ *             chunk (pslot), summed by the *_fin kernels in chunk order;
+ *   halo      every node is owned by one chunk, and the chunk also evaluates the "halo" elements of other
+ *             chunks that touch its owned nodes, so each owned node is completed inside the workgroup (no
+ *             shared nodes, no second kernel).  Halo threads only add into LDS; per-element outputs
+ *             (moduli, geometry, adjoint gradient) are written by the element's core chunk. */
 typedef struct dfea_chunk {
-    int32_t e0, ne;        /* internal element range [e0, e0+ne) */
+    int32_t e0, ne;        /* core elements [e0, e0+ne) (internal order) = threads 0..ne-1 */
     int32_t k0, nk;        /* local node table range [k0, k0+nk) in lnode */
-    int32_t nown;          /* local nodes k0..k0+nown-1 belong to this chunk only; the rest are shared */
-    int32_t pbase;         /* partial slot of the first shared local node */
-    int32_t ncol;          /* element colours; colour c = local elements [col[c], col[c+1]) */
-    int32_t pad;
-    int32_t col[DFEA_MAXCOL + 1];
-    int32_t pad2[7];       /* 128 bytes */
+    int32_t nown;          /* local nodes k0..k0+nown-1 are owned (final values written by this chunk) */
+    int32_t pbase;         /* partials mode: first pslot entry of the chunk's shared local nodes */
+    int32_t ncol;          /* element colours */
+    int32_t h0, nh;        /* halo mode: halo entries [h0, h0+nh) of hel / hlelem = threads ne..ne+nh-1 */
+    int32_t pad[3];
+    int32_t col[DFEA_MAXCOL + 1];   /* colour c = core threads col[c]..col[c+1]-1 */
+    int32_t hcol[DFEA_MAXCOL + 1];  /* colour c = halo threads ne+hcol[c]..ne+hcol[c+1]-1 */
+    int32_t pad2[18];      /* 256 bytes */
 } dfea_chunk;
 
 /* Everything the kernels need about one solve (device pointers). */
@@ -70,6 +82,8 @@ typedef struct dfea_desc {
     const int32_t* sh_off;      /* [n_shared+1] */
     const int32_t* pslot;       /* [n_part] partial slot of each (chunk, shared node) entry, chunk order;
                                    the partials of shared node i are slots sh_off[i]..sh_off[i+1]-1 */
+    const int32_t* hel;         /* [n_halo] halo mode: internal element id of each halo entry */
+    const uint16_t* hlelem;     /* [nen][n_halo] halo mode: local node index of halo entry h, node a */
     const int32_t* conn;        /* [nen][n_el] node ids, internal element order */
     const int32_t* perm;        /* [n_el] original element id of internal element e */
     const double* nodes;        /* [N][2] reference coordinates */
@@ -93,7 +107,7 @@ typedef struct dfea_desc {
     double mu0, lam0;
     int32_t n_chunks, n_lnodes, n_shared, n_part;
     int32_t n_el, n_nodes, nen, nq;
-    int32_t precision, n_finblk, n_vblk, pad0;
+    int32_t precision, n_finblk, n_vblk, n_halo;
 } dfea_desc;
 
 /* ---- library ---------------------------------------------------------------------------------------- */
@@ -106,6 +120,7 @@ int dfea_vblk(int n_dof);                               /* dof blocks of the vec
 int dfea_finblk(int n_shared);                          /* blocks of the shared-node kernels */
 int dfea_chunk_elems(int nen);                          /* maximum elements per chunk for this element */
 int dfea_chunk_nodes(void);                             /* maximum local nodes per chunk */
+int dfea_chunk_threads(int nen);                        /* workgroup size: core + halo elements per chunk */
 
 /* ---- once per mesh ---------------------------------------------------------------------------------- */
 /* Geometry cache dN_dX, w_detJ (dN32 too when non-null).

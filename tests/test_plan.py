@@ -32,15 +32,15 @@ MESHES = {
 
 
 def _emulate_apply(plan, C, dN, v):
-    """numpy emulation of k_apply + k_apply_fin: per chunk gather -> element -> colour-ordered LDS
-    accumulation -> owned write / shared partial; then shared nodes sum their partials in slot order."""
-    nen = plan.nen
+    """numpy emulation of k_apply (+ k_apply_fin): per chunk gather -> core and halo elements ->
+    colour-ordered LDS accumulation -> owned write / shared partial; then shared nodes sum their partials
+    in slot order (partials plans)."""
     N = plan.n_nodes
     y = np.full((N, 2), np.nan)
     part = np.zeros((plan.n_part, 2))
     vv = v.reshape(N, 2)
     for c in range(plan.n_chunks):
-        e0, ne, k0, nk, nown, pbase, ncol = plan.chunks[c, :7]
+        e0, ne, k0, nk, nown, pbase, ncol, h0, nh = plan.chunks[c, :9]
         ln = plan.lnode[k0:k0 + nk]
         nodes = (ln & _lib.NODE_MASK).astype(np.int64)
         bits = ln >> 30
@@ -48,21 +48,25 @@ def _emulate_apply(plan, C, dN, v):
         masked = raw.copy()
         masked[bits & 1 == 1, 0] = 0
         masked[bits & 2 == 2, 1] = 0
-        loc = plan.lelem[:, e0:e0 + ne].T.astype(np.int64)                  # (ne, nen)
-        ve = masked[loc]                                                    # (ne, nen, 2)
-        d = dN[e0:e0 + ne]                                                  # (ne, nq, nen, 2)
-        g = np.einsum("eai,eqaj->eqij", ve, d).reshape(ne, -1, 4)
-        s = np.einsum("eqmn,eqn->eqm", C[e0:e0 + ne], g).reshape(ne, -1, 2, 2)
-        fe = np.einsum("eqij,eqaj->eai", s, d)
+        els = np.r_[np.arange(e0, e0 + ne), plan.hel[h0:h0 + nh]].astype(np.int64)
+        loc = np.r_[plan.lelem[:, e0:e0 + ne].T, plan.hlelem[:, h0:h0 + nh].T].astype(np.int64)   # (threads, nen)
+        ve = masked[loc]
+        d = dN[els]
+        g = np.einsum("eai,eqaj->eqij", ve, d).reshape(len(els), -1, 4)
+        s_ = np.einsum("eqmn,eqn->eqm", C[els], g).reshape(len(els), -1, 2, 2)
+        fe = np.einsum("eqij,eqaj->eai", s_, d)
         acc = np.zeros((nk, 2))
-        col = plan.chunks[c, 8:8 + ncol + 1]
+        col = plan.chunks[c, _lib.COL_OFF:_lib.COL_OFF + ncol + 1]
+        hcol = plan.chunks[c, _lib.HCOL_OFF:_lib.HCOL_OFF + ncol + 1]
         for k in range(ncol):
-            idx = loc[col[k]:col[k + 1]].reshape(-1)
+            thr = np.r_[np.arange(col[k], col[k + 1]), ne + np.arange(hcol[k], hcol[k + 1])]
+            idx = loc[thr].reshape(-1)
             assert len(np.unique(idx)) == len(idx), "two elements of one colour share a node"
-            acc[idx] += fe[col[k]:col[k + 1]].reshape(-1, 2)
+            acc[idx] += fe[thr].reshape(-1, 2)
         out = np.where(np.stack([bits & 1, bits & 2], 1) > 0, raw, acc)
         y[nodes[:nown]] = out[:nown]
-        part[plan.pslot[pbase:pbase + nk - nown]] = acc[nown:]
+        if plan.n_part:
+            part[plan.pslot[pbase:pbase + nk - nown]] = acc[nown:]
     for i in range(plan.n_shared):
         g_ = int(plan.sh_node[i])
         n = g_ & _lib.NODE_MASK
@@ -71,13 +75,17 @@ def _emulate_apply(plan, C, dN, v):
             acc += part[j]
         y[n, 0] = vv[n, 0] if g_ & _lib.FIXED_X else acc[0]
         y[n, 1] = vv[n, 1] if g_ & _lib.FIXED_Y else acc[1]
+    assert not np.isnan(y).any(), "a node was written by no chunk"
     return y.reshape(-1)
 
 
+@pytest.mark.parametrize("mode", ["halo", "partials"])
 @pytest.mark.parametrize("name", list(MESHES))
-def test_plan_tables_and_emulated_assembly_match_oracle(name):
+def test_plan_tables_and_emulated_assembly_match_oracle(name, mode):
     p = fem.precompute_geometry(MESHES[name]())
-    plan = _plan(p)
+    plan = _plan(p, mode=mode)
+    if mode == "halo" and plan.mode != "halo":
+        pytest.skip("halo chunks would exceed the workgroup for this element type")
     plan.validate()
     # local tables reconstruct the connectivity
     nodes_of = (plan.lnode & _lib.NODE_MASK).astype(np.int64)
@@ -85,8 +93,11 @@ def test_plan_tables_and_emulated_assembly_match_oracle(name):
     glob = nodes_of[plan.chunks[ch_of_e, 2][None, :] + plan.lelem.astype(np.int64)]
     assert np.array_equal(glob, plan.conn_int)
     assert np.array_equal(plan.conn_int.T, p["conn"].numpy()[plan.perm])
-    # every node is owned by exactly one chunk or shared with a partial in each touching chunk
-    assert plan.n_lnodes - plan.n_part + plan.n_shared == plan.n_nodes
+    # every node is owned by exactly one chunk (or shared, with a partial in each touching chunk)
+    if plan.mode == "partials":
+        assert plan.n_lnodes - plan.n_part + plan.n_shared == plan.n_nodes
+    else:
+        assert plan.chunks[:, 4].sum() == plan.n_nodes
     th = torch.ones(p["conn"].shape[0], dtype=torch.float64)
     u, v = state(p, 2, scale=1e-3)
     C, dN = fem.qp_setup(u, th, p)
@@ -98,9 +109,11 @@ def test_plan_tables_and_emulated_assembly_match_oracle(name):
 def test_structured_meshes_get_square_tiles_and_four_colours():
     plan = _plan(unit_square("quad4", 400))
     st = plan.stats()
-    assert st["n_chunks"] == 625 and st["max_chunk_nodes"] == 289 and st["max_colours"] == 4
+    # 16x16 core tiles + 33 halo elements; 16x16 owned nodes + the halo's nodes (18x18)
+    assert st["mode"] == "halo" and st["n_chunks"] == 625 and st["max_threads"] == 289
+    assert st["max_chunk_nodes"] == 324 and st["max_colours"] == 4
     plan9 = _plan(unit_square("quad9", 200))
-    assert plan9.stats()["max_chunk_nodes"] == 289
+    assert plan9.stats()["mode"] == "partials" and plan9.stats()["max_chunk_nodes"] == 289
 
 
 def test_jones_plassmann_is_a_valid_colouring():
