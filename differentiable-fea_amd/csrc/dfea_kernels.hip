@@ -157,11 +157,14 @@ __device__ __forceinline__ ChunkView chunk_view(const dfea_chunk* c) {
 }
 
 // element of thread t: core threads 0..ne-1 take the chunk's contiguous elements, halo threads ne..ne+nh-1
-// the halo list; returns false for idle threads.  l[] receives the element's local node indices.
+// the halo list.  e = plane column of the thread's data (n_el + halo entry for halo threads), el = the
+// element id; returns false for idle threads.  l[] receives the element's local node indices.
 template <int NEN>
-__device__ __forceinline__ bool thread_element(const dfea_desc& D, const ChunkView& ch, int t, int& e, int* l) {
+__device__ __forceinline__ bool thread_element(const dfea_desc& D, const ChunkView& ch, int t, int& e, int* l,
+                                               int* el = nullptr) {
     if (t < ch.ne) {
         e = ch.e0 + t;
+        if (el) *el = e;
 #pragma unroll
         for (int a = 0; a < NEN; ++a) l[a] = D.lelem[a * D.n_el + e];
         return true;
@@ -169,12 +172,14 @@ __device__ __forceinline__ bool thread_element(const dfea_desc& D, const ChunkVi
     const int h = t - ch.ne;
     if (h < ch.nh) {
         const int hh = ch.h0 + h;
-        e = D.hel[hh];
+        e = D.n_el + hh;
+        if (el) *el = D.hel[hh];
 #pragma unroll
         for (int a = 0; a < NEN; ++a) l[a] = D.hlelem[a * D.n_halo + hh];
         return true;
     }
     e = 0;
+    if (el) *el = 0;
     return false;
 }
 
@@ -205,13 +210,13 @@ __global__ __launch_bounds__(BS) void k_geometry(const dfea_desc* __restrict__ t
     const dfea_desc& D = tab[it.s];
     const ChunkView ch = chunk_view(D.chunks + it.b);
     const int t = threadIdx.x;
-    if (t >= ch.ne) return;
-    const int e = ch.e0 + t;
-    const int ne = D.n_el;
+    int l[NEN], e, el;
+    if (!thread_element<NEN>(D, ch, t, e, l, &el)) return;   // halo threads fill their copy columns
+    const int ne = D.n_cols;
     double X[NEN][2];
 #pragma unroll
     for (int a = 0; a < NEN; ++a) {
-        const int n = D.conn[a * ne + e];
+        const int n = D.conn[a * D.n_el + el];
         X[a][0] = D.nodes[2 * n];
         X[a][1] = D.nodes[2 * n + 1];
     }
@@ -300,7 +305,7 @@ __global__ __launch_bounds__(BS) void k_apply(const dfea_desc* __restrict__ tab,
     if (PCG) pcg_dir_buffers<T>(D, p_rd, p_wr);
     const T* z = static_cast<const T*>(D.vec[DFEA_Z]);
     const double beta = PCG ? D.sc[DFEA_SC_BETA] : 0.0;
-    const int ne = D.n_el;
+    const int ne = D.n_cols;
     const T* dN = std::is_same<T, double>::value ? (const T*)D.dN64 : (const T*)D.dN32;
     const T* C = std::is_same<T, double>::value ? (const T*)D.C64 : (const T*)D.C32;
 
@@ -489,7 +494,7 @@ __global__ __launch_bounds__(BS) void k_residual(const dfea_desc* __restrict__ t
     const bool has_el = thread_element<NEN>(D, ch, t, e, l);
     V2<double> fe[NEN];
     if (has_el) {
-        const int ne = D.n_el;
+        const int ne = D.n_cols;
         const double th = D.theta[e];
         const double mu = D.mu0 * th, lam = D.lam0 * th;
         V2<double> ue[NEN];
@@ -582,10 +587,9 @@ __global__ __launch_bounds__(BS) void k_moduli(const dfea_desc* __restrict__ tab
     int l[NEN];
     int e;
     const bool has_el = thread_element<NEN>(D, ch, t, e, l);
-    const bool core = t < ch.ne;                  // halo threads add to the diagonal but store no moduli
     V2<double> fe[NEN];
     if (has_el) {
-        const int ne = D.n_el;
+        const int ne = D.n_cols;
         const double th = D.theta[e];
         const double mu = D.mu0 * th, lam = D.lam0 * th;
         CT* Cout = std::is_same<CT, double>::value ? (CT*)D.C64 : (CT*)D.C32;
@@ -608,10 +612,8 @@ __global__ __launch_bounds__(BS) void k_moduli(const dfea_desc* __restrict__ tab
             const Kin k = kinematics(F00, F01, F10, F11);
             double c[10];
             moduli10(k, mu, lam, D.w[(size_t)q * ne + e], c);
-            if (core) {
 #pragma unroll
-                for (int kk = 0; kk < 10; ++kk) Cout[(size_t)(kk * NQ + q) * ne + e] = (CT)c[kk];
-            }
+            for (int kk = 0; kk < 10; ++kk) Cout[(size_t)(kk * NQ + q) * ne + e] = (CT)c[kk];   // halo: copy column
             // K^e_{(a,i),(a,i)} += sum_{j,l} C[2i+j][2i+l] dN[a][j] dN[a][l]
 #pragma unroll
             for (int a = 0; a < NEN; ++a) {
@@ -690,12 +692,12 @@ __global__ __launch_bounds__(BS) void k_adjoint(const dfea_desc* __restrict__ ta
     }
     __syncthreads();
     if (t >= ch.ne) return;                       // per-element output: core elements only
-    const int ne = D.n_el;
+    const int ne = D.n_cols;
     const int e = ch.e0 + t;
     V2<double> ue[NEN], le[NEN];
 #pragma unroll
     for (int a = 0; a < NEN; ++a) {
-        const int la = D.lelem[a * ne + e];
+        const int la = D.lelem[a * D.n_el + e];
         ue[a] = L.v[la];
         le[a] = L.acc[la];
     }
@@ -1012,7 +1014,8 @@ int dfea_desc_offset(int field) {
         (int)offsetof(dfea_desc, n_chunks), (int)offsetof(dfea_desc, n_lnodes), (int)offsetof(dfea_desc, n_shared),
         (int)offsetof(dfea_desc, n_part), (int)offsetof(dfea_desc, n_el), (int)offsetof(dfea_desc, n_nodes),
         (int)offsetof(dfea_desc, nen), (int)offsetof(dfea_desc, nq), (int)offsetof(dfea_desc, precision),
-        (int)offsetof(dfea_desc, n_finblk), (int)offsetof(dfea_desc, n_vblk), (int)offsetof(dfea_desc, n_halo)};
+        (int)offsetof(dfea_desc, n_finblk), (int)offsetof(dfea_desc, n_vblk), (int)offsetof(dfea_desc, n_halo),
+        (int)offsetof(dfea_desc, n_cols), (int)offsetof(dfea_desc, pad1)};
     if (field < 0 || field >= (int)(sizeof off / sizeof off[0])) return -1;
     return off[field];
 }

@@ -40,6 +40,7 @@ class Desc(ctypes.Structure):
         ("n_part", ctypes.c_int32), ("n_el", ctypes.c_int32), ("n_nodes", ctypes.c_int32),
         ("nen", ctypes.c_int32), ("nq", ctypes.c_int32), ("precision", ctypes.c_int32),
         ("n_finblk", ctypes.c_int32), ("n_vblk", ctypes.c_int32), ("n_halo", ctypes.c_int32),
+        ("n_cols", ctypes.c_int32), ("pad1", ctypes.c_int32),
     ]
 
 

@@ -84,9 +84,13 @@ class HipMesh:
         self.t_dNdxi = torch.from_numpy(reference_gradient_table(self.element_type, GP)).contiguous().to(dv)
         self.t_gw = torch.from_numpy(GW).to(dv)
         self.t_nfix = torch.from_numpy(p.nfix).to(dv)
-        self.dN64 = torch.empty(self.nq * self.nen * 2, self.n_el, dtype=F64, device=dv)
-        self.dN32 = torch.empty(self.nq * self.nen * 2, self.n_el, dtype=F32, device=dv)
-        self.w = torch.empty(self.nq, self.n_el, dtype=F64, device=dv)
+        # per-element planes: n_el columns + one copy column per halo entry
+        self.n_cols = self.n_el + len(p.hel)
+        self.dN64 = torch.empty(self.nq * self.nen * 2, self.n_cols, dtype=F64, device=dv)
+        self.dN32 = torch.empty(self.nq * self.nen * 2, self.n_cols, dtype=F32, device=dv)
+        self.w = torch.empty(self.nq, self.n_cols, dtype=F64, device=dv)
+        # original element id of every plane column (core columns, then halo copies)
+        self.col_elem = torch.from_numpy(np.r_[p.perm, p.perm[p.hel]].astype(np.int64)).to(dv)
         self.n_vblk = self.lib.dfea_vblk(self.n_dof)
         self.n_finblk = self.lib.dfea_finblk(p.n_shared)
         self.red_size = self.lib.dfea_red_size(p.n_chunks, self.n_finblk, self.n_vblk)
@@ -107,6 +111,7 @@ class HipMesh:
         d.chunks, d.lnode, d.lelem = self.t_chunks.data_ptr(), self.t_lnode.data_ptr(), self.t_lelem.data_ptr()
         d.sh_node, d.sh_off, d.pslot = self.t_sh_node.data_ptr(), self.t_sh_off.data_ptr(), self.t_pslot.data_ptr()
         d.hel, d.hlelem, d.n_halo = self.t_hel.data_ptr(), self.t_hlelem.data_ptr(), len(p.hel)
+        d.n_cols = self.n_cols
         d.conn, d.perm, d.nodes = self.t_conn.data_ptr(), self.t_perm.data_ptr(), self.t_nodes.data_ptr()
         d.dNdxi, d.gw, d.nfix = self.t_dNdxi.data_ptr(), self.t_gw.data_ptr(), self.t_nfix.data_ptr()
         d.dN64, d.dN32, d.w = self.dN64.data_ptr(), self.dN32.data_ptr(), self.w.data_ptr()
@@ -121,8 +126,8 @@ class HipMesh:
 
     # -- element-order helpers ---------------------------------------------------------------------------
     def to_internal(self, per_element):
-        """Per-element values (original order) -> internal chunk order."""
-        return per_element.to(self.device, F64)[self.perm].contiguous()
+        """Per-element values (original order) -> plane columns (internal order + halo copies)."""
+        return per_element.to(self.device, F64)[self.col_elem].contiguous()
 
 
 class HipSystem:
@@ -176,7 +181,7 @@ class HipSystem:
     def _ensure_moduli_buffers(self):
         if self.C is None:
             m = self.mesh
-            self.C = torch.empty(10 * m.nq, m.n_el, dtype=self.vdt, device=m.device)
+            self.C = torch.empty(10 * m.nq, m.n_cols, dtype=self.vdt, device=m.device)
             self.diag = torch.empty(m.n_dof, dtype=self.vdt, device=m.device)
             self.release_graphs()             # graphs hold the previous table's address
             self.desc = self._make_desc()

@@ -13,9 +13,12 @@
  *     simply {0, 0}, {0, 1}, ... ; for a batch, the concatenation over solves.
  *
  * Data layout in HBM (internal element order = chunk order, see dfea_chunk)
- *   dN[(q*nen + a)*2 + j][e]   shape-function gradients dN_a/dX_j at Gauss point q (planes of n_el)
- *   w[q][e]                    Gauss weight * det(dX/dxi)
- *   C[k*nq + q][e]             the 10 unique symmetric tangent moduli, k over
+ *   Per-element arrays are planes of n_cols = n_el + n_halo columns: column e < n_el is element e, column
+ *   n_el + h holds a copy for halo entry h (halo plans), so halo threads read coalesced like core threads.
+ *   dN[(q*nen + a)*2 + j][col] shape-function gradients dN_a/dX_j at Gauss point q
+ *   w[q][col]                  Gauss weight * det(dX/dxi)
+ *   theta[col]                 design factor
+ *   C[k*nq + q][col]           the 10 unique symmetric tangent moduli, k over
  *                              (00,01,02,03,11,12,13,22,23,33) of the flat index m = 2i+j,
  *                              already multiplied by w (`src/difffea/operators.py:316-342`)
  *   vectors                    node-major dofs 2a+i, as the reference (`boundary_conditions.py:171-179`)
@@ -93,7 +96,7 @@ typedef struct dfea_desc {
     double* dN64;
     float* dN32;
     double* w;
-    double* theta;              /* [n_el] internal order */
+    double* theta;              /* [n_cols] internal order (+ halo copies) */
     double* C64;
     float* C32;
     double* diag64;
@@ -108,6 +111,7 @@ typedef struct dfea_desc {
     int32_t n_chunks, n_lnodes, n_shared, n_part;
     int32_t n_el, n_nodes, nen, nq;
     int32_t precision, n_finblk, n_vblk, n_halo;
+    int32_t n_cols, pad1;      /* plane length n_el + n_halo */
 } dfea_desc;
 
 /* ---- library ---------------------------------------------------------------------------------------- */
