@@ -288,10 +288,11 @@ __device__ __forceinline__ void pcg_dir_buffers(const dfea_desc& D, const T*& p_
     p_wr = static_cast<T*>(D.vec[odd ? DFEA_P : DFEA_P2]);
 }
 
-template <int NEN, int NQ, int BS, typename T, bool PCG>
-__global__ __launch_bounds__(BS) void k_apply(const dfea_desc* __restrict__ tab, const int* __restrict__ items,
-                                             const T* __restrict__ vin, T* __restrict__ vout) {
-    constexpr int PF = NEN == 4 ? NQ : 2;
+// PF: Gauss points whose data is loaded before the gather barrier; MINW: minimum waves per SIMD the
+// register allocation must allow (1 = no constraint)
+template <int NEN, int NQ, int BS, typename T, bool PCG, int PF = (NEN == 4 ? NQ : 2), int MINW = 1>
+__global__ __launch_bounds__(BS, MINW) void k_apply(const dfea_desc* __restrict__ tab, const int* __restrict__ items,
+                                                   const T* __restrict__ vin, T* __restrict__ vout) {
     __shared__ ChunkLDS<T> L;
     const Item it = load_item(items);
     const dfea_desc& D = tab[it.s];

@@ -43,7 +43,7 @@ def _items(s, n, device):
 class HipMesh:
     """Mesh plan + geometry on the device.  ``params``: the reference's params contract."""
 
-    def __init__(self, params, device=None):
+    def __init__(self, params, device=None, plan_mode="auto"):
         self.lib = _lib.load()
         device = torch.device(device or "cuda")
         if device.type != "cuda":
@@ -61,7 +61,7 @@ class HipMesh:
             raise ValueError(f"the hip kernels support quad4 with 2x2 and quad9 with 3x3 Gauss points, "
                              f"got nen={self.nen}, n_q={self.nq}")
         cij = params.get("cell_ij")
-        self.plan = p = MeshPlan(nodes.numpy(), conn.numpy(), fixed.numpy(),
+        self.plan = p = MeshPlan(nodes.numpy(), conn.numpy(), fixed.numpy(), mode=plan_mode,
                                  cell_ij=None if cij is None else cij.detach().cpu().numpy())
         if p.validate() > self.lib.dfea_chunk_threads(self.nen):
             raise ValueError("chunk larger than the kernel's workgroup")
