@@ -29,7 +29,8 @@ namespace {
 
 constexpr int CHUNK_Q4 = 256;     // core elements per chunk for quad4
 constexpr int CHUNK_Q9 = 64;      // core elements per chunk for quad9
-constexpr int BS_Q4 = 320;        // workgroup size for quad4: core + halo elements (16x16 + 33 on a grid)
+constexpr int BS_Q4 = 256;        // workgroup size for quad4 partials plans
+constexpr int BS_Q4H = 320;       // workgroup size for quad4 halo plans: core + halo elements (16x16 + 33)
 constexpr int BS_Q9 = 64;         // workgroup size for quad9
 constexpr int MAXLN = 640;        // local nodes per chunk (LDS capacity)
 constexpr int VBS = 256;          // workgroup size of the vector kernels
@@ -290,7 +291,7 @@ __device__ __forceinline__ void pcg_dir_buffers(const dfea_desc& D, const T*& p_
 
 // PF: Gauss points whose data is loaded before the gather barrier; MINW: minimum waves per SIMD the
 // register allocation must allow (1 = no constraint)
-template <int NEN, int NQ, int BS, typename T, bool PCG, int PF = (NEN == 4 ? NQ : 2), int MINW = 1>
+template <int NEN, int NQ, int BS, typename T, bool PCG, int PF = 2, int MINW = 1>
 __global__ __launch_bounds__(BS, MINW) void k_apply(const dfea_desc* __restrict__ tab, const int* __restrict__ items,
                                                    const T* __restrict__ vin, T* __restrict__ vout) {
     __shared__ ChunkLDS<T> L;
@@ -971,7 +972,25 @@ int launch(K kern, int grid, int block, void* stream, const char* name, A... arg
     return hip_status(name);
 }
 
-bool bad_nen(int nen) { return nen != 4 && nen != 9; }
+// element kind of a launch: nen, with DFEA_HALO set for halo plans
+enum class Kind { Q4, Q4H, Q9, Bad };
+Kind kind_of(int k) {
+    if (k == 4) return Kind::Q4;
+    if (k == (4 | DFEA_HALO)) return Kind::Q4H;
+    if (k == 9) return Kind::Q9;
+    return Kind::Bad;
+}
+
+// call f.template run<NEN, BS>() for the kind (one instantiation per element type and plan mode)
+template <typename F>
+int dispatch(int kind, F f) {
+    switch (kind_of(kind)) {
+        case Kind::Q4: return f.template run<4, BS_Q4>();
+        case Kind::Q4H: return f.template run<4, BS_Q4H>();
+        case Kind::Q9: return f.template run<9, BS_Q9>();
+        default: return fail(DFEA_EINVAL, "unknown element kind (expected 4, 9 or 4 | DFEA_HALO)");
+    }
+}
 
 template <int NEN, int BS, typename T>
 int pcg_iter(const dfea_desc* table, const int32_t* items, int n_items, const int32_t* fin_items,
@@ -989,6 +1008,66 @@ int pcg_iter(const dfea_desc* table, const int32_t* items, int n_items, const in
     return 0;
 }
 
+
+// launchers of the C ABI, one instantiation per element kind
+struct Launch_geometry {
+    const dfea_desc* t; const int32_t* it; int n; void* st;
+    template <int NEN, int BS> int run() {
+        return launch(k_geometry<NEN, NEN, BS>, n, BS, st, "k_geometry", t, it);
+    }
+};
+
+struct Launch_moduli {
+    const dfea_desc* t; const int32_t* it; int n; const int32_t* fi; int nf; const double* u; int prec; void* st;
+    template <int NEN, int BS> int run() {
+        int rc = prec == DFEA_FP64 ? launch(k_moduli<NEN, NEN, BS, double>, n, BS, st, "k_moduli", t, it, u)
+                                   : launch(k_moduli<NEN, NEN, BS, float>, n, BS, st, "k_moduli", t, it, u);
+        if (rc) return rc;
+        return prec == DFEA_FP64 ? launch(k_moduli_fin<double>, nf, FBS, st, "k_moduli_fin", t, fi)
+                                 : launch(k_moduli_fin<float>, nf, FBS, st, "k_moduli_fin", t, fi);
+    }
+};
+
+struct Launch_residual {
+    const dfea_desc* t; const int32_t* it; int n; const int32_t* fi; int nf; const double* u; double* R;
+    double lf; void* st;
+    template <int NEN, int BS> int run() {
+        int rc = launch(k_residual<NEN, NEN, BS>, n, BS, st, "k_residual", t, it, u, R, lf);
+        if (rc) return rc;
+        return launch(k_residual_fin, nf, FBS, st, "k_residual_fin", t, fi, R, lf);
+    }
+};
+
+struct Launch_apply {
+    const dfea_desc* t; const int32_t* it; int n; const int32_t* fi; int nf; const void* v; void* y; int prec;
+    void* st;
+    template <typename T, int NEN, int BS> int go() {
+        const T* vv = static_cast<const T*>(v);
+        T* yy = static_cast<T*>(y);
+        int rc = launch(k_apply<NEN, NEN, BS, T, false>, n, BS, st, "k_apply", t, it, vv, yy);
+        if (rc) return rc;
+        return launch(k_apply_fin<T, false>, nf, FBS, st, "k_apply_fin", t, fi, vv, yy);
+    }
+    template <int NEN, int BS> int run() {
+        return prec == DFEA_FP64 ? go<double, NEN, BS>() : go<float, NEN, BS>();
+    }
+};
+
+struct Launch_pcg_iterate {
+    const dfea_desc* t; const int32_t* it; int n; const int32_t* fi; int nf; const int32_t* vi; int nv;
+    const int32_t* si; int ns; int k; int prec; void* st;
+    template <int NEN, int BS> int run() {
+        return prec == DFEA_FP64 ? pcg_iter<NEN, BS, double>(t, it, n, fi, nf, vi, nv, si, ns, k, st)
+                                 : pcg_iter<NEN, BS, float>(t, it, n, fi, nf, vi, nv, si, ns, k, st);
+    }
+};
+
+struct Launch_adjoint {
+    const dfea_desc* t; const int32_t* it; int n; const double* u; const double* lam; double* g; void* st;
+    template <int NEN, int BS> int run() {
+        return launch(k_adjoint<NEN, NEN, BS>, n, BS, st, "k_adjoint", t, it, u, lam, g);
+    }
+};
 
 }  // namespace
 
@@ -1024,61 +1103,40 @@ int dfea_desc_offset(int field) {
 int dfea_red_size(int n_chunks, int n_finblk, int n_vblk) { return n_chunks + n_finblk + 3 * n_vblk; }
 int dfea_vblk(int n_dof) { return (n_dof + VBLK - 1) / VBLK; }
 int dfea_finblk(int n_shared) { return (n_shared + FBS - 1) / FBS; }
-int dfea_chunk_elems(int nen) { return nen == 4 ? CHUNK_Q4 : nen == 9 ? CHUNK_Q9 : -1; }
+int dfea_chunk_elems(int nen) { return (nen & ~DFEA_HALO) == 4 ? CHUNK_Q4 : nen == 9 ? CHUNK_Q9 : -1; }
 int dfea_chunk_nodes(void) { return MAXLN; }
-int dfea_chunk_threads(int nen) { return nen == 4 ? BS_Q4 : nen == 9 ? BS_Q9 : -1; }
+int dfea_chunk_threads(int nen) {
+    switch (kind_of(nen)) {
+        case Kind::Q4: return BS_Q4;
+        case Kind::Q4H: return BS_Q4H;
+        case Kind::Q9: return BS_Q9;
+        default: return -1;
+    }
+}
 
 int dfea_geometry(const dfea_desc* table, const int32_t* items, int n_items, int nen, void* stream) {
-    if (!table || !items || bad_nen(nen)) return fail(DFEA_EINVAL, "dfea_geometry: bad argument");
-    if (nen == 4) return launch(k_geometry<4, 4, BS_Q4>, n_items, BS_Q4, stream, "k_geometry", table, items);
-    return launch(k_geometry<9, 9, BS_Q9>, n_items, BS_Q9, stream, "k_geometry", table, items);
+    if (!table || !items) return fail(DFEA_EINVAL, "dfea_geometry: bad argument");
+    return dispatch(nen, Launch_geometry{table, items, n_items, stream});
 }
 
 int dfea_moduli(const dfea_desc* table, const int32_t* items, int n_items, const int32_t* fin_items,
                 int n_fin_items, const double* u, int nen, int precision, void* stream) {
-    if (!table || !items || bad_nen(nen) || (precision != DFEA_FP64 && precision != DFEA_FP32))
+    if (!table || !items || (precision != DFEA_FP64 && precision != DFEA_FP32))
         return fail(DFEA_EINVAL, "dfea_moduli: bad argument");
-    int rc;
-    if (precision == DFEA_FP64) {
-        rc = nen == 4 ? launch(k_moduli<4, 4, BS_Q4, double>, n_items, BS_Q4, stream, "k_moduli", table, items, u)
-                      : launch(k_moduli<9, 9, BS_Q9, double>, n_items, BS_Q9, stream, "k_moduli", table, items, u);
-        if (rc) return rc;
-        return launch(k_moduli_fin<double>, n_fin_items, FBS, stream, "k_moduli_fin", table, fin_items);
-    }
-    rc = nen == 4 ? launch(k_moduli<4, 4, BS_Q4, float>, n_items, BS_Q4, stream, "k_moduli", table, items, u)
-                  : launch(k_moduli<9, 9, BS_Q9, float>, n_items, BS_Q9, stream, "k_moduli", table, items, u);
-    if (rc) return rc;
-    return launch(k_moduli_fin<float>, n_fin_items, FBS, stream, "k_moduli_fin", table, fin_items);
+    return dispatch(nen, Launch_moduli{table, items, n_items, fin_items, n_fin_items, u, precision, stream});
 }
 
 int dfea_residual(const dfea_desc* table, const int32_t* items, int n_items, const int32_t* fin_items,
                   int n_fin_items, const double* u, double* R, double lf, int nen, void* stream) {
-    if (!table || !items || bad_nen(nen)) return fail(DFEA_EINVAL, "dfea_residual: bad argument");
-    int rc = nen == 4 ? launch(k_residual<4, 4, BS_Q4>, n_items, BS_Q4, stream, "k_residual", table, items, u, R, lf)
-                      : launch(k_residual<9, 9, BS_Q9>, n_items, BS_Q9, stream, "k_residual", table, items, u, R, lf);
-    if (rc) return rc;
-    return launch(k_residual_fin, n_fin_items, FBS, stream, "k_residual_fin", table, fin_items, R, lf);
+    if (!table || !items) return fail(DFEA_EINVAL, "dfea_residual: bad argument");
+    return dispatch(nen, Launch_residual{table, items, n_items, fin_items, n_fin_items, u, R, lf, stream});
 }
 
 int dfea_apply(const dfea_desc* table, const int32_t* items, int n_items, const int32_t* fin_items,
                int n_fin_items, const void* v, void* y, int nen, int precision, void* stream) {
-    if (!table || !items || bad_nen(nen) || (precision != DFEA_FP64 && precision != DFEA_FP32))
+    if (!table || !items || (precision != DFEA_FP64 && precision != DFEA_FP32))
         return fail(DFEA_EINVAL, "dfea_apply: bad argument");
-    int rc;
-    if (precision == DFEA_FP64) {
-        const double* vv = static_cast<const double*>(v);
-        double* yy = static_cast<double*>(y);
-        rc = nen == 4 ? launch(k_apply<4, 4, BS_Q4, double, false>, n_items, BS_Q4, stream, "k_apply", table, items, vv, yy)
-                      : launch(k_apply<9, 9, BS_Q9, double, false>, n_items, BS_Q9, stream, "k_apply", table, items, vv, yy);
-        if (rc) return rc;
-        return launch(k_apply_fin<double, false>, n_fin_items, FBS, stream, "k_apply_fin", table, fin_items, vv, yy);
-    }
-    const float* vv = static_cast<const float*>(v);
-    float* yy = static_cast<float*>(y);
-    rc = nen == 4 ? launch(k_apply<4, 4, BS_Q4, float, false>, n_items, BS_Q4, stream, "k_apply", table, items, vv, yy)
-                  : launch(k_apply<9, 9, BS_Q9, float, false>, n_items, BS_Q9, stream, "k_apply", table, items, vv, yy);
-    if (rc) return rc;
-    return launch(k_apply_fin<float, false>, n_fin_items, FBS, stream, "k_apply_fin", table, fin_items, vv, yy);
+    return dispatch(nen, Launch_apply{table, items, n_items, fin_items, n_fin_items, v, y, precision, stream});
 }
 
 int dfea_dot(const dfea_desc* table, const int32_t* items, int n_items, const int32_t* solve_items, int n_solves,
@@ -1113,17 +1171,11 @@ int dfea_pcg_init(const dfea_desc* table, const int32_t* vitems, int n_vitems, c
 int dfea_pcg_iterate(const dfea_desc* table, const int32_t* items, int n_items, const int32_t* fin_items,
                      int n_fin_items, const int32_t* vitems, int n_vitems, const int32_t* solve_items, int n_solves,
                      int n_iter, int nen, int precision, void* stream) {
-    if (!table || !items || !vitems || !solve_items || bad_nen(nen) || n_iter < 0)
+    if (!table || !items || !vitems || !solve_items || n_iter < 0 ||
+        (precision != DFEA_FP64 && precision != DFEA_FP32))
         return fail(DFEA_EINVAL, "dfea_pcg_iterate: bad argument");
-    if (precision == DFEA_FP64)
-        return nen == 4 ? pcg_iter<4, BS_Q4, double>(table, items, n_items, fin_items, n_fin_items, vitems, n_vitems,
-                                                        solve_items, n_solves, n_iter, stream)
-                        : pcg_iter<9, BS_Q9, double>(table, items, n_items, fin_items, n_fin_items, vitems, n_vitems,
-                                                        solve_items, n_solves, n_iter, stream);
-    return nen == 4 ? pcg_iter<4, BS_Q4, float>(table, items, n_items, fin_items, n_fin_items, vitems, n_vitems,
-                                                   solve_items, n_solves, n_iter, stream)
-                    : pcg_iter<9, BS_Q9, float>(table, items, n_items, fin_items, n_fin_items, vitems, n_vitems,
-                                                   solve_items, n_solves, n_iter, stream);
+    return dispatch(nen, Launch_pcg_iterate{table, items, n_items, fin_items, n_fin_items, vitems, n_vitems, solve_items, n_solves,
+                           n_iter, precision, stream});
 }
 
 int dfea_pcg_graph_create(const dfea_desc* table, const int32_t* items, int n_items, const int32_t* fin_items,
@@ -1173,9 +1225,8 @@ int dfea_graph_destroy(void* graph) {
 
 int dfea_adjoint_grad(const dfea_desc* table, const int32_t* items, int n_items, const double* u, const double* lam,
                       double* g, int nen, void* stream) {
-    if (!table || !items || !u || !lam || !g || bad_nen(nen)) return fail(DFEA_EINVAL, "dfea_adjoint_grad: bad argument");
-    if (nen == 4) return launch(k_adjoint<4, 4, BS_Q4>, n_items, BS_Q4, stream, "k_adjoint", table, items, u, lam, g);
-    return launch(k_adjoint<9, 9, BS_Q9>, n_items, BS_Q9, stream, "k_adjoint", table, items, u, lam, g);
+    if (!table || !items || !u || !lam || !g) return fail(DFEA_EINVAL, "dfea_adjoint_grad: bad argument");
+    return dispatch(nen, Launch_adjoint{table, items, n_items, u, lam, g, stream});
 }
 
 }  // extern "C"

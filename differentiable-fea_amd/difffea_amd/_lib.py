@@ -16,6 +16,7 @@ LIB_PATH = os.path.join(_HERE, LIB_NAME)
 FP64, FP32 = 0, 1
 MAXCOL = 16
 FIXED_X, FIXED_Y, NODE_MASK = 0x40000000, 0x80000000, 0x3FFFFFFF
+HALO = 0x100          # element-kind flag of halo plans
 VEC = dict(x=0, r=1, z=2, p=3, Ap=4, b=5, u=6, R=7, uacc=8, uD=9, p2=10)
 NVEC = 11
 SC = dict(rz=0, pAp=1, alpha=2, beta=3, rr=4, bb=5, rtol=6, status=7, k=8, maxit=9, iters=10, norm=11, lf=12,

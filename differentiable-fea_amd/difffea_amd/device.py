@@ -63,7 +63,9 @@ class HipMesh:
         cij = params.get("cell_ij")
         self.plan = p = MeshPlan(nodes.numpy(), conn.numpy(), fixed.numpy(), mode=plan_mode,
                                  cell_ij=None if cij is None else cij.detach().cpu().numpy())
-        if p.validate() > self.lib.dfea_chunk_threads(self.nen):
+        # element kind of every launch: nen, flagged for halo plans
+        self.kind = self.nen | (_lib.HALO if p.mode == "halo" else 0)
+        if p.validate() > self.lib.dfea_chunk_threads(self.kind):
             raise ValueError("chunk larger than the kernel's workgroup")
         self.n_el, self.n_nodes = p.n_el, p.n_nodes
         self.n_dof = 2 * p.n_nodes
@@ -102,7 +104,7 @@ class HipMesh:
         # geometry, once
         d = self.fill_desc(_lib.Desc())
         tab = self.upload(d)
-        _lib.check(self.lib.dfea_geometry(_ptr(tab), _ptr(self.items), p.n_chunks, self.nen, _stream()),
+        _lib.check(self.lib.dfea_geometry(_ptr(tab), _ptr(self.items), p.n_chunks, self.kind, _stream()),
                    "dfea_geometry")
         self._geom_table = tab
 
@@ -192,14 +194,14 @@ class HipSystem:
         m = self.mesh
         u = u.to(F64).contiguous()
         _lib.check(self.lib.dfea_moduli(_ptr(self.table), _ptr(m.items), m.plan.n_chunks, _ptr(m.fin_items),
-                                        m.n_finblk, _ptr(u), m.nen, self.prec_id, _stream()), "dfea_moduli")
+                                        m.n_finblk, _ptr(u), m.kind, self.prec_id, _stream()), "dfea_moduli")
 
     def residual(self, u, out=None, lf=1.0):
         m = self.mesh
         u = u.to(F64).contiguous()
         out = torch.empty_like(u) if out is None else out
         _lib.check(self.lib.dfea_residual(_ptr(self.table), _ptr(m.items), m.plan.n_chunks, _ptr(m.fin_items),
-                                          m.n_finblk, _ptr(u), _ptr(out), float(lf), m.nen, _stream()),
+                                          m.n_finblk, _ptr(u), _ptr(out), float(lf), m.kind, _stream()),
                    "dfea_residual")
         return out
 
@@ -209,7 +211,7 @@ class HipSystem:
             v = v.to(self.vdt).contiguous()
         out = torch.empty_like(v) if out is None else out
         _lib.check(self.lib.dfea_apply(_ptr(self.table), _ptr(m.items), m.plan.n_chunks, _ptr(m.fin_items),
-                                       m.n_finblk, _ptr(v), _ptr(out), m.nen, self.prec_id, _stream()),
+                                       m.n_finblk, _ptr(v), _ptr(out), m.kind, self.prec_id, _stream()),
                    "dfea_apply")
         return out
 
@@ -240,7 +242,7 @@ class HipSystem:
         g = torch.empty(m.n_el, dtype=F64, device=m.device)
         _lib.check(self.lib.dfea_adjoint_grad(_ptr(self.table), _ptr(m.items), m.plan.n_chunks,
                                               _ptr(u.to(F64).contiguous()), _ptr(lam.to(F64).contiguous()), _ptr(g),
-                                              m.nen, _stream()), "dfea_adjoint_grad")
+                                              m.kind, _stream()), "dfea_adjoint_grad")
         return g
 
     # -- fused PCG ---------------------------------------------------------------------------------------------
@@ -266,7 +268,7 @@ class HipSystem:
             h = ctypes.c_void_p()
             _lib.check(self.lib.dfea_pcg_graph_create(
                 _ptr(self.table), _ptr(m.items), m.plan.n_chunks, _ptr(m.fin_items), m.n_finblk, _ptr(m.vitems),
-                m.n_vblk, _ptr(m.solve_items), 1, int(n_iter), m.nen, self.prec_id, ctypes.byref(h)),
+                m.n_vblk, _ptr(m.solve_items), 1, int(n_iter), m.kind, self.prec_id, ctypes.byref(h)),
                 "dfea_pcg_graph_create")
             self._graphs[n_iter] = g = h
         return g
@@ -280,7 +282,7 @@ class HipSystem:
             return
         _lib.check(self.lib.dfea_pcg_iterate(_ptr(self.table), _ptr(m.items), m.plan.n_chunks, _ptr(m.fin_items),
                                              m.n_finblk, _ptr(m.vitems), m.n_vblk, _ptr(m.solve_items), 1,
-                                             int(n_iter), m.nen, self.prec_id, _stream()), "dfea_pcg_iterate")
+                                             int(n_iter), m.kind, self.prec_id, _stream()), "dfea_pcg_iterate")
 
     def release_graphs(self):
         for g in self._graphs.values():

@@ -16,13 +16,14 @@ Built once per (mesh, Dirichlet mask) on the host with vectorised numpy; uploade
    then chunk: the partials of shared node i are slots ``sh_off[i]..sh_off[i+1]-1``, summed in that order.
    ``pslot`` maps a chunk's shared entries (chunk order) to their slots.
 
-Halo mode (default for quad4).  Every node is owned by exactly one chunk: the chunk of its adjacent element
+Halo mode (quad4, on request).  Every node is owned by exactly one chunk: the chunk of its adjacent element
 latest in Morton order (on a grid: the element up and to the right, so a 16x16 tile owns a 16x16 block of
 nodes).  A chunk evaluates its core elements plus the "halo" elements of other chunks that touch its owned
 nodes (33 per interior tile), so each owned node is completed inside one workgroup: no shared nodes, no
-partials, no second kernel, at the price of ~13 % more element reads.  Halo elements are listed per chunk
+partials, no second kernel, at the price of ~13 % more element reads (on MI355X the partials plan with its
+finalize kernel measured faster, so it is the default).  Halo elements are listed per chunk
 (``hel``, with their local indices in ``hlelem``), sorted by colour like the core elements.  Partials mode
-(default for quad9, and the fallback when a chunk would exceed the workgroup) is described in 4.
+(the default, and the fallback when a halo chunk would exceed the workgroup) is described in 4.
 
 Nothing here depends on which other solves run in the same batch, so results are batch-independent.
 """
@@ -147,8 +148,9 @@ class MeshPlan:
         E = chunk_elems or lib.dfea_chunk_elems(nen)
         MAXLN = max_nodes or lib.dfea_chunk_nodes()
         BS = lib.dfea_chunk_threads(nen)
+        BS_halo = lib.dfea_chunk_threads(nen | _lib.HALO)
         if mode == "auto":
-            mode = "halo" if nen == 4 else "partials"
+            mode = "partials"          # measured faster than halo plans on MI355X
         if mode not in ("halo", "partials"):
             raise ValueError(f"plan mode must be 'auto', 'halo' or 'partials', got {mode!r}")
         if E <= 0:
@@ -187,7 +189,7 @@ class MeshPlan:
         self.conn_int = np.ascontiguousarray(conn[perm].T).astype(np.int32)     # [nen][n_el]
         self.n_chunks = n_chunks
         self.structured = structured
-        if mode == "halo" and self._build_halo(conn, perm, colour, E, BS, MAXLN, bits):
+        if mode == "halo" and BS_halo > 0 and self._build_halo(conn, perm, colour, E, BS_halo, MAXLN, bits):
             self.mode = "halo"
             return
         self.mode = "partials"

@@ -87,6 +87,24 @@ def request_problem(req):
     return p, theta
 
 
+LIFT_PROBE = 1e-6
+
+
+def reference_residual_norm(sys_, params, device):
+    """The norm that relative residual tolerances refer to.
+
+    Force-driven problems: ||m f||.  Displacement-driven problems: ||K(0) u_D|| restricted to the free dofs,
+    the linear out-of-balance force of the bare prescribed displacement, evaluated as ||R(d lift)|| / d at
+    d = 1e-6.  (The residual of the full lift itself can be NaN: moving only the boundary nodes by the whole
+    prescribed displacement inverts the boundary elements of fine meshes.)
+    """
+    uD = params["u_D"].to(device, torch.float64)
+    if not bool(torch.any(uD != 0)):
+        return float(torch.norm((params["f"] * params["m"]).to(device)))
+    R = sys_.residual(sys_.lift(torch.zeros_like(uD), uD, LIFT_PROBE), lf=LIFT_PROBE)
+    return float(torch.norm(R)) / LIFT_PROBE
+
+
 def footprint_bytes(element_type, n, precision):
     """HBM bytes a request holds while admitted (geometry, moduli, vectors, plan), an upper estimate."""
     nen, nq = (4, 4) if element_type == "quad4" else (9, 9)
@@ -224,14 +242,7 @@ class Server:
         pl = mesh.plan
         job.items = {"chunk": self._items(slot, pl.n_chunks), "fin": self._items(slot, mesh.n_finblk),
                      "vec": self._items(slot, mesh.n_vblk), "solve": self._items(slot, 1)}
-        # tolerance: relative to the residual of the state lifted to the full load (f-loaded: ||m f||)
-        if req.adjoint:
-            ref = float(torch.norm(params["f"] * params["m"]))
-        else:
-            uD = params["u_D"].to(self.device)
-            R = sys_.residual(sys_.lift(torch.zeros_like(uD), uD, 1.0), lf=1.0)
-            ref = float(torch.norm(R))
-        job.tol = self.rel_tol * ref
+        job.tol = self.rel_tol * reference_residual_norm(sys_, params, self.device)
         job.saved = torch.zeros_like(sys_.vec["uacc"])
         job.gen = solve_ops(job, job.tol, cg_max_iter=self.cg_max_iter)
         job.op = next(job.gen)
@@ -293,8 +304,8 @@ class Server:
         if js:
             idx = torch.tensor([j.slot for j in js])
             self.sc[idx, _lib.SC["lf"]] = torch.tensor([float(j.op[1]) for j in js], dtype=torch.float64).to(self.device)
-            for nen in (4, 9):
-                sub = [j for j in js if j.mesh.nen == nen]
+            for nen in (4, 9, 4 | _lib.HALO):
+                sub = [j for j in js if j.mesh.kind == nen]
                 it, n = self._dev_items(sub, "chunk")
                 if not n:
                     continue
@@ -305,8 +316,8 @@ class Server:
             _lib.check(lib.dfea_dot(T, _ptr(it), n, _ptr(si), ns, None, None, st), "dfea_dot")
         # moduli + diagonal at vec[u]
         js = by.get("moduli", [])
-        for nen in (4, 9):
-            sub = [j for j in js if j.mesh.nen == nen]
+        for nen in (4, 9, 4 | _lib.HALO):
+            sub = [j for j in js if j.mesh.kind == nen]
             it, n = self._dev_items(sub, "chunk")
             if n:
                 fi, nf = self._dev_items(sub, "fin")
@@ -314,6 +325,8 @@ class Server:
         # PCG start (new solves) then a chunk of iterations for every running solve
         js = by.get("pcg", []) + by.get("adjoint", [])
         new = [j for j in js if not getattr(j, "pcg_running", False)]
+        for j in new:
+            j.pcg_rounds = 0
         if new:
             idx = torch.tensor([j.slot for j in new])
             adj_rtol = max(self.adjoint_rtol, MIXED_RTOL_FLOOR) if self.precision == "mixed" else self.adjoint_rtol
@@ -335,16 +348,20 @@ class Server:
             for j in new:
                 j.pcg_running = True
         if js:
-            for nen in (4, 9):
-                sub = [j for j in js if j.mesh.nen == nen]
+            for nen in (4, 9, 4 | _lib.HALO):
+                sub = [j for j in js if j.mesh.kind == nen]
                 if not sub:
                     continue
+                # chunk length: grows while a solve keeps iterating (finished solves skip on the device)
+                chunk = min(self.pcg_chunk << min(max(j.pcg_rounds for j in sub), 3), 64)
+                for j in sub:
+                    j.pcg_rounds += 1
                 it, n = self._dev_items(sub, "chunk")
                 fi, nf = self._dev_items(sub, "fin")
                 vi, nv = self._dev_items(sub, "vec")
                 si, ns = self._dev_items(sub, "solve")
                 _lib.check(lib.dfea_pcg_iterate(T, _ptr(it), n, _ptr(fi), nf, _ptr(vi), nv, _ptr(si), ns,
-                                                self.pcg_chunk, nen, self.prec_id, st), "dfea_pcg_iterate")
+                                                chunk, nen, self.prec_id, st), "dfea_pcg_iterate")
         # one host sync: status, iterations, norms
         sc = self.sc.cpu().numpy()
         finished = []

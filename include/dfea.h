@@ -9,6 +9,7 @@
  *     synchronises the stream except where stated (dfea_pcg_run reads a status block at chunk ends).
  *   - The caller allocates and owns every buffer.  Descriptors (dfea_desc) hold device pointers into those
  *     buffers; a descriptor table in device memory describes a batch of independent solves.
+ *   - Element kind (`nen` arguments): 4 (quad4), 9 (quad9), or 4 | DFEA_HALO for quad4 halo plans.
  *   - Work items: an int32 pair {solve index, block index} per workgroup.  For one solve, the items are
  *     simply {0, 0}, {0, 1}, ... ; for a batch, the concatenation over solves.
  *
@@ -37,6 +38,7 @@ extern "C" {
 #define DFEA_FIXED_X 0x40000000u
 #define DFEA_FIXED_Y 0x80000000u
 #define DFEA_NODE_MASK 0x3fffffffu
+#define DFEA_HALO 0x100        /* element-kind flag: nen | DFEA_HALO selects the halo-plan kernels */
 
 enum { DFEA_FP64 = 0, DFEA_FP32 = 1 };
 
@@ -124,7 +126,7 @@ int dfea_vblk(int n_dof);                               /* dof blocks of the vec
 int dfea_finblk(int n_shared);                          /* blocks of the shared-node kernels */
 int dfea_chunk_elems(int nen);                          /* maximum elements per chunk for this element */
 int dfea_chunk_nodes(void);                             /* maximum local nodes per chunk */
-int dfea_chunk_threads(int nen);                        /* workgroup size: core + halo elements per chunk */
+int dfea_chunk_threads(int kind);                       /* workgroup size of an element kind */
 
 /* ---- once per mesh ---------------------------------------------------------------------------------- */
 /* Geometry cache dN_dX, w_detJ (dN32 too when non-null).

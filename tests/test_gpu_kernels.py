@@ -140,3 +140,27 @@ def test_pcg_graph_replay_equals_direct_launches(et, n):
         torch.cuda.synchronize()
         assert (k1, st1) == (k2, st2)
         assert torch.equal(x1, x2)
+
+
+@pytest.mark.parametrize("n", [37, 64])
+def test_halo_plan_matches_oracle_and_partials_plan(n):
+    """Quad4 halo plans (no shared nodes, halo elements evaluated by the owner chunk): A(v), the Jacobi
+    diagonal, the residual and the fused PCG agree with the oracle and with the default partials plan."""
+    from difffea_amd.device import HipMesh, HipSystem
+    from difffea_amd.operators import _check_material
+    p, u, v, th = _setup("quad4", n, seed=12, nonuniform=True, amp=0.01)
+    mh = HipMesh(p, DEV, plan_mode="halo")
+    assert mh.plan.mode == "halo" and mh.plan.n_shared == 0 and len(mh.plan.hel) > 0
+    sh = HipSystem(mh, th, _check_material(p), p["f"], "fp64")
+    sh.moduli(u.to(DEV))
+    assert rel(sh.apply(v.to(DEV)), fem.make_A_operator_qp(u, th, p)(v)) < 1e-11
+    assert rel(sh.diag, fem.compute_jacobi_diagonal(u, th, p)) < 1e-11
+    assert rel(sh.residual(u.to(DEV), lf=0.3), fem.global_residual(u, th, p, 0.3)) < 1e-11
+    b = (v * p["m"]).to(DEV)
+    x_h, k_h, st_h = sh.pcg_solve(b, rtol=1e-9, max_iter=3000)
+    mp = HipMesh(p, DEV, plan_mode="partials")
+    sp = HipSystem(mp, th, _check_material(p), p["f"], "fp64")
+    sp.moduli(u.to(DEV))
+    x_p, k_p, st_p = sp.pcg_solve(b, rtol=1e-9, max_iter=3000)
+    assert st_h == st_p == 1 and abs(k_h - k_p) <= 1
+    assert rel(x_h, x_p) < 1e-8

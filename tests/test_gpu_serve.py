@@ -45,11 +45,15 @@ def test_served_solution_matches_oracle_and_single_solve_api():
     for req in (reqs[1], reqs[3]):
         p, th = request_problem(req)
         u_srv = recs[req.rid]["u"]
-        # oracle solve with the server's tolerance
-        R1 = fem.global_residual(torch.where(p["fixed"], p["u_D"], torch.zeros_like(p["u_D"])), th,
-                                 fem.precompute_geometry(p), 1.0)
-        tol = 1e-8 * float(torch.norm(R1))
-        u_ref, info = fem.newton_krylov_solve(fem.precompute_geometry(p), th, req.n_load_steps, tol)
+        # oracle solve with the server's tolerance; the server's reference norm is ||K(0) u_D|| (||m f|| for
+        # force loads): check it against the oracle's tangent at u = 0 applied to the lift
+        pg = fem.precompute_geometry(p)
+        tol = recs[req.rid]["tol_abs"]
+        lift = torch.where(p["fixed"], p["u_D"], torch.zeros_like(p["u_D"]))
+        _, Klift = torch.func.jvp(lambda w: fem.global_residual(w, th, pg, 0.0), (torch.zeros_like(lift),), (lift,))
+        ref = float(torch.norm(Klift)) if not req.adjoint else float(torch.norm(p["f"] * p["m"]))
+        assert tol == pytest.approx(1e-8 * ref, rel=1e-5)
+        u_ref, info = fem.newton_krylov_solve(pg, th, req.n_load_steps, tol)
         assert info["converged"]
         assert rel(u_srv, u_ref) < 1e-7          # both converged to 1e-8 relative residual
         # the single-solve API gives the same bits

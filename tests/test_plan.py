@@ -109,9 +109,12 @@ def test_plan_tables_and_emulated_assembly_match_oracle(name, mode):
 def test_structured_meshes_get_square_tiles_and_four_colours():
     plan = _plan(unit_square("quad4", 400))
     st = plan.stats()
+    assert st["mode"] == "partials" and st["n_chunks"] == 625 and st["max_chunk_nodes"] == 289
+    assert st["max_colours"] == 4 and st["max_threads"] == 256
+    halo = _plan(unit_square("quad4", 400), mode="halo").stats()
     # 16x16 core tiles + 33 halo elements; 16x16 owned nodes + the halo's nodes (18x18)
-    assert st["mode"] == "halo" and st["n_chunks"] == 625 and st["max_threads"] == 289
-    assert st["max_chunk_nodes"] == 324 and st["max_colours"] == 4
+    assert halo["mode"] == "halo" and halo["n_chunks"] == 625 and halo["max_threads"] == 289
+    assert halo["max_chunk_nodes"] == 324 and halo["n_shared"] == 0
     plan9 = _plan(unit_square("quad9", 200))
     assert plan9.stats()["mode"] == "partials" and plan9.stats()["max_chunk_nodes"] == 289
 
