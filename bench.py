@@ -70,6 +70,38 @@ def time_apply(p, u, v, theta, precision, steps, warmup):
     return dict(ms=ms, gbs=B / (ms * 1e-3) / 1e9, bytes=B, products_per_s=1e3 / ms, y=y)
 
 
+def time_pcg_iteration(p, u, v, theta, precision, iters=400, chunk=50):
+    """One fused PCG iteration (direction/product kernel + update kernel) on the same mesh and state,
+    replayed from HIP graphs of `chunk` iterations; rtol = 0 keeps the solve running.  Algorithmic bytes per
+    iteration: the A(v) bytes plus 10 dof vectors (z, p, p', A p written / read again, x, r read and
+    written, diag, z written)."""
+    from difffea_amd import _lib
+    from difffea_amd.operators import make_system
+    dev = torch.device("cuda")
+    s = make_system(u.to(dev), theta.to(dev), p, precision)
+    s.moduli(u.to(dev))
+    b = (v * p["m"]).to(dev)
+    s.pcg_start(b, rtol=0.0, max_iter=iters + 2 * chunk + 1)
+    s.pcg_iterate(chunk)                       # warm-up (graph capture + first replay)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    n = (iters // chunk) * chunk
+    e0.record()
+    for _ in range(n // chunk):
+        s.pcg_iterate(chunk)
+    e1.record()
+    torch.cuda.synchronize()
+    st, _, k = s.pcg_status()
+    if st != _lib.PCG_RUNNING or k != n + chunk:
+        raise RuntimeError(f"PCG timing run stopped early (status {st}, k {k})")
+    us = e0.elapsed_time(e1) * 1e3 / n
+    m = s.mesh
+    sz = 8 if precision == "fp64" else 4
+    B = b_alg(m.n_el, m.nen, m.nq, m.n_dof, sz) + 10 * m.n_dof * sz
+    return dict(us_per_iteration=round(us, 3), gbs=round(B / (us * 1e-6) / 1e9, 2), bytes_per_iteration=B,
+                iterations=n)
+
+
 def cpu_baseline(p, u, v, theta, seconds=10.0):
     """The oracle's 'qp' restatement (gather, einsum, batched 4x4 matmul, einsum, index_add_), FP64."""
     from oracle import fem
@@ -193,6 +225,10 @@ def main():
     print(f"# Q4 400x400 fp64 A(v): {r64['ms'] * 1e3:.2f} us, {r64['gbs']:.0f} GB/s", file=sys.stderr, flush=True)
     r32 = time_apply(p, u, v, th, "mixed", args.steps, args.warmup)
     print(f"# Q4 400x400 fp32 A(v): {r32['ms'] * 1e3:.2f} us, {r32['gbs']:.0f} GB/s", file=sys.stderr, flush=True)
+    pcg64 = time_pcg_iteration(p, u, v, th, "fp64")
+    pcg32 = time_pcg_iteration(p, u, v, th, "mixed")
+    print(f"# Q4 400x400 PCG iteration: fp64 {pcg64['us_per_iteration']:.2f} us, "
+          f"fp32 {pcg32['us_per_iteration']:.2f} us", file=sys.stderr, flush=True)
     p9, u9, v9, th9 = problem("quad9", 200)
     q9 = time_apply(p9, u9, v9, th9, "fp64", args.steps, args.warmup)
     print(f"# Q9 200x200 fp64 A(v): {q9['ms'] * 1e3:.2f} us, {q9['gbs']:.0f} GB/s", file=sys.stderr, flush=True)
@@ -215,7 +251,10 @@ def main():
             rate = 0.8 * sat64["solves_per_s"]
             seq = sequential_fp64(args.serve_requests, side=tuple(args.serve_side))
             print(f"# plain FP64 GPU solver (sequential): {seq}", file=sys.stderr, flush=True)
-            srv = {"saturated_fp64": sat64, "plain_fp64_sequential": seq,
+            sat32 = serve_stream(args.serve_requests, "mixed", side=tuple(args.serve_side))
+            print(f"# served mixed (saturated): {sat32}", file=sys.stderr, flush=True)
+            srv = {"saturated_fp64": sat64, "saturated_mixed": sat32, "plain_fp64_sequential": seq,
+                   "saturated_mixed_vs_plain_fp64": round(sat32["solves_per_s"] / seq["solves_per_s"], 3),
                    "fp64": serve_stream(args.serve_requests, "fp64", rate, side=tuple(args.serve_side)),
                    "mixed": serve_stream(args.serve_requests, "mixed", rate, side=tuple(args.serve_side)),
                    "arrival_rate_per_s": rate}
@@ -242,6 +281,7 @@ def main():
                         "bytes_per_step": r32["bytes"]},
         "q9_200x200_fp64": {"us_per_product": round(q9["ms"] * 1e3, 3), "gbs": round(q9["gbs"], 2),
                             "bytes_per_step": q9["bytes"]},
+        "pcg_iteration_fp64": pcg64, "pcg_iteration_mixed": pcg32,
         "cpu_baseline": None if cpu is None else {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")},
         "solves": solves,
     }

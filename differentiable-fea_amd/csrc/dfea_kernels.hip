@@ -199,7 +199,6 @@ __device__ __forceinline__ Item load_item(const int* items) {
     return it;
 }
 
-__device__ __forceinline__ double pcg_status(const dfea_desc& D) { return D.sc[DFEA_SC_STATUS]; }
 
 // ======================================================================================================
 // geometry: dN_dX = dN_dxi J^-1, w = gw det J   (`src/difffea/operators.py:85-120`)
@@ -278,15 +277,67 @@ __device__ __forceinline__ void colour_accumulate(const ChunkView& ch, int t, co
 }
 
 // ---- tangent product  y = m J (m v) + (1 - m) v  (`operators.py:385-412`) ----------------------------------
-// PCG: the direction update p' = z + beta p happens on the gather (p read from the buffer of this iteration's
-// parity, p' written to the other one), and partials of p'.Ap' are produced.
+// PCG: the kernel opens iteration k with the scalar step that closes iteration k-1 (pcg_step); the direction
+// update p' = z + beta p happens on the gather (p read from the buffer of iteration k's parity, p' written
+// to the other one), and partials of p'.Ap' are produced, shared nodes included.
 // The element data of the first PF Gauss points is loaded before the gather barrier, so the HBM stream
 // starts at once instead of after the gather's two dependent round trips.
 template <typename T>
-__device__ __forceinline__ void pcg_dir_buffers(const dfea_desc& D, const T*& p_rd, T*& p_wr) {
-    const bool odd = ((int)D.sc[DFEA_SC_K]) & 1;
+__device__ __forceinline__ void pcg_dir_buffers(double k, const dfea_desc& D, const T*& p_rd, T*& p_wr) {
+    const bool odd = ((int)k) & 1;
     p_rd = static_cast<const T*>(D.vec[odd ? DFEA_P2 : DFEA_P]);
     p_wr = static_cast<T*>(D.vec[odd ? DFEA_P : DFEA_P2]);
+}
+
+// The scalar step between PCG iterations k-1 and k (`newton_krylov.py:78-86`): convergence test on the
+// updated residual, beta = rz' / rz, iteration cap.  Every workgroup of the solve evaluates it from the same
+// partials in the same order, so all take the same decision; workgroup 0 hands the outcome to the update
+// kernel in the DFEA_SC_A_* slots (this kernel reads only the caller-visible slots, which the update kernel
+// writes).
+struct PcgStep {
+    double beta, k;
+    bool running;
+};
+
+template <int BS>
+__device__ PcgStep pcg_step(const dfea_desc& D, int blk, double* s_red, double* s_b) {
+    const double* sc = D.sc;
+    double st = sc[DFEA_SC_STATUS];
+    const double k = sc[DFEA_SC_K];
+    double rz = sc[DFEA_SC_RZ], rr = sc[DFEA_SC_RR], iters = sc[DFEA_SC_ITERS], beta = 0.0;
+    if (st == DFEA_PCG_RUNNING && k > 0.0) {            // consume the partials of update k-1
+        const double* ru = D.red + D.n_chunks + D.n_finblk;
+        double a = 0.0, b = 0.0;
+        for (int i = threadIdx.x; i < D.n_vblk; i += BS) {
+            a += ru[2 * i];
+            b += ru[2 * i + 1];
+        }
+        a = block_sum<BS>(a, s_red);
+        b = block_sum<BS>(b, s_red);
+        const double rz_new = bcast0(a, s_b);
+        rr = bcast0(b, s_b);
+        if (sqrt(rr) <= sc[DFEA_SC_RTOL] * sqrt(sc[DFEA_SC_BB])) {
+            st = DFEA_PCG_CONVERGED;
+            iters = k;
+        } else {
+            beta = rz_new / rz;
+            rz = rz_new;
+        }
+    }
+    if (st == DFEA_PCG_RUNNING && k >= sc[DFEA_SC_MAXIT]) {
+        st = DFEA_PCG_MAXITER;
+        iters = sc[DFEA_SC_MAXIT];
+    }
+    if (blk == 0 && threadIdx.x == 0) {
+        double* w = D.sc;
+        w[DFEA_SC_A_STATUS] = st;
+        w[DFEA_SC_A_K] = k;
+        w[DFEA_SC_A_ITERS] = iters;
+        w[DFEA_SC_A_RZ] = rz;
+        w[DFEA_SC_A_BETA] = beta;
+        w[DFEA_SC_A_RR] = rr;
+    }
+    return PcgStep{beta, k, st == DFEA_PCG_RUNNING};
 }
 
 // PF: Gauss points whose data is loaded before the gather barrier; MINW: minimum waves per SIMD the
@@ -297,16 +348,11 @@ __global__ __launch_bounds__(BS, MINW) void k_apply(const dfea_desc* __restrict_
     __shared__ ChunkLDS<T> L;
     const Item it = load_item(items);
     const dfea_desc& D = tab[it.s];
-    if (PCG && pcg_status(D) != DFEA_PCG_RUNNING) return;
     const ChunkView ch = chunk_view(D.chunks + it.b);
     const int t = threadIdx.x;
     const T* v = PCG ? nullptr : (vin ? vin : static_cast<const T*>(D.vec[DFEA_P]));
     T* y = PCG ? static_cast<T*>(D.vec[DFEA_AP]) : (vout ? vout : static_cast<T*>(D.vec[DFEA_AP]));
-    const T* p_rd = nullptr;
-    T* p_wr = nullptr;
-    if (PCG) pcg_dir_buffers<T>(D, p_rd, p_wr);
     const T* z = static_cast<const T*>(D.vec[DFEA_Z]);
-    const double beta = PCG ? D.sc[DFEA_SC_BETA] : 0.0;
     const int ne = D.n_cols;
     const T* dN = std::is_same<T, double>::value ? (const T*)D.dN64 : (const T*)D.dN32;
     const T* C = std::is_same<T, double>::value ? (const T*)D.C64 : (const T*)D.C32;
@@ -328,6 +374,17 @@ __global__ __launch_bounds__(BS, MINW) void k_apply(const dfea_desc* __restrict_
 #pragma unroll
             for (int k = 0; k < 10; ++k) c[q][k] = C[(size_t)(k * NQ + q) * ne + e];
         }
+    }
+
+    // PCG: scalar step (uniform over the workgroup), direction buffers of iteration k
+    double beta = 0.0;
+    const T* p_rd = nullptr;
+    T* p_wr = nullptr;
+    if (PCG) {
+        const PcgStep ps = pcg_step<BS>(D, it.b, L.red, L.red + 15);
+        if (!ps.running) return;
+        beta = ps.beta;
+        pcg_dir_buffers<T>(ps.k, D, p_rd, p_wr);
     }
 
     // 1. gather
@@ -413,7 +470,22 @@ __global__ __launch_bounds__(BS, MINW) void k_apply(const dfea_desc* __restrict_
             reinterpret_cast<V2<T>*>(y)[n] = out;
             if (PCG) dot += (double)raw.x * (double)out.x + (double)raw.y * (double)out.y;
         } else if (D.n_part) {
-            part[D.pslot[ch.pbase + (k - ch.nown)]] = L.acc[k];
+            const int slot = D.pslot[ch.pbase + (k - ch.nown)];
+            const V2<T> a = L.acc[k];
+            part[slot] = a;
+            if (PCG) {   // p.Ap over a shared node is linear in its partials: p . this chunk's partial
+                const V2<T> raw = L.v[k];
+                const unsigned char b = L.bits[k];
+                if (!(b & 1)) dot += (double)raw.x * (double)a.x;
+                if (!(b & 2)) dot += (double)raw.y * (double)a.y;
+                if (b) {     // fixed components (Ap = p there) once, by the chunk holding the node's first slot
+                    const int n = (int)(D.lnode[ch.k0 + k] & DFEA_NODE_MASK);
+                    if (slot == D.shslot[2 * n]) {
+                        if (b & 1) dot += (double)raw.x * (double)raw.x;
+                        if (b & 2) dot += (double)raw.y * (double)raw.y;
+                    }
+                }
+            }
         }
     }
     if (PCG) {
@@ -422,18 +494,15 @@ __global__ __launch_bounds__(BS, MINW) void k_apply(const dfea_desc* __restrict_
     }
 }
 
-// shared nodes of the tangent product (partials plans)
-template <typename T, bool PCG>
+// shared nodes of the tangent product (partials plans): partials summed in slot order
+template <typename T>
 __global__ __launch_bounds__(FBS) void k_apply_fin(const dfea_desc* __restrict__ tab, const int* __restrict__ items,
                                                   const T* __restrict__ vin, T* __restrict__ vout) {
-    __shared__ double s_red[16];
     const Item it = load_item(items);
     const dfea_desc& D = tab[it.s];
-    if (PCG && pcg_status(D) != DFEA_PCG_RUNNING) return;
     const int i = it.b * FBS + threadIdx.x;
-    const T* v = PCG ? nullptr : (vin ? vin : static_cast<const T*>(D.vec[DFEA_P]));
-    T* y = PCG ? static_cast<T*>(D.vec[DFEA_AP]) : (vout ? vout : static_cast<T*>(D.vec[DFEA_AP]));
-    double dot = 0.0;
+    const T* v = vin ? vin : static_cast<const T*>(D.vec[DFEA_P]);
+    T* y = vout ? vout : static_cast<T*>(D.vec[DFEA_AP]);
     if (i < D.n_shared) {
         const V2<T>* part = reinterpret_cast<const V2<T>*>(D.part);
         V2<T> acc{0, 0};
@@ -444,29 +513,11 @@ __global__ __launch_bounds__(FBS) void k_apply_fin(const dfea_desc* __restrict__
         }
         const uint32_t g = D.sh_node[i];
         const int n = (int)(g & DFEA_NODE_MASK);
-        V2<T> raw;
-        if (PCG) {
-            const T* p_rd;
-            T* p_wr;
-            pcg_dir_buffers<T>(D, p_rd, p_wr);
-            const V2<T> zz = static_cast<const V2<T>*>(D.vec[DFEA_Z])[n];
-            const V2<T> pp = reinterpret_cast<const V2<T>*>(p_rd)[n];
-            const double beta = D.sc[DFEA_SC_BETA];
-            raw.x = (T)((double)zz.x + beta * (double)pp.x);
-            raw.y = (T)((double)zz.y + beta * (double)pp.y);
-            reinterpret_cast<V2<T>*>(p_wr)[n] = raw;
-        } else {
-            raw = reinterpret_cast<const V2<T>*>(v)[n];
-        }
+        const V2<T> raw = reinterpret_cast<const V2<T>*>(v)[n];
         V2<T> out;
         out.x = (g & DFEA_FIXED_X) ? raw.x : acc.x;
         out.y = (g & DFEA_FIXED_Y) ? raw.y : acc.y;
         reinterpret_cast<V2<T>*>(y)[n] = out;
-        if (PCG) dot = (double)raw.x * (double)out.x + (double)raw.y * (double)out.y;
-    }
-    if (PCG) {
-        const double s = block_sum<FBS>(dot, s_red);
-        if (threadIdx.x == 0) D.red[D.n_chunks + it.b] = s;
     }
 }
 
@@ -860,103 +911,162 @@ __global__ __launch_bounds__(VBS) void k_pcg_init_final(const dfea_desc* __restr
     }
 }
 
-// x += alpha p, r -= alpha Ap, z = r / diag, partials r.z and r.r  (alpha = rz / p.Ap from the partials)
+// PCG iteration, second kernel (`newton_krylov.py:62-79`): alpha = rz / p.Ap (the chunk partials summed in
+// a fixed order by every workgroup), A p of shared nodes summed from their partials in slot order (the
+// direction of shared nodes, p = z + beta p, is formed and stored here), x += alpha p, r -= alpha A p,
+// z = r / diag, partials of r.z and r.r.  Reads only the DFEA_SC_A_* slots; workgroup 0 of each solve writes
+// the caller-visible state.  A thread handles NPT nodes (both dofs) of the workgroup's dof block.  Every
+// load that does not depend on a scalar (both direction buffers included) is issued before the scalars and
+// the p.Ap partials are read, so one HBM round trip covers all of them.
+constexpr int UPD_MAXP = 4;        // partials per shared node summed from registers (more: loop)
+
 template <typename T>
 __global__ __launch_bounds__(VBS) void k_pcg_update(const dfea_desc* __restrict__ tab, const int* __restrict__ items) {
+    constexpr int NPT = VPT / 2;
     __shared__ double s_red[16];
     __shared__ double s_b;
     const Item it = load_item(items);
     const dfea_desc& D = tab[it.s];
-    if (pcg_status(D) != DFEA_PCG_RUNNING) return;
-    const double pAp = bcast0(sum_partials<VBS>(D.red, D.n_chunks + D.n_finblk, s_red), &s_b);
-    const int n = ndof(D);
-    T* x = static_cast<T*>(D.vec[DFEA_X]);
-    T* r = static_cast<T*>(D.vec[DFEA_R]);
-    T* z = static_cast<T*>(D.vec[DFEA_Z]);
-    const T* p_rd;
-    T* p_wr;
-    pcg_dir_buffers<T>(D, p_rd, p_wr);
-    const T* p = p_wr;                            // this iteration's direction
-    const T* Ap = static_cast<const T*>(D.vec[DFEA_AP]);
-    const T* dg = std::is_same<T, double>::value ? (const T*)D.diag64 : (const T*)D.diag32;
-    const int base = it.b * VBLK;
-    if (!(pAp > 0.0)) {                 // non-positive curvature: stop; at k = 0 the answer is z
-        if (D.sc[DFEA_SC_K] == 0.0) {
+    const int nn = D.n_nodes;
+    const int base = it.b * (VBLK / 2);
+    V2<T>* x = static_cast<V2<T>*>(D.vec[DFEA_X]);
+    V2<T>* r = static_cast<V2<T>*>(D.vec[DFEA_R]);
+    V2<T>* z = static_cast<V2<T>*>(D.vec[DFEA_Z]);
+    V2<T>* Ap = static_cast<V2<T>*>(D.vec[DFEA_AP]);
+    V2<T>* P0 = static_cast<V2<T>*>(D.vec[DFEA_P]);
+    V2<T>* P1 = static_cast<V2<T>*>(D.vec[DFEA_P2]);
+    const V2<T>* dg = reinterpret_cast<const V2<T>*>(std::is_same<T, double>::value ? (const T*)D.diag64
+                                                                                       : (const T*)D.diag32);
+    const V2<T>* part = reinterpret_cast<const V2<T>*>(D.part);
+    const int2* shs = reinterpret_cast<const int2*>(D.shslot);
+
+    // 1. loads independent of the scalars
+    int2 ss[NPT];
+    V2<T> xv[NPT], rv[NPT], dv[NPT], av[NPT], zv[NPT], p0[NPT], p1[NPT];
+    V2<T> pq[NPT][UPD_MAXP];
+    unsigned char fx[NPT];
 #pragma unroll
-            for (int j = 0; j < VPT; ++j) {
-                const int i = base + j * VBS + threadIdx.x;
-                if (i < n) x[i] = z[i];
-            }
+    for (int j = 0; j < NPT; ++j) {
+        const int nd = base + j * VBS + threadIdx.x;
+        ss[j] = make_int2(0, 0);
+        if (nd < nn) {
+            ss[j] = shs[nd];
+            xv[j] = x[nd];
+            rv[j] = r[nd];
+            dv[j] = dg[nd];
+            av[j] = Ap[nd];
+            zv[j] = z[nd];
+            p0[j] = P0[nd];
+            p1[j] = P1[nd];
+            fx[j] = D.nfix[nd];
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < NPT; ++j) {
+#pragma unroll
+        for (int q = 0; q < UPD_MAXP; ++q)
+            if (q < ss[j].y) pq[j][q] = part[ss[j].x + q];
+    }
+    double* sc = D.sc;
+    const bool lead = it.b == 0 && threadIdx.x == 0;
+    const double st = sc[DFEA_SC_A_STATUS];
+    const double k = sc[DFEA_SC_A_K];
+    const double rz = sc[DFEA_SC_A_RZ], beta = sc[DFEA_SC_A_BETA];
+    if (st != DFEA_PCG_RUNNING) {       // finished by the scalar step: publish the outcome
+        if (lead) {
+            sc[DFEA_SC_STATUS] = st;
+            sc[DFEA_SC_ITERS] = sc[DFEA_SC_A_ITERS];
+            sc[DFEA_SC_RR] = sc[DFEA_SC_A_RR];
         }
         return;
     }
-    const double alpha = D.sc[DFEA_SC_RZ] / pAp;
-    double rz = 0.0, rr = 0.0;
+    const double pAp = bcast0(sum_partials<VBS>(D.red, D.n_chunks, s_red), &s_b);
+    const bool odd = ((int)k) & 1;      // iteration k reads the direction from buffer k mod 2
+    V2<T>* p_wr = odd ? P0 : P1;
+    if (!(pAp > 0.0)) {                 // non-positive curvature: stop; at k = 0 the answer is z
+        if (k == 0.0) {
 #pragma unroll
-    for (int j = 0; j < VPT; ++j) {
-        const int i = base + j * VBS + threadIdx.x;
-        if (i >= n) continue;
-        const T xi = (T)((double)x[i] + alpha * (double)p[i]);
-        const T ri = (T)((double)r[i] - alpha * (double)Ap[i]);
-        const T zi = ri / dg[i];
-        x[i] = xi;
-        r[i] = ri;
-        z[i] = zi;
-        rz += (double)ri * (double)zi;
-        rr += (double)ri * (double)ri;
-    }
-    rz = block_sum<VBS>(rz, s_red);
-    rr = block_sum<VBS>(rr, s_red);
-    if (threadIdx.x == 0) {
-        double* ru = D.red + D.n_chunks + D.n_finblk;
-        ru[2 * it.b] = rz;
-        ru[2 * it.b + 1] = rr;
-    }
-}
-
-// end of one PCG iteration: convergence / curvature / iteration cap, beta   (one workgroup per solve)
-__global__ __launch_bounds__(VBS) void k_pcg_scalar(const dfea_desc* __restrict__ tab, const int* __restrict__ items) {
-    __shared__ double s_red[16];
-    __shared__ double s_b;
-    const Item it = load_item(items);
-    const dfea_desc& D = tab[it.s];
-    if (pcg_status(D) != DFEA_PCG_RUNNING) return;
-    const double pAp = bcast0(sum_partials<VBS>(D.red, D.n_chunks + D.n_finblk, s_red), &s_b);
-    double* sc = D.sc;
-    const double k = sc[DFEA_SC_K];
-    if (!(pAp > 0.0)) {
-        __syncthreads();
-        if (threadIdx.x == 0) {
+            for (int j = 0; j < NPT; ++j) {
+                const int nd = base + j * VBS + threadIdx.x;
+                if (nd < nn) x[nd] = zv[j];
+            }
+        }
+        if (lead) {
             sc[DFEA_SC_PAP] = pAp;
             sc[DFEA_SC_STATUS] = DFEA_PCG_CURVATURE;
             sc[DFEA_SC_ITERS] = k;
         }
         return;
     }
-    const double* ru = D.red + D.n_chunks + D.n_finblk;
-    double rz = 0.0, rr = 0.0;
-    for (int i = threadIdx.x; i < D.n_vblk; i += VBS) {
-        rz += ru[2 * i];
-        rr += ru[2 * i + 1];
+    const double alpha = rz / pAp;
+
+    // 2. shared nodes: direction and A p; 3. vector update
+    double rzs = 0.0, rr = 0.0;
+    V2<T> pv[NPT];
+#pragma unroll
+    for (int j = 0; j < NPT; ++j) {
+        const int nd = base + j * VBS + threadIdx.x;
+        if (nd >= nn) continue;
+        pv[j] = odd ? p0[j] : p1[j];
+        if (ss[j].y > 0) {
+            const V2<T> q = odd ? p1[j] : p0[j];
+            pv[j].x = (T)((double)zv[j].x + beta * (double)q.x);
+            pv[j].y = (T)((double)zv[j].y + beta * (double)q.y);
+            V2<T> acc{0, 0};
+#pragma unroll
+            for (int c = 0; c < UPD_MAXP; ++c) {
+                if (c < ss[j].y) {
+                    acc.x += pq[j][c].x;
+                    acc.y += pq[j][c].y;
+                }
+            }
+            for (int c = ss[j].x + UPD_MAXP; c < ss[j].x + ss[j].y; ++c) {
+                const V2<T> pc = part[c];
+                acc.x += pc.x;
+                acc.y += pc.y;
+            }
+            av[j].x = (fx[j] & 1) ? pv[j].x : acc.x;
+            av[j].y = (fx[j] & 2) ? pv[j].y : acc.y;
+        }
+        V2<T> xi, ri, zi;
+        xi.x = (T)((double)xv[j].x + alpha * (double)pv[j].x);
+        xi.y = (T)((double)xv[j].y + alpha * (double)pv[j].y);
+        ri.x = (T)((double)rv[j].x - alpha * (double)av[j].x);
+        ri.y = (T)((double)rv[j].y - alpha * (double)av[j].y);
+        zi.x = ri.x / dv[j].x;
+        zi.y = ri.y / dv[j].y;
+        xv[j] = xi;
+        rv[j] = ri;
+        zv[j] = zi;
+        rzs += (double)ri.x * (double)zi.x + (double)ri.y * (double)zi.y;
+        rr += (double)ri.x * (double)ri.x + (double)ri.y * (double)ri.y;
     }
-    rz = block_sum<VBS>(rz, s_red);
+#pragma unroll
+    for (int j = 0; j < NPT; ++j) {
+        const int nd = base + j * VBS + threadIdx.x;
+        if (nd >= nn) continue;
+        x[nd] = xv[j];
+        r[nd] = rv[j];
+        z[nd] = zv[j];
+        if (ss[j].y > 0) {
+            p_wr[nd] = pv[j];
+            Ap[nd] = av[j];
+        }
+    }
+    rzs = block_sum<VBS>(rzs, s_red);
     rr = block_sum<VBS>(rr, s_red);
     if (threadIdx.x == 0) {
+        double* ru = D.red + D.n_chunks + D.n_finblk;
+        ru[2 * it.b] = rzs;
+        ru[2 * it.b + 1] = rr;
+    }
+    if (lead) {
+        sc[DFEA_SC_K] = k + 1.0;
+        sc[DFEA_SC_RZ] = rz;
         sc[DFEA_SC_PAP] = pAp;
-        sc[DFEA_SC_ALPHA] = sc[DFEA_SC_RZ] / pAp;
-        sc[DFEA_SC_RR] = rr;
-        if (sqrt(rr) <= sc[DFEA_SC_RTOL] * sqrt(sc[DFEA_SC_BB])) {
-            sc[DFEA_SC_STATUS] = DFEA_PCG_CONVERGED;
-            sc[DFEA_SC_ITERS] = k + 1.0;
-        } else {
-            sc[DFEA_SC_BETA] = rz / sc[DFEA_SC_RZ];
-            sc[DFEA_SC_RZ] = rz;
-            sc[DFEA_SC_K] = k + 1.0;
-            if (k + 1.0 >= sc[DFEA_SC_MAXIT]) {
-                sc[DFEA_SC_STATUS] = DFEA_PCG_MAXITER;
-                sc[DFEA_SC_ITERS] = sc[DFEA_SC_MAXIT];
-            }
-        }
+        sc[DFEA_SC_ALPHA] = alpha;
+        sc[DFEA_SC_BETA] = beta;
+        sc[DFEA_SC_RR] = sc[DFEA_SC_A_RR];
     }
 }
 
@@ -993,16 +1103,12 @@ int dispatch(int kind, F f) {
 }
 
 template <int NEN, int BS, typename T>
-int pcg_iter(const dfea_desc* table, const int32_t* items, int n_items, const int32_t* fin_items,
-                    int n_fin_items, const int32_t* vitems, int n_vitems, const int32_t* solve_items, int n_solves,
-                    int n_iter, void* stream) {
+int pcg_iter(const dfea_desc* table, const int32_t* items, int n_items, const int32_t* vitems, int n_vitems,
+             int n_iter, void* stream) {
     for (int i = 0; i < n_iter; ++i) {
         int rc = launch(k_apply<NEN, NEN, BS, T, true>, n_items, BS, stream, "k_apply(pcg)", table, items,
                         (const T*)nullptr, (T*)nullptr);
-        if (!rc) rc = launch(k_apply_fin<T, true>, n_fin_items, FBS, stream, "k_apply_fin(pcg)", table, fin_items,
-                             (const T*)nullptr, (T*)nullptr);
         if (!rc) rc = launch(k_pcg_update<T>, n_vitems, VBS, stream, "k_pcg_update", table, vitems);
-        if (!rc) rc = launch(k_pcg_scalar, n_solves, VBS, stream, "k_pcg_scalar", table, solve_items);
         if (rc) return rc;
     }
     return 0;
@@ -1046,7 +1152,7 @@ struct Launch_apply {
         T* yy = static_cast<T*>(y);
         int rc = launch(k_apply<NEN, NEN, BS, T, false>, n, BS, st, "k_apply", t, it, vv, yy);
         if (rc) return rc;
-        return launch(k_apply_fin<T, false>, nf, FBS, st, "k_apply_fin", t, fi, vv, yy);
+        return launch(k_apply_fin<T>, nf, FBS, st, "k_apply_fin", t, fi, vv, yy);
     }
     template <int NEN, int BS> int run() {
         return prec == DFEA_FP64 ? go<double, NEN, BS>() : go<float, NEN, BS>();
@@ -1054,11 +1160,10 @@ struct Launch_apply {
 };
 
 struct Launch_pcg_iterate {
-    const dfea_desc* t; const int32_t* it; int n; const int32_t* fi; int nf; const int32_t* vi; int nv;
-    const int32_t* si; int ns; int k; int prec; void* st;
+    const dfea_desc* t; const int32_t* it; int n; const int32_t* vi; int nv; int k; int prec; void* st;
     template <int NEN, int BS> int run() {
-        return prec == DFEA_FP64 ? pcg_iter<NEN, BS, double>(t, it, n, fi, nf, vi, nv, si, ns, k, st)
-                                 : pcg_iter<NEN, BS, float>(t, it, n, fi, nf, vi, nv, si, ns, k, st);
+        return prec == DFEA_FP64 ? pcg_iter<NEN, BS, double>(t, it, n, vi, nv, k, st)
+                                 : pcg_iter<NEN, BS, float>(t, it, n, vi, nv, k, st);
     }
 };
 
@@ -1084,6 +1189,7 @@ int dfea_desc_offset(int field) {
     static const int off[] = {
         (int)offsetof(dfea_desc, chunks), (int)offsetof(dfea_desc, lnode), (int)offsetof(dfea_desc, lelem),
         (int)offsetof(dfea_desc, sh_node), (int)offsetof(dfea_desc, sh_off), (int)offsetof(dfea_desc, pslot), (int)offsetof(dfea_desc, hel), (int)offsetof(dfea_desc, hlelem),
+        (int)offsetof(dfea_desc, shslot),
         (int)offsetof(dfea_desc, conn), (int)offsetof(dfea_desc, perm), (int)offsetof(dfea_desc, nodes),
         (int)offsetof(dfea_desc, dNdxi), (int)offsetof(dfea_desc, gw), (int)offsetof(dfea_desc, nfix),
         (int)offsetof(dfea_desc, dN64), (int)offsetof(dfea_desc, dN32), (int)offsetof(dfea_desc, w),
@@ -1168,19 +1274,15 @@ int dfea_pcg_init(const dfea_desc* table, const int32_t* vitems, int n_vitems, c
     return launch(k_pcg_init_final, n_solves, VBS, stream, "k_pcg_init_final", table, solve_items);
 }
 
-int dfea_pcg_iterate(const dfea_desc* table, const int32_t* items, int n_items, const int32_t* fin_items,
-                     int n_fin_items, const int32_t* vitems, int n_vitems, const int32_t* solve_items, int n_solves,
-                     int n_iter, int nen, int precision, void* stream) {
-    if (!table || !items || !vitems || !solve_items || n_iter < 0 ||
-        (precision != DFEA_FP64 && precision != DFEA_FP32))
+int dfea_pcg_iterate(const dfea_desc* table, const int32_t* items, int n_items, const int32_t* vitems,
+                     int n_vitems, int n_iter, int nen, int precision, void* stream) {
+    if (!table || !items || !vitems || n_iter < 0 || (precision != DFEA_FP64 && precision != DFEA_FP32))
         return fail(DFEA_EINVAL, "dfea_pcg_iterate: bad argument");
-    return dispatch(nen, Launch_pcg_iterate{table, items, n_items, fin_items, n_fin_items, vitems, n_vitems, solve_items, n_solves,
-                           n_iter, precision, stream});
+    return dispatch(nen, Launch_pcg_iterate{table, items, n_items, vitems, n_vitems, n_iter, precision, stream});
 }
 
-int dfea_pcg_graph_create(const dfea_desc* table, const int32_t* items, int n_items, const int32_t* fin_items,
-                          int n_fin_items, const int32_t* vitems, int n_vitems, const int32_t* solve_items,
-                          int n_solves, int n_iter, int nen, int precision, void** graph) {
+int dfea_pcg_graph_create(const dfea_desc* table, const int32_t* items, int n_items, const int32_t* vitems,
+                          int n_vitems, int n_iter, int nen, int precision, void** graph) {
     if (!graph || n_iter <= 0) return fail(DFEA_EINVAL, "dfea_pcg_graph_create: bad argument");
     *graph = nullptr;
     hipStream_t s = nullptr;
@@ -1191,8 +1293,7 @@ int dfea_pcg_graph_create(const dfea_desc* table, const int32_t* items, int n_it
         (void)hipStreamDestroy(s);
         return fail(-1000 - (int)e, "dfea_pcg_graph_create: hipStreamBeginCapture failed");
     }
-    int rc = dfea_pcg_iterate(table, items, n_items, fin_items, n_fin_items, vitems, n_vitems, solve_items,
-                              n_solves, n_iter, nen, precision, s);
+    int rc = dfea_pcg_iterate(table, items, n_items, vitems, n_vitems, n_iter, nen, precision, s);
     hipGraph_t g = nullptr;
     e = hipStreamEndCapture(s, &g);
     (void)hipStreamDestroy(s);

@@ -21,7 +21,7 @@ VEC = dict(x=0, r=1, z=2, p=3, Ap=4, b=5, u=6, R=7, uacc=8, uD=9, p2=10)
 NVEC = 11
 SC = dict(rz=0, pAp=1, alpha=2, beta=3, rr=4, bb=5, rtol=6, status=7, k=8, maxit=9, iters=10, norm=11, lf=12,
           step=13)
-NSC = 16
+NSC = 24
 PCG_RUNNING, PCG_CONVERGED, PCG_CURVATURE, PCG_MAXITER = 0, 1, 2, 3
 
 _P = ctypes.c_void_p
@@ -32,7 +32,7 @@ _D = ctypes.c_double
 class Desc(ctypes.Structure):
     """Mirror of ``dfea_desc`` (field order = declaration order in include/dfea.h)."""
     _fields_ = [
-        ("chunks", _P), ("lnode", _P), ("lelem", _P), ("sh_node", _P), ("sh_off", _P), ("pslot", _P), ("hel", _P), ("hlelem", _P),
+        ("chunks", _P), ("lnode", _P), ("lelem", _P), ("sh_node", _P), ("sh_off", _P), ("pslot", _P), ("hel", _P), ("hlelem", _P), ("shslot", _P),
         ("conn", _P), ("perm", _P), ("nodes", _P), ("dNdxi", _P), ("gw", _P), ("nfix", _P),
         ("dN64", _P), ("dN32", _P), ("w", _P), ("theta", _P), ("C64", _P), ("C32", _P),
         ("diag64", _P), ("diag32", _P), ("f", _P), ("part", _P), ("red", _P), ("sc", _P),
@@ -66,9 +66,9 @@ _SIGNATURES = {
     "dfea_dot": (_I, [_P, _P, _I, _P, _I, _P, _P, _P]),
     "dfea_axpy": (_I, [_P, _P, _I, _I, _P, _P, _P, _P, _P, _I, _P]),
     "dfea_pcg_init": (_I, [_P, _P, _I, _P, _I, _I, _I, _P]),
-    "dfea_pcg_iterate": (_I, [_P, _P, _I, _P, _I, _P, _I, _P, _I, _I, _I, _I, _P]),
+    "dfea_pcg_iterate": (_I, [_P, _P, _I, _P, _I, _I, _I, _I, _P]),
     "dfea_adjoint_grad": (_I, [_P, _P, _I, _P, _P, _P, _I, _P]),
-    "dfea_pcg_graph_create": (_I, [_P, _P, _I, _P, _I, _P, _I, _P, _I, _I, _I, _I, ctypes.POINTER(_P)]),
+    "dfea_pcg_graph_create": (_I, [_P, _P, _I, _P, _I, _I, _I, _I, ctypes.POINTER(_P)]),
     "dfea_graph_launch": (_I, [_P, _P]),
     "dfea_graph_destroy": (_I, [_P]),
 }

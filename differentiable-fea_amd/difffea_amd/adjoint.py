@@ -15,7 +15,6 @@ adjoint solve + the hand-written gradient kernel, so ``loss(u(theta)).backward()
 import torch
 from torch.func import grad
 
-from . import _lib
 from .newton_krylov import HipNewton, MIXED_RTOL_FLOOR
 
 
@@ -28,12 +27,7 @@ def solve_adjoint(solver, u_star, dL_du, rtol=1e-6, max_iter=500):
     if s.precision == "mixed":
         rtol = max(rtol, MIXED_RTOL_FLOOR)
     s.pcg_start(rtol=rtol, max_iter=max_iter, neg_res=False)
-    st, its, k = s.pcg_status()
-    chunk = 16
-    while st == _lib.PCG_RUNNING:
-        s.pcg_iterate(min(chunk, max_iter - k))
-        st, its, k = s.pcg_status()
-        chunk = min(2 * chunk, 128)
+    its, _ = s.pcg_run(max_iter)
     return s.vec["x"].to(torch.float64).clone(), its
 
 

@@ -3,8 +3,8 @@
 ``HipMesh``    plan tables + geometry cache (dN_dX planes in FP64 and FP32, w_detJ) of one mesh + mask.
 ``HipSystem``  one solve on a mesh: theta, precision, moduli, Jacobi diagonal, PCG / Newton vectors and
                its scratch (shared-node partials, reduction partials, scalar block); owns a descriptor.
-``Batch``      a descriptor table + work-item tables over several systems of one element type and
-               precision, so that one launch advances all of them.
+Several systems are advanced by one launch through a descriptor table of their descriptors and concatenated
+work-item tables (``serve.Server`` builds those).
 
 All memory is allocated through torch (the caching allocator) and lives on the current CUDA(HIP) device;
 kernels are enqueued on torch's current stream.
@@ -77,6 +77,7 @@ class HipMesh:
         self.t_sh_off = _dev_int32(p.sh_off, dv)
         self.t_pslot = _dev_int32(p.pslot if len(p.pslot) else np.zeros(1, np.int32), dv)
         self.t_hel = _dev_int32(p.hel if len(p.hel) else np.zeros(1, np.int32), dv)
+        self.t_shslot = _dev_int32(p.shslot, dv)
         self.t_hlelem = torch.from_numpy(np.ascontiguousarray(p.hlelem).view(np.int16).reshape(-1) if p.hlelem.size
                                          else np.zeros(1, np.int16)).to(dv)
         self.t_conn = _dev_int32(p.conn_int, dv)
@@ -113,6 +114,7 @@ class HipMesh:
         d.chunks, d.lnode, d.lelem = self.t_chunks.data_ptr(), self.t_lnode.data_ptr(), self.t_lelem.data_ptr()
         d.sh_node, d.sh_off, d.pslot = self.t_sh_node.data_ptr(), self.t_sh_off.data_ptr(), self.t_pslot.data_ptr()
         d.hel, d.hlelem, d.n_halo = self.t_hel.data_ptr(), self.t_hlelem.data_ptr(), len(p.hel)
+        d.shslot = self.t_shslot.data_ptr()
         d.n_cols = self.n_cols
         d.conn, d.perm, d.nodes = self.t_conn.data_ptr(), self.t_perm.data_ptr(), self.t_nodes.data_ptr()
         d.dNdxi, d.gw, d.nfix = self.t_dNdxi.data_ptr(), self.t_gw.data_ptr(), self.t_nfix.data_ptr()
@@ -267,9 +269,8 @@ class HipSystem:
             m = self.mesh
             h = ctypes.c_void_p()
             _lib.check(self.lib.dfea_pcg_graph_create(
-                _ptr(self.table), _ptr(m.items), m.plan.n_chunks, _ptr(m.fin_items), m.n_finblk, _ptr(m.vitems),
-                m.n_vblk, _ptr(m.solve_items), 1, int(n_iter), m.kind, self.prec_id, ctypes.byref(h)),
-                "dfea_pcg_graph_create")
+                _ptr(self.table), _ptr(m.items), m.plan.n_chunks, _ptr(m.vitems), m.n_vblk, int(n_iter), m.kind,
+                self.prec_id, ctypes.byref(h)), "dfea_pcg_graph_create")
             self._graphs[n_iter] = g = h
         return g
 
@@ -280,9 +281,9 @@ class HipSystem:
         if self.use_graphs if graph is None else graph:
             _lib.check(self.lib.dfea_graph_launch(self._graph(int(n_iter)), _stream()), "dfea_graph_launch")
             return
-        _lib.check(self.lib.dfea_pcg_iterate(_ptr(self.table), _ptr(m.items), m.plan.n_chunks, _ptr(m.fin_items),
-                                             m.n_finblk, _ptr(m.vitems), m.n_vblk, _ptr(m.solve_items), 1,
-                                             int(n_iter), m.kind, self.prec_id, _stream()), "dfea_pcg_iterate")
+        _lib.check(self.lib.dfea_pcg_iterate(_ptr(self.table), _ptr(m.items), m.plan.n_chunks, _ptr(m.vitems),
+                                             m.n_vblk, int(n_iter), m.kind, self.prec_id, _stream()),
+                   "dfea_pcg_iterate")
 
     def release_graphs(self):
         for g in self._graphs.values():
@@ -299,15 +300,21 @@ class HipSystem:
         s = self.sc[[_lib.SC["status"], _lib.SC["iters"], _lib.SC["k"]]].cpu()
         return int(s[0]), int(s[1]), int(s[2])
 
-    def pcg_solve(self, b=None, rtol=1e-5, max_iter=500, neg_res=False, chunk=16, graph=None):
-        """Run the fused PCG to completion; returns (x view, iterations, status).
+    def pcg_run(self, max_iter, chunk=16, graph=None):
+        """Iterate a started PCG until it stops; returns (iterations, status).
 
-        Iterations run in chunks (graph replays); solves that finish inside a chunk skip the remaining
-        iterations on the device.  The status block is read once per chunk."""
-        self.pcg_start(b, rtol, max_iter, neg_res)
+        Iterations run in chunks (graph replays) whose length doubles up to 128; solves that finish inside
+        a chunk skip the remaining iterations on the device, and the status block is read once per chunk.
+        The test on the last update is made at the start of the following iteration, hence the + 1."""
         st, its, k = self.pcg_status()
         while st == _lib.PCG_RUNNING:
-            self.pcg_iterate(min(chunk, max_iter - k), graph)
+            self.pcg_iterate(min(chunk, max_iter - k + 1), graph)
             st, its, k = self.pcg_status()
             chunk = min(2 * chunk, 128)
+        return its, st
+
+    def pcg_solve(self, b=None, rtol=1e-5, max_iter=500, neg_res=False, chunk=16, graph=None):
+        """Run the fused PCG to completion; returns (x view, iterations, status)."""
+        self.pcg_start(b, rtol, max_iter, neg_res)
+        its, st = self.pcg_run(max_iter, chunk, graph)
         return self.vec["x"], its, st

@@ -5,7 +5,7 @@
 * ``newton_krylov_solve(params, theta, ..., tangent="hip", precision="fp64")`` (`newton_krylov.py:186-264`):
   load continuation with cut-backs, Eisenstat-Walker forcing, Armijo line search -- the same state machine
   and ``info`` dictionary -- where every vector operation runs in the HIP library: residual, moduli +
-  Jacobi diagonal (one fused kernel per Newton step), the fused PCG (four kernels per iteration, device
+  Jacobi diagonal (one fused kernel per Newton step), the fused PCG (two kernels per iteration, device
   scalars, status read once per chunk of iterations), line-search trials.
 * ``precision="mixed"``: moduli, A(v) and the inner CG vectors in FP32 (dot products accumulated in FP64),
   while the Newton residual, the state u and the line search stay FP64 -- Newton itself is the outer
@@ -81,12 +81,7 @@ class HipNewton:
         if s.precision == "mixed":
             rtol = max(rtol, MIXED_RTOL_FLOOR)
         s.pcg_start(rtol=rtol, max_iter=max_iter, neg_res=True)
-        st, its, k = s.pcg_status()
-        chunk = self.pcg_chunk
-        while st == _lib.PCG_RUNNING:
-            s.pcg_iterate(min(chunk, max_iter - k))
-            st, its, k = s.pcg_status()
-            chunk = min(2 * chunk, 128)
+        its, _ = s.pcg_run(max_iter, self.pcg_chunk)
         return s.vec["x"], its
 
     def trial(self, u, du, alpha, out):

@@ -150,7 +150,7 @@ class MeshPlan:
         BS = lib.dfea_chunk_threads(nen)
         BS_halo = lib.dfea_chunk_threads(nen | _lib.HALO)
         if mode == "auto":
-            mode = "partials"          # measured faster than halo plans on MI355X
+            mode = "partials"          # measured faster than halo plans on MI355X (fewer VGPRs, full waves)
         if mode not in ("halo", "partials"):
             raise ValueError(f"plan mode must be 'auto', 'halo' or 'partials', got {mode!r}")
         if E <= 0:
@@ -352,6 +352,22 @@ class MeshPlan:
         self.sh_off = np.zeros(1, np.int32)
         return True
 
+    @property
+    def shmap(self):
+        """Shared-node index of every node, -1 for nodes owned by one chunk."""
+        m = np.full(self.n_nodes, -1, np.int32)
+        m[(self.sh_node & _lib.NODE_MASK).astype(np.int64)] = np.arange(self.n_shared, dtype=np.int32)
+        return m
+
+    @property
+    def shslot(self):
+        """[n_nodes][2] int32: first partial slot and number of partials of every node ({0, 0} if owned)."""
+        out = np.zeros((self.n_nodes, 2), np.int32)
+        nodes = (self.sh_node & _lib.NODE_MASK).astype(np.int64)
+        out[nodes, 0] = self.sh_off[:-1]
+        out[nodes, 1] = np.diff(self.sh_off)
+        return out
+
     def validate(self):
         """Host-side bounds checks of every index the kernels follow (run before any launch).
         Returns the largest number of threads a chunk needs (core + halo elements)."""
@@ -382,6 +398,8 @@ class MeshPlan:
             assert self.sh_off[0] == 0 and self.sh_off[-1] == len(self.pslot) == self.n_part
             assert np.array_equal(np.sort(self.pslot), np.arange(self.n_part))
             assert (self.sh_node & _lib.NODE_MASK).max() < self.n_nodes
+            assert len(np.unique(self.sh_node & _lib.NODE_MASK)) == self.n_shared
+            assert np.array_equal(self.sh_node >> 30, self.nfix[self.sh_node & _lib.NODE_MASK])
         assert self.conn_int.min() >= 0 and self.conn_int.max() < self.n_nodes
         return int((ch[:, 1] + ch[:, 8]).max())
 

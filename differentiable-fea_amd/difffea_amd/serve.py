@@ -202,6 +202,11 @@ def solve_ops(job, tol, max_newton=25, max_cutbacks=8, cg_max_iter=500):
 # ------------------------------------------------------------------------------------------------------------
 # the server
 # ------------------------------------------------------------------------------------------------------------
+def _kinds(jobs):
+    """Element kinds (nen plus plan flags) present among the jobs, in a fixed order."""
+    return sorted({j.mesh.kind for j in jobs})
+
+
 class Server:
     """Continuous-batching solver for a stream of requests on one GPU (see module docstring)."""
 
@@ -304,7 +309,7 @@ class Server:
         if js:
             idx = torch.tensor([j.slot for j in js])
             self.sc[idx, _lib.SC["lf"]] = torch.tensor([float(j.op[1]) for j in js], dtype=torch.float64).to(self.device)
-            for nen in (4, 9, 4 | _lib.HALO):
+            for nen in _kinds(js):
                 sub = [j for j in js if j.mesh.kind == nen]
                 it, n = self._dev_items(sub, "chunk")
                 if not n:
@@ -316,7 +321,7 @@ class Server:
             _lib.check(lib.dfea_dot(T, _ptr(it), n, _ptr(si), ns, None, None, st), "dfea_dot")
         # moduli + diagonal at vec[u]
         js = by.get("moduli", [])
-        for nen in (4, 9, 4 | _lib.HALO):
+        for nen in _kinds(js):
             sub = [j for j in js if j.mesh.kind == nen]
             it, n = self._dev_items(sub, "chunk")
             if n:
@@ -348,7 +353,7 @@ class Server:
             for j in new:
                 j.pcg_running = True
         if js:
-            for nen in (4, 9, 4 | _lib.HALO):
+            for nen in _kinds(js):
                 sub = [j for j in js if j.mesh.kind == nen]
                 if not sub:
                     continue
@@ -357,11 +362,9 @@ class Server:
                 for j in sub:
                     j.pcg_rounds += 1
                 it, n = self._dev_items(sub, "chunk")
-                fi, nf = self._dev_items(sub, "fin")
                 vi, nv = self._dev_items(sub, "vec")
-                si, ns = self._dev_items(sub, "solve")
-                _lib.check(lib.dfea_pcg_iterate(T, _ptr(it), n, _ptr(fi), nf, _ptr(vi), nv, _ptr(si), ns,
-                                                chunk, nen, self.prec_id, st), "dfea_pcg_iterate")
+                _lib.check(lib.dfea_pcg_iterate(T, _ptr(it), n, _ptr(vi), nv, chunk, nen, self.prec_id, st),
+                           "dfea_pcg_iterate")
         # one host sync: status, iterations, norms
         sc = self.sc.cpu().numpy()
         finished = []

@@ -49,10 +49,14 @@ enum { DFEA_X = 0, DFEA_R = 1, DFEA_Z = 2, DFEA_P = 3, DFEA_AP = 4, DFEA_B = 5, 
  * the old direction from buffer k mod 2 and writes the new one to the other, so no workgroup overwrites a
  * value another workgroup still has to read. */
 
-/* scalar block of a solve (doubles) */
+/* scalar block of a solve (doubles).  The PCG state read by the caller is slots 0..10; slots DFEA_SC_A_*
+ * hold the state the direction/product kernel of an iteration hands to its update kernel (each slot has one
+ * writer kernel, and no kernel reads a slot it writes). */
 enum { DFEA_SC_RZ = 0, DFEA_SC_PAP = 1, DFEA_SC_ALPHA = 2, DFEA_SC_BETA = 3, DFEA_SC_RR = 4, DFEA_SC_BB = 5,
        DFEA_SC_RTOL = 6, DFEA_SC_STATUS = 7, DFEA_SC_K = 8, DFEA_SC_MAXIT = 9, DFEA_SC_ITERS = 10,
-       DFEA_SC_NORM = 11, DFEA_SC_LF = 12, DFEA_SC_STEP = 13, DFEA_NSC = 16 };
+       DFEA_SC_NORM = 11, DFEA_SC_LF = 12, DFEA_SC_STEP = 13,
+       DFEA_SC_A_STATUS = 16, DFEA_SC_A_K = 17, DFEA_SC_A_ITERS = 18, DFEA_SC_A_RZ = 19, DFEA_SC_A_BETA = 20,
+       DFEA_SC_A_RR = 21, DFEA_NSC = 24 };
 
 /* PCG status values stored in sc[DFEA_SC_STATUS] */
 enum { DFEA_PCG_RUNNING = 0, DFEA_PCG_CONVERGED = 1, DFEA_PCG_CURVATURE = 2, DFEA_PCG_MAXITER = 3 };
@@ -89,6 +93,8 @@ typedef struct dfea_desc {
                                    the partials of shared node i are slots sh_off[i]..sh_off[i+1]-1 */
     const int32_t* hel;         /* [n_halo] halo mode: internal element id of each halo entry */
     const uint16_t* hlelem;     /* [nen][n_halo] halo mode: local node index of halo entry h, node a */
+    const int32_t* shslot;      /* [n_nodes][2] {first partial slot, number of partials} of each node; {0, 0}
+                                   for nodes owned by one chunk */
     const int32_t* conn;        /* [nen][n_el] node ids, internal element order */
     const int32_t* perm;        /* [n_el] original element id of internal element e */
     const double* nodes;        /* [N][2] reference coordinates */
@@ -179,19 +185,26 @@ int dfea_axpy(const dfea_desc* table, const int32_t* items, int n_items, int mod
  * Replaces `newton_krylov.py:51-60`. */
 int dfea_pcg_init(const dfea_desc* table, const int32_t* vitems, int n_vitems, const int32_t* solve_items,
                   int n_solves, int neg_res, int precision, void* stream);
-/* n_iter PCG iterations for every running solve of the batch (fused: apply + dot, shared finalize,
- * axpy + dots, scalar update); finished solves skip their work.  Semantics of `newton_krylov.py:62-88`. */
-int dfea_pcg_iterate(const dfea_desc* table, const int32_t* items, int n_items, const int32_t* fin_items,
-                     int n_fin_items, const int32_t* vitems, int n_vitems, const int32_t* solve_items,
-                     int n_solves, int n_iter, int nen, int precision, void* stream);
+/* n_iter PCG iterations for every running solve of the batch, two kernels each:
+ *   1. per chunk: the scalar step of the previous iteration (convergence test on r.r, beta = rz'/rz,
+ *      iteration cap), evaluated redundantly and identically by every workgroup from the previous update's
+ *      partials; the direction p = z + beta p on the gather; A p; partials of p.Ap that already include
+ *      the shared nodes (each chunk contributes p . its own partial);
+ *   2. per dof block: alpha = rz / p.Ap, shared nodes' A p summed from their partials in slot order,
+ *      x += alpha p, r -= alpha A p, z = r / diag, partials of r.z and r.r.
+ * The test on the last update's residual is made by the first kernel of the next iteration, so a solve
+ * whose k = max_iter updates are done is reported finished one iteration call later: callers iterate while
+ * the status is DFEA_PCG_RUNNING, with n_iter >= 1.  Finished solves skip their work.
+ * items: chunk items; vitems: dof-block items.  Semantics of `newton_krylov.py:62-88`. */
+int dfea_pcg_iterate(const dfea_desc* table, const int32_t* items, int n_items, const int32_t* vitems,
+                     int n_vitems, int n_iter, int nen, int precision, void* stream);
 
 /* Capture n_iter iterations of dfea_pcg_iterate (same arguments) into an executable HIP graph; replay it
  * with dfea_graph_launch on any stream.  The tables and buffers the descriptors point to must outlive the
  * graph.  No memset nodes are recorded.  Graph capture of the iteration loop replaces the reference's
  * per-iteration Python loop with two host syncs (`newton_krylov.py:62-86`). */
-int dfea_pcg_graph_create(const dfea_desc* table, const int32_t* items, int n_items, const int32_t* fin_items,
-                          int n_fin_items, const int32_t* vitems, int n_vitems, const int32_t* solve_items,
-                          int n_solves, int n_iter, int nen, int precision, void** graph);
+int dfea_pcg_graph_create(const dfea_desc* table, const int32_t* items, int n_items, const int32_t* vitems,
+                          int n_vitems, int n_iter, int nen, int precision, void** graph);
 int dfea_graph_launch(void* graph, void* stream);
 int dfea_graph_destroy(void* graph);
 

@@ -164,3 +164,20 @@ def test_halo_plan_matches_oracle_and_partials_plan(n):
     x_p, k_p, st_p = sp.pcg_solve(b, rtol=1e-9, max_iter=3000)
     assert st_h == st_p == 1 and abs(k_h - k_p) <= 1
     assert rel(x_h, x_p) < 1e-8
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_pcg_negative_curvature_at_first_iteration_returns_z(prec):
+    """theta = -1 makes the tangent negative definite: p.Ap < 0 at k = 0, so PCG stops with status
+    'curvature' after 0 iterations and returns z = M^-1 b, as the reference's pcg does."""
+    from difffea_amd.operators import make_system
+    p, u, v, th = _setup("quad4", 8, seed=14, amp=0.01)
+    th = -th
+    b = v * p["m"]
+    x_ref, k_ref = fem.pcg(fem.make_A_operator_qp(u, th, p), b, fem.make_jacobi_preconditioner(u, th, p),
+                           rtol=1e-8, max_iter=100)
+    s = make_system(u.to(DEV), th.to(DEV), p, prec)
+    s.moduli(u.to(DEV))
+    x, k, st = s.pcg_solve(b.to(DEV).to(s.vdt), rtol=1e-8, max_iter=100)
+    assert (st, k, k_ref) == (2, 0, 0)
+    assert rel(x, x_ref) < (1e-12 if prec == "fp64" else 1e-6)

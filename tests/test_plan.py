@@ -31,14 +31,17 @@ MESHES = {
 }
 
 
-def _emulate_apply(plan, C, dN, v):
+def _emulate_apply(plan, C, dN, v, dots=None):
     """numpy emulation of k_apply (+ k_apply_fin): per chunk gather -> core and halo elements ->
     colour-ordered LDS accumulation -> owned write / shared partial; then shared nodes sum their partials
-    in slot order (partials plans)."""
+    in slot order (partials plans).  ``dots``: a list that receives each chunk's PCG partial of v.Av (owned
+    nodes v.y, shared nodes v . the chunk's partial on free components, fixed components v^2 from the chunk
+    holding the node's first slot)."""
     N = plan.n_nodes
     y = np.full((N, 2), np.nan)
     part = np.zeros((plan.n_part, 2))
     vv = v.reshape(N, 2)
+    shmap = plan.shmap
     for c in range(plan.n_chunks):
         e0, ne, k0, nk, nown, pbase, ncol, h0, nh = plan.chunks[c, :9]
         ln = plan.lnode[k0:k0 + nk]
@@ -65,8 +68,21 @@ def _emulate_apply(plan, C, dN, v):
             acc[idx] += fe[thr].reshape(-1, 2)
         out = np.where(np.stack([bits & 1, bits & 2], 1) > 0, raw, acc)
         y[nodes[:nown]] = out[:nown]
+        slots = plan.pslot[pbase:pbase + nk - nown]
         if plan.n_part:
-            part[plan.pslot[pbase:pbase + nk - nown]] = acc[nown:]
+            part[slots] = acc[nown:]
+        if dots is not None:
+            d = float((raw[:nown] * out[:nown]).sum())
+            for j in range(nk - nown if plan.n_part else 0):   # halo plans: the rest are read-only nodes
+                kk = nown + j
+                fx = [bool(bits[kk] & 1), bool(bits[kk] & 2)]
+                first = slots[j] == plan.sh_off[shmap[nodes[kk]]]
+                for comp in range(2):
+                    if not fx[comp]:
+                        d += raw[kk, comp] * acc[kk, comp]
+                    elif first:
+                        d += raw[kk, comp] ** 2
+            dots.append(d)
     for i in range(plan.n_shared):
         g_ = int(plan.sh_node[i])
         n = g_ & _lib.NODE_MASK
@@ -101,9 +117,21 @@ def test_plan_tables_and_emulated_assembly_match_oracle(name, mode):
     th = torch.ones(p["conn"].shape[0], dtype=torch.float64)
     u, v = state(p, 2, scale=1e-3)
     C, dN = fem.qp_setup(u, th, p)
-    y = _emulate_apply(plan, C.numpy()[plan.perm], dN.numpy()[plan.perm], v.numpy())
+    dots = []
+    y = _emulate_apply(plan, C.numpy()[plan.perm], dN.numpy()[plan.perm], v.numpy(), dots)
     ref = fem.make_A_operator_qp(u, th, p, C=C)(v).numpy()
     assert np.abs(y - ref).max() < 1e-12 * np.abs(ref).max()
+    # the PCG's p.Ap from chunk partials alone (shared nodes by linearity) equals v.A(v)
+    vAv = float(v.numpy() @ ref)
+    assert abs(sum(dots) - vAv) < 1e-12 * float(np.abs(v.numpy() * ref).sum())
+    # the update kernel's reconstruction of A v on shared nodes from shmap / sh_off / nfix
+    sm, ss = plan.shmap, plan.shslot
+    assert (sm >= 0).sum() == plan.n_shared and (ss[:, 1] > 0).sum() == plan.n_shared
+    assert (ss[sm < 0] == 0).all()
+    for nd in np.flatnonzero(sm >= 0):
+        s_ = sm[nd]
+        assert plan.sh_node[s_] & _lib.NODE_MASK == nd
+        assert tuple(ss[nd]) == (plan.sh_off[s_], plan.sh_off[s_ + 1] - plan.sh_off[s_])
 
 
 def test_structured_meshes_get_square_tiles_and_four_colours():
