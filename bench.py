@@ -27,6 +27,20 @@ import torch  # noqa: E402
 
 METRIC = "A(v) HBM GB/s at 160k Q4; solves/s + p50/p99 latency, mixed vs FP64"
 HBM_PEAK_GBS = 8000.0
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "apply_q4_400_fp64_pmc.md")
+
+
+def measured_traffic():
+    """HBM bytes per FP64 A(v) at 400x400 Q4 from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary
+    (scaled by the factors stated there); None when the summary is absent."""
+    try:
+        with open(PMC_SUMMARY) as fh:
+            for line in fh:
+                if line.startswith("traffic_bytes_per_step:"):
+                    return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
 
 
 def b_alg(n_el, nen, nq, n_dof, s):
@@ -276,7 +290,9 @@ def main():
                    "bytes_per_step": r64["bytes"]},
         "products_per_s": round(r64["products_per_s"], 1),
         "roofline": {"bound": "hbm", "achieved": round(r64["gbs"], 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(r64["gbs"] / HBM_PEAK_GBS, 4), "traffic": None},
+                     "frac": round(r64["gbs"] / HBM_PEAK_GBS, 4), "traffic": measured_traffic(),
+                     "step": "one A(v) = k_apply + k_apply_fin (shared nodes); traffic from "
+                             "profiles/apply_q4_400_fp64_pmc.md"},
         "fp32_moduli": {"us_per_product": round(r32["ms"] * 1e3, 3), "gbs": round(r32["gbs"], 2),
                         "bytes_per_step": r32["bytes"]},
         "q9_200x200_fp64": {"us_per_product": round(q9["ms"] * 1e3, 3), "gbs": round(q9["gbs"], 2),
