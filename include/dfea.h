@@ -6,7 +6,7 @@
  *   - Every entry point returns 0 on success, DFEA_EINVAL (-1) for an invalid argument, or
  *     -(1000 + hipError_t) when a HIP call failed.  dfea_last_error() gives a message for the last failure.
  *   - All work is enqueued on the caller's stream (hipStream_t passed as void*); no entry point
- *     synchronises the stream except where stated (dfea_pcg_run reads a status block at chunk ends).
+ *     synchronises the stream (callers read the scalar block, e.g. the PCG status, when they choose).
  *   - The caller allocates and owns every buffer.  Descriptors (dfea_desc) hold device pointers into those
  *     buffers; a descriptor table in device memory describes a batch of independent solves.
  *   - Element kind (`nen` arguments): 4 (quad4), 9 (quad9), or 4 | DFEA_HALO for quad4 halo plans.
