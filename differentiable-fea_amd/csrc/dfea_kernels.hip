@@ -190,12 +190,18 @@ __device__ __forceinline__ bool in_colour(const ChunkView& ch, int t, int c) {
     return h >= ch.hcol[c] && h < ch.hcol[c + 1];
 }
 
+// work item of this workgroup; items == NULL: one solve (descriptor 0), block = workgroup index
 struct Item { int s, b; };
 __device__ __forceinline__ Item load_item(const int* items) {
     const int b = xcd_remap(blockIdx.x, gridDim.x);
     Item it;
-    it.s = items[2 * b];
-    it.b = items[2 * b + 1];
+    if (items) {
+        it.s = items[2 * b];
+        it.b = items[2 * b + 1];
+    } else {
+        it.s = 0;
+        it.b = b;
+    }
     return it;
 }
 
@@ -1221,33 +1227,33 @@ int dfea_chunk_threads(int nen) {
 }
 
 int dfea_geometry(const dfea_desc* table, const int32_t* items, int n_items, int nen, void* stream) {
-    if (!table || !items) return fail(DFEA_EINVAL, "dfea_geometry: bad argument");
+    if (!table) return fail(DFEA_EINVAL, "dfea_geometry: bad argument");
     return dispatch(nen, Launch_geometry{table, items, n_items, stream});
 }
 
 int dfea_moduli(const dfea_desc* table, const int32_t* items, int n_items, const int32_t* fin_items,
                 int n_fin_items, const double* u, int nen, int precision, void* stream) {
-    if (!table || !items || (precision != DFEA_FP64 && precision != DFEA_FP32))
+    if (!table || (precision != DFEA_FP64 && precision != DFEA_FP32))
         return fail(DFEA_EINVAL, "dfea_moduli: bad argument");
     return dispatch(nen, Launch_moduli{table, items, n_items, fin_items, n_fin_items, u, precision, stream});
 }
 
 int dfea_residual(const dfea_desc* table, const int32_t* items, int n_items, const int32_t* fin_items,
                   int n_fin_items, const double* u, double* R, double lf, int nen, void* stream) {
-    if (!table || !items) return fail(DFEA_EINVAL, "dfea_residual: bad argument");
+    if (!table) return fail(DFEA_EINVAL, "dfea_residual: bad argument");
     return dispatch(nen, Launch_residual{table, items, n_items, fin_items, n_fin_items, u, R, lf, stream});
 }
 
 int dfea_apply(const dfea_desc* table, const int32_t* items, int n_items, const int32_t* fin_items,
                int n_fin_items, const void* v, void* y, int nen, int precision, void* stream) {
-    if (!table || !items || (precision != DFEA_FP64 && precision != DFEA_FP32))
+    if (!table || (precision != DFEA_FP64 && precision != DFEA_FP32))
         return fail(DFEA_EINVAL, "dfea_apply: bad argument");
     return dispatch(nen, Launch_apply{table, items, n_items, fin_items, n_fin_items, v, y, precision, stream});
 }
 
 int dfea_dot(const dfea_desc* table, const int32_t* items, int n_items, const int32_t* solve_items, int n_solves,
              const double* x, const double* y, void* stream) {
-    if (!table || !items || !solve_items) return fail(DFEA_EINVAL, "dfea_dot: bad argument");
+    if (!table) return fail(DFEA_EINVAL, "dfea_dot: bad argument");
     int rc = launch(k_dot_part, n_items, VBS, stream, "k_dot_part", table, items, x, y);
     if (rc) return rc;
     return launch(k_dot_final, n_solves, VBS, stream, "k_dot_final", table, solve_items);
@@ -1255,7 +1261,7 @@ int dfea_dot(const dfea_desc* table, const int32_t* items, int n_items, const in
 
 int dfea_axpy(const dfea_desc* table, const int32_t* items, int n_items, int mode, const double* u, const void* dx,
               const double* uD, const double* sarg, double* out, int precision, void* stream) {
-    if (!table || !items || (mode != 0 && mode != 1 && mode != 2) || (mode != 2 && !sarg))
+    if (!table || (mode != 0 && mode != 1 && mode != 2) || (mode != 2 && !sarg))
         return fail(DFEA_EINVAL, "dfea_axpy: bad argument");
     if (precision == DFEA_FP64)
         return launch(k_axpy<double>, n_items, VBS, stream, "k_axpy", table, items, mode, u,
@@ -1266,7 +1272,7 @@ int dfea_axpy(const dfea_desc* table, const int32_t* items, int n_items, int mod
 
 int dfea_pcg_init(const dfea_desc* table, const int32_t* vitems, int n_vitems, const int32_t* solve_items,
                   int n_solves, int neg_res, int precision, void* stream) {
-    if (!table || !vitems || !solve_items) return fail(DFEA_EINVAL, "dfea_pcg_init: bad argument");
+    if (!table) return fail(DFEA_EINVAL, "dfea_pcg_init: bad argument");
     int rc = precision == DFEA_FP64
                  ? launch(k_pcg_init<double>, n_vitems, VBS, stream, "k_pcg_init", table, vitems, neg_res)
                  : launch(k_pcg_init<float>, n_vitems, VBS, stream, "k_pcg_init", table, vitems, neg_res);
@@ -1276,7 +1282,7 @@ int dfea_pcg_init(const dfea_desc* table, const int32_t* vitems, int n_vitems, c
 
 int dfea_pcg_iterate(const dfea_desc* table, const int32_t* items, int n_items, const int32_t* vitems,
                      int n_vitems, int n_iter, int nen, int precision, void* stream) {
-    if (!table || !items || !vitems || n_iter < 0 || (precision != DFEA_FP64 && precision != DFEA_FP32))
+    if (!table || n_iter < 0 || (precision != DFEA_FP64 && precision != DFEA_FP32))
         return fail(DFEA_EINVAL, "dfea_pcg_iterate: bad argument");
     return dispatch(nen, Launch_pcg_iterate{table, items, n_items, vitems, n_vitems, n_iter, precision, stream});
 }
@@ -1326,7 +1332,7 @@ int dfea_graph_destroy(void* graph) {
 
 int dfea_adjoint_grad(const dfea_desc* table, const int32_t* items, int n_items, const double* u, const double* lam,
                       double* g, int nen, void* stream) {
-    if (!table || !items || !u || !lam || !g) return fail(DFEA_EINVAL, "dfea_adjoint_grad: bad argument");
+    if (!table || !u || !lam || !g) return fail(DFEA_EINVAL, "dfea_adjoint_grad: bad argument");
     return dispatch(nen, Launch_adjoint{table, items, n_items, u, lam, g, stream});
 }
 

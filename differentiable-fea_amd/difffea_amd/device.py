@@ -33,17 +33,10 @@ def _dev_int32(a, device):
     return torch.from_numpy(np.ascontiguousarray(a).view(np.int32)).to(device)
 
 
-def _items(s, n, device):
-    it = np.zeros((max(n, 0), 2), np.int32)
-    it[:, 0] = s
-    it[:, 1] = np.arange(n)
-    return torch.from_numpy(it).to(device)
-
-
 class HipMesh:
     """Mesh plan + geometry on the device.  ``params``: the reference's params contract."""
 
-    def __init__(self, params, device=None, plan_mode="auto"):
+    def __init__(self, params, device=None, plan_mode="auto", plan=None):
         self.lib = _lib.load()
         device = torch.device(device or "cuda")
         if device.type != "cuda":
@@ -61,8 +54,12 @@ class HipMesh:
             raise ValueError(f"the hip kernels support quad4 with 2x2 and quad9 with 3x3 Gauss points, "
                              f"got nen={self.nen}, n_q={self.nq}")
         cij = params.get("cell_ij")
-        self.plan = p = MeshPlan(nodes.numpy(), conn.numpy(), fixed.numpy(), mode=plan_mode,
-                                 cell_ij=None if cij is None else cij.detach().cpu().numpy())
+        # plan: a MeshPlan of exactly these tables, built beforehand (e.g. while the GPU was busy)
+        self.plan = p = plan if plan is not None else MeshPlan(
+            nodes.numpy(), conn.numpy(), fixed.numpy(), mode=plan_mode,
+            cell_ij=None if cij is None else cij.detach().cpu().numpy())
+        if p.n_el != conn.shape[0] or p.n_nodes != nodes.shape[0]:
+            raise ValueError("the plan was built for another mesh")
         # element kind of every launch: nen, flagged for halo plans
         self.kind = self.nen | (_lib.HALO if p.mode == "halo" else 0)
         if p.validate() > self.lib.dfea_chunk_threads(self.kind):
@@ -98,10 +95,8 @@ class HipMesh:
         self.n_finblk = self.lib.dfea_finblk(p.n_shared)
         self.red_size = self.lib.dfea_red_size(p.n_chunks, self.n_finblk, self.n_vblk)
         # single-solve item tables (solve index 0)
-        self.items = _items(0, p.n_chunks, dv)
-        self.fin_items = _items(0, self.n_finblk, dv) if self.n_finblk else torch.zeros(1, 2, dtype=torch.int32, device=dv)
-        self.vitems = _items(0, self.n_vblk, dv)
-        self.solve_items = _items(0, 1, dv)
+        # one solve: NULL item tables (solve 0, blocks 0 .. n-1)
+        self.items = self.fin_items = self.vitems = self.solve_items = None
         # geometry, once
         d = self.fill_desc(_lib.Desc())
         tab = self.upload(d)

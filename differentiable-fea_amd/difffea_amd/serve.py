@@ -27,6 +27,7 @@ import torch
 
 from . import _lib
 from .device import HipMesh, HipSystem, _ptr, _stream
+from .plan import MeshPlan
 from .problem import build_problem
 
 MIXED_RTOL_FLOOR = 1e-6
@@ -230,12 +231,34 @@ class Server:
         self.free = list(range(max_slots))[::-1]
         self.jobs = {}
         self.used_bytes = 0
+        self._item_cache = {}
+        self.prep_per_round = 2
+
+    # -- host <-> device without stalls ----------------------------------------------------------------------
+    def _upload(self, a):
+        """Host array -> device tensor through pinned memory, ordered on the stream, without a host wait."""
+        return torch.from_numpy(np.ascontiguousarray(a)).pin_memory().to(self.device, non_blocking=True)
+
+    def _set_sc(self, slots, field, values):
+        """sc[slot, field] = value for every pair, as one device scatter (no host wait)."""
+        idx = np.asarray(slots, np.int64) * _lib.NSC + _lib.SC[field]
+        self.sc.view(-1).index_copy_(0, self._upload(idx), self._upload(np.asarray(values, np.float64)))
 
     # -- admission -----------------------------------------------------------------------------------------
-    def _admit(self, req):
-        slot = self.free.pop()
+    @staticmethod
+    def prepare(req):
+        """The host-only part of admitting a request (problem set-up and mesh plan); it needs no GPU and
+        runs while the GPU works on a round."""
         params, theta = request_problem(req)
-        mesh = HipMesh(params, self.device)
+        cij = params.get("cell_ij")
+        plan = MeshPlan(params["nodes"].numpy(), params["conn"].numpy(), params["fixed"].numpy(),
+                        cell_ij=None if cij is None else cij.numpy())
+        return params, theta, plan
+
+    def _admit(self, req, prepared=None):
+        slot = self.free.pop()
+        params, theta, plan = prepared if prepared is not None else self.prepare(req)
+        mesh = HipMesh(params, self.device, plan=plan)
         mp = params["mat_p"]
         sys_ = HipSystem(mesh, theta, mp, params["f"], self.precision, uD=params["u_D"], sc=self.sc[slot])
         sys_._ensure_moduli_buffers()
@@ -248,6 +271,9 @@ class Server:
         job.items = {"chunk": self._items(slot, pl.n_chunks), "fin": self._items(slot, mesh.n_finblk),
                      "vec": self._items(slot, mesh.n_vblk), "solve": self._items(slot, 1)}
         job.tol = self.rel_tol * reference_residual_norm(sys_, params, self.device)
+        if req.adjoint:              # right-hand side of the compliance adjoint, m f
+            job.bvec = (params["m"] * params["f"]).to(self.device, sys_.vdt)
+        self._item_cache.clear()
         job.saved = torch.zeros_like(sys_.vec["uacc"])
         job.gen = solve_ops(job, job.tol, cg_max_iter=self.cg_max_iter)
         job.op = next(job.gen)
@@ -256,6 +282,7 @@ class Server:
         return job
 
     def _retire(self, job, info):
+        self._item_cache.clear()
         self.free.append(job.slot)
         self.used_bytes -= job.bytes
         del self.jobs[job.slot]
@@ -269,14 +296,24 @@ class Server:
         return it
 
     def _dev_items(self, jobs, kind):
+        """Concatenated work items of the jobs on the device (cached while the set of jobs is unchanged)."""
+        key = (kind, tuple(j.slot for j in jobs))
+        hit = self._item_cache.get(key)
+        if hit is not None:
+            return hit
         arrs = [j.items[kind] for j in jobs if len(j.items[kind])]
         if not arrs:
-            return None, 0
-        a = np.concatenate(arrs)
-        return torch.from_numpy(a).to(self.device, non_blocking=False), len(a)
+            hit = (None, 0)
+        else:
+            a = np.concatenate(arrs)
+            hit = (self._upload(a), len(a))
+        self._item_cache[key] = hit
+        return hit
 
     # -- one round -------------------------------------------------------------------------------------------
-    def _round(self):
+    def _round(self, host_work=None):
+        """One round: enqueue every job's next operation (no host waits), run ``host_work`` while the GPU
+        executes, then one sync reads every job's scalar block."""
         lib, T, st = self.lib, _ptr(self.table), _stream()
         by = {}
         for j in self.jobs.values():
@@ -292,7 +329,7 @@ class Server:
             for j in js:
                 sarg_host[j.slot] = j.op[1]
         if sarg_host is not None:
-            self.sarg.copy_(torch.from_numpy(sarg_host))
+            self.sarg.copy_(torch.from_numpy(sarg_host).pin_memory(), non_blocking=True)
         for kind, mode in (("lift", 0), ("trial", 1), ("accept", 2)):
             js = by.get(kind, [])
             for prec in (_lib.FP64, _lib.FP32):
@@ -307,8 +344,7 @@ class Server:
         # residual + norm (lf per solve in sc[LF])
         js = by.get("resnorm", [])
         if js:
-            idx = torch.tensor([j.slot for j in js])
-            self.sc[idx, _lib.SC["lf"]] = torch.tensor([float(j.op[1]) for j in js], dtype=torch.float64).to(self.device)
+            self._set_sc([j.slot for j in js], "lf", [float(j.op[1]) for j in js])
             for nen in _kinds(js):
                 sub = [j for j in js if j.mesh.kind == nen]
                 it, n = self._dev_items(sub, "chunk")
@@ -333,20 +369,18 @@ class Server:
         for j in new:
             j.pcg_rounds = 0
         if new:
-            idx = torch.tensor([j.slot for j in new])
             adj_rtol = max(self.adjoint_rtol, MIXED_RTOL_FLOOR) if self.precision == "mixed" else self.adjoint_rtol
-            vals = torch.tensor([[float(j.op[1]) if j.op[0] == "pcg" else adj_rtol,
-                                  float(j.op[2]) if j.op[0] == "pcg" else float(self.cg_max_iter)] for j in new],
-                                dtype=torch.float64)
-            self.sc[idx, _lib.SC["rtol"]] = vals[:, 0].to(self.device)
-            self.sc[idx, _lib.SC["maxit"]] = vals[:, 1].to(self.device)
+            slots = [j.slot for j in new]
+            self._set_sc(slots, "rtol", [float(j.op[1]) if j.op[0] == "pcg" else adj_rtol for j in new])
+            self._set_sc(slots, "maxit", [float(j.op[2]) if j.op[0] == "pcg" else float(self.cg_max_iter)
+                                          for j in new])
             for neg in (True, False):
                 sub = [j for j in new if (j.op[0] == "pcg") == neg]
                 if not sub:
                     continue
                 if not neg:                      # adjoint right-hand side m f (compliance), and the moduli at u*
                     for j in sub:
-                        j.sys.vec["b"].copy_((j.params["m"] * j.params["f"]).to(self.device, j.sys.vdt))
+                        j.sys.vec["b"].copy_(j.bvec)
                 it, n = self._dev_items(sub, "vec")
                 si, ns = self._dev_items(sub, "solve")
                 _lib.check(lib.dfea_pcg_init(T, _ptr(it), n, _ptr(si), ns, int(neg), self.prec_id, st), "dfea_pcg_init")
@@ -365,6 +399,8 @@ class Server:
                 vi, nv = self._dev_items(sub, "vec")
                 _lib.check(lib.dfea_pcg_iterate(T, _ptr(it), n, _ptr(vi), nv, chunk, nen, self.prec_id, st),
                            "dfea_pcg_iterate")
+        if host_work is not None:
+            host_work()
         # one host sync: status, iterations, norms
         sc = self.sc.cpu().numpy()
         finished = []
@@ -397,8 +433,19 @@ class Server:
         """Serve ``requests`` (sorted by arrival).  Returns the per-request records and a summary."""
         pending = sorted(requests, key=lambda r: r.arrival)
         records = []
+        prepared = {}
         t0 = time.perf_counter()
         i = 0
+
+        def prepare_arrived():
+            # host-only admission work for requests that have arrived, while the GPU runs the round
+            now, k, done = time.perf_counter() - t0, i, 0
+            while k < len(pending) and done < self.prep_per_round and (not realtime or pending[k].arrival <= now):
+                if k not in prepared:
+                    prepared[k] = self.prepare(pending[k])
+                    done += 1
+                k += 1
+
         while i < len(pending) or self.jobs:
             now = time.perf_counter() - t0
             while i < len(pending) and (not realtime or pending[i].arrival <= now) and self.free:
@@ -406,14 +453,16 @@ class Server:
                 fb = footprint_bytes(r.element_type, r.n, self.precision)
                 if self.jobs and self.used_bytes + fb > self.hbm_budget:
                     break
-                job = self._admit(r)
+                if self.jobs and i not in prepared:
+                    break                      # its host set-up runs during the next round
+                job = self._admit(r, prepared.pop(i, None))
                 job.t_arrival = t0 + r.arrival if realtime else job.t_admit
                 i += 1
             if not self.jobs:
                 if i < len(pending):
                     time.sleep(max(0.0, min(0.01, pending[i].arrival - (time.perf_counter() - t0))))
                 continue
-            for job, info in self._round():
+            for job, info in self._round(prepare_arrived):
                 t_done = time.perf_counter()
                 rec = dict(rid=job.req.rid, element_type=job.req.element_type, n=job.req.n, adjoint=job.req.adjoint,
                            n_dof=job.mesh.n_dof, tol_abs=job.tol, latency=t_done - job.t_arrival,

@@ -10,8 +10,9 @@
  *   - The caller allocates and owns every buffer.  Descriptors (dfea_desc) hold device pointers into those
  *     buffers; a descriptor table in device memory describes a batch of independent solves.
  *   - Element kind (`nen` arguments): 4 (quad4), 9 (quad9), or 4 | DFEA_HALO for quad4 halo plans.
- *   - Work items: an int32 pair {solve index, block index} per workgroup.  For one solve, the items are
- *     simply {0, 0}, {0, 1}, ... ; for a batch, the concatenation over solves.
+ *   - Work items: an int32 pair {solve index, block index} per workgroup; for a batch, the concatenation
+ *     over solves.  Any item table may be NULL for one solve: solve 0 (the table's first descriptor),
+ *     blocks 0 .. n_items-1, which saves the workgroups one dependent load.
  *
  * Data layout in HBM (internal element order = chunk order, see dfea_chunk)
  *   Per-element arrays are planes of n_cols = n_el + n_halo columns: column e < n_el is element e, column
