@@ -85,10 +85,10 @@ def time_apply(p, u, v, theta, precision, steps, warmup):
 
 
 def time_pcg_iteration(p, u, v, theta, precision, iters=400, chunk=50):
-    """One fused PCG iteration (direction/product kernel + update kernel) on the same mesh and state,
-    replayed from HIP graphs of `chunk` iterations; rtol = 0 keeps the solve running.  Algorithmic bytes per
-    iteration: the A(v) bytes plus 10 dof vectors (z, p, p', A p written / read again, x, r read and
-    written, diag, z written)."""
+    """One fused PCG iteration (one kernel: scalar step, update, next product) on the same mesh and state,
+    replayed from HIP graphs of `chunk` kernels; rtol = 0 keeps the solve running.  Algorithmic bytes per
+    iteration: the A(v) bytes plus 11 dof vectors (x, r, p, s = A p, w = A M r read and written, diag read;
+    v read / y written are already in the A(v) bytes, so 9 more)."""
     from difffea_amd import _lib
     from difffea_amd.operators import make_system
     dev = torch.device("cuda")
@@ -106,12 +106,12 @@ def time_pcg_iteration(p, u, v, theta, precision, iters=400, chunk=50):
     e1.record()
     torch.cuda.synchronize()
     st, _, k = s.pcg_status()
-    if st != _lib.PCG_RUNNING or k != n + chunk:
+    if st != _lib.PCG_RUNNING or k != n + chunk - 1:        # the first kernel only forms A M b
         raise RuntimeError(f"PCG timing run stopped early (status {st}, k {k})")
     us = e0.elapsed_time(e1) * 1e3 / n
     m = s.mesh
     sz = 8 if precision == "fp64" else 4
-    B = b_alg(m.n_el, m.nen, m.nq, m.n_dof, sz) + 10 * m.n_dof * sz
+    B = b_alg(m.n_el, m.nen, m.nq, m.n_dof, sz) + 9 * m.n_dof * sz
     return dict(us_per_iteration=round(us, 3), gbs=round(B / (us * 1e-6) / 1e9, 2), bytes_per_iteration=B,
                 iterations=n)
 

@@ -190,6 +190,11 @@ __device__ __forceinline__ bool in_colour(const ChunkView& ch, int t, int c) {
     return h >= ch.hcol[c] && h < ch.hcol[c + 1];
 }
 
+// reduction partials (dfea_desc.red): [0, 6 n_chunks) the PCG kernel's chunk partials (two parities x
+// {r.u, w.u, r.r} per chunk), then n_finblk unused, then 3 n_vblk for the vector kernels
+__device__ __forceinline__ double* pcg_red(const dfea_desc& D, int par) { return D.red + 3 * par * D.n_chunks; }
+__device__ __forceinline__ double* vec_red(const dfea_desc& D) { return D.red + 6 * D.n_chunks + D.n_finblk; }
+
 // work item of this workgroup; items == NULL: one solve (descriptor 0), block = workgroup index
 struct Item { int s, b; };
 __device__ __forceinline__ Item load_item(const int* items) {
@@ -283,137 +288,44 @@ __device__ __forceinline__ void colour_accumulate(const ChunkView& ch, int t, co
 }
 
 // ---- tangent product  y = m J (m v) + (1 - m) v  (`operators.py:385-412`) ----------------------------------
-// PCG: the kernel opens iteration k with the scalar step that closes iteration k-1 (pcg_step); the direction
-// update p' = z + beta p happens on the gather (p read from the buffer of iteration k's parity, p' written
-// to the other one), and partials of p'.Ap' are produced, shared nodes included.
-// The element data of the first PF Gauss points is loaded before the gather barrier, so the HBM stream
-// starts at once instead of after the gather's two dependent round trips.
-template <typename T>
-__device__ __forceinline__ void pcg_dir_buffers(double k, const dfea_desc& D, const T*& p_rd, T*& p_wr) {
-    const bool odd = ((int)k) & 1;
-    p_rd = static_cast<const T*>(D.vec[odd ? DFEA_P2 : DFEA_P]);
-    p_wr = static_cast<T*>(D.vec[odd ? DFEA_P : DFEA_P2]);
+// Shared by the product kernel and the PCG kernel: the thread's element, the first PF Gauss points' moduli
+// and gradients loaded before the gather barrier (so the HBM stream starts at once instead of after the
+// gather's dependent round trips), and the element product accumulated colour by colour into LDS.
+
+// Thread t's element: partials plans cut the internal element order into chunks of exactly CE elements (the
+// host checks it), so the element follows from the workgroup index alone and its loads need not wait for the
+// chunk record; halo plans read the chunk record.
+template <int NEN, int BS>
+__device__ __forceinline__ bool chunk_element(const dfea_desc& D, const ChunkView& ch, int blk, int t, int& e, int* l) {
+    constexpr int CE = NEN == 4 ? CHUNK_Q4 : CHUNK_Q9;
+    if (BS == CE) {
+        e = blk * CE + t;
+        if (e >= D.n_el) return false;
+#pragma unroll
+        for (int a = 0; a < NEN; ++a) l[a] = D.lelem[a * D.n_el + e];
+        return true;
+    }
+    return thread_element<NEN>(D, ch, t, e, l);
 }
 
-// The scalar step between PCG iterations k-1 and k (`newton_krylov.py:78-86`): convergence test on the
-// updated residual, beta = rz' / rz, iteration cap.  Every workgroup of the solve evaluates it from the same
-// partials in the same order, so all take the same decision; workgroup 0 hands the outcome to the update
-// kernel in the DFEA_SC_A_* slots (this kernel reads only the caller-visible slots, which the update kernel
-// writes).
-struct PcgStep {
-    double beta, k;
-    bool running;
-};
-
-template <int BS>
-__device__ PcgStep pcg_step(const dfea_desc& D, int blk, double* s_red, double* s_b) {
-    const double* sc = D.sc;
-    double st = sc[DFEA_SC_STATUS];
-    const double k = sc[DFEA_SC_K];
-    double rz = sc[DFEA_SC_RZ], rr = sc[DFEA_SC_RR], iters = sc[DFEA_SC_ITERS], beta = 0.0;
-    if (st == DFEA_PCG_RUNNING && k > 0.0) {            // consume the partials of update k-1
-        const double* ru = D.red + D.n_chunks + D.n_finblk;
-        double a = 0.0, b = 0.0;
-        for (int i = threadIdx.x; i < D.n_vblk; i += BS) {
-            a += ru[2 * i];
-            b += ru[2 * i + 1];
-        }
-        a = block_sum<BS>(a, s_red);
-        b = block_sum<BS>(b, s_red);
-        const double rz_new = bcast0(a, s_b);
-        rr = bcast0(b, s_b);
-        if (sqrt(rr) <= sc[DFEA_SC_RTOL] * sqrt(sc[DFEA_SC_BB])) {
-            st = DFEA_PCG_CONVERGED;
-            iters = k;
-        } else {
-            beta = rz_new / rz;
-            rz = rz_new;
-        }
+template <int NEN, int NQ, typename T>
+__device__ __forceinline__ void load_gauss_point(const T* dN, const T* C, int ne, int e, int q, T (&d)[NEN][2],
+                                                 T (&c)[10]) {
+#pragma unroll
+    for (int a = 0; a < NEN; ++a) {
+        d[a][0] = dN[(size_t)((q * NEN + a) * 2) * ne + e];
+        d[a][1] = dN[(size_t)((q * NEN + a) * 2 + 1) * ne + e];
     }
-    if (st == DFEA_PCG_RUNNING && k >= sc[DFEA_SC_MAXIT]) {
-        st = DFEA_PCG_MAXITER;
-        iters = sc[DFEA_SC_MAXIT];
-    }
-    if (blk == 0 && threadIdx.x == 0) {
-        double* w = D.sc;
-        w[DFEA_SC_A_STATUS] = st;
-        w[DFEA_SC_A_K] = k;
-        w[DFEA_SC_A_ITERS] = iters;
-        w[DFEA_SC_A_RZ] = rz;
-        w[DFEA_SC_A_BETA] = beta;
-        w[DFEA_SC_A_RR] = rr;
-    }
-    return PcgStep{beta, k, st == DFEA_PCG_RUNNING};
+#pragma unroll
+    for (int k = 0; k < 10; ++k) c[k] = C[(size_t)(k * NQ + q) * ne + e];
 }
 
-// PF: Gauss points whose data is loaded before the gather barrier; MINW: minimum waves per SIMD the
-// register allocation must allow (1 = no constraint)
-template <int NEN, int NQ, int BS, typename T, bool PCG, int PF = 2, int MINW = 1>
-__global__ __launch_bounds__(BS, MINW) void k_apply(const dfea_desc* __restrict__ tab, const int* __restrict__ items,
-                                                   const T* __restrict__ vin, T* __restrict__ vout) {
-    __shared__ ChunkLDS<T> L;
-    const Item it = load_item(items);
-    const dfea_desc& D = tab[it.s];
-    const ChunkView ch = chunk_view(D.chunks + it.b);
-    const int t = threadIdx.x;
-    const T* v = PCG ? nullptr : (vin ? vin : static_cast<const T*>(D.vec[DFEA_P]));
-    T* y = PCG ? static_cast<T*>(D.vec[DFEA_AP]) : (vout ? vout : static_cast<T*>(D.vec[DFEA_AP]));
-    const T* z = static_cast<const T*>(D.vec[DFEA_Z]);
-    const int ne = D.n_cols;
-    const T* dN = std::is_same<T, double>::value ? (const T*)D.dN64 : (const T*)D.dN32;
-    const T* C = std::is_same<T, double>::value ? (const T*)D.C64 : (const T*)D.C32;
-
-    // 0. element data of the first PF Gauss points, local node indices
-    int l[NEN];
-    int e;
-    const bool has_el = thread_element<NEN>(D, ch, t, e, l);
-    T d[NQ][NEN][2];
-    T c[NQ][10];
-    if (has_el) {
-#pragma unroll
-        for (int q = 0; q < PF; ++q) {
-#pragma unroll
-            for (int a = 0; a < NEN; ++a) {
-                d[q][a][0] = dN[(size_t)((q * NEN + a) * 2) * ne + e];
-                d[q][a][1] = dN[(size_t)((q * NEN + a) * 2 + 1) * ne + e];
-            }
-#pragma unroll
-            for (int k = 0; k < 10; ++k) c[q][k] = C[(size_t)(k * NQ + q) * ne + e];
-        }
-    }
-
-    // PCG: scalar step (uniform over the workgroup), direction buffers of iteration k
-    double beta = 0.0;
-    const T* p_rd = nullptr;
-    T* p_wr = nullptr;
-    if (PCG) {
-        const PcgStep ps = pcg_step<BS>(D, it.b, L.red, L.red + 15);
-        if (!ps.running) return;
-        beta = ps.beta;
-        pcg_dir_buffers<T>(ps.k, D, p_rd, p_wr);
-    }
-
-    // 1. gather
-    for (int k = t; k < ch.nk; k += BS) {
-        const uint32_t g = D.lnode[ch.k0 + k];
-        const int n = (int)(g & DFEA_NODE_MASK);
-        V2<T> val;
-        if (PCG) {
-            const V2<T> zz = reinterpret_cast<const V2<T>*>(z)[n];
-            const V2<T> pp = reinterpret_cast<const V2<T>*>(p_rd)[n];
-            val.x = (T)((double)zz.x + beta * (double)pp.x);
-            val.y = (T)((double)zz.y + beta * (double)pp.y);
-            if (k < ch.nown) reinterpret_cast<V2<T>*>(p_wr)[n] = val;   // owned: written by this chunk only
-        } else {
-            val = reinterpret_cast<const V2<T>*>(v)[n];
-        }
-        L.v[k] = val;
-        L.acc[k] = V2<T>{0, 0};
-        L.bits[k] = (unsigned char)(g >> 30);
-    }
-    __syncthreads();
-
-    // 2. element
+// fe = K_e (m v_e) from the gathered LDS values, then the colour-ordered sum into L.acc.  d / c hold the
+// first PF Gauss points' data on entry; the rest is loaded here.
+template <int NEN, int NQ, int PF, typename T>
+__device__ __forceinline__ void element_product(const ChunkView& ch, int t, bool has_el, const int* l, const T* dN,
+                                                const T* C, int ne, int e, T (&d)[NQ][NEN][2], T (&c)[NQ][10],
+                                                ChunkLDS<T>& L) {
     V2<T> fe[NEN];
     if (has_el) {
         V2<T> ve[NEN];
@@ -427,15 +339,7 @@ __global__ __launch_bounds__(BS, MINW) void k_apply(const dfea_desc* __restrict_
         }
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
-            if (q >= PF) {
-#pragma unroll
-                for (int a = 0; a < NEN; ++a) {
-                    d[q][a][0] = dN[(size_t)((q * NEN + a) * 2) * ne + e];
-                    d[q][a][1] = dN[(size_t)((q * NEN + a) * 2 + 1) * ne + e];
-                }
-#pragma unroll
-                for (int k = 0; k < 10; ++k) c[q][k] = C[(size_t)(k * NQ + q) * ne + e];
-            }
+            if (q >= PF) load_gauss_point<NEN, NQ, T>(dN, C, ne, e, q, d[q], c[q]);
             T g[4] = {0, 0, 0, 0};    // g[2i+j] = sum_a ve[a][i] dN[a][j]
 #pragma unroll
             for (int a = 0; a < NEN; ++a) {
@@ -460,9 +364,48 @@ __global__ __launch_bounds__(BS, MINW) void k_apply(const dfea_desc* __restrict_
         }
     }
     colour_accumulate<T, NEN>(ch, t, l, fe, L.acc);
+}
+
+// PF: Gauss points whose data is loaded before the gather barrier; MINW: minimum waves per SIMD the
+// register allocation must allow (1 = no constraint)
+template <int NEN, int NQ, int BS, typename T, int PF = 2, int MINW = 1>
+__global__ __launch_bounds__(BS, MINW) void k_apply(const dfea_desc* __restrict__ tab, const int* __restrict__ items,
+                                                   const T* __restrict__ vin, T* __restrict__ vout) {
+    __shared__ ChunkLDS<T> L;
+    const Item it = load_item(items);
+    const dfea_desc& D = tab[it.s];
+    const ChunkView ch = chunk_view(D.chunks + it.b);
+    const int t = threadIdx.x;
+    const T* v = vin ? vin : static_cast<const T*>(D.vec[DFEA_P]);
+    T* y = vout ? vout : static_cast<T*>(D.vec[DFEA_W]);
+    const int ne = D.n_cols;
+    const T* dN = std::is_same<T, double>::value ? (const T*)D.dN64 : (const T*)D.dN32;
+    const T* C = std::is_same<T, double>::value ? (const T*)D.C64 : (const T*)D.C32;
+
+    // 0. element, first PF Gauss points
+    int l[NEN];
+    int e;
+    const bool has_el = chunk_element<NEN, BS>(D, ch, it.b, t, e, l);
+    T d[NQ][NEN][2];
+    T c[NQ][10];
+    if (has_el) {
+#pragma unroll
+        for (int q = 0; q < PF; ++q) load_gauss_point<NEN, NQ, T>(dN, C, ne, e, q, d[q], c[q]);
+    }
+
+    // 1. gather
+    for (int k = t; k < ch.nk; k += BS) {
+        const uint32_t g = D.lnode[ch.k0 + k];
+        L.v[k] = reinterpret_cast<const V2<T>*>(v)[(int)(g & DFEA_NODE_MASK)];
+        L.acc[k] = V2<T>{0, 0};
+        L.bits[k] = (unsigned char)(g >> 30);
+    }
+    __syncthreads();
+
+    // 2. element products, colour-ordered sums
+    element_product<NEN, NQ, PF, T>(ch, t, has_el, l, dN, C, ne, e, d, c, L);
 
     // 3. owned nodes: final value; shared nodes (partials plans): partial
-    double dot = 0.0;
     V2<T>* part = reinterpret_cast<V2<T>*>(D.part);
     for (int k = t; k < ch.nk; k += BS) {
         if (k < ch.nown) {
@@ -472,31 +415,10 @@ __global__ __launch_bounds__(BS, MINW) void k_apply(const dfea_desc* __restrict_
             V2<T> out;
             out.x = (b & 1) ? raw.x : acc.x;
             out.y = (b & 2) ? raw.y : acc.y;
-            const int n = (int)(D.lnode[ch.k0 + k] & DFEA_NODE_MASK);
-            reinterpret_cast<V2<T>*>(y)[n] = out;
-            if (PCG) dot += (double)raw.x * (double)out.x + (double)raw.y * (double)out.y;
+            reinterpret_cast<V2<T>*>(y)[(int)(D.lnode[ch.k0 + k] & DFEA_NODE_MASK)] = out;
         } else if (D.n_part) {
-            const int slot = D.pslot[ch.pbase + (k - ch.nown)];
-            const V2<T> a = L.acc[k];
-            part[slot] = a;
-            if (PCG) {   // p.Ap over a shared node is linear in its partials: p . this chunk's partial
-                const V2<T> raw = L.v[k];
-                const unsigned char b = L.bits[k];
-                if (!(b & 1)) dot += (double)raw.x * (double)a.x;
-                if (!(b & 2)) dot += (double)raw.y * (double)a.y;
-                if (b) {     // fixed components (Ap = p there) once, by the chunk holding the node's first slot
-                    const int n = (int)(D.lnode[ch.k0 + k] & DFEA_NODE_MASK);
-                    if (slot == D.shslot[2 * n]) {
-                        if (b & 1) dot += (double)raw.x * (double)raw.x;
-                        if (b & 2) dot += (double)raw.y * (double)raw.y;
-                    }
-                }
-            }
+            part[D.pslot[ch.pbase + (k - ch.nown)]] = L.acc[k];
         }
-    }
-    if (PCG) {
-        const double s = block_sum<BS>(dot, L.red);
-        if (t == 0) D.red[it.b] = s;
     }
 }
 
@@ -508,7 +430,7 @@ __global__ __launch_bounds__(FBS) void k_apply_fin(const dfea_desc* __restrict__
     const dfea_desc& D = tab[it.s];
     const int i = it.b * FBS + threadIdx.x;
     const T* v = vin ? vin : static_cast<const T*>(D.vec[DFEA_P]);
-    T* y = vout ? vout : static_cast<T*>(D.vec[DFEA_AP]);
+    T* y = vout ? vout : static_cast<T*>(D.vec[DFEA_W]);
     if (i < D.n_shared) {
         const V2<T>* part = reinterpret_cast<const V2<T>*>(D.part);
         V2<T> acc{0, 0};
@@ -806,14 +728,14 @@ __global__ __launch_bounds__(VBS) void k_dot_part(const dfea_desc* __restrict__ 
         if (i < n) s += x[i] * y[i];
     }
     s = block_sum<VBS>(s, s_red);
-    if (threadIdx.x == 0) D.red[D.n_chunks + D.n_finblk + 2 * D.n_vblk + it.b] = s;
+    if (threadIdx.x == 0) vec_red(D)[2 * D.n_vblk + it.b] = s;
 }
 
 __global__ __launch_bounds__(VBS) void k_dot_final(const dfea_desc* __restrict__ tab, const int* __restrict__ items) {
     __shared__ double s_red[16];
     const Item it = load_item(items);
     const dfea_desc& D = tab[it.s];
-    const double s = sum_partials<VBS>(D.red + D.n_chunks + D.n_finblk + 2 * D.n_vblk, D.n_vblk, s_red);
+    const double s = sum_partials<VBS>(vec_red(D) + 2 * D.n_vblk, D.n_vblk, s_red);
     if (threadIdx.x == 0) D.sc[DFEA_SC_NORM] = s;
 }
 
@@ -847,7 +769,52 @@ __global__ __launch_bounds__(VBS) void k_axpy(const dfea_desc* __restrict__ tab,
     }
 }
 
-// PCG start: x = 0, r = b, z = r / diag, p = z; partials of r.z and b.b
+// ======================================================================================================
+// PCG (`newton_krylov.py:28-88`) in one kernel per iteration.
+//
+// The recurrences are those of Jacobi-PCG in the Chronopoulos-Gear arrangement, which yields the same iterates
+// as the textbook form in exact arithmetic but needs one reduction phase per iteration: with u = M r and
+// w = A u, gamma = r.u and delta = w.u are summed together, and
+//     beta_k = gamma_k / gamma_{k-1},  p.Ap = delta_k - beta_k gamma_k / alpha_{k-1},  alpha_k = gamma_k / p.Ap,
+//     p_k = u_k + beta_k p_{k-1},  s_k = w_k + beta_k s_{k-1} (= A p_k),
+//     x_{k+1} = x_k + alpha_k p_k,  r_{k+1} = r_k - alpha_k s_k,  u_{k+1} = M r_{k+1},  w_{k+1} = A u_{k+1}.
+// Kernel K_k (k = -1 computes w_0 = A u_0 only) opens with the scalar step from the previous kernel's chunk
+// partials, summed in a fixed order by every workgroup (all take the same decision): convergence of r_k,
+// the iteration cap, curvature p.Ap <= 0 (x = z = u_0 at k = 0) -- the reference's tests, in its order.  It
+// then updates every node of its chunk (shared nodes redundantly, with w of shared nodes summed from the
+// previous kernel's partials in slot order; owned nodes and a shared node's first-slot chunk write), and
+// applies A to u_{k+1} in LDS.  gamma and r.r come from the nodes the chunk writes; delta of a shared node is
+// linear in its partials (u . this chunk's partial, fixed components u^2 from the first-slot chunk).
+//
+// Kernels are launched in pairs with parity arguments 0, 1: K_k has parity (k + 1) mod 2, reads the state
+// buffers of parity 1 - par (x_k, r_k, p_{k-1}, s_{k-1}, w_k), the partials and the scalar set of parity
+// 1 - par / par, and writes those of the other parity, so no workgroup reads what another workgroup of the
+// same launch writes.  The solution ends in vec[DFEA_X].
+// ======================================================================================================
+enum { PS_ST = 0, PS_K = 1, PS_GAM = 2, PS_ALPHA = 3 };    // fields of a parity scalar set
+
+// the PCG state buffers of parity q (0: the slot, 1: its twin)
+template <typename T>
+struct PcgSet {
+    V2<T>* x;
+    V2<T>* r;
+    V2<T>* w;
+    V2<T>* p;
+    V2<T>* s;
+};
+
+template <typename T>
+__device__ __forceinline__ PcgSet<T> pcg_set(const dfea_desc& D, int q) {
+    PcgSet<T> v;
+    v.x = static_cast<V2<T>*>(D.vec[q ? DFEA_X2 : DFEA_X]);
+    v.r = static_cast<V2<T>*>(D.vec[q ? DFEA_R2 : DFEA_R]);
+    v.w = static_cast<V2<T>*>(D.vec[q ? DFEA_W2 : DFEA_W]);
+    v.p = static_cast<V2<T>*>(D.vec[q ? DFEA_P2 : DFEA_P]);
+    v.s = static_cast<V2<T>*>(D.vec[q ? DFEA_S2 : DFEA_S]);
+    return v;
+}
+
+// PCG start: x = 0, r = b, p = s = 0 (parity-0 buffers); partials of b.b
 template <typename T>
 __global__ __launch_bounds__(VBS) void k_pcg_init(const dfea_desc* __restrict__ tab, const int* __restrict__ items,
                                                  int neg_res) {
@@ -857,222 +824,274 @@ __global__ __launch_bounds__(VBS) void k_pcg_init(const dfea_desc* __restrict__ 
     const int n = ndof(D);
     T* x = static_cast<T*>(D.vec[DFEA_X]);
     T* r = static_cast<T*>(D.vec[DFEA_R]);
-    T* z = static_cast<T*>(D.vec[DFEA_Z]);
     T* p = static_cast<T*>(D.vec[DFEA_P]);
+    T* sv = static_cast<T*>(D.vec[DFEA_S]);
     const T* bT = static_cast<const T*>(D.vec[DFEA_B]);
     const double* res = static_cast<const double*>(D.vec[DFEA_RES]);
-    const T* dg = std::is_same<T, double>::value ? (const T*)D.diag64 : (const T*)D.diag32;
-    double rz = 0.0, bb = 0.0;
+    double bb = 0.0;
     const int base = it.b * VBLK;
 #pragma unroll
     for (int j = 0; j < VPT; ++j) {
         const int i = base + j * VBS + threadIdx.x;
         if (i >= n) continue;
         const T b = neg_res ? (T)(-res[i]) : bT[i];
-        const T zz = b / dg[i];
         x[i] = 0;
         r[i] = b;
-        z[i] = zz;
-        p[i] = zz;
-        rz += (double)b * (double)zz;
+        p[i] = 0;
+        sv[i] = 0;
         bb += (double)b * (double)b;
     }
-    rz = block_sum<VBS>(rz, s_red);
     bb = block_sum<VBS>(bb, s_red);
-    if (threadIdx.x == 0) {
-        double* ru = D.red + D.n_chunks + D.n_finblk;
-        ru[2 * it.b] = rz;
-        ru[2 * it.b + 1] = bb;
-    }
+    if (threadIdx.x == 0) vec_red(D)[it.b] = bb;
 }
 
+// ||b||^2 and the scalar block: both parity sets start the solve (k = -1: the first kernel computes w_0)
 __global__ __launch_bounds__(VBS) void k_pcg_init_final(const dfea_desc* __restrict__ tab, const int* __restrict__ items) {
     __shared__ double s_red[16];
     const Item it = load_item(items);
     const dfea_desc& D = tab[it.s];
-    const double* ru = D.red + D.n_chunks + D.n_finblk;
-    double rz = 0.0, bb = 0.0;
-    for (int i = threadIdx.x; i < D.n_vblk; i += VBS) {
-        rz += ru[2 * i];
-        bb += ru[2 * i + 1];
-    }
-    rz = block_sum<VBS>(rz, s_red);
-    bb = block_sum<VBS>(bb, s_red);
+    const double bb = sum_partials<VBS>(vec_red(D), D.n_vblk, s_red);
     if (threadIdx.x == 0) {
         double* sc = D.sc;
-        sc[DFEA_SC_RZ] = rz;
+        // nothing to do when the right-hand side is zero: x = 0 after 0 iterations
+        const double st = sqrt(bb) == 0.0 ? (double)DFEA_PCG_CONVERGED : (double)DFEA_PCG_RUNNING;
         sc[DFEA_SC_BB] = bb;
+        sc[DFEA_SC_RR] = bb;
+        sc[DFEA_SC_RZ] = 0.0;
+        sc[DFEA_SC_PAP] = 0.0;
+        sc[DFEA_SC_ALPHA] = 0.0;
         sc[DFEA_SC_BETA] = 0.0;
         sc[DFEA_SC_K] = 0.0;
-        sc[DFEA_SC_ALPHA] = 0.0;
-        sc[DFEA_SC_PAP] = 0.0;
-        sc[DFEA_SC_RR] = bb;
-        if (sqrt(bb) == 0.0) {            // nothing to do: the right-hand side is zero
-            sc[DFEA_SC_STATUS] = DFEA_PCG_CONVERGED;
-            sc[DFEA_SC_ITERS] = 0.0;
-        } else {
-            sc[DFEA_SC_STATUS] = DFEA_PCG_RUNNING;
-            sc[DFEA_SC_ITERS] = 0.0;
+        sc[DFEA_SC_ITERS] = 0.0;
+        sc[DFEA_SC_STATUS] = st;
+        for (int par = 0; par < 2; ++par) {
+            double* S = sc + (par ? DFEA_SC_S1 : DFEA_SC_S0);
+            S[PS_ST] = st;
+            S[PS_K] = -1.0;
+            S[PS_GAM] = 0.0;
+            S[PS_ALPHA] = 0.0;
         }
     }
 }
 
-// PCG iteration, second kernel (`newton_krylov.py:62-79`): alpha = rz / p.Ap (the chunk partials summed in
-// a fixed order by every workgroup), A p of shared nodes summed from their partials in slot order (the
-// direction of shared nodes, p = z + beta p, is formed and stored here), x += alpha p, r -= alpha A p,
-// z = r / diag, partials of r.z and r.r.  Reads only the DFEA_SC_A_* slots; workgroup 0 of each solve writes
-// the caller-visible state.  A thread handles NPT nodes (both dofs) of the workgroup's dof block.  Every
-// load that does not depend on a scalar (both direction buffers included) is issued before the scalars and
-// the p.Ap partials are read, so one HBM round trip covers all of them.
-constexpr int UPD_MAXP = 4;        // partials per shared node summed from registers (more: loop)
-
-template <typename T>
-__global__ __launch_bounds__(VBS) void k_pcg_update(const dfea_desc* __restrict__ tab, const int* __restrict__ items) {
-    constexpr int NPT = VPT / 2;
-    __shared__ double s_red[16];
-    __shared__ double s_b;
+// one PCG iteration (see above); par = launch parity; MINW: minimum waves per SIMD for the register budget
+template <int NEN, int NQ, int BS, typename T, int PF = 2, int MINW = 1>
+__global__ __launch_bounds__(BS, MINW) void k_pcg(const dfea_desc* __restrict__ tab, const int* __restrict__ items, int par) {
+    __shared__ ChunkLDS<T> L;
+    __shared__ double s_b[3];
     const Item it = load_item(items);
     const dfea_desc& D = tab[it.s];
-    const int nn = D.n_nodes;
-    const int base = it.b * (VBLK / 2);
-    V2<T>* x = static_cast<V2<T>*>(D.vec[DFEA_X]);
-    V2<T>* r = static_cast<V2<T>*>(D.vec[DFEA_R]);
-    V2<T>* z = static_cast<V2<T>*>(D.vec[DFEA_Z]);
-    V2<T>* Ap = static_cast<V2<T>*>(D.vec[DFEA_AP]);
-    V2<T>* P0 = static_cast<V2<T>*>(D.vec[DFEA_P]);
-    V2<T>* P1 = static_cast<V2<T>*>(D.vec[DFEA_P2]);
+    double* sc = D.sc;
+    const double* S = sc + (par ? DFEA_SC_S1 : DFEA_SC_S0);
+    if (S[PS_ST] != DFEA_PCG_RUNNING) return;
+    const double k = S[PS_K];
+    const ChunkView ch = chunk_view(D.chunks + it.b);
+    const int t = threadIdx.x;
+    const bool lead = it.b == 0 && t == 0;
+    const int ne = D.n_cols;
+    const T* dN = std::is_same<T, double>::value ? (const T*)D.dN64 : (const T*)D.dN32;
+    const T* C = std::is_same<T, double>::value ? (const T*)D.C64 : (const T*)D.C32;
     const V2<T>* dg = reinterpret_cast<const V2<T>*>(std::is_same<T, double>::value ? (const T*)D.diag64
                                                                                        : (const T*)D.diag32);
-    const V2<T>* part = reinterpret_cast<const V2<T>*>(D.part);
+    const int qr = k < 0.0 ? par : 1 - par;     // state buffers read (x_k, r_k, p_{k-1}, s_{k-1}, w_k)
+    const PcgSet<T> Rd = pcg_set<T>(D, qr);
+    const PcgSet<T> Wr = pcg_set<T>(D, par);   // state buffers written
+    const V2<T>* part_rd = reinterpret_cast<const V2<T>*>(D.part) + (size_t)(1 - par) * D.n_part;
+    V2<T>* part_wr = reinterpret_cast<V2<T>*>(D.part) + (size_t)par * D.n_part;
     const int2* shs = reinterpret_cast<const int2*>(D.shslot);
 
-    // 1. loads independent of the scalars
-    int2 ss[NPT];
-    V2<T> xv[NPT], rv[NPT], dv[NPT], av[NPT], zv[NPT], p0[NPT], p1[NPT];
-    V2<T> pq[NPT][UPD_MAXP];
-    unsigned char fx[NPT];
+    // element and its first Gauss points (independent of the scalars)
+    int l[NEN];
+    int e;
+    const bool has_el = chunk_element<NEN, BS>(D, ch, it.b, t, e, l);
+    T d[NQ][NEN][2];
+    T c[NQ][10];
+    if (has_el) {
 #pragma unroll
-    for (int j = 0; j < NPT; ++j) {
-        const int nd = base + j * VBS + threadIdx.x;
-        ss[j] = make_int2(0, 0);
-        if (nd < nn) {
-            ss[j] = shs[nd];
-            xv[j] = x[nd];
-            rv[j] = r[nd];
-            dv[j] = dg[nd];
-            av[j] = Ap[nd];
-            zv[j] = z[nd];
-            p0[j] = P0[nd];
-            p1[j] = P1[nd];
-            fx[j] = D.nfix[nd];
+        for (int q = 0; q < PF; ++q) load_gauss_point<NEN, NQ, T>(dN, C, ne, e, q, d[q], c[q]);
+    }
+
+    // scalar step
+    double alpha = 0.0, beta = 0.0, gam = 0.0, den = 0.0, rr = 0.0;
+    if (k >= 0.0) {
+        const double* rd = pcg_red(D, 1 - par);
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+        for (int i = t; i < D.n_chunks; i += BS) {
+            a0 += rd[3 * i];
+            a1 += rd[3 * i + 1];
+            a2 += rd[3 * i + 2];
         }
-    }
-#pragma unroll
-    for (int j = 0; j < NPT; ++j) {
-#pragma unroll
-        for (int q = 0; q < UPD_MAXP; ++q)
-            if (q < ss[j].y) pq[j][q] = part[ss[j].x + q];
-    }
-    double* sc = D.sc;
-    const bool lead = it.b == 0 && threadIdx.x == 0;
-    const double st = sc[DFEA_SC_A_STATUS];
-    const double k = sc[DFEA_SC_A_K];
-    const double rz = sc[DFEA_SC_A_RZ], beta = sc[DFEA_SC_A_BETA];
-    if (st != DFEA_PCG_RUNNING) {       // finished by the scalar step: publish the outcome
-        if (lead) {
-            sc[DFEA_SC_STATUS] = st;
-            sc[DFEA_SC_ITERS] = sc[DFEA_SC_A_ITERS];
-            sc[DFEA_SC_RR] = sc[DFEA_SC_A_RR];
+        a0 = block_sum<BS>(a0, L.red);
+        a1 = block_sum<BS>(a1, L.red);
+        a2 = block_sum<BS>(a2, L.red);
+        if (t == 0) {
+            s_b[0] = a0;
+            s_b[1] = a1;
+            s_b[2] = a2;
         }
-        return;
-    }
-    const double pAp = bcast0(sum_partials<VBS>(D.red, D.n_chunks, s_red), &s_b);
-    const bool odd = ((int)k) & 1;      // iteration k reads the direction from buffer k mod 2
-    V2<T>* p_wr = odd ? P0 : P1;
-    if (!(pAp > 0.0)) {                 // non-positive curvature: stop; at k = 0 the answer is z
-        if (k == 0.0) {
-#pragma unroll
-            for (int j = 0; j < NPT; ++j) {
-                const int nd = base + j * VBS + threadIdx.x;
-                if (nd < nn) x[nd] = zv[j];
+        __syncthreads();
+        gam = s_b[0];
+        const double delta = s_b[1];
+        rr = s_b[2];
+        double st = DFEA_PCG_RUNNING, iters = 0.0;
+        if (k >= 1.0 && sqrt(rr) <= sc[DFEA_SC_RTOL] * sqrt(sc[DFEA_SC_BB])) {
+            st = DFEA_PCG_CONVERGED;
+            iters = k;
+        } else if (k >= sc[DFEA_SC_MAXIT]) {
+            st = DFEA_PCG_MAXITER;
+            iters = sc[DFEA_SC_MAXIT];
+        } else {
+            beta = k >= 1.0 ? gam / S[PS_GAM] : 0.0;
+            den = k >= 1.0 ? delta - beta * gam / S[PS_ALPHA] : delta;
+            if (!(den > 0.0)) {
+                st = DFEA_PCG_CURVATURE;
+                iters = k;
+            } else {
+                alpha = gam / den;
             }
         }
-        if (lead) {
-            sc[DFEA_SC_PAP] = pAp;
-            sc[DFEA_SC_STATUS] = DFEA_PCG_CURVATURE;
-            sc[DFEA_SC_ITERS] = k;
-        }
-        return;
-    }
-    const double alpha = rz / pAp;
-
-    // 2. shared nodes: direction and A p; 3. vector update
-    double rzs = 0.0, rr = 0.0;
-    V2<T> pv[NPT];
-#pragma unroll
-    for (int j = 0; j < NPT; ++j) {
-        const int nd = base + j * VBS + threadIdx.x;
-        if (nd >= nn) continue;
-        pv[j] = odd ? p0[j] : p1[j];
-        if (ss[j].y > 0) {
-            const V2<T> q = odd ? p1[j] : p0[j];
-            pv[j].x = (T)((double)zv[j].x + beta * (double)q.x);
-            pv[j].y = (T)((double)zv[j].y + beta * (double)q.y);
-            V2<T> acc{0, 0};
-#pragma unroll
-            for (int c = 0; c < UPD_MAXP; ++c) {
-                if (c < ss[j].y) {
-                    acc.x += pq[j][c].x;
-                    acc.y += pq[j][c].y;
+        if (st != DFEA_PCG_RUNNING) {
+            if (lead) {
+                double* O = sc + (par ? DFEA_SC_S0 : DFEA_SC_S1);
+                O[PS_ST] = st;
+                O[PS_K] = k;
+                sc[DFEA_SC_STATUS] = st;
+                sc[DFEA_SC_ITERS] = iters;
+                sc[DFEA_SC_K] = k;
+                sc[DFEA_SC_RR] = rr;
+                sc[DFEA_SC_PAP] = den;
+            }
+            // the answer into vec[DFEA_X]: x_k lives in buffer k mod 2; curvature at k = 0 answers z = u_0
+            const bool at0 = st == DFEA_PCG_CURVATURE && k == 0.0;
+            if (at0 || qr == 1) {
+                V2<T>* x0 = static_cast<V2<T>*>(D.vec[DFEA_X]);
+                for (int kk = t; kk < ch.nk; kk += BS) {
+                    const int n = (int)(D.lnode[ch.k0 + kk] & DFEA_NODE_MASK);
+                    if (kk >= ch.nown && (!D.n_part || shs[n].x != D.pslot[ch.pbase + (kk - ch.nown)])) continue;
+                    if (at0) {
+                        const V2<T> rv = Rd.r[n], dv = dg[n];
+                        x0[n] = V2<T>{rv.x / dv.x, rv.y / dv.y};
+                    } else {
+                        x0[n] = Rd.x[n];
+                    }
                 }
             }
-            for (int c = ss[j].x + UPD_MAXP; c < ss[j].x + ss[j].y; ++c) {
-                const V2<T> pc = part[c];
-                acc.x += pc.x;
-                acc.y += pc.y;
+            return;
+        }
+    }
+
+    // update of the chunk's nodes, u_{k+1} into LDS
+    double g_part = 0.0, r_part = 0.0;
+    for (int kk = t; kk < ch.nk; kk += BS) {
+        const uint32_t g = D.lnode[ch.k0 + kk];
+        const int n = (int)(g & DFEA_NODE_MASK);
+        const unsigned char b = (unsigned char)(g >> 30);
+        // writer: owned nodes, and a shared node's first-slot chunk (halo plans: nodes past nown are owned
+        // by another chunk and only read)
+        const bool shared = kk >= ch.nown && D.n_part;
+        bool writer = kk < ch.nown;
+        int2 ss = make_int2(0, 0);
+        if (shared) {
+            ss = shs[n];
+            writer = ss.x == D.pslot[ch.pbase + (kk - ch.nown)];
+        }
+        const V2<T> dv = dg[n];
+        V2<T> rn = Rd.r[n];
+        if (k >= 0.0) {
+            V2<T> w;
+            if (!shared) {
+                w = Rd.w[n];
+            } else {                    // shared node: w from the previous kernel's partials, slot order
+                V2<T> acc{0, 0};
+                for (int j = ss.x; j < ss.x + ss.y; ++j) {
+                    const V2<T> pj = part_rd[j];
+                    acc.x += pj.x;
+                    acc.y += pj.y;
+                }
+                w = acc;
             }
-            av[j].x = (fx[j] & 1) ? pv[j].x : acc.x;
-            av[j].y = (fx[j] & 2) ? pv[j].y : acc.y;
+            const V2<T> xo = Rd.x[n], po = Rd.p[n], so = Rd.s[n];
+            const V2<T> u{rn.x / dv.x, rn.y / dv.y};
+            if (b & 1) w.x = u.x;       // fixed dofs: A acts as the identity
+            if (b & 2) w.y = u.y;
+            V2<T> pn, sn, xn;
+            pn.x = (T)((double)u.x + beta * (double)po.x);
+            pn.y = (T)((double)u.y + beta * (double)po.y);
+            sn.x = (T)((double)w.x + beta * (double)so.x);
+            sn.y = (T)((double)w.y + beta * (double)so.y);
+            xn.x = (T)((double)xo.x + alpha * (double)pn.x);
+            xn.y = (T)((double)xo.y + alpha * (double)pn.y);
+            rn.x = (T)((double)rn.x - alpha * (double)sn.x);
+            rn.y = (T)((double)rn.y - alpha * (double)sn.y);
+            if (writer) {
+                Wr.p[n] = pn;
+                Wr.s[n] = sn;
+                Wr.x[n] = xn;
+                Wr.r[n] = rn;
+            }
         }
-        V2<T> xi, ri, zi;
-        xi.x = (T)((double)xv[j].x + alpha * (double)pv[j].x);
-        xi.y = (T)((double)xv[j].y + alpha * (double)pv[j].y);
-        ri.x = (T)((double)rv[j].x - alpha * (double)av[j].x);
-        ri.y = (T)((double)rv[j].y - alpha * (double)av[j].y);
-        zi.x = ri.x / dv[j].x;
-        zi.y = ri.y / dv[j].y;
-        xv[j] = xi;
-        rv[j] = ri;
-        zv[j] = zi;
-        rzs += (double)ri.x * (double)zi.x + (double)ri.y * (double)zi.y;
-        rr += (double)ri.x * (double)ri.x + (double)ri.y * (double)ri.y;
+        const V2<T> un{rn.x / dv.x, rn.y / dv.y};
+        if (writer) {
+            g_part += (double)rn.x * (double)un.x + (double)rn.y * (double)un.y;
+            r_part += (double)rn.x * (double)rn.x + (double)rn.y * (double)rn.y;
+        }
+        L.v[kk] = un;
+        L.acc[kk] = V2<T>{0, 0};
+        L.bits[kk] = b;
     }
-#pragma unroll
-    for (int j = 0; j < NPT; ++j) {
-        const int nd = base + j * VBS + threadIdx.x;
-        if (nd >= nn) continue;
-        x[nd] = xv[j];
-        r[nd] = rv[j];
-        z[nd] = zv[j];
-        if (ss[j].y > 0) {
-            p_wr[nd] = pv[j];
-            Ap[nd] = av[j];
+    __syncthreads();
+
+    // w_{k+1} = A u_{k+1}
+    element_product<NEN, NQ, PF, T>(ch, t, has_el, l, dN, C, ne, e, d, c, L);
+
+    double d_part = 0.0;
+    for (int kk = t; kk < ch.nk; kk += BS) {
+        const V2<T> u = L.v[kk];
+        const V2<T> acc = L.acc[kk];
+        const unsigned char b = L.bits[kk];
+        if (kk < ch.nown) {
+            V2<T> out;
+            out.x = (b & 1) ? u.x : acc.x;
+            out.y = (b & 2) ? u.y : acc.y;
+            Wr.w[(int)(D.lnode[ch.k0 + kk] & DFEA_NODE_MASK)] = out;
+            d_part += (double)u.x * (double)out.x + (double)u.y * (double)out.y;
+        } else if (D.n_part) {
+            const int slot = D.pslot[ch.pbase + (kk - ch.nown)];
+            part_wr[slot] = acc;
+            if (!(b & 1)) d_part += (double)u.x * (double)acc.x;
+            if (!(b & 2)) d_part += (double)u.y * (double)acc.y;
+            if (b) {     // fixed components (w = u there) once, by the chunk holding the node's first slot
+                const int n = (int)(D.lnode[ch.k0 + kk] & DFEA_NODE_MASK);
+                if (slot == shs[n].x) {
+                    if (b & 1) d_part += (double)u.x * (double)u.x;
+                    if (b & 2) d_part += (double)u.y * (double)u.y;
+                }
+            }
         }
     }
-    rzs = block_sum<VBS>(rzs, s_red);
-    rr = block_sum<VBS>(rr, s_red);
-    if (threadIdx.x == 0) {
-        double* ru = D.red + D.n_chunks + D.n_finblk;
-        ru[2 * it.b] = rzs;
-        ru[2 * it.b + 1] = rr;
+    g_part = block_sum<BS>(g_part, L.red);
+    d_part = block_sum<BS>(d_part, L.red);
+    r_part = block_sum<BS>(r_part, L.red);
+    if (t == 0) {
+        double* wr = pcg_red(D, par) + 3 * it.b;
+        wr[0] = g_part;
+        wr[1] = d_part;
+        wr[2] = r_part;
     }
     if (lead) {
-        sc[DFEA_SC_K] = k + 1.0;
-        sc[DFEA_SC_RZ] = rz;
-        sc[DFEA_SC_PAP] = pAp;
-        sc[DFEA_SC_ALPHA] = alpha;
-        sc[DFEA_SC_BETA] = beta;
-        sc[DFEA_SC_RR] = sc[DFEA_SC_A_RR];
+        double* O = sc + (par ? DFEA_SC_S0 : DFEA_SC_S1);
+        O[PS_ST] = DFEA_PCG_RUNNING;
+        O[PS_K] = k + 1.0;
+        O[PS_GAM] = gam;
+        O[PS_ALPHA] = alpha;
+        if (k >= 0.0) {
+            sc[DFEA_SC_K] = k + 1.0;
+            sc[DFEA_SC_RZ] = gam;
+            sc[DFEA_SC_PAP] = den;
+            sc[DFEA_SC_ALPHA] = alpha;
+            sc[DFEA_SC_BETA] = beta;
+            sc[DFEA_SC_RR] = rr;
+        }
     }
 }
 
@@ -1108,13 +1127,11 @@ int dispatch(int kind, F f) {
     }
 }
 
+// PCG kernels in pairs of parities 0, 1 (n_iter rounded up to even)
 template <int NEN, int BS, typename T>
-int pcg_iter(const dfea_desc* table, const int32_t* items, int n_items, const int32_t* vitems, int n_vitems,
-             int n_iter, void* stream) {
-    for (int i = 0; i < n_iter; ++i) {
-        int rc = launch(k_apply<NEN, NEN, BS, T, true>, n_items, BS, stream, "k_apply(pcg)", table, items,
-                        (const T*)nullptr, (T*)nullptr);
-        if (!rc) rc = launch(k_pcg_update<T>, n_vitems, VBS, stream, "k_pcg_update", table, vitems);
+int pcg_iter(const dfea_desc* table, const int32_t* items, int n_items, int n_iter, void* stream) {
+    for (int i = 0; i < n_iter + (n_iter & 1); ++i) {
+        const int rc = launch(k_pcg<NEN, NEN, BS, T>, n_items, BS, stream, "k_pcg", table, items, i & 1);
         if (rc) return rc;
     }
     return 0;
@@ -1156,7 +1173,7 @@ struct Launch_apply {
     template <typename T, int NEN, int BS> int go() {
         const T* vv = static_cast<const T*>(v);
         T* yy = static_cast<T*>(y);
-        int rc = launch(k_apply<NEN, NEN, BS, T, false>, n, BS, st, "k_apply", t, it, vv, yy);
+        int rc = launch(k_apply<NEN, NEN, BS, T>, n, BS, st, "k_apply", t, it, vv, yy);
         if (rc) return rc;
         return launch(k_apply_fin<T>, nf, FBS, st, "k_apply_fin", t, fi, vv, yy);
     }
@@ -1166,10 +1183,10 @@ struct Launch_apply {
 };
 
 struct Launch_pcg_iterate {
-    const dfea_desc* t; const int32_t* it; int n; const int32_t* vi; int nv; int k; int prec; void* st;
+    const dfea_desc* t; const int32_t* it; int n; int k; int prec; void* st;
     template <int NEN, int BS> int run() {
-        return prec == DFEA_FP64 ? pcg_iter<NEN, BS, double>(t, it, n, vi, nv, k, st)
-                                 : pcg_iter<NEN, BS, float>(t, it, n, vi, nv, k, st);
+        return prec == DFEA_FP64 ? pcg_iter<NEN, BS, double>(t, it, n, k, st)
+                                 : pcg_iter<NEN, BS, float>(t, it, n, k, st);
     }
 };
 
@@ -1212,7 +1229,7 @@ int dfea_desc_offset(int field) {
     return off[field];
 }
 
-int dfea_red_size(int n_chunks, int n_finblk, int n_vblk) { return n_chunks + n_finblk + 3 * n_vblk; }
+int dfea_red_size(int n_chunks, int n_finblk, int n_vblk) { return 6 * n_chunks + n_finblk + 3 * n_vblk; }
 int dfea_vblk(int n_dof) { return (n_dof + VBLK - 1) / VBLK; }
 int dfea_finblk(int n_shared) { return (n_shared + FBS - 1) / FBS; }
 int dfea_chunk_elems(int nen) { return (nen & ~DFEA_HALO) == 4 ? CHUNK_Q4 : nen == 9 ? CHUNK_Q9 : -1; }
@@ -1280,15 +1297,15 @@ int dfea_pcg_init(const dfea_desc* table, const int32_t* vitems, int n_vitems, c
     return launch(k_pcg_init_final, n_solves, VBS, stream, "k_pcg_init_final", table, solve_items);
 }
 
-int dfea_pcg_iterate(const dfea_desc* table, const int32_t* items, int n_items, const int32_t* vitems,
-                     int n_vitems, int n_iter, int nen, int precision, void* stream) {
+int dfea_pcg_iterate(const dfea_desc* table, const int32_t* items, int n_items, int n_iter, int nen, int precision,
+                     void* stream) {
     if (!table || n_iter < 0 || (precision != DFEA_FP64 && precision != DFEA_FP32))
         return fail(DFEA_EINVAL, "dfea_pcg_iterate: bad argument");
-    return dispatch(nen, Launch_pcg_iterate{table, items, n_items, vitems, n_vitems, n_iter, precision, stream});
+    return dispatch(nen, Launch_pcg_iterate{table, items, n_items, n_iter, precision, stream});
 }
 
-int dfea_pcg_graph_create(const dfea_desc* table, const int32_t* items, int n_items, const int32_t* vitems,
-                          int n_vitems, int n_iter, int nen, int precision, void** graph) {
+int dfea_pcg_graph_create(const dfea_desc* table, const int32_t* items, int n_items, int n_iter, int nen,
+                          int precision, void** graph) {
     if (!graph || n_iter <= 0) return fail(DFEA_EINVAL, "dfea_pcg_graph_create: bad argument");
     *graph = nullptr;
     hipStream_t s = nullptr;
@@ -1299,7 +1316,7 @@ int dfea_pcg_graph_create(const dfea_desc* table, const int32_t* items, int n_it
         (void)hipStreamDestroy(s);
         return fail(-1000 - (int)e, "dfea_pcg_graph_create: hipStreamBeginCapture failed");
     }
-    int rc = dfea_pcg_iterate(table, items, n_items, vitems, n_vitems, n_iter, nen, precision, s);
+    int rc = dfea_pcg_iterate(table, items, n_items, n_iter, nen, precision, s);
     hipGraph_t g = nullptr;
     e = hipStreamEndCapture(s, &g);
     (void)hipStreamDestroy(s);

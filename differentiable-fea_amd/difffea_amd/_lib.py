@@ -17,8 +17,9 @@ FP64, FP32 = 0, 1
 MAXCOL = 16
 FIXED_X, FIXED_Y, NODE_MASK = 0x40000000, 0x80000000, 0x3FFFFFFF
 HALO = 0x100          # element-kind flag of halo plans
-VEC = dict(x=0, r=1, z=2, p=3, Ap=4, b=5, u=6, R=7, uacc=8, uD=9, p2=10)
-NVEC = 11
+VEC = dict(x=0, r=1, w=2, p=3, s=4, b=5, u=6, R=7, uacc=8, uD=9, x2=10, r2=11, w2=12, p2=13, s2=14)
+NVEC = 15
+PCG_VECS = ("x", "r", "w", "p", "s", "b", "x2", "r2", "w2", "p2", "s2")   # in the solve's precision
 SC = dict(rz=0, pAp=1, alpha=2, beta=3, rr=4, bb=5, rtol=6, status=7, k=8, maxit=9, iters=10, norm=11, lf=12,
           step=13)
 NSC = 24
@@ -66,9 +67,9 @@ _SIGNATURES = {
     "dfea_dot": (_I, [_P, _P, _I, _P, _I, _P, _P, _P]),
     "dfea_axpy": (_I, [_P, _P, _I, _I, _P, _P, _P, _P, _P, _I, _P]),
     "dfea_pcg_init": (_I, [_P, _P, _I, _P, _I, _I, _I, _P]),
-    "dfea_pcg_iterate": (_I, [_P, _P, _I, _P, _I, _I, _I, _I, _P]),
+    "dfea_pcg_iterate": (_I, [_P, _P, _I, _I, _I, _I, _P]),
     "dfea_adjoint_grad": (_I, [_P, _P, _I, _P, _P, _P, _I, _P]),
-    "dfea_pcg_graph_create": (_I, [_P, _P, _I, _P, _I, _I, _I, _I, ctypes.POINTER(_P)]),
+    "dfea_pcg_graph_create": (_I, [_P, _P, _I, _I, _I, _I, ctypes.POINTER(_P)]),
     "dfea_graph_launch": (_I, [_P, _P]),
     "dfea_graph_destroy": (_I, [_P]),
 }

@@ -64,6 +64,9 @@ class HipMesh:
         self.kind = self.nen | (_lib.HALO if p.mode == "halo" else 0)
         if p.validate() > self.lib.dfea_chunk_threads(self.kind):
             raise ValueError("chunk larger than the kernel's workgroup")
+        E = self.lib.dfea_chunk_elems(self.nen)
+        if p.mode == "partials" and not np.array_equal(p.chunks[:, 0], np.arange(p.n_chunks) * E):
+            raise ValueError(f"partials plans must cut the element order into chunks of {E} elements")
         self.n_el, self.n_nodes = p.n_el, p.n_nodes
         self.n_dof = 2 * p.n_nodes
         dv = device
@@ -148,11 +151,11 @@ class HipSystem:
         nq, n_el, nd = m.nq, m.n_el, m.n_dof
         self.C = None                    # moduli and diagonal: allocated by the first moduli() call
         self.diag = None
-        self.part = torch.empty(max(1, m.plan.n_part) * 2, dtype=F64, device=dv)
+        self.part = torch.empty(max(1, 2 * m.plan.n_part) * 2, dtype=F64, device=dv)   # two parities of pairs
         self.red = torch.zeros(max(1, m.red_size), dtype=F64, device=dv)
         # the scalar block may be a row of a pool shared by a batch of solves
         self.sc = torch.zeros(_lib.NSC, dtype=F64, device=dv) if sc is None else sc
-        self.vec = {k: torch.zeros(nd, dtype=self.vdt, device=dv) for k in ("x", "r", "z", "p", "Ap", "b", "p2")}
+        self.vec = {k: torch.zeros(nd, dtype=self.vdt, device=dv) for k in _lib.PCG_VECS}
         self.vec["u"] = torch.zeros(nd, dtype=F64, device=dv)
         self.vec["R"] = torch.zeros(nd, dtype=F64, device=dv)
         self.vec["uacc"] = torch.zeros(nd, dtype=F64, device=dv)
@@ -264,8 +267,8 @@ class HipSystem:
             m = self.mesh
             h = ctypes.c_void_p()
             _lib.check(self.lib.dfea_pcg_graph_create(
-                _ptr(self.table), _ptr(m.items), m.plan.n_chunks, _ptr(m.vitems), m.n_vblk, int(n_iter), m.kind,
-                self.prec_id, ctypes.byref(h)), "dfea_pcg_graph_create")
+                _ptr(self.table), _ptr(m.items), m.plan.n_chunks, int(n_iter), m.kind, self.prec_id,
+                ctypes.byref(h)), "dfea_pcg_graph_create")
             self._graphs[n_iter] = g = h
         return g
 
@@ -276,9 +279,8 @@ class HipSystem:
         if self.use_graphs if graph is None else graph:
             _lib.check(self.lib.dfea_graph_launch(self._graph(int(n_iter)), _stream()), "dfea_graph_launch")
             return
-        _lib.check(self.lib.dfea_pcg_iterate(_ptr(self.table), _ptr(m.items), m.plan.n_chunks, _ptr(m.vitems),
-                                             m.n_vblk, int(n_iter), m.kind, self.prec_id, _stream()),
-                   "dfea_pcg_iterate")
+        _lib.check(self.lib.dfea_pcg_iterate(_ptr(self.table), _ptr(m.items), m.plan.n_chunks, int(n_iter), m.kind,
+                                             self.prec_id, _stream()), "dfea_pcg_iterate")
 
     def release_graphs(self):
         for g in self._graphs.values():
@@ -298,12 +300,12 @@ class HipSystem:
     def pcg_run(self, max_iter, chunk=16, graph=None):
         """Iterate a started PCG until it stops; returns (iterations, status).
 
-        Iterations run in chunks (graph replays) whose length doubles up to 128; solves that finish inside
-        a chunk skip the remaining iterations on the device, and the status block is read once per chunk.
-        The test on the last update is made at the start of the following iteration, hence the + 1."""
+        Kernels run in chunks (graph replays) of even length doubling up to 128; solves that finish inside
+        a chunk skip the remaining kernels on the device, and the status block is read once per chunk.  The
+        first kernel forms A M b and the test on the last update is made by the kernel after it, hence + 2."""
         st, its, k = self.pcg_status()
         while st == _lib.PCG_RUNNING:
-            self.pcg_iterate(min(chunk, max_iter - k + 1), graph)
+            self.pcg_iterate(min(chunk, max_iter - k + 2), graph)
             st, its, k = self.pcg_status()
             chunk = min(2 * chunk, 128)
         return its, st

@@ -114,7 +114,7 @@ def footprint_bytes(element_type, n, precision):
     n_dof = 2 * side * side
     s = 8 if precision == "fp64" else 4
     per_qp = 2 * nen * 12 + 8 + 10 * s
-    return n_el * nq * per_qp + n_dof * (6 * s + 6 * 8) + n_el * nen * 8 + 64 * n_el + (1 << 20)
+    return n_el * nq * per_qp + n_dof * (12 * s + 5 * 8) + n_el * nen * 8 + 64 * n_el + (1 << 20)
 
 
 # ------------------------------------------------------------------------------------------------------------
@@ -396,9 +396,7 @@ class Server:
                 for j in sub:
                     j.pcg_rounds += 1
                 it, n = self._dev_items(sub, "chunk")
-                vi, nv = self._dev_items(sub, "vec")
-                _lib.check(lib.dfea_pcg_iterate(T, _ptr(it), n, _ptr(vi), nv, chunk, nen, self.prec_id, st),
-                           "dfea_pcg_iterate")
+                _lib.check(lib.dfea_pcg_iterate(T, _ptr(it), n, chunk, nen, self.prec_id, st), "dfea_pcg_iterate")
         if host_work is not None:
             host_work()
         # one host sync: status, iterations, norms

@@ -43,21 +43,20 @@ extern "C" {
 
 enum { DFEA_FP64 = 0, DFEA_FP32 = 1 };
 
-/* vector slots of a descriptor */
-enum { DFEA_X = 0, DFEA_R = 1, DFEA_Z = 2, DFEA_P = 3, DFEA_AP = 4, DFEA_B = 5, DFEA_U = 6, DFEA_RES = 7,
-       DFEA_UACC = 8, DFEA_UD = 9, DFEA_P2 = 10, DFEA_NVEC = 11 };
-/* PCG keeps the search direction in two buffers, p (vec[DFEA_P]) and p' (vec[DFEA_P2]): iteration k reads
- * the old direction from buffer k mod 2 and writes the new one to the other, so no workgroup overwrites a
- * value another workgroup still has to read. */
+/* vector slots of a descriptor.  The PCG state x, r, w = A M r, p, s = A p lives in two buffers each (the
+ * slot and its "2" twin): iteration k reads the buffers of parity k mod 2 and writes the others, so no
+ * workgroup overwrites a value another workgroup still has to read.  The PCG solution ends in vec[DFEA_X]. */
+enum { DFEA_X = 0, DFEA_R = 1, DFEA_W = 2, DFEA_P = 3, DFEA_S = 4, DFEA_B = 5, DFEA_U = 6, DFEA_RES = 7,
+       DFEA_UACC = 8, DFEA_UD = 9, DFEA_X2 = 10, DFEA_R2 = 11, DFEA_W2 = 12, DFEA_P2 = 13, DFEA_S2 = 14,
+       DFEA_NVEC = 15 };
 
-/* scalar block of a solve (doubles).  The PCG state read by the caller is slots 0..10; slots DFEA_SC_A_*
- * hold the state the direction/product kernel of an iteration hands to its update kernel (each slot has one
- * writer kernel, and no kernel reads a slot it writes). */
+/* scalar block of a solve (doubles).  Slots 0..10 are the PCG state read by the caller (RZ = r.M r,
+ * PAP = p.A p and ALPHA / BETA of the last update, RR = r.r of the last tested residual, K = updates done);
+ * the PCG kernels themselves carry their state in two parity sets of 4 doubles at DFEA_SC_S0 / DFEA_SC_S1
+ * (status, k, r.M r, alpha), each written by the kernels of one parity and read by those of the other. */
 enum { DFEA_SC_RZ = 0, DFEA_SC_PAP = 1, DFEA_SC_ALPHA = 2, DFEA_SC_BETA = 3, DFEA_SC_RR = 4, DFEA_SC_BB = 5,
        DFEA_SC_RTOL = 6, DFEA_SC_STATUS = 7, DFEA_SC_K = 8, DFEA_SC_MAXIT = 9, DFEA_SC_ITERS = 10,
-       DFEA_SC_NORM = 11, DFEA_SC_LF = 12, DFEA_SC_STEP = 13,
-       DFEA_SC_A_STATUS = 16, DFEA_SC_A_K = 17, DFEA_SC_A_ITERS = 18, DFEA_SC_A_RZ = 19, DFEA_SC_A_BETA = 20,
-       DFEA_SC_A_RR = 21, DFEA_NSC = 24 };
+       DFEA_SC_NORM = 11, DFEA_SC_LF = 12, DFEA_SC_STEP = 13, DFEA_SC_S0 = 16, DFEA_SC_S1 = 20, DFEA_NSC = 24 };
 
 /* PCG status values stored in sc[DFEA_SC_STATUS] */
 enum { DFEA_PCG_RUNNING = 0, DFEA_PCG_CONVERGED = 1, DFEA_PCG_CURVATURE = 2, DFEA_PCG_MAXITER = 3 };
@@ -157,7 +156,7 @@ int dfea_residual(const dfea_desc* table, const int32_t* items, int n_items, con
 
 /* ---- tangent product -------------------------------------------------------------------------------- */
 /* y = m * J(u)(m v) + (1-m) v from the stored moduli (FP64 or FP32 per descriptor precision).
- * v/y NULL -> vec[DFEA_P] / vec[DFEA_AP].  Deterministic (no atomics).
+ * v/y NULL -> vec[DFEA_P] / vec[DFEA_W].  Deterministic (no atomics).
  * Replaces the closure of `operators.py:345-412` (make_A_operator_qp / _qp_ew). */
 int dfea_apply(const dfea_desc* table, const int32_t* items, int n_items, const int32_t* fin_items,
                int n_fin_items, const void* v, void* y, int nen, int precision, void* stream);
@@ -181,31 +180,30 @@ int dfea_axpy(const dfea_desc* table, const int32_t* items, int n_items, int mod
               const void* dx, const double* uD, const double* sarg, double* out, int precision, void* stream);
 
 /* ---- PCG ---------------------------------------------------------------------------------------------- */
-/* Initialise x=0, r=b, z=r/diag, p=z and the scalar block (rtol, max_iter from sc).  b: vec[DFEA_B]
- * (solve precision) or, when b64 != NULL (one solve) / flag neg_res (batch), b = -R from vec[DFEA_RES].
- * Replaces `newton_krylov.py:51-60`. */
+/* Start a PCG solve: x = 0, r = b, p = s = 0 and the scalar block (rtol, max_iter are read from sc).
+ * b: vec[DFEA_B] (solve precision), or b = -R from vec[DFEA_RES] with neg_res.  A zero b finishes at once
+ * (converged, 0 iterations, x = 0).  Replaces `newton_krylov.py:51-60`. */
 int dfea_pcg_init(const dfea_desc* table, const int32_t* vitems, int n_vitems, const int32_t* solve_items,
                   int n_solves, int neg_res, int precision, void* stream);
-/* n_iter PCG iterations for every running solve of the batch, two kernels each:
- *   1. per chunk: the scalar step of the previous iteration (convergence test on r.r, beta = rz'/rz,
- *      iteration cap), evaluated redundantly and identically by every workgroup from the previous update's
- *      partials; the direction p = z + beta p on the gather; A p; partials of p.Ap that already include
- *      the shared nodes (each chunk contributes p . its own partial);
- *   2. per dof block: alpha = rz / p.Ap, shared nodes' A p summed from their partials in slot order,
- *      x += alpha p, r -= alpha A p, z = r / diag, partials of r.z and r.r.
- * The test on the last update's residual is made by the first kernel of the next iteration, so a solve
- * whose k = max_iter updates are done is reported finished one iteration call later: callers iterate while
- * the status is DFEA_PCG_RUNNING, with n_iter >= 1.  Finished solves skip their work.
- * items: chunk items; vitems: dof-block items.  Semantics of `newton_krylov.py:62-88`. */
-int dfea_pcg_iterate(const dfea_desc* table, const int32_t* items, int n_items, const int32_t* vitems,
-                     int n_vitems, int n_iter, int nen, int precision, void* stream);
+/* PCG kernels for every running solve of the batch, one kernel per iteration (per chunk: the scalar step
+ * -- convergence of the last residual, the iteration cap, curvature, alpha and beta -- from the previous
+ * kernel's partials, evaluated identically by every workgroup; the update of the chunk's nodes; the next
+ * product A M r in LDS; partials of r.M r, (A M r).M r and r.r).  The recurrences are the Chronopoulos-Gear
+ * arrangement of Jacobi-PCG: the iterates of `newton_krylov.py:62-88` in exact arithmetic, with its tests
+ * in its order (non-positive curvature at the first iteration answers z = M b).
+ * Kernels run in pairs of parities 0, 1: n_iter is rounded up to even, and dfea_pcg_init may only be
+ * called between two calls.  The first kernel after dfea_pcg_init forms A M b; each further kernel is one
+ * iteration, whose test on the residual it produced is made by the next kernel.  Callers launch while
+ * sc[DFEA_SC_STATUS] reads DFEA_PCG_RUNNING; finished solves skip their work.  items: chunk items. */
+int dfea_pcg_iterate(const dfea_desc* table, const int32_t* items, int n_items, int n_iter, int nen, int precision,
+                     void* stream);
 
-/* Capture n_iter iterations of dfea_pcg_iterate (same arguments) into an executable HIP graph; replay it
+/* Capture n_iter kernels of dfea_pcg_iterate (same arguments) into an executable HIP graph; replay it
  * with dfea_graph_launch on any stream.  The tables and buffers the descriptors point to must outlive the
  * graph.  No memset nodes are recorded.  Graph capture of the iteration loop replaces the reference's
  * per-iteration Python loop with two host syncs (`newton_krylov.py:62-86`). */
-int dfea_pcg_graph_create(const dfea_desc* table, const int32_t* items, int n_items, const int32_t* vitems,
-                          int n_vitems, int n_iter, int nen, int precision, void** graph);
+int dfea_pcg_graph_create(const dfea_desc* table, const int32_t* items, int n_items, int n_iter, int nen,
+                          int precision, void** graph);
 int dfea_graph_launch(void* graph, void* stream);
 int dfea_graph_destroy(void* graph);
 

@@ -34,4 +34,4 @@ def test_size_queries():
     lib = _lib.load()
     assert lib.dfea_vblk(1024) == 1 and lib.dfea_vblk(1025) == 2
     assert lib.dfea_chunk_elems(4) > 0 and lib.dfea_chunk_elems(9) > 0 and lib.dfea_chunk_elems(3) == -1
-    assert lib.dfea_red_size(3, 2, 1) == 3 + 2 + 3
+    assert lib.dfea_red_size(3, 2, 1) == 6 * 3 + 2 + 3
