@@ -32,6 +32,7 @@ constexpr int CHUNK_Q9 = 64;      // core elements per chunk for quad9
 constexpr int BS_Q4 = 256;        // workgroup size for quad4 partials plans
 constexpr int BS_Q4H = 320;       // workgroup size for quad4 halo plans: core + halo elements (16x16 + 33)
 constexpr int BS_Q9 = 64;         // workgroup size for quad9
+constexpr int BS_Q9P = 128;       // workgroup size of the quad9 PCG kernel (node updates on 128 lanes)
 constexpr int MAXLN = 640;        // local nodes per chunk (LDS capacity)
 constexpr int VBS = 256;          // workgroup size of the vector kernels
 constexpr int VPT = 4;            // dofs per thread in the vector kernels
@@ -1127,11 +1128,14 @@ int dispatch(int kind, F f) {
     }
 }
 
-// PCG kernels in pairs of parities 0, 1 (n_iter rounded up to even)
+// PCG kernels in pairs of parities 0, 1 (n_iter rounded up to even).  Quad9 chunks (64 elements, up to 289
+// nodes) run the PCG kernel with BS_Q9P threads, so the node updates are spread over more lanes than the
+// element threads.
 template <int NEN, int BS, typename T>
 int pcg_iter(const dfea_desc* table, const int32_t* items, int n_items, int n_iter, void* stream) {
+    constexpr int BSP = NEN == 9 ? BS_Q9P : BS;
     for (int i = 0; i < n_iter + (n_iter & 1); ++i) {
-        const int rc = launch(k_pcg<NEN, NEN, BS, T>, n_items, BS, stream, "k_pcg", table, items, i & 1);
+        const int rc = launch(k_pcg<NEN, NEN, BSP, T>, n_items, BSP, stream, "k_pcg", table, items, i & 1);
         if (rc) return rc;
     }
     return 0;
