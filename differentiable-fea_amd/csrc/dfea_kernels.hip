@@ -32,7 +32,10 @@ constexpr int CHUNK_Q9 = 64;      // core elements per chunk for quad9
 constexpr int BS_Q4 = 256;        // workgroup size for quad4 partials plans
 constexpr int BS_Q4H = 320;       // workgroup size for quad4 halo plans: core + halo elements (16x16 + 33)
 constexpr int BS_Q9 = 64;         // workgroup size for quad9
-constexpr int BS_Q9P = 128;       // workgroup size of the quad9 PCG kernel (node updates on 128 lanes)
+// workgroup size of the quad9 product and PCG kernels: Gauss-point groups of 64 threads (2 in FP64, where
+// registers limit the waves per SIMD, 3 in FP32; measured fastest on 200x200 and the 64x256 O-grid)
+template <typename T>
+constexpr int bs_q9() { return std::is_same<T, double>::value ? 128 : 192; }
 constexpr int MAXLN = 640;        // local nodes per chunk (LDS capacity)
 constexpr int VBS = 256;          // workgroup size of the vector kernels
 constexpr int VPT = 4;            // dofs per thread in the vector kernels
@@ -293,14 +296,29 @@ __device__ __forceinline__ void colour_accumulate(const ChunkView& ch, int t, co
 // and gradients loaded before the gather barrier (so the HBM stream starts at once instead of after the
 // gather's dependent round trips), and the element product accumulated colour by colour into LDS.
 
+// Gauss-point groups: a workgroup of BS = G * CE threads over a partials-plan chunk of CE elements gives each
+// element G threads, thread t taking element t mod CE and the Gauss points [g NQ / G, (g+1) NQ / G) of group
+// g = t / CE; the groups' element vectors are added in group order.  (G = 1: one thread per element; halo
+// plans, whose workgroups are not a multiple of the chunk, always use G = 1.)
+template <int NEN, int BS>
+struct Groups {
+    static constexpr int CE = NEN == 4 ? CHUNK_Q4 : CHUNK_Q9;
+    static constexpr int G = BS % CE == 0 ? BS / CE : 1;
+};
+
+template <int NQ, int G>
+struct GaussSplit {
+    static constexpr int QG = (NQ + G - 1) / G;     // Gauss points per group, at most
+};
+
 // Thread t's element: partials plans cut the internal element order into chunks of exactly CE elements (the
 // host checks it), so the element follows from the workgroup index alone and its loads need not wait for the
 // chunk record; halo plans read the chunk record.
 template <int NEN, int BS>
 __device__ __forceinline__ bool chunk_element(const dfea_desc& D, const ChunkView& ch, int blk, int t, int& e, int* l) {
-    constexpr int CE = NEN == 4 ? CHUNK_Q4 : CHUNK_Q9;
-    if (BS == CE) {
-        e = blk * CE + t;
+    constexpr int CE = Groups<NEN, BS>::CE;
+    if (BS % CE == 0) {
+        e = blk * CE + t % CE;
         if (e >= D.n_el) return false;
 #pragma unroll
         for (int a = 0; a < NEN; ++a) l[a] = D.lelem[a * D.n_el + e];
@@ -321,13 +339,40 @@ __device__ __forceinline__ void load_gauss_point(const T* dN, const T* C, int ne
     for (int k = 0; k < 10; ++k) c[k] = C[(size_t)(k * NQ + q) * ne + e];
 }
 
-// fe = K_e (m v_e) from the gathered LDS values, then the colour-ordered sum into L.acc.  d / c hold the
-// first PF Gauss points' data on entry; the rest is loaded here.
-template <int NEN, int NQ, int PF, typename T>
+// the first PF Gauss points of the thread's group
+template <int NEN, int NQ, int BS, int PF, typename T>
+__device__ __forceinline__ void prefetch_gauss_points(const T* dN, const T* C, int ne, int e, int t,
+                                                      T (&d)[GaussSplit<NQ, Groups<NEN, BS>::G>::QG][NEN][2],
+                                                      T (&c)[GaussSplit<NQ, Groups<NEN, BS>::G>::QG][10]) {
+    constexpr int G = Groups<NEN, BS>::G, CE = Groups<NEN, BS>::CE, QG = GaussSplit<NQ, G>::QG;
+    const int g = G > 1 ? t / CE : 0;
+    const int q0 = g * NQ / G, q1 = (g + 1) * NQ / G;
+#pragma unroll
+    for (int j = 0; j < (PF < QG ? PF : QG); ++j)
+        if (q0 + j < q1) load_gauss_point<NEN, NQ, T>(dN, C, ne, e, q0 + j, d[j], c[j]);
+}
+
+// LDS for the element vectors of groups 1..G-1
+template <int NEN, int BS, typename T>
+struct GroupLDS {
+    static constexpr int G = Groups<NEN, BS>::G, CE = Groups<NEN, BS>::CE;
+    V2<T> fe[G > 1 ? G - 1 : 1][G > 1 ? CE : 1][NEN];
+};
+
+// fe = K_e (m v_e) from the gathered LDS values (the group's Gauss points, the groups added in order), then
+// the colour-ordered sum into L.acc.  d / c hold the group's first PF Gauss points on entry.
+template <int NEN, int NQ, int BS, int PF, typename T>
 __device__ __forceinline__ void element_product(const ChunkView& ch, int t, bool has_el, const int* l, const T* dN,
-                                                const T* C, int ne, int e, T (&d)[NQ][NEN][2], T (&c)[NQ][10],
-                                                ChunkLDS<T>& L) {
+                                                const T* C, int ne, int e,
+                                                T (&d)[GaussSplit<NQ, Groups<NEN, BS>::G>::QG][NEN][2],
+                                                T (&c)[GaussSplit<NQ, Groups<NEN, BS>::G>::QG][10],
+                                                ChunkLDS<T>& L, GroupLDS<NEN, BS, T>& GL) {
+    constexpr int G = Groups<NEN, BS>::G, CE = Groups<NEN, BS>::CE, QG = GaussSplit<NQ, G>::QG;
+    const int g = G > 1 ? t / CE : 0;
+    const int q0 = g * NQ / G, q1 = (g + 1) * NQ / G;
     V2<T> fe[NEN];
+#pragma unroll
+    for (int a = 0; a < NEN; ++a) fe[a] = V2<T>{0, 0};
     if (has_el) {
         V2<T> ve[NEN];
 #pragma unroll
@@ -336,31 +381,49 @@ __device__ __forceinline__ void element_product(const ChunkView& ch, int t, bool
             const unsigned char b = L.bits[l[a]];
             if (b & 1) ve[a].x = 0;
             if (b & 2) ve[a].y = 0;
-            fe[a] = V2<T>{0, 0};
         }
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) {
-            if (q >= PF) load_gauss_point<NEN, NQ, T>(dN, C, ne, e, q, d[q], c[q]);
-            T g[4] = {0, 0, 0, 0};    // g[2i+j] = sum_a ve[a][i] dN[a][j]
+        for (int j = 0; j < QG; ++j) {
+            const int q = q0 + j;
+            if (q >= q1) break;
+            if (j >= PF) load_gauss_point<NEN, NQ, T>(dN, C, ne, e, q, d[j], c[j]);
+            T gr[4] = {0, 0, 0, 0};    // gr[2i+k] = sum_a ve[a][i] dN[a][k]
 #pragma unroll
             for (int a = 0; a < NEN; ++a) {
-                g[0] += ve[a].x * d[q][a][0];
-                g[1] += ve[a].x * d[q][a][1];
-                g[2] += ve[a].y * d[q][a][0];
-                g[3] += ve[a].y * d[q][a][1];
+                gr[0] += ve[a].x * d[j][a][0];
+                gr[1] += ve[a].x * d[j][a][1];
+                gr[2] += ve[a].y * d[j][a][0];
+                gr[3] += ve[a].y * d[j][a][1];
             }
             T sq[4];
 #pragma unroll
             for (int m = 0; m < 4; ++m) {
                 T acc = 0;
 #pragma unroll
-                for (int n = 0; n < 4; ++n) acc += c[q][uidx(m, n)] * g[n];
+                for (int n = 0; n < 4; ++n) acc += c[j][uidx(m, n)] * gr[n];
                 sq[m] = acc;
             }
 #pragma unroll
             for (int a = 0; a < NEN; ++a) {
-                fe[a].x += sq[0] * d[q][a][0] + sq[1] * d[q][a][1];
-                fe[a].y += sq[2] * d[q][a][0] + sq[3] * d[q][a][1];
+                fe[a].x += sq[0] * d[j][a][0] + sq[1] * d[j][a][1];
+                fe[a].y += sq[2] * d[j][a][0] + sq[3] * d[j][a][1];
+            }
+        }
+    }
+    if (G > 1) {
+        if (g > 0 && has_el) {
+#pragma unroll
+            for (int a = 0; a < NEN; ++a) GL.fe[g - 1][t % CE][a] = fe[a];
+        }
+        __syncthreads();
+        if (g == 0 && has_el) {
+            for (int h = 1; h < G; ++h) {
+#pragma unroll
+                for (int a = 0; a < NEN; ++a) {
+                    const V2<T> o = GL.fe[h - 1][t][a];
+                    fe[a].x += o.x;
+                    fe[a].y += o.y;
+                }
             }
         }
     }
@@ -373,6 +436,7 @@ template <int NEN, int NQ, int BS, typename T, int PF = 2, int MINW = 1>
 __global__ __launch_bounds__(BS, MINW) void k_apply(const dfea_desc* __restrict__ tab, const int* __restrict__ items,
                                                    const T* __restrict__ vin, T* __restrict__ vout) {
     __shared__ ChunkLDS<T> L;
+    __shared__ GroupLDS<NEN, BS, T> GL;
     const Item it = load_item(items);
     const dfea_desc& D = tab[it.s];
     const ChunkView ch = chunk_view(D.chunks + it.b);
@@ -387,12 +451,10 @@ __global__ __launch_bounds__(BS, MINW) void k_apply(const dfea_desc* __restrict_
     int l[NEN];
     int e;
     const bool has_el = chunk_element<NEN, BS>(D, ch, it.b, t, e, l);
-    T d[NQ][NEN][2];
-    T c[NQ][10];
-    if (has_el) {
-#pragma unroll
-        for (int q = 0; q < PF; ++q) load_gauss_point<NEN, NQ, T>(dN, C, ne, e, q, d[q], c[q]);
-    }
+    constexpr int QG = GaussSplit<NQ, Groups<NEN, BS>::G>::QG;
+    T d[QG][NEN][2];
+    T c[QG][10];
+    if (has_el) prefetch_gauss_points<NEN, NQ, BS, PF, T>(dN, C, ne, e, t, d, c);
 
     // 1. gather
     for (int k = t; k < ch.nk; k += BS) {
@@ -404,7 +466,7 @@ __global__ __launch_bounds__(BS, MINW) void k_apply(const dfea_desc* __restrict_
     __syncthreads();
 
     // 2. element products, colour-ordered sums
-    element_product<NEN, NQ, PF, T>(ch, t, has_el, l, dN, C, ne, e, d, c, L);
+    element_product<NEN, NQ, BS, PF, T>(ch, t, has_el, l, dN, C, ne, e, d, c, L, GL);
 
     // 3. owned nodes: final value; shared nodes (partials plans): partial
     V2<T>* part = reinterpret_cast<V2<T>*>(D.part);
@@ -879,6 +941,7 @@ __global__ __launch_bounds__(VBS) void k_pcg_init_final(const dfea_desc* __restr
 template <int NEN, int NQ, int BS, typename T, int PF = 2, int MINW = 1>
 __global__ __launch_bounds__(BS, MINW) void k_pcg(const dfea_desc* __restrict__ tab, const int* __restrict__ items, int par) {
     __shared__ ChunkLDS<T> L;
+    __shared__ GroupLDS<NEN, BS, T> GL;
     __shared__ double s_b[3];
     const Item it = load_item(items);
     const dfea_desc& D = tab[it.s];
@@ -905,12 +968,10 @@ __global__ __launch_bounds__(BS, MINW) void k_pcg(const dfea_desc* __restrict__ 
     int l[NEN];
     int e;
     const bool has_el = chunk_element<NEN, BS>(D, ch, it.b, t, e, l);
-    T d[NQ][NEN][2];
-    T c[NQ][10];
-    if (has_el) {
-#pragma unroll
-        for (int q = 0; q < PF; ++q) load_gauss_point<NEN, NQ, T>(dN, C, ne, e, q, d[q], c[q]);
-    }
+    constexpr int QG = GaussSplit<NQ, Groups<NEN, BS>::G>::QG;
+    T d[QG][NEN][2];
+    T c[QG][10];
+    if (has_el) prefetch_gauss_points<NEN, NQ, BS, PF, T>(dN, C, ne, e, t, d, c);
 
     // scalar step
     double alpha = 0.0, beta = 0.0, gam = 0.0, den = 0.0, rr = 0.0;
@@ -1043,7 +1104,7 @@ __global__ __launch_bounds__(BS, MINW) void k_pcg(const dfea_desc* __restrict__ 
     __syncthreads();
 
     // w_{k+1} = A u_{k+1}
-    element_product<NEN, NQ, PF, T>(ch, t, has_el, l, dN, C, ne, e, d, c, L);
+    element_product<NEN, NQ, BS, PF, T>(ch, t, has_el, l, dN, C, ne, e, d, c, L, GL);
 
     double d_part = 0.0;
     for (int kk = t; kk < ch.nk; kk += BS) {
@@ -1129,11 +1190,10 @@ int dispatch(int kind, F f) {
 }
 
 // PCG kernels in pairs of parities 0, 1 (n_iter rounded up to even).  Quad9 chunks (64 elements, up to 289
-// nodes) run the PCG kernel with BS_Q9P threads, so the node updates are spread over more lanes than the
-// element threads.
+// nodes) run with bs_q9 threads: Gauss-point groups per element, node updates on all lanes.
 template <int NEN, int BS, typename T>
 int pcg_iter(const dfea_desc* table, const int32_t* items, int n_items, int n_iter, void* stream) {
-    constexpr int BSP = NEN == 9 ? BS_Q9P : BS;
+    constexpr int BSP = NEN == 9 ? bs_q9<T>() : BS;
     for (int i = 0; i < n_iter + (n_iter & 1); ++i) {
         const int rc = launch(k_pcg<NEN, NEN, BSP, T>, n_items, BSP, stream, "k_pcg", table, items, i & 1);
         if (rc) return rc;
@@ -1174,7 +1234,8 @@ struct Launch_residual {
 struct Launch_apply {
     const dfea_desc* t; const int32_t* it; int n; const int32_t* fi; int nf; const void* v; void* y; int prec;
     void* st;
-    template <typename T, int NEN, int BS> int go() {
+    template <typename T, int NEN, int BS0> int go() {
+        constexpr int BS = NEN == 9 ? bs_q9<T>() : BS0;
         const T* vv = static_cast<const T*>(v);
         T* yy = static_cast<T*>(y);
         int rc = launch(k_apply<NEN, NEN, BS, T>, n, BS, st, "k_apply", t, it, vv, yy);
