@@ -20,6 +20,7 @@ import ctypes
 import heapq
 import math
 import time
+from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -232,7 +233,17 @@ class Server:
         self.jobs = {}
         self.used_bytes = 0
         self._item_cache = {}
-        self.prep_per_round = 2
+        self._streams = {}
+        self.pcg_chunk_max = 64
+        self.prep_ahead = 16         # requests whose host set-up runs ahead on worker threads
+        self.prep_workers = 4
+
+    def _side_stream(self, idx):
+        """Extra stream number idx (default priority) for launches that may overlap the main stream's."""
+        s = self._streams.get(idx)
+        if s is None:
+            s = self._streams[idx] = torch.cuda.Stream(device=self.device)
+        return s
 
     # -- host <-> device without stalls ----------------------------------------------------------------------
     def _upload(self, a):
@@ -248,7 +259,7 @@ class Server:
     @staticmethod
     def prepare(req):
         """The host-only part of admitting a request (problem set-up and mesh plan); it needs no GPU and
-        runs while the GPU works on a round."""
+        runs on a worker thread while the GPU works."""
         params, theta = request_problem(req)
         cij = params.get("cell_ij")
         plan = MeshPlan(params["nodes"].numpy(), params["conn"].numpy(), params["fixed"].numpy(),
@@ -387,16 +398,36 @@ class Server:
             for j in new:
                 j.pcg_running = True
         if js:
+            # one stream per element kind: the kinds' PCG launches are independent and overlap on the GPU.
+            # Every group's work items are uploaded (on the main stream) before the fork event, so the side
+            # streams' kernels see them.
+            groups = []
             for nen in _kinds(js):
                 sub = [j for j in js if j.mesh.kind == nen]
-                if not sub:
-                    continue
                 # chunk length: grows while a solve keeps iterating (finished solves skip on the device)
-                chunk = min(self.pcg_chunk << min(max(j.pcg_rounds for j in sub), 3), 64)
+                chunk = min(self.pcg_chunk << min(max(j.pcg_rounds for j in sub), 3), self.pcg_chunk_max)
                 for j in sub:
                     j.pcg_rounds += 1
                 it, n = self._dev_items(sub, "chunk")
-                _lib.check(lib.dfea_pcg_iterate(T, _ptr(it), n, chunk, nen, self.prec_id, st), "dfea_pcg_iterate")
+                groups.append((nen, chunk, it, n))
+            main = torch.cuda.current_stream(self.device)
+            fork = None
+            if len(groups) > 1:
+                fork = torch.cuda.Event()
+                fork.record(main)
+            joins = []
+            for idx, (nen, chunk, it, n) in enumerate(groups):
+                stream = main if idx == 0 else self._side_stream(idx)
+                if idx > 0:
+                    stream.wait_event(fork)
+                _lib.check(lib.dfea_pcg_iterate(T, _ptr(it), n, chunk, nen, self.prec_id,
+                                                ctypes.c_void_p(stream.cuda_stream)), "dfea_pcg_iterate")
+                if idx > 0:
+                    done = torch.cuda.Event()
+                    done.record(stream)
+                    joins.append(done)
+            for done in joins:
+                main.wait_event(done)
         if host_work is not None:
             host_work()
         # one host sync: status, iterations, norms
@@ -431,31 +462,34 @@ class Server:
         """Serve ``requests`` (sorted by arrival).  Returns the per-request records and a summary."""
         pending = sorted(requests, key=lambda r: r.arrival)
         records = []
-        prepared = {}
+        futures = {}                 # request index -> host set-up running on the worker thread
+        pool = ThreadPoolExecutor(max_workers=self.prep_workers)
         t0 = time.perf_counter()
         i = 0
 
         def prepare_arrived():
-            # host-only admission work for requests that have arrived, while the GPU runs the round
-            now, k, done = time.perf_counter() - t0, i, 0
-            while k < len(pending) and done < self.prep_per_round and (not realtime or pending[k].arrival <= now):
-                if k not in prepared:
-                    prepared[k] = self.prepare(pending[k])
-                    done += 1
+            # start the host-only admission work of the next arrived requests on the worker thread
+            now, k = time.perf_counter() - t0, i
+            while k < len(pending) and k < i + self.prep_ahead and (not realtime or pending[k].arrival <= now):
+                if k not in futures:
+                    futures[k] = pool.submit(self.prepare, pending[k])
                 k += 1
 
         while i < len(pending) or self.jobs:
             now = time.perf_counter() - t0
+            prepare_arrived()
             while i < len(pending) and (not realtime or pending[i].arrival <= now) and self.free:
                 r = pending[i]
                 fb = footprint_bytes(r.element_type, r.n, self.precision)
                 if self.jobs and self.used_bytes + fb > self.hbm_budget:
                     break
-                if self.jobs and i not in prepared:
-                    break                      # its host set-up runs during the next round
-                job = self._admit(r, prepared.pop(i, None))
+                fut = futures.get(i)
+                if self.jobs and (fut is None or not fut.done()):
+                    break                      # admitted once its host set-up is done; the GPU keeps working
+                job = self._admit(r, futures.pop(i).result() if fut is not None else None)
                 job.t_arrival = t0 + r.arrival if realtime else job.t_admit
                 i += 1
+                prepare_arrived()
             if not self.jobs:
                 if i < len(pending):
                     time.sleep(max(0.0, min(0.01, pending[i].arrival - (time.perf_counter() - t0))))
@@ -472,6 +506,7 @@ class Server:
                 records.append(rec)
                 self._retire(job, info)
         wall = time.perf_counter() - t0
+        pool.shutdown(wait=True)
         lat = np.array([r["latency"] for r in records])
         summary = dict(precision=self.precision, n_requests=len(records), wall_s=wall,
                        solves_per_s=len(records) / wall, p50_latency_s=float(np.percentile(lat, 50)),
