@@ -181,3 +181,28 @@ def test_pcg_negative_curvature_at_first_iteration_returns_z(prec):
     x, k, st = s.pcg_solve(b.to(DEV).to(s.vdt), rtol=1e-8, max_iter=100)
     assert (st, k, k_ref) == (2, 0, 0)
     assert rel(x, x_ref) < (1e-12 if prec == "fp64" else 1e-6)
+
+
+def test_apply_properties_at_the_largest_served_size():
+    """Side-512 quad9 (262,144 elements, 2,101,250 dofs; the largest mesh of the served stream), no oracle at
+    this size: the product is symmetric (a.A b = b.A a), linear, repeatable bit for bit, the identity on
+    fixed dofs, and its FP32-moduli form agrees with FP64 to single precision."""
+    from difffea_amd.operators import make_tangent_operator
+    p = unit_square("quad9", 512)
+    n = p["nodes"].shape[0] * 2
+    u = stretch_state(p, 1.0 / 512, seed=3)
+    g = torch.Generator().manual_seed(4)
+    a = torch.randn(n, dtype=torch.float64, generator=g).to(DEV)
+    b = torch.randn(n, dtype=torch.float64, generator=g).to(DEV)
+    th = torch.ones(p["conn"].shape[0], dtype=torch.float64)
+    A = make_tangent_operator("hip", u.to(DEV), th.to(DEV), p)
+    Aa, Ab = A(a).clone(), A(b).clone()
+    assert torch.equal(A(a), Aa)
+    s1, s2 = float(torch.dot(b, Aa)), float(torch.dot(a, Ab))
+    assert abs(s1 - s2) <= 1e-12 * float(torch.dot(Aa.abs(), b.abs()))
+    lin = A(2.0 * a - 0.5 * b)
+    assert rel(lin, 2.0 * Aa - 0.5 * Ab) < 1e-13
+    fixed = p["fixed"].to(DEV)
+    assert torch.equal(Aa[fixed], a[fixed])
+    A32 = make_tangent_operator("hip", u.to(DEV), th.to(DEV), p, precision="mixed")
+    assert rel(A32(a.float()), Aa) < 1e-5
