@@ -6,10 +6,12 @@
 //     element from structure-of-arrays moduli / geometry planes (coalesced HBM reads), and the element
 //     vectors are summed into LDS node accumulators colour by colour (elements of one colour share no
 //     node), so the summation order is fixed: no atomics, bit-reproducible results.
-//   * Halo plans (quad4): every node is owned by one chunk, and a chunk also evaluates the neighbouring
+//   * Partials plans (the default): nodes used by several chunks receive one partial per chunk, stored
+//     contiguously per node in chunk order, and the next kernel sums them in that order.  Halo plans
+//     (quad4, optional): every node is owned by one chunk, and a chunk also evaluates the neighbouring
 //     ("halo") elements that touch its owned nodes, so every owned node is completed inside its workgroup.
-//     Partials plans (quad9): nodes used by several chunks receive one partial per chunk, stored
-//     contiguously per node in chunk order, and a second kernel sums them in that order.
+//   * PCG runs one kernel per iteration (Chronopoulos-Gear recurrences, see k_pcg), launched in pairs of
+//     parities so that every buffer a launch writes is one no workgroup of that launch reads.
 //   * Every launch processes a list of work items {solve, block}: a batch of independent solves of the
 //     same element type and precision goes through one launch; per-solve reductions depend only on the
 //     solve's own blocks, so a solve gives the same bits alone and inside a batch.

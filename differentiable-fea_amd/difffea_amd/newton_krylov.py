@@ -5,7 +5,7 @@
 * ``newton_krylov_solve(params, theta, ..., tangent="hip", precision="fp64")`` (`newton_krylov.py:186-264`):
   load continuation with cut-backs, Eisenstat-Walker forcing, Armijo line search -- the same state machine
   and ``info`` dictionary -- where every vector operation runs in the HIP library: residual, moduli +
-  Jacobi diagonal (one fused kernel per Newton step), the fused PCG (two kernels per iteration, device
+  Jacobi diagonal (one fused kernel per Newton step), the fused PCG (one kernel per iteration, device
   scalars, status read once per chunk of iterations), line-search trials.
 * ``precision="mixed"``: moduli, A(v) and the inner CG vectors in FP32 (dot products accumulated in FP64),
   while the Newton residual, the state u and the line search stay FP64 -- Newton itself is the outer
