@@ -198,6 +198,9 @@ def sequential_fp64(n_requests, seed=5, side=(32, 512)):
             compute_implicit_vjp(solver, u, lam)
         torch.cuda.synchronize()
         lat.append(time.perf_counter() - t1)
+        if len(lat) % 32 == 0:
+            print(f"# plain FP64 solver: {len(lat)} of {len(reqs)} requests, {time.perf_counter() - t0:.1f} s",
+                  file=sys.stderr, flush=True)
     wall = time.perf_counter() - t0
     return dict(precision="fp64", mode="sequential newton_krylov_solve", n_requests=len(reqs), wall_s=wall,
                 solves_per_s=len(reqs) / wall, p50_latency_s=float(np.percentile(lat, 50)),
@@ -208,7 +211,12 @@ def serve_stream(n_requests, precision, rate=None, seed=5, side=(32, 512)):
     from difffea_amd.serve import Server, make_stream
     reqs = make_stream(n_requests, rate=rate or 1e9, seed=seed, side=side)
     srv = Server(precision, max_slots=64)
-    recs, summ = srv.run(reqs, realtime=rate is not None)
+
+    def progress(done, total):
+        if done % 32 == 0:
+            print(f"# served {precision}: {done} of {total} requests", file=sys.stderr, flush=True)
+
+    recs, summ = srv.run(reqs, realtime=rate is not None, progress=progress)
     summ["n_newton"] = int(sum(r["total_newton"] for r in recs))
     summ["n_cg"] = int(sum(r["total_cg"] for r in recs))
     return summ

@@ -458,8 +458,9 @@ class Server:
         return finished
 
     # -- driver ------------------------------------------------------------------------------------------------
-    def run(self, requests, realtime=True, keep_solution=False):
-        """Serve ``requests`` (sorted by arrival).  Returns the per-request records and a summary."""
+    def run(self, requests, realtime=True, keep_solution=False, progress=None):
+        """Serve ``requests`` (sorted by arrival).  Returns the per-request records and a summary.
+        ``progress``: optional callable(n_done, n_total), called as requests complete."""
         pending = sorted(requests, key=lambda r: r.arrival)
         records = []
         futures = {}                 # request index -> host set-up running on the worker thread
@@ -475,40 +476,45 @@ class Server:
                     futures[k] = pool.submit(self.prepare, pending[k])
                 k += 1
 
-        while i < len(pending) or self.jobs:
-            now = time.perf_counter() - t0
-            prepare_arrived()
-            while i < len(pending) and (not realtime or pending[i].arrival <= now) and self.free:
-                r = pending[i]
-                fb = footprint_bytes(r.element_type, r.n, self.precision)
-                if self.jobs and self.used_bytes + fb > self.hbm_budget:
-                    break
-                fut = futures.get(i)
-                if self.jobs and (fut is None or not fut.done()):
-                    break                      # admitted once its host set-up is done; the GPU keeps working
-                job = self._admit(r, futures.pop(i).result() if fut is not None else None)
-                job.t_arrival = t0 + r.arrival if realtime else job.t_admit
-                i += 1
+        try:
+            while i < len(pending) or self.jobs:
+                now = time.perf_counter() - t0
                 prepare_arrived()
-            if not self.jobs:
-                if i < len(pending):
-                    time.sleep(max(0.0, min(0.01, pending[i].arrival - (time.perf_counter() - t0))))
-                continue
-            for job, info in self._round(prepare_arrived):
-                t_done = time.perf_counter()
-                rec = dict(rid=job.req.rid, element_type=job.req.element_type, n=job.req.n, adjoint=job.req.adjoint,
-                           n_dof=job.mesh.n_dof, tol_abs=job.tol, latency=t_done - job.t_arrival,
-                           queue=job.t_admit - job.t_arrival, **info)
-                if keep_solution:
-                    rec["u"] = job.sys.vec["uacc"].detach().cpu().clone()
-                    if "grad" in job.result:
-                        rec["grad"] = job.result["grad"].detach().cpu().clone()
-                records.append(rec)
-                self._retire(job, info)
+                while i < len(pending) and (not realtime or pending[i].arrival <= now) and self.free:
+                    r = pending[i]
+                    fb = footprint_bytes(r.element_type, r.n, self.precision)
+                    if self.jobs and self.used_bytes + fb > self.hbm_budget:
+                        break
+                    fut = futures.get(i)
+                    if self.jobs and (fut is None or not fut.done()):
+                        break                      # admitted once its host set-up is done; the GPU keeps working
+                    job = self._admit(r, futures.pop(i).result() if fut is not None else None)
+                    job.t_arrival = t0 + r.arrival if realtime else job.t_admit
+                    i += 1
+                    prepare_arrived()
+                if not self.jobs:
+                    if i < len(pending):
+                        time.sleep(max(0.0, min(0.01, pending[i].arrival - (time.perf_counter() - t0))))
+                    continue
+                for job, info in self._round(prepare_arrived):
+                    t_done = time.perf_counter()
+                    rec = dict(rid=job.req.rid, element_type=job.req.element_type, n=job.req.n, adjoint=job.req.adjoint,
+                               n_dof=job.mesh.n_dof, tol_abs=job.tol, latency=t_done - job.t_arrival,
+                               queue=job.t_admit - job.t_arrival, **info)
+                    if keep_solution:
+                        rec["u"] = job.sys.vec["uacc"].detach().cpu().clone()
+                        if "grad" in job.result:
+                            rec["grad"] = job.result["grad"].detach().cpu().clone()
+                    records.append(rec)
+                    self._retire(job, info)
+                    if progress is not None:
+                        progress(len(records), len(pending))
+        finally:
+            pool.shutdown(wait=True)
         wall = time.perf_counter() - t0
-        pool.shutdown(wait=True)
         lat = np.array([r["latency"] for r in records])
         summary = dict(precision=self.precision, n_requests=len(records), wall_s=wall,
+                       latency_from="arrival" if realtime else "admission",
                        solves_per_s=len(records) / wall, p50_latency_s=float(np.percentile(lat, 50)),
                        p99_latency_s=float(np.percentile(lat, 99)),
                        converged=int(sum(r["converged"] for r in records)))
