@@ -38,6 +38,9 @@ constexpr int BS_Q9 = 64;         // workgroup size for quad9
 // registers limit the waves per SIMD, 3 in FP32; measured fastest on 200x200 and the 64x256 O-grid)
 template <typename T>
 constexpr int bs_q9() { return std::is_same<T, double>::value ? 128 : 192; }
+// PCG launches of at most this many workgroups (one per CU or fewer: latency-bound) load the first node's
+// state before the scalar step
+constexpr int PCG_EARLY_ITEMS = 256;
 constexpr int MAXLN = 640;        // local nodes per chunk (LDS capacity)
 constexpr int VBS = 256;          // workgroup size of the vector kernels
 constexpr int VPT = 4;            // dofs per thread in the vector kernels
@@ -947,8 +950,59 @@ __global__ __launch_bounds__(VBS) void k_pcg_init_final(const dfea_desc* __restr
     }
 }
 
-// one PCG iteration (see above); par = launch parity; MINW: minimum waves per SIMD for the register budget
-template <int NEN, int NQ, int BS, typename T, int PF = 2, int MINW = 1>
+// the state of local node kk of a chunk for the PCG update: x_k, r_k, p_{k-1}, s_{k-1}, w_k (w of a shared
+// node summed from the previous kernel's partials in slot order), diag, fixed bits, and whether this chunk
+// writes the node (owned nodes; a shared node's first-slot chunk; halo plans: nodes past nown are owned by
+// another chunk and only read)
+template <typename T>
+struct NodeState {
+    V2<T> x, r, p, s, w, d;
+    int n;
+    unsigned char b;
+    bool writer;
+};
+
+template <typename T>
+__device__ __forceinline__ NodeState<T> load_node_state(const dfea_desc& D, const ChunkView& ch, int kk, bool upd,
+                                                        const PcgSet<T>& Rd, const V2<T>* dg, const V2<T>* part_rd,
+                                                        const int2* shs) {
+    NodeState<T> ns;
+    const uint32_t g = D.lnode[ch.k0 + kk];
+    ns.n = (int)(g & DFEA_NODE_MASK);
+    ns.b = (unsigned char)(g >> 30);
+    const bool shared = kk >= ch.nown && D.n_part;
+    ns.writer = kk < ch.nown;
+    int2 ss = make_int2(0, 0);
+    if (shared) {
+        ss = shs[ns.n];
+        ns.writer = ss.x == D.pslot[ch.pbase + (kk - ch.nown)];
+    }
+    ns.d = dg[ns.n];
+    ns.r = Rd.r[ns.n];
+    if (upd) {
+        if (!shared) {
+            ns.w = Rd.w[ns.n];
+        } else {
+            V2<T> acc{0, 0};
+            for (int j = ss.x; j < ss.x + ss.y; ++j) {
+                const V2<T> pj = part_rd[j];
+                acc.x += pj.x;
+                acc.y += pj.y;
+            }
+            ns.w = acc;
+        }
+        ns.x = Rd.x[ns.n];
+        ns.p = Rd.p[ns.n];
+        ns.s = Rd.s[ns.n];
+    }
+    return ns;
+}
+
+// one PCG iteration (see above); par = launch parity; MINW: minimum waves per SIMD for the register budget.
+// EARLY: the state of the thread's first node is loaded before the scalar step, so its loads overlap the
+// reduction; it costs registers (waves per SIMD), which matters only when the launch has more workgroups
+// than the GPU has CUs.
+template <int NEN, int NQ, int BS, typename T, int PF = 2, int MINW = 1, bool EARLY = false>
 __global__ __launch_bounds__(BS, MINW) void k_pcg(const dfea_desc* __restrict__ tab, const int* __restrict__ items, int par) {
     __shared__ ChunkLDS<T> L;
     __shared__ GroupLDS<NEN, BS, T> GL;
@@ -982,6 +1036,10 @@ __global__ __launch_bounds__(BS, MINW) void k_pcg(const dfea_desc* __restrict__ 
     T d[QG][NEN][2];
     T c[QG][10];
     if (has_el) prefetch_gauss_points<NEN, NQ, BS, PF, T>(dN, C, ne, e, t, d, c);
+
+    // the first node's state (independent of the scalars)
+    NodeState<T> ns0;
+    if (EARLY && t < ch.nk) ns0 = load_node_state<T>(D, ch, t, k >= 0.0, Rd, dg, part_rd, shs);
 
     // scalar step
     double alpha = 0.0, beta = 0.0, gam = 0.0, den = 0.0, rr = 0.0;
@@ -1055,47 +1113,27 @@ __global__ __launch_bounds__(BS, MINW) void k_pcg(const dfea_desc* __restrict__ 
     // update of the chunk's nodes, u_{k+1} into LDS
     double g_part = 0.0, r_part = 0.0;
     for (int kk = t; kk < ch.nk; kk += BS) {
-        const uint32_t g = D.lnode[ch.k0 + kk];
-        const int n = (int)(g & DFEA_NODE_MASK);
-        const unsigned char b = (unsigned char)(g >> 30);
-        // writer: owned nodes, and a shared node's first-slot chunk (halo plans: nodes past nown are owned
-        // by another chunk and only read)
-        const bool shared = kk >= ch.nown && D.n_part;
-        bool writer = kk < ch.nown;
-        int2 ss = make_int2(0, 0);
-        if (shared) {
-            ss = shs[n];
-            writer = ss.x == D.pslot[ch.pbase + (kk - ch.nown)];
-        }
-        const V2<T> dv = dg[n];
-        V2<T> rn = Rd.r[n];
+        const bool first = EARLY && kk == t;
+        NodeState<T> ns = first ? ns0 : load_node_state<T>(D, ch, kk, k >= 0.0, Rd, dg, part_rd, shs);
+        const int n = ns.n;
+        const unsigned char b = ns.b;
+        V2<T> rn = ns.r;
+        const V2<T> dv = ns.d;
         if (k >= 0.0) {
-            V2<T> w;
-            if (!shared) {
-                w = Rd.w[n];
-            } else {                    // shared node: w from the previous kernel's partials, slot order
-                V2<T> acc{0, 0};
-                for (int j = ss.x; j < ss.x + ss.y; ++j) {
-                    const V2<T> pj = part_rd[j];
-                    acc.x += pj.x;
-                    acc.y += pj.y;
-                }
-                w = acc;
-            }
-            const V2<T> xo = Rd.x[n], po = Rd.p[n], so = Rd.s[n];
+            V2<T> w = ns.w;
             const V2<T> u{rn.x / dv.x, rn.y / dv.y};
             if (b & 1) w.x = u.x;       // fixed dofs: A acts as the identity
             if (b & 2) w.y = u.y;
             V2<T> pn, sn, xn;
-            pn.x = (T)((double)u.x + beta * (double)po.x);
-            pn.y = (T)((double)u.y + beta * (double)po.y);
-            sn.x = (T)((double)w.x + beta * (double)so.x);
-            sn.y = (T)((double)w.y + beta * (double)so.y);
-            xn.x = (T)((double)xo.x + alpha * (double)pn.x);
-            xn.y = (T)((double)xo.y + alpha * (double)pn.y);
+            pn.x = (T)((double)u.x + beta * (double)ns.p.x);
+            pn.y = (T)((double)u.y + beta * (double)ns.p.y);
+            sn.x = (T)((double)w.x + beta * (double)ns.s.x);
+            sn.y = (T)((double)w.y + beta * (double)ns.s.y);
+            xn.x = (T)((double)ns.x.x + alpha * (double)pn.x);
+            xn.y = (T)((double)ns.x.y + alpha * (double)pn.y);
             rn.x = (T)((double)rn.x - alpha * (double)sn.x);
             rn.y = (T)((double)rn.y - alpha * (double)sn.y);
-            if (writer) {
+            if (ns.writer) {
                 Wr.p[n] = pn;
                 Wr.s[n] = sn;
                 Wr.x[n] = xn;
@@ -1103,7 +1141,7 @@ __global__ __launch_bounds__(BS, MINW) void k_pcg(const dfea_desc* __restrict__ 
             }
         }
         const V2<T> un{rn.x / dv.x, rn.y / dv.y};
-        if (writer) {
+        if (ns.writer) {
             g_part += (double)rn.x * (double)un.x + (double)rn.y * (double)un.y;
             r_part += (double)rn.x * (double)rn.x + (double)rn.y * (double)rn.y;
         }
@@ -1204,8 +1242,12 @@ int dispatch(int kind, F f) {
 template <int NEN, int BS, typename T>
 int pcg_iter(const dfea_desc* table, const int32_t* items, int n_items, int n_iter, void* stream) {
     constexpr int BSP = NEN == 9 ? bs_q9<T>() : BS;
+    const bool early = n_items <= PCG_EARLY_ITEMS;
     for (int i = 0; i < n_iter + (n_iter & 1); ++i) {
-        const int rc = launch(k_pcg<NEN, NEN, BSP, T>, n_items, BSP, stream, "k_pcg", table, items, i & 1);
+        const int rc = early ? launch(k_pcg<NEN, NEN, BSP, T, 2, 1, true>, n_items, BSP, stream, "k_pcg", table, items,
+                                      i & 1)
+                             : launch(k_pcg<NEN, NEN, BSP, T, 2, 1, false>, n_items, BSP, stream, "k_pcg", table,
+                                      items, i & 1);
         if (rc) return rc;
     }
     return 0;
