@@ -498,6 +498,31 @@ __global__ __launch_bounds__(BS, MINW) void k_apply(const dfea_desc* __restrict_
     }
 }
 
+// sum of the cnt partials of a shared node starting at slot j0, in slot order; the first SUM_UNROLL are
+// loaded together, so the loads are in flight at once rather than one after another
+constexpr int SUM_UNROLL = 4;
+template <typename T>
+__device__ __forceinline__ V2<T> sum_slots(const V2<T>* part, int j0, int cnt) {
+    V2<T> pv[SUM_UNROLL];
+#pragma unroll
+    for (int q = 0; q < SUM_UNROLL; ++q)
+        if (q < cnt) pv[q] = part[j0 + q];
+    V2<T> acc{0, 0};
+#pragma unroll
+    for (int q = 0; q < SUM_UNROLL; ++q) {
+        if (q < cnt) {
+            acc.x += pv[q].x;
+            acc.y += pv[q].y;
+        }
+    }
+    for (int j = j0 + SUM_UNROLL; j < j0 + cnt; ++j) {
+        const V2<T> pj = part[j];
+        acc.x += pj.x;
+        acc.y += pj.y;
+    }
+    return acc;
+}
+
 // shared nodes of the tangent product (partials plans): partials summed in slot order
 template <typename T>
 __global__ __launch_bounds__(FBS) void k_apply_fin(const dfea_desc* __restrict__ tab, const int* __restrict__ items,
@@ -509,12 +534,8 @@ __global__ __launch_bounds__(FBS) void k_apply_fin(const dfea_desc* __restrict__
     T* y = vout ? vout : static_cast<T*>(D.vec[DFEA_W]);
     if (i < D.n_shared) {
         const V2<T>* part = reinterpret_cast<const V2<T>*>(D.part);
-        V2<T> acc{0, 0};
-        for (int j = D.sh_off[i]; j < D.sh_off[i + 1]; ++j) {
-            const V2<T> pj = part[j];
-            acc.x += pj.x;
-            acc.y += pj.y;
-        }
+        const int j0 = D.sh_off[i];
+        const V2<T> acc = sum_slots<T>(part, j0, D.sh_off[i + 1] - j0);
         const uint32_t g = D.sh_node[i];
         const int n = (int)(g & DFEA_NODE_MASK);
         const V2<T> raw = reinterpret_cast<const V2<T>*>(v)[n];
@@ -609,12 +630,8 @@ __global__ __launch_bounds__(FBS) void k_residual_fin(const dfea_desc* __restric
     double* R = rout ? rout : static_cast<double*>(D.vec[DFEA_RES]);
     const double lf = lf_arg >= 0.0 ? lf_arg : D.sc[DFEA_SC_LF];
     const V2<double>* part = reinterpret_cast<const V2<double>*>(D.part);
-    V2<double> acc{0, 0};
-    for (int j = D.sh_off[i]; j < D.sh_off[i + 1]; ++j) {
-        const V2<double> pj = part[j];
-        acc.x += pj.x;
-        acc.y += pj.y;
-    }
+    const int j0 = D.sh_off[i];
+    const V2<double> acc = sum_slots<double>(part, j0, D.sh_off[i + 1] - j0);
     const uint32_t g = D.sh_node[i];
     const int n = (int)(g & DFEA_NODE_MASK);
     V2<double> out;
@@ -709,12 +726,8 @@ __global__ __launch_bounds__(FBS) void k_moduli_fin(const dfea_desc* __restrict_
     const int i = it.b * FBS + threadIdx.x;
     if (i >= D.n_shared) return;
     const V2<double>* part = reinterpret_cast<const V2<double>*>(D.part);
-    V2<double> acc{0, 0};
-    for (int j = D.sh_off[i]; j < D.sh_off[i + 1]; ++j) {
-        const V2<double> pj = part[j];
-        acc.x += pj.x;
-        acc.y += pj.y;
-    }
+    const int j0 = D.sh_off[i];
+    const V2<double> acc = sum_slots<double>(part, j0, D.sh_off[i + 1] - j0);
     const uint32_t g = D.sh_node[i];
     const int n = (int)(g & DFEA_NODE_MASK);
     const double dx = (g & DFEA_FIXED_X) ? 1.0 : fmax(acc.x, 1e-6);
@@ -983,13 +996,7 @@ __device__ __forceinline__ NodeState<T> load_node_state(const dfea_desc& D, cons
         if (!shared) {
             ns.w = Rd.w[ns.n];
         } else {
-            V2<T> acc{0, 0};
-            for (int j = ss.x; j < ss.x + ss.y; ++j) {
-                const V2<T> pj = part_rd[j];
-                acc.x += pj.x;
-                acc.y += pj.y;
-            }
-            ns.w = acc;
+            ns.w = sum_slots<T>(part_rd, ss.x, ss.y);
         }
         ns.x = Rd.x[ns.n];
         ns.p = Rd.p[ns.n];
