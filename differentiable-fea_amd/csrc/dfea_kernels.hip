@@ -89,6 +89,38 @@ __device__ double block_sum(double v, double* s_red) {
     return t;
 }
 
+// three workgroup sums at once, each in the order block_sum uses.  ALL: every thread computes the totals
+// (same order, same bits) after one barrier, and s_red must not be written again before a later barrier;
+// otherwise the totals are valid on thread 0 and a second barrier frees s_red.  s_red holds 3 * BS / 64
+// doubles.
+template <int BS, bool ALL = false>
+__device__ void block_sum3(double& a, double& b, double& c, double* s_red) {
+    for (int o = 32; o > 0; o >>= 1) {
+        a += __shfl_xor(a, o);
+        b += __shfl_xor(b, o);
+        c += __shfl_xor(c, o);
+    }
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        s_red[3 * w] = a;
+        s_red[3 * w + 1] = b;
+        s_red[3 * w + 2] = c;
+    }
+    __syncthreads();
+    if (ALL || threadIdx.x == 0) {
+        double ta = 0.0, tb = 0.0, tc = 0.0;
+        for (int i = 0; i < BS / 64; ++i) {
+            ta += s_red[3 * i];
+            tb += s_red[3 * i + 1];
+            tc += s_red[3 * i + 2];
+        }
+        a = ta;
+        b = tb;
+        c = tc;
+    }
+    if (!ALL) __syncthreads();
+}
+
 // sum of n partials in a fixed order, identical in every workgroup that calls it; valid on thread 0
 template <int BS>
 __device__ double sum_partials(const double* p, int n, double* s_red) {
@@ -1011,9 +1043,9 @@ __device__ __forceinline__ NodeState<T> load_node_state(const dfea_desc& D, cons
 // than the GPU has CUs.
 template <int NEN, int NQ, int BS, typename T, int PF = 2, int MINW = 1, bool EARLY = false>
 __global__ __launch_bounds__(BS, MINW) void k_pcg(const dfea_desc* __restrict__ tab, const int* __restrict__ items, int par) {
+    static_assert(3 * (BS / 64) <= 16, "block_sum3 needs 3 slots per wave in ChunkLDS::red");
     __shared__ ChunkLDS<T> L;
     __shared__ GroupLDS<NEN, BS, T> GL;
-    __shared__ double s_b[3];
     const Item it = load_item(items);
     const dfea_desc& D = tab[it.s];
     double* sc = D.sc;
@@ -1058,18 +1090,10 @@ __global__ __launch_bounds__(BS, MINW) void k_pcg(const dfea_desc* __restrict__ 
             a1 += rd[3 * i + 1];
             a2 += rd[3 * i + 2];
         }
-        a0 = block_sum<BS>(a0, L.red);
-        a1 = block_sum<BS>(a1, L.red);
-        a2 = block_sum<BS>(a2, L.red);
-        if (t == 0) {
-            s_b[0] = a0;
-            s_b[1] = a1;
-            s_b[2] = a2;
-        }
-        __syncthreads();
-        gam = s_b[0];
-        const double delta = s_b[1];
-        rr = s_b[2];
+        block_sum3<BS, true>(a0, a1, a2, L.red);     // L.red is next written after the gather barrier
+        gam = a0;
+        const double delta = a1;
+        rr = a2;
         double st = DFEA_PCG_RUNNING, iters = 0.0;
         if (k >= 1.0 && sqrt(rr) <= sc[DFEA_SC_RTOL] * sqrt(sc[DFEA_SC_BB])) {
             st = DFEA_PCG_CONVERGED;
@@ -1186,9 +1210,7 @@ __global__ __launch_bounds__(BS, MINW) void k_pcg(const dfea_desc* __restrict__ 
             }
         }
     }
-    g_part = block_sum<BS>(g_part, L.red);
-    d_part = block_sum<BS>(d_part, L.red);
-    r_part = block_sum<BS>(r_part, L.red);
+    block_sum3<BS, true>(g_part, d_part, r_part, L.red);     // last use of L.red
     if (t == 0) {
         double* wr = pcg_red(D, par) + 3 * it.b;
         wr[0] = g_part;
