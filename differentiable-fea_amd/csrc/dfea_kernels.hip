@@ -1010,7 +1010,7 @@ struct NodeState {
 template <typename T>
 __device__ __forceinline__ NodeState<T> load_node_state(const dfea_desc& D, const ChunkView& ch, int kk, bool upd,
                                                         const PcgSet<T>& Rd, const V2<T>* dg, const V2<T>* part_rd,
-                                                        const int2* shs) {
+                                                        const int2* prg) {
     NodeState<T> ns;
     const uint32_t g = D.lnode[ch.k0 + kk];
     ns.n = (int)(g & DFEA_NODE_MASK);
@@ -1018,9 +1018,10 @@ __device__ __forceinline__ NodeState<T> load_node_state(const dfea_desc& D, cons
     const bool shared = kk >= ch.nown && D.n_part;
     ns.writer = kk < ch.nown;
     int2 ss = make_int2(0, 0);
-    if (shared) {
-        ss = shs[ns.n];
-        ns.writer = ss.x == D.pslot[ch.pbase + (kk - ch.nown)];
+    if (shared) {                 // entry tables, loaded together with the node id
+        const int ent = ch.pbase + (kk - ch.nown);
+        ss = prg[ent];
+        ns.writer = ss.x == D.pslot[ent];
     }
     ns.d = dg[ns.n];
     ns.r = Rd.r[ns.n];
@@ -1065,7 +1066,7 @@ __global__ __launch_bounds__(BS, MINW) void k_pcg(const dfea_desc* __restrict__ 
     const PcgSet<T> Wr = pcg_set<T>(D, par);   // state buffers written
     const V2<T>* part_rd = reinterpret_cast<const V2<T>*>(D.part) + (size_t)(1 - par) * D.n_part;
     V2<T>* part_wr = reinterpret_cast<V2<T>*>(D.part) + (size_t)par * D.n_part;
-    const int2* shs = reinterpret_cast<const int2*>(D.shslot);
+    const int2* prg = reinterpret_cast<const int2*>(D.prange);
 
     // element and its first Gauss points (independent of the scalars)
     int l[NEN];
@@ -1078,7 +1079,7 @@ __global__ __launch_bounds__(BS, MINW) void k_pcg(const dfea_desc* __restrict__ 
 
     // the first node's state (independent of the scalars)
     NodeState<T> ns0;
-    if (EARLY && t < ch.nk) ns0 = load_node_state<T>(D, ch, t, k >= 0.0, Rd, dg, part_rd, shs);
+    if (EARLY && t < ch.nk) ns0 = load_node_state<T>(D, ch, t, k >= 0.0, Rd, dg, part_rd, prg);
 
     // scalar step
     double alpha = 0.0, beta = 0.0, gam = 0.0, den = 0.0, rr = 0.0;
@@ -1128,7 +1129,8 @@ __global__ __launch_bounds__(BS, MINW) void k_pcg(const dfea_desc* __restrict__ 
                 V2<T>* x0 = static_cast<V2<T>*>(D.vec[DFEA_X]);
                 for (int kk = t; kk < ch.nk; kk += BS) {
                     const int n = (int)(D.lnode[ch.k0 + kk] & DFEA_NODE_MASK);
-                    if (kk >= ch.nown && (!D.n_part || shs[n].x != D.pslot[ch.pbase + (kk - ch.nown)])) continue;
+                    if (kk >= ch.nown && (!D.n_part || prg[ch.pbase + (kk - ch.nown)].x != D.pslot[ch.pbase + (kk - ch.nown)]))
+                        continue;
                     if (at0) {
                         const V2<T> rv = Rd.r[n], dv = dg[n];
                         x0[n] = V2<T>{rv.x / dv.x, rv.y / dv.y};
@@ -1145,7 +1147,7 @@ __global__ __launch_bounds__(BS, MINW) void k_pcg(const dfea_desc* __restrict__ 
     double g_part = 0.0, r_part = 0.0;
     for (int kk = t; kk < ch.nk; kk += BS) {
         const bool first = EARLY && kk == t;
-        NodeState<T> ns = first ? ns0 : load_node_state<T>(D, ch, kk, k >= 0.0, Rd, dg, part_rd, shs);
+        NodeState<T> ns = first ? ns0 : load_node_state<T>(D, ch, kk, k >= 0.0, Rd, dg, part_rd, prg);
         const int n = ns.n;
         const unsigned char b = ns.b;
         V2<T> rn = ns.r;
@@ -1197,13 +1199,13 @@ __global__ __launch_bounds__(BS, MINW) void k_pcg(const dfea_desc* __restrict__ 
             Wr.w[(int)(D.lnode[ch.k0 + kk] & DFEA_NODE_MASK)] = out;
             d_part += (double)u.x * (double)out.x + (double)u.y * (double)out.y;
         } else if (D.n_part) {
-            const int slot = D.pslot[ch.pbase + (kk - ch.nown)];
+            const int ent = ch.pbase + (kk - ch.nown);
+            const int slot = D.pslot[ent];
             part_wr[slot] = acc;
             if (!(b & 1)) d_part += (double)u.x * (double)acc.x;
             if (!(b & 2)) d_part += (double)u.y * (double)acc.y;
             if (b) {     // fixed components (w = u there) once, by the chunk holding the node's first slot
-                const int n = (int)(D.lnode[ch.k0 + kk] & DFEA_NODE_MASK);
-                if (slot == shs[n].x) {
+                if (slot == prg[ent].x) {
                     if (b & 1) d_part += (double)u.x * (double)u.x;
                     if (b & 2) d_part += (double)u.y * (double)u.y;
                 }
@@ -1358,7 +1360,7 @@ int dfea_desc_offset(int field) {
     static const int off[] = {
         (int)offsetof(dfea_desc, chunks), (int)offsetof(dfea_desc, lnode), (int)offsetof(dfea_desc, lelem),
         (int)offsetof(dfea_desc, sh_node), (int)offsetof(dfea_desc, sh_off), (int)offsetof(dfea_desc, pslot), (int)offsetof(dfea_desc, hel), (int)offsetof(dfea_desc, hlelem),
-        (int)offsetof(dfea_desc, shslot),
+        (int)offsetof(dfea_desc, prange),
         (int)offsetof(dfea_desc, conn), (int)offsetof(dfea_desc, perm), (int)offsetof(dfea_desc, nodes),
         (int)offsetof(dfea_desc, dNdxi), (int)offsetof(dfea_desc, gw), (int)offsetof(dfea_desc, nfix),
         (int)offsetof(dfea_desc, dN64), (int)offsetof(dfea_desc, dN32), (int)offsetof(dfea_desc, w),

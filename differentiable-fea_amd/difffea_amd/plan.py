@@ -368,6 +368,13 @@ class MeshPlan:
         out[nodes, 1] = np.diff(self.sh_off)
         return out
 
+    @property
+    def prange(self):
+        """[n_part][2] int32: first slot and number of slots of the shared node of every partials entry
+        (chunk order, like pslot)."""
+        s = np.searchsorted(self.sh_off, self.pslot, side="right") - 1      # shared index of each entry
+        return np.stack([self.sh_off[s], self.sh_off[s + 1] - self.sh_off[s]], axis=1).astype(np.int32)
+
     def validate(self):
         """Host-side bounds checks of every index the kernels follow (run before any launch).
         Returns the largest number of threads a chunk needs (core + halo elements)."""

@@ -93,8 +93,8 @@ typedef struct dfea_desc {
                                    the partials of shared node i are slots sh_off[i]..sh_off[i+1]-1 */
     const int32_t* hel;         /* [n_halo] halo mode: internal element id of each halo entry */
     const uint16_t* hlelem;     /* [nen][n_halo] halo mode: local node index of halo entry h, node a */
-    const int32_t* shslot;      /* [n_nodes][2] {first partial slot, number of partials} of each node; {0, 0}
-                                   for nodes owned by one chunk */
+    const int32_t* prange;      /* [n_part][2] {first slot, number of slots} of the shared node of each (chunk,
+                                   shared node) entry, chunk order (entry tables load without the node id) */
     const int32_t* conn;        /* [nen][n_el] node ids, internal element order */
     const int32_t* perm;        /* [n_el] original element id of internal element e */
     const double* nodes;        /* [N][2] reference coordinates */

@@ -132,6 +132,14 @@ def test_plan_tables_and_emulated_assembly_match_oracle(name, mode):
         s_ = sm[nd]
         assert plan.sh_node[s_] & _lib.NODE_MASK == nd
         assert tuple(ss[nd]) == (plan.sh_off[s_], plan.sh_off[s_ + 1] - plan.sh_off[s_])
+    # the PCG kernel's per-entry copy of the slot ranges: entry -> its node's range, which holds its slot
+    if plan.n_part:
+        pr = plan.prange
+        nodes_of_entry = np.concatenate([(plan.lnode[c[2] + c[4]:c[2] + c[3]] & _lib.NODE_MASK).astype(np.int64)
+                                         for c in plan.chunks])
+        assert np.array_equal(pr, ss[nodes_of_entry][np.argsort(np.concatenate(
+            [np.arange(c[5], c[5] + c[3] - c[4]) for c in plan.chunks]))])
+        assert ((plan.pslot >= pr[:, 0]) & (plan.pslot < pr[:, 0] + pr[:, 1])).all()
 
 
 def test_structured_meshes_get_square_tiles_and_four_colours():
