@@ -254,6 +254,46 @@ def make_jacobi_preconditioner(u, theta, params):
 # ---------------------------------------------------------------------------------------------------------
 # Krylov + Newton                                                   (newton_krylov.py:28-264)
 # ---------------------------------------------------------------------------------------------------------
+def element_tangent_blocks(u, theta, params):
+    """The 2x2 diagonal blocks of every element tangent, (n_el, nen, 2, 2) [e, a, i, j] = dR^e[a,i]/du[a,j]
+    (`operators.py:588-615`): the element Jacobian column by column (one jvp per local unit vector), then its
+    node-diagonal blocks."""
+    conn = params["conn"]
+    N = params["nodes"].shape[0]
+    nen = conn.shape[1]
+    ue = u.reshape(N, 2)[conn]
+    pe = element_material_parameters(theta, params)
+    dN_dX, w_detJ = get_geometry(params)
+    units = torch.eye(2 * nen, dtype=u.dtype).reshape(-1, nen, 2)
+
+    def jacobian(u_e, dN_e, w_e, p_e):
+        res = lambda w: elem_residual_precomputed(w, dN_e, w_e, p_e)
+        cols = vmap(lambda e: jvp(res, (u_e,), (e,))[1])(units)      # [(b, j), a, i]
+        cols = cols.reshape(nen, 2, nen, 2)                           # [b, j, a, i]
+        return torch.diagonal(cols, dim1=0, dim2=2).permute(2, 1, 0)  # [a, i, j], a == b
+
+    return vmap(jacobian)(ue, dN_dX, w_detJ, pe)
+
+
+def block_jacobi_blocks(u, theta, params):
+    """Nodal 2x2 blocks of A: scatter of the element blocks, identity rows / columns on fixed dofs, plus the
+    1e-12 I guard (`operators.py:618-638`); returns (N, 2, 2)."""
+    conn, m = params["conn"], params["m"]
+    N = params["nodes"].shape[0]
+    blocks = element_tangent_blocks(u, theta, params)
+    B = torch.zeros(N, 2, 2, dtype=u.dtype).index_add_(0, conn.reshape(-1), blocks.reshape(-1, 2, 2))
+    mn = m.reshape(N, 2)
+    B = B * (mn[:, :, None] * mn[:, None, :]) + torch.diag_embed(1.0 - mn)
+    return B + 1e-12 * torch.eye(2, dtype=u.dtype)
+
+
+def make_block_jacobi_preconditioner(u, theta, params):
+    """r -> B_n^-1 r_n per node (`operators.py:618-638`)."""
+    N = params["nodes"].shape[0]
+    Binv = torch.linalg.inv(block_jacobi_blocks(u, theta, params))
+    return lambda r: torch.einsum("nij,nj->ni", Binv, r.reshape(N, 2)).reshape(-1)
+
+
 def pcg(A_fn, b, Minv_fn, rtol=1e-5, max_iter=500):
     """Textbook Jacobi-PCG with x0 = 0; curvature guard; stop when ||r|| <= rtol ||b||."""
     x = torch.zeros_like(b)
