@@ -672,7 +672,31 @@ __global__ __launch_bounds__(FBS) void k_residual_fin(const dfea_desc* __restric
     reinterpret_cast<V2<double>*>(R)[n] = out;
 }
 
-// ---- moduli + Jacobi diagonal  (`operators.py:316-342`, `:477-556`) ------------------------------------------
+// ---- moduli + preconditioner  (`operators.py:316-342`, `:477-556`, `:588-638`) -------------------------------
+// the preconditioner of one node from its summed diagonal entries a = (A_xx, A_yy) and, for block Jacobi, the
+// off-diagonal entry axy of its 2x2 block: point Jacobi keeps the clamped diagonal (1 on fixed dofs); block
+// Jacobi pivots fixed dofs (identity row and column), adds 1e-12 I and stores the closed-form inverse,
+// {m00, m11} at the node's dofs and m01 after the 2N diagonal entries
+template <typename CT>
+__device__ __forceinline__ void store_precond(const dfea_desc& D, int n, unsigned char b, V2<double> a, double axy) {
+    CT* out = std::is_same<CT, double>::value ? (CT*)D.diag64 : (CT*)D.diag32;
+    double d0, d1;
+    if (D.precond == DFEA_PRECOND_BLOCK) {
+        const double b00 = ((b & 1) ? 1.0 : a.x) + 1e-12;
+        const double b11 = ((b & 2) ? 1.0 : a.y) + 1e-12;
+        const double b01 = b ? 0.0 : axy;
+        const double det = b00 * b11 - b01 * b01;
+        d0 = b11 / det;
+        d1 = b00 / det;
+        out[2 * D.n_nodes + n] = (CT)(-b01 / det);
+    } else {
+        d0 = (b & 1) ? 1.0 : fmax(a.x, 1e-6);
+        d1 = (b & 2) ? 1.0 : fmax(a.y, 1e-6);
+    }
+    out[2 * n] = (CT)d0;
+    out[2 * n + 1] = (CT)d1;
+}
+
 template <int NEN, int NQ, int BS, typename CT>
 __global__ __launch_bounds__(BS) void k_moduli(const dfea_desc* __restrict__ tab, const int* __restrict__ items,
                                               const double* __restrict__ uin) {
@@ -693,7 +717,9 @@ __global__ __launch_bounds__(BS) void k_moduli(const dfea_desc* __restrict__ tab
     int l[NEN];
     int e;
     const bool has_el = thread_element<NEN>(D, ch, t, e, l);
+    const bool blk = D.precond == DFEA_PRECOND_BLOCK;
     V2<double> fe[NEN];
+    V2<double> fo[NEN];     // block Jacobi: {A_xy, 0} of the element's nodal blocks
     if (has_el) {
         const int ne = D.n_cols;
         const double th = D.theta[e];
@@ -704,6 +730,7 @@ __global__ __launch_bounds__(BS) void k_moduli(const dfea_desc* __restrict__ tab
         for (int a = 0; a < NEN; ++a) {
             ue[a] = L.v[l[a]];
             fe[a] = V2<double>{0, 0};
+            fo[a] = V2<double>{0, 0};
         }
         for (int q = 0; q < NQ; ++q) {
             double d[NEN][2];
@@ -726,27 +753,31 @@ __global__ __launch_bounds__(BS) void k_moduli(const dfea_desc* __restrict__ tab
                 const double d0 = d[a][0], d1 = d[a][1];
                 fe[a].x += c[uidx(0, 0)] * d0 * d0 + 2.0 * c[uidx(0, 1)] * d0 * d1 + c[uidx(1, 1)] * d1 * d1;
                 fe[a].y += c[uidx(2, 2)] * d0 * d0 + 2.0 * c[uidx(2, 3)] * d0 * d1 + c[uidx(3, 3)] * d1 * d1;
+                // K^e_{(a,0),(a,1)} = sum_{j,l} C[j][2+l] dN[a][j] dN[a][l]
+                if (blk)
+                    fo[a].x += c[uidx(0, 2)] * d0 * d0 + (c[uidx(0, 3)] + c[uidx(1, 2)]) * d0 * d1 +
+                               c[uidx(1, 3)] * d1 * d1;
             }
         }
     }
     colour_accumulate<double, NEN>(ch, t, l, fe, L.acc);
+    if (blk) {      // the off-diagonal entries, accumulated in L.v once every thread has read its gathered values
+        __syncthreads();
+        for (int k = t; k < ch.nk; k += BS) L.v[k] = V2<double>{0, 0};
+        __syncthreads();
+        colour_accumulate<double, NEN>(ch, t, l, fo, L.v);
+    }
     V2<double>* part = reinterpret_cast<V2<double>*>(D.part);
     for (int k = t; k < ch.nk; k += BS) {
         const V2<double> acc = L.acc[k];
+        const double axy = blk ? L.v[k].x : 0.0;
         if (k < ch.nown) {
             const int n = (int)(D.lnode[ch.k0 + k] & DFEA_NODE_MASK);
-            const unsigned char b = L.bits[k];
-            const double dx = (b & 1) ? 1.0 : fmax(acc.x, 1e-6);
-            const double dy = (b & 2) ? 1.0 : fmax(acc.y, 1e-6);
-            if (std::is_same<CT, double>::value) {
-                D.diag64[2 * n] = dx;
-                D.diag64[2 * n + 1] = dy;
-            } else {
-                D.diag32[2 * n] = (float)dx;
-                D.diag32[2 * n + 1] = (float)dy;
-            }
+            store_precond<CT>(D, n, L.bits[k], acc, axy);
         } else if (D.n_part) {
-            part[D.pslot[ch.pbase + (k - ch.nown)]] = acc;
+            const int slot = D.pslot[ch.pbase + (k - ch.nown)];
+            part[slot] = acc;
+            if (blk) part[D.n_part + slot] = V2<double>{axy, 0.0};
         }
     }
 }
@@ -760,17 +791,10 @@ __global__ __launch_bounds__(FBS) void k_moduli_fin(const dfea_desc* __restrict_
     const V2<double>* part = reinterpret_cast<const V2<double>*>(D.part);
     const int j0 = D.sh_off[i];
     const V2<double> acc = sum_slots<double>(part, j0, D.sh_off[i + 1] - j0);
+    const double axy = D.precond == DFEA_PRECOND_BLOCK ? sum_slots<double>(part + D.n_part, j0, D.sh_off[i + 1] - j0).x
+                                                       : 0.0;
     const uint32_t g = D.sh_node[i];
-    const int n = (int)(g & DFEA_NODE_MASK);
-    const double dx = (g & DFEA_FIXED_X) ? 1.0 : fmax(acc.x, 1e-6);
-    const double dy = (g & DFEA_FIXED_Y) ? 1.0 : fmax(acc.y, 1e-6);
-    if (std::is_same<CT, double>::value) {
-        D.diag64[2 * n] = dx;
-        D.diag64[2 * n + 1] = dy;
-    } else {
-        D.diag32[2 * n] = (float)dx;
-        D.diag32[2 * n + 1] = (float)dy;
-    }
+    store_precond<CT>(D, (int)(g & DFEA_NODE_MASK), (unsigned char)(g >> 30), acc, axy);
 }
 
 // ---- adjoint: g[e] = -sum (m lam)_e . R^e_int(u; mu0, lam0)  (`adjoint.py:75-106`) ----------------------------
@@ -996,21 +1020,30 @@ __global__ __launch_bounds__(VBS) void k_pcg_init_final(const dfea_desc* __restr
 }
 
 // the state of local node kk of a chunk for the PCG update: x_k, r_k, p_{k-1}, s_{k-1}, w_k (w of a shared
-// node summed from the previous kernel's partials in slot order), diag, fixed bits, and whether this chunk
-// writes the node (owned nodes; a shared node's first-slot chunk; halo plans: nodes past nown are owned by
-// another chunk and only read)
+// node summed from the previous kernel's partials in slot order), the preconditioner (d: the Jacobi diagonal,
+// or the inverse block's diagonal with its off-diagonal entry o), fixed bits, and whether this chunk writes
+// the node (owned nodes; a shared node's first-slot chunk; halo plans: nodes past nown are owned by another
+// chunk and only read)
 template <typename T>
 struct NodeState {
     V2<T> x, r, p, s, w, d;
+    T o;
     int n;
     unsigned char b;
     bool writer;
 };
 
+// u = M r: r / diag (point Jacobi) or the inverse nodal block times r (block Jacobi, dgo != nullptr)
+template <typename T>
+__device__ __forceinline__ V2<T> precond_apply(V2<T> r, V2<T> d, T o, bool blk) {
+    if (blk) return V2<T>{d.x * r.x + o * r.y, o * r.x + d.y * r.y};
+    return V2<T>{r.x / d.x, r.y / d.y};
+}
+
 template <typename T>
 __device__ __forceinline__ NodeState<T> load_node_state(const dfea_desc& D, const ChunkView& ch, int kk, bool upd,
-                                                        const PcgSet<T>& Rd, const V2<T>* dg, const V2<T>* part_rd,
-                                                        const int2* prg) {
+                                                        const PcgSet<T>& Rd, const V2<T>* dg, const T* dgo,
+                                                        const V2<T>* part_rd, const int2* prg) {
     NodeState<T> ns;
     const uint32_t g = D.lnode[ch.k0 + kk];
     ns.n = (int)(g & DFEA_NODE_MASK);
@@ -1024,6 +1057,7 @@ __device__ __forceinline__ NodeState<T> load_node_state(const dfea_desc& D, cons
         ns.writer = ss.x == D.pslot[ent];
     }
     ns.d = dg[ns.n];
+    ns.o = dgo ? dgo[ns.n] : (T)0;
     ns.r = Rd.r[ns.n];
     if (upd) {
         if (!shared) {
@@ -1061,6 +1095,8 @@ __global__ __launch_bounds__(BS, MINW) void k_pcg(const dfea_desc* __restrict__ 
     const T* C = std::is_same<T, double>::value ? (const T*)D.C64 : (const T*)D.C32;
     const V2<T>* dg = reinterpret_cast<const V2<T>*>(std::is_same<T, double>::value ? (const T*)D.diag64
                                                                                        : (const T*)D.diag32);
+    const bool blk = D.precond == DFEA_PRECOND_BLOCK;
+    const T* dgo = blk ? reinterpret_cast<const T*>(dg + D.n_nodes) : nullptr;
     const int qr = k < 0.0 ? par : 1 - par;     // state buffers read (x_k, r_k, p_{k-1}, s_{k-1}, w_k)
     const PcgSet<T> Rd = pcg_set<T>(D, qr);
     const PcgSet<T> Wr = pcg_set<T>(D, par);   // state buffers written
@@ -1079,7 +1115,7 @@ __global__ __launch_bounds__(BS, MINW) void k_pcg(const dfea_desc* __restrict__ 
 
     // the first node's state (independent of the scalars)
     NodeState<T> ns0;
-    if (EARLY && t < ch.nk) ns0 = load_node_state<T>(D, ch, t, k >= 0.0, Rd, dg, part_rd, prg);
+    if (EARLY && t < ch.nk) ns0 = load_node_state<T>(D, ch, t, k >= 0.0, Rd, dg, dgo, part_rd, prg);
 
     // scalar step
     double alpha = 0.0, beta = 0.0, gam = 0.0, den = 0.0, rr = 0.0;
@@ -1132,8 +1168,7 @@ __global__ __launch_bounds__(BS, MINW) void k_pcg(const dfea_desc* __restrict__ 
                     if (kk >= ch.nown && (!D.n_part || prg[ch.pbase + (kk - ch.nown)].x != D.pslot[ch.pbase + (kk - ch.nown)]))
                         continue;
                     if (at0) {
-                        const V2<T> rv = Rd.r[n], dv = dg[n];
-                        x0[n] = V2<T>{rv.x / dv.x, rv.y / dv.y};
+                        x0[n] = precond_apply<T>(Rd.r[n], dg[n], blk ? dgo[n] : (T)0, blk);
                     } else {
                         x0[n] = Rd.x[n];
                     }
@@ -1147,14 +1182,14 @@ __global__ __launch_bounds__(BS, MINW) void k_pcg(const dfea_desc* __restrict__ 
     double g_part = 0.0, r_part = 0.0;
     for (int kk = t; kk < ch.nk; kk += BS) {
         const bool first = EARLY && kk == t;
-        NodeState<T> ns = first ? ns0 : load_node_state<T>(D, ch, kk, k >= 0.0, Rd, dg, part_rd, prg);
+        NodeState<T> ns = first ? ns0 : load_node_state<T>(D, ch, kk, k >= 0.0, Rd, dg, dgo, part_rd, prg);
         const int n = ns.n;
         const unsigned char b = ns.b;
         V2<T> rn = ns.r;
         const V2<T> dv = ns.d;
         if (k >= 0.0) {
             V2<T> w = ns.w;
-            const V2<T> u{rn.x / dv.x, rn.y / dv.y};
+            const V2<T> u = precond_apply<T>(rn, dv, ns.o, blk);
             if (b & 1) w.x = u.x;       // fixed dofs: A acts as the identity
             if (b & 2) w.y = u.y;
             V2<T> pn, sn, xn;
@@ -1173,7 +1208,7 @@ __global__ __launch_bounds__(BS, MINW) void k_pcg(const dfea_desc* __restrict__ 
                 Wr.r[n] = rn;
             }
         }
-        const V2<T> un{rn.x / dv.x, rn.y / dv.y};
+        const V2<T> un = precond_apply<T>(rn, dv, ns.o, blk);
         if (ns.writer) {
             g_part += (double)rn.x * (double)un.x + (double)rn.y * (double)un.y;
             r_part += (double)rn.x * (double)rn.x + (double)rn.y * (double)rn.y;
@@ -1372,7 +1407,7 @@ int dfea_desc_offset(int field) {
         (int)offsetof(dfea_desc, n_part), (int)offsetof(dfea_desc, n_el), (int)offsetof(dfea_desc, n_nodes),
         (int)offsetof(dfea_desc, nen), (int)offsetof(dfea_desc, nq), (int)offsetof(dfea_desc, precision),
         (int)offsetof(dfea_desc, n_finblk), (int)offsetof(dfea_desc, n_vblk), (int)offsetof(dfea_desc, n_halo),
-        (int)offsetof(dfea_desc, n_cols), (int)offsetof(dfea_desc, pad1)};
+        (int)offsetof(dfea_desc, n_cols), (int)offsetof(dfea_desc, precond)};
     if (field < 0 || field >= (int)(sizeof off / sizeof off[0])) return -1;
     return off[field];
 }

@@ -14,6 +14,7 @@ LIB_NAME = "libdfea_hip.so"
 LIB_PATH = os.path.join(_HERE, LIB_NAME)
 
 FP64, FP32 = 0, 1
+PRECOND = {"jacobi": 0, "block_jacobi": 1}     # dfea_desc.precond
 MAXCOL = 16
 FIXED_X, FIXED_Y, NODE_MASK = 0x40000000, 0x80000000, 0x3FFFFFFF
 HALO = 0x100          # element-kind flag of halo plans
@@ -42,7 +43,7 @@ class Desc(ctypes.Structure):
         ("n_part", ctypes.c_int32), ("n_el", ctypes.c_int32), ("n_nodes", ctypes.c_int32),
         ("nen", ctypes.c_int32), ("nq", ctypes.c_int32), ("precision", ctypes.c_int32),
         ("n_finblk", ctypes.c_int32), ("n_vblk", ctypes.c_int32), ("n_halo", ctypes.c_int32),
-        ("n_cols", ctypes.c_int32), ("pad1", ctypes.c_int32),
+        ("n_cols", ctypes.c_int32), ("precond", ctypes.c_int32),
     ]
 
 

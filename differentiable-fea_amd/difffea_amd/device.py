@@ -135,13 +135,16 @@ class HipMesh:
 class HipSystem:
     """One solve (mesh + theta + material + precision) with all its device buffers."""
 
-    def __init__(self, mesh, theta, mat_p, f, precision="fp64", uD=None, sc=None):
+    def __init__(self, mesh, theta, mat_p, f, precision="fp64", uD=None, sc=None, preconditioner="jacobi"):
         self.mesh = m = mesh
         self.lib = mesh.lib
         dv = mesh.device
         if precision not in ("fp64", "mixed"):
             raise ValueError(f"precision must be 'fp64' or 'mixed', got {precision!r}")
+        if preconditioner not in _lib.PRECOND:
+            raise ValueError(f"Unknown preconditioner '{preconditioner}'. Choose one of {tuple(_lib.PRECOND)}.")
         self.precision = precision
+        self.preconditioner = preconditioner
         self.prec_id = _lib.FP64 if precision == "fp64" else _lib.FP32
         self.vdt = F64 if precision == "fp64" else F32
         self.theta = mesh.to_internal(theta)
@@ -149,8 +152,8 @@ class HipSystem:
         self.mu0, self.lam0 = float(mp[0]), float(mp[1])
         self.f = f.to(dv, F64).contiguous()
         nq, n_el, nd = m.nq, m.n_el, m.n_dof
-        self.C = None                    # moduli and diagonal: allocated by the first moduli() call
-        self.diag = None
+        self.C = None                    # moduli and preconditioner: allocated by the first moduli() call
+        self.diag = None                 # Jacobi: diag(A); block Jacobi: inverse blocks {m00, m11} per node, m01
         self.part = torch.empty(max(1, 2 * m.plan.n_part) * 2, dtype=F64, device=dv)   # two parities of pairs
         self.red = torch.zeros(max(1, m.red_size), dtype=F64, device=dv)
         # the scalar block may be a row of a pool shared by a batch of solves
@@ -177,6 +180,7 @@ class HipSystem:
             d.vec[i] = self.vec[k].data_ptr()
         d.mu0, d.lam0 = self.mu0, self.lam0
         d.precision = self.prec_id
+        d.precond = _lib.PRECOND[self.preconditioner]
         return d
 
     # -- kernels on this single system -------------------------------------------------------------------
@@ -184,7 +188,8 @@ class HipSystem:
         if self.C is None:
             m = self.mesh
             self.C = torch.empty(10 * m.nq, m.n_cols, dtype=self.vdt, device=m.device)
-            self.diag = torch.empty(m.n_dof, dtype=self.vdt, device=m.device)
+            n = m.n_dof + (m.n_nodes if self.preconditioner == "block_jacobi" else 0)
+            self.diag = torch.empty(n, dtype=self.vdt, device=m.device)
             self.release_graphs()             # graphs hold the previous table's address
             self.desc = self._make_desc()
             self.table = m.upload(self.desc)
@@ -195,6 +200,17 @@ class HipSystem:
         u = u.to(F64).contiguous()
         _lib.check(self.lib.dfea_moduli(_ptr(self.table), _ptr(m.items), m.plan.n_chunks, _ptr(m.fin_items),
                                         m.n_finblk, _ptr(u), m.kind, self.prec_id, _stream()), "dfea_moduli")
+
+    def precondition(self, r):
+        """u = M r with the preconditioner of the last moduli() call (device tensors, the system's precision)."""
+        r = r.to(self.vdt)
+        if self.preconditioner == "jacobi":
+            return r / self.diag
+        nd = self.mesh.n_dof
+        rn = r.reshape(-1, 2)
+        d = self.diag[:nd].reshape(-1, 2)
+        o = self.diag[nd:]
+        return torch.stack((d[:, 0] * rn[:, 0] + o * rn[:, 1], o * rn[:, 0] + d[:, 1] * rn[:, 1]), 1).reshape(-1)
 
     def residual(self, u, out=None, lf=1.0):
         m = self.mesh

@@ -5,8 +5,9 @@
 * ``newton_krylov_solve(params, theta, ..., tangent="hip", precision="fp64")`` (`newton_krylov.py:186-264`):
   load continuation with cut-backs, Eisenstat-Walker forcing, Armijo line search -- the same state machine
   and ``info`` dictionary -- where every vector operation runs in the HIP library: residual, moduli +
-  Jacobi diagonal (one fused kernel per Newton step), the fused PCG (one kernel per iteration, device
-  scalars, status read once per chunk of iterations), line-search trials.
+  preconditioner (one fused kernel per Newton step: the Jacobi diagonal, or the inverse nodal 2x2 blocks
+  for ``preconditioner="block_jacobi"``), the fused PCG (one kernel per iteration, device scalars, status
+  read once per chunk of iterations), line-search trials.
 * ``precision="mixed"``: moduli, A(v) and the inner CG vectors in FP32 (dot products accumulated in FP64),
   while the Newton residual, the state u and the line search stay FP64 -- Newton itself is the outer
   refinement loop, so the converged residual meets the same FP64 tolerance.  The inner relative tolerance
@@ -54,13 +55,14 @@ def pcg(A_fn, b, Minv_fn, rtol=1e-5, max_iter=500):
 class HipNewton:
     """The Newton-Krylov state machine of one solve on one ``HipSystem`` (device-resident)."""
 
-    def __init__(self, params, theta, precision="fp64", device=None):
+    def __init__(self, params, theta, precision="fp64", device=None, preconditioner="jacobi"):
         if precision not in PRECISIONS:
             raise ValueError(f"Unknown precision '{precision}'. Choose one of {PRECISIONS}.")
         device = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
         self.params = params
         self.mesh = get_mesh(params, device)
-        self.sys = HipSystem(self.mesh, theta.detach(), _check_material(params), params["f"], precision)
+        self.sys = HipSystem(self.mesh, theta.detach(), _check_material(params), params["f"], precision,
+                             preconditioner=preconditioner)
         self.device = self.mesh.device
         self.uD = params["u_D"].to(self.device, torch.float64).contiguous()
         nd = self.mesh.n_dof
@@ -198,7 +200,7 @@ def newton_krylov_solve(params, theta, n_load_steps=5, tol=1e-8, max_newton=25, 
         raise ValueError(f"Unknown preconditioner '{preconditioner}'. Choose one of {PRECONDITIONERS}.")
     if device is None:
         device = theta.device if theta.is_cuda else torch.device("cuda", torch.cuda.current_device())
-    solver = HipNewton(params, theta.to(device), precision, device)
+    solver = HipNewton(params, theta.to(device), precision, device, preconditioner)
     t0 = time.perf_counter()
     u, info = solver.solve(n_load_steps, tol, max_newton, verbose, callback, max_cutbacks, cg_rtol, cg_max_iter)
     torch.cuda.synchronize(solver.device)

@@ -8,7 +8,9 @@ Mirrors `src/difffea/operators.py`:
 * ``make_AT_operator(u, theta, params)``                       (`operators.py:441-471`)
   the hyperelastic tangent is symmetric, so the transpose product is the same kernel.
 * ``make_preconditioner(kind, u, theta, params)``              (`operators.py:585, 702-710`)
-  ``"jacobi"``: ``r -> r / diag`` with the diagonal from the fused moduli kernel (clamp 1e-6, pivot 1).
+  ``"jacobi"``: ``r -> r / diag`` with the diagonal from the fused moduli kernel (clamp 1e-6, pivot 1);
+  ``"block_jacobi"``: ``r -> B_n^-1 r_n`` per node with the inverse nodal 2x2 blocks from the same kernel
+  (`operators.py:588-638`).  ``"ilu"`` needs the assembled sparse matrix on the host and is not offered.
 * ``compute_jacobi_diagonal(u, theta, params)``                (`operators.py:477-556`)
 * ``global_residual(u, theta, params, load_factor)``           (`operators.py:159-220`)
 
@@ -23,7 +25,7 @@ from . import _lib
 from .device import HipMesh, HipSystem
 
 TANGENT_OPERATORS = ("auto", "hip")
-PRECONDITIONERS = ("jacobi",)
+PRECONDITIONERS = ("jacobi", "block_jacobi")
 PRECISIONS = ("fp64", "mixed")
 
 _MESHES = {}
@@ -58,18 +60,19 @@ def get_mesh(params, device=None):
     return mesh
 
 
-def make_system(u, theta, params, precision="fp64"):
+def make_system(u, theta, params, precision="fp64", preconditioner="jacobi"):
     if precision not in PRECISIONS:
         raise ValueError(f"Unknown precision '{precision}'. Choose one of {PRECISIONS}.")
     mesh = get_mesh(params, u.device)
-    return HipSystem(mesh, theta.detach(), _check_material(params), params["f"], precision)
+    return HipSystem(mesh, theta.detach(), _check_material(params), params["f"], precision,
+                     preconditioner=preconditioner)
 
 
 class HipTangent:
-    """Callable closure ``v -> A(v)`` holding the moduli and diagonal at its linearisation point."""
+    """Callable closure ``v -> A(v)`` holding the moduli and preconditioner at its linearisation point."""
 
-    def __init__(self, u, theta, params, precision="fp64"):
-        self.system = make_system(u, theta, params, precision)
+    def __init__(self, u, theta, params, precision="fp64", preconditioner="jacobi"):
+        self.system = make_system(u, theta, params, precision, preconditioner)
         self.system.moduli(u.detach())
         self._u = u
 
@@ -123,10 +126,18 @@ def make_jacobi_preconditioner(u, theta, params, precision="fp64"):
     return lambda r: r / (diag if diag.dtype == r.dtype else diag.to(r.dtype))
 
 
+def make_block_jacobi_preconditioner(u, theta, params, precision="fp64"):
+    """r -> B_n^-1 r_n per node, the inverse blocks from the moduli kernel (`operators.py:618-638`)."""
+    system = HipTangent(u, theta, params, precision, "block_jacobi").system
+    return lambda r: system.precondition(r).to(r.dtype)
+
+
 def make_preconditioner(kind, u, theta, params, precision="fp64"):
-    """Build a preconditioner by name: ``"jacobi"``."""
+    """Build a preconditioner by name: ``"jacobi"`` or ``"block_jacobi"``."""
     if kind == "jacobi":
         return make_jacobi_preconditioner(u, theta, params, precision)
+    if kind == "block_jacobi":
+        return make_block_jacobi_preconditioner(u, theta, params, precision)
     raise ValueError(f"Unknown preconditioner '{kind}'. Choose one of {PRECONDITIONERS}.")
 
 
@@ -137,5 +148,6 @@ def global_residual(u, theta, params, load_factor=1.0):
 
 
 __all__ = ["TANGENT_OPERATORS", "PRECONDITIONERS", "make_tangent_operator", "make_AT_operator",
-           "make_preconditioner", "make_jacobi_preconditioner", "compute_jacobi_diagonal", "global_residual",
+           "make_preconditioner", "make_jacobi_preconditioner", "make_block_jacobi_preconditioner",
+           "compute_jacobi_diagonal", "global_residual",
            "get_mesh", "make_system", "HipTangent", "_lib"]

@@ -42,6 +42,8 @@ extern "C" {
 #define DFEA_HALO 0x100        /* element-kind flag: nen | DFEA_HALO selects the halo-plan kernels */
 
 enum { DFEA_FP64 = 0, DFEA_FP32 = 1 };
+/* dfea_desc.precond: point Jacobi (`operators.py:477-570`) or nodal block Jacobi (`operators.py:588-638`) */
+enum { DFEA_PRECOND_JACOBI = 0, DFEA_PRECOND_BLOCK = 1 };
 
 /* vector slots of a descriptor.  The PCG state x, r, w = A M r, p, s = A p lives in two buffers each (the
  * slot and its "2" twin): iteration k reads the buffers of parity k mod 2 and writes the others, so no
@@ -107,7 +109,8 @@ typedef struct dfea_desc {
     double* theta;              /* [n_cols] internal order (+ halo copies) */
     double* C64;
     float* C32;
-    double* diag64;
+    double* diag64;             /* Jacobi: [2N] diag(A); block Jacobi: [3N], the inverse nodal blocks {m00, m11}
+                                   per node, then m01 per node */
     float* diag32;
     double* f;                  /* [2N] external force at load factor 1 */
     double* part;               /* [n_part] pairs (double2 or float2) */
@@ -119,7 +122,8 @@ typedef struct dfea_desc {
     int32_t n_chunks, n_lnodes, n_shared, n_part;
     int32_t n_el, n_nodes, nen, nq;
     int32_t precision, n_finblk, n_vblk, n_halo;
-    int32_t n_cols, pad1;      /* plane length n_el + n_halo */
+    int32_t n_cols;            /* plane length n_el + n_halo */
+    int32_t precond;           /* DFEA_PRECOND_JACOBI or DFEA_PRECOND_BLOCK */
 } dfea_desc;
 
 /* ---- library ---------------------------------------------------------------------------------------- */
@@ -141,9 +145,12 @@ int dfea_geometry(const dfea_desc* table, const int32_t* items, int n_items, int
 
 /* ---- once per Newton step --------------------------------------------------------------------------- */
 /* Tangent moduli C (10 unique entries, weight folded in) at state u, stored FP64 (precision FP64) or FP32,
- * fused with the Jacobi diagonal (clamp 1e-6 on free dofs, 1 on fixed dofs).
- * u == NULL uses vec[DFEA_U]; diag_out == NULL uses diag64/diag32 of the solve.
- * Replaces `operators.py:316-342` (_qp_setup) and `operators.py:477-570` (compute_jacobi_diagonal).
+ * fused with the preconditioner of desc.precond into diag64/diag32: the Jacobi diagonal (clamp 1e-6 on free
+ * dofs, 1 on fixed dofs), or the inverses of the nodal 2x2 blocks (fixed dofs: identity rows and columns;
+ * + 1e-12 I), summed in FP64 and inverted in closed form.  The shared-node partials of the block's
+ * off-diagonal entry use the second half of desc.part.  u == NULL uses vec[DFEA_U].
+ * Replaces `operators.py:316-342` (_qp_setup), `operators.py:477-570` (compute_jacobi_diagonal) and
+ * `operators.py:588-638` (make_block_jacobi_preconditioner).
  * items: chunk items; fin_items: shared-node items (dfea_finblk blocks per solve). */
 int dfea_moduli(const dfea_desc* table, const int32_t* items, int n_items, const int32_t* fin_items,
                 int n_fin_items, const double* u, int nen, int precision, void* stream);

@@ -294,6 +294,9 @@ def make_block_jacobi_preconditioner(u, theta, params):
     return lambda r: torch.einsum("nij,nj->ni", Binv, r.reshape(N, 2)).reshape(-1)
 
 
+PRECONDITIONERS = {"jacobi": make_jacobi_preconditioner, "block_jacobi": make_block_jacobi_preconditioner}
+
+
 def pcg(A_fn, b, Minv_fn, rtol=1e-5, max_iter=500):
     """Textbook Jacobi-PCG with x0 = 0; curvature guard; stop when ||r|| <= rtol ||b||."""
     x = torch.zeros_like(b)
@@ -334,7 +337,7 @@ def armijo_line_search(u, du, rn, theta, params, load_factor, c1=1e-4, min_alpha
 
 
 def newton_at_load(u0, theta, params, lf, tol, max_newton, step_id, tangent="qp",
-                   cg_rtol=None, cg_max_iter=500, callback=None):
+                   cg_rtol=None, cg_max_iter=500, callback=None, preconditioner="jacobi"):
     u = torch.where(params["fixed"], lf * params["u_D"], u0)
     R = global_residual(u, theta, params, lf)
     r0 = torch.norm(R)
@@ -347,7 +350,7 @@ def newton_at_load(u0, theta, params, lf, tol, max_newton, step_id, tangent="qp"
             break
         eta = min(0.5, float((rn / (r0 + 1e-14)) ** 0.5)) if cg_rtol is None else cg_rtol
         A = make_tangent_operator(tangent, u, theta, params)
-        Minv = make_jacobi_preconditioner(u, theta, params)
+        Minv = PRECONDITIONERS[preconditioner](u, theta, params)
         du, k = pcg(A, -R, Minv, rtol=eta, max_iter=cg_max_iter)
         u, Rn, rn, alpha, ok = armijo_line_search(u, du, rn, theta, params, lf)
         rec = dict(step=step_id, load_factor=lf, iteration=it, residual=float(rn), cg_iters=int(k), alpha=alpha)
@@ -362,7 +365,7 @@ def newton_at_load(u0, theta, params, lf, tol, max_newton, step_id, tangent="qp"
 
 @torch.no_grad()
 def newton_krylov_solve(params, theta, n_load_steps=5, tol=1e-8, max_newton=25, max_cutbacks=8,
-                        tangent="qp", cg_rtol=None, cg_max_iter=500, callback=None):
+                        tangent="qp", cg_rtol=None, cg_max_iter=500, callback=None, preconditioner="jacobi"):
     """Adaptive load continuation around ``newton_at_load``; returns (u, info) like the reference."""
     n = 2 * params["nodes"].shape[0]
     u = torch.zeros(n, dtype=DT)
@@ -376,7 +379,7 @@ def newton_krylov_solve(params, theta, n_load_steps=5, tol=1e-8, max_newton=25, 
         lf_try = min(1.0, lf + d)
         step += 1
         u_try, ok, recs = newton_at_load(u, theta, params, lf_try, tol, max_newton, step, tangent,
-                                         cg_rtol, cg_max_iter, callback)
+                                         cg_rtol, cg_max_iter, callback, preconditioner)
         info["history"] += recs
         info["total_newton"] += len(recs)
         info["total_cg"] += sum(r["cg_iters"] for r in recs)
