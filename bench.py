@@ -141,7 +141,7 @@ def cpu_baseline(p, u, v, theta, seconds=10.0):
                 ms_per_product=med * 1e3)
 
 
-def solve_config1(precision, cg_rtol=None, cg_max_iter=500):
+def solve_config1(precision, cg_rtol=None, cg_max_iter=500, preconditioner="jacobi"):
     """Unit-square 100x100 Q4, E=1 nu=0.3, left edge clamped, right edge u_x = 0.1 in 3 load steps; Newton
     tolerance 1e-10 relative to ||K(0) u_D|| (the linear out-of-balance force of the prescribed motion)."""
     from difffea_amd.problem import build_problem
@@ -152,14 +152,15 @@ def solve_config1(precision, cg_rtol=None, cg_max_iter=500):
     th = torch.ones(p["conn"].shape[0], dtype=torch.float64, device="cuda")
     probe = HipNewton(p, th)
     tol = 1e-10 * reference_residual_norm(probe.sys, p, probe.device)
-    newton_krylov_solve(p, th, 3, tol, verbose=False, precision=precision, cg_rtol=cg_rtol, cg_max_iter=cg_max_iter)
-    u, info = newton_krylov_solve(p, th, 3, tol, verbose=False, precision=precision, cg_rtol=cg_rtol,
-                                  cg_max_iter=cg_max_iter)
+    kw = dict(verbose=False, precision=precision, cg_rtol=cg_rtol, cg_max_iter=cg_max_iter,
+              preconditioner=preconditioner)
+    newton_krylov_solve(p, th, 3, tol, **kw)
+    u, info = newton_krylov_solve(p, th, 3, tol, **kw)
     return u, dict(seconds=round(info["seconds"], 4), solves_per_s=round(1.0 / info["seconds"], 3),
                    newton=info["total_newton"], cg=info["total_cg"], converged=info["converged"], tol_abs=tol)
 
 
-def solve_config3(precision):
+def solve_config3(precision, preconditioner="jacobi"):
     """O-grid plate with a hole r=0.2, 64 x 256 Q9, E_e lognormal (mean 1, sigma 0.3, seed 2), nu=0.3,
     left edge clamped, dead traction on the right edge (total 0.1) in 5 load steps; Newton tolerance
     1e-10 relative to ||m f||."""
@@ -173,7 +174,8 @@ def solve_config3(precision):
     n_el = p["conn"].shape[0]
     th = torch.exp(torch.randn(n_el, dtype=torch.float64, generator=g) * 0.3 - 0.045).cuda()
     tol = 1e-10 * float(torch.norm(p["f"] * p["m"]))
-    u, info = newton_krylov_solve(p, th, 5, tol, verbose=False, precision=precision, cg_max_iter=20000)
+    u, info = newton_krylov_solve(p, th, 5, tol, verbose=False, precision=precision, cg_max_iter=20000,
+                                  preconditioner=preconditioner)
     return u, dict(seconds=round(info["seconds"], 4), newton=info["total_newton"], cg=info["total_cg"],
                    converged=info["converged"], tol_abs=tol)
 
@@ -207,14 +209,15 @@ def sequential_fp64(n_requests, seed=5, side=(32, 512)):
                 p99_latency_s=float(np.percentile(lat, 99)))
 
 
-def serve_stream(n_requests, precision, rate=None, seed=5, side=(32, 512)):
+def serve_stream(n_requests, precision, rate=None, seed=5, side=(32, 512), preconditioner="jacobi"):
     from difffea_amd.serve import Server, make_stream
     reqs = make_stream(n_requests, rate=rate or 1e9, seed=seed, side=side)
-    srv = Server(precision, max_slots=64)
+    srv = Server(precision, max_slots=64, preconditioner=preconditioner)
 
     def progress(done, total):
         if done % 32 == 0:
-            print(f"# served {precision}: {done} of {total} requests", file=sys.stderr, flush=True)
+            print(f"# served {precision} {preconditioner}: {done} of {total} requests", file=sys.stderr,
+                  flush=True)
 
     recs, summ = srv.run(reqs, realtime=rate is not None, progress=progress)
     summ["n_newton"] = int(sum(r["total_newton"] for r in recs))
@@ -262,11 +265,18 @@ def main():
         u32, solves["c1_mixed_ew"] = solve_config1("mixed")
         solves["c1_mixed_vs_fp64_rel"] = float((u32 - u64).abs().max() / u64.abs().max())
         _, solves["c1_fp64_cg1e-12"] = solve_config1("fp64", cg_rtol=1e-12, cg_max_iter=5000)
+        ub, solves["c1_fp64_block_jacobi"] = solve_config1("fp64", preconditioner="block_jacobi")
+        _, solves["c1_mixed_block_jacobi"] = solve_config1("mixed", preconditioner="block_jacobi")
+        solves["c1_block_vs_jacobi_rel"] = float((ub - u64).abs().max() / u64.abs().max())
         print(f"# config-1 solves: {solves}", file=sys.stderr, flush=True)
         v64, solves["c3_fp64"] = solve_config3("fp64")
         v32, solves["c3_mixed"] = solve_config3("mixed")
         solves["c3_mixed_vs_fp64_rel"] = float((v32 - v64).abs().max() / v64.abs().max())
-        print(f"# config-3 solves: {solves['c3_fp64']} {solves['c3_mixed']}", file=sys.stderr, flush=True)
+        vb, solves["c3_fp64_block_jacobi"] = solve_config3("fp64", "block_jacobi")
+        _, solves["c3_mixed_block_jacobi"] = solve_config3("mixed", "block_jacobi")
+        solves["c3_block_vs_jacobi_rel"] = float((vb - v64).abs().max() / v64.abs().max())
+        print(f"# config-3 solves: {solves['c3_fp64']} {solves['c3_mixed']} {solves['c3_fp64_block_jacobi']} "
+              f"{solves['c3_mixed_block_jacobi']}", file=sys.stderr, flush=True)
         if args.serve_requests > 0:
             sat64 = serve_stream(args.serve_requests, "fp64", side=tuple(args.serve_side))
             print(f"# served fp64 (saturated): {sat64}", file=sys.stderr, flush=True)
@@ -275,8 +285,14 @@ def main():
             print(f"# plain FP64 GPU solver (sequential): {seq}", file=sys.stderr, flush=True)
             sat32 = serve_stream(args.serve_requests, "mixed", side=tuple(args.serve_side))
             print(f"# served mixed (saturated): {sat32}", file=sys.stderr, flush=True)
-            srv = {"saturated_fp64": sat64, "saturated_mixed": sat32, "plain_fp64_sequential": seq,
+            sat32b = serve_stream(args.serve_requests, "mixed", side=tuple(args.serve_side),
+                                  preconditioner="block_jacobi")
+            print(f"# served mixed block-Jacobi (saturated): {sat32b}", file=sys.stderr, flush=True)
+            srv = {"saturated_fp64": sat64, "saturated_mixed": sat32, "saturated_mixed_block_jacobi": sat32b,
+                   "plain_fp64_sequential": seq,
                    "saturated_mixed_vs_plain_fp64": round(sat32["solves_per_s"] / seq["solves_per_s"], 3),
+                   "saturated_mixed_block_jacobi_vs_plain_fp64": round(sat32b["solves_per_s"] / seq["solves_per_s"],
+                                                                       3),
                    "fp64": serve_stream(args.serve_requests, "fp64", rate, side=tuple(args.serve_side)),
                    "mixed": serve_stream(args.serve_requests, "mixed", rate, side=tuple(args.serve_side)),
                    "arrival_rate_per_s": rate}

@@ -107,15 +107,17 @@ def reference_residual_norm(sys_, params, device):
     return float(torch.norm(R)) / LIFT_PROBE
 
 
-def footprint_bytes(element_type, n, precision):
-    """HBM bytes a request holds while admitted (geometry, moduli, vectors, plan), an upper estimate."""
+def footprint_bytes(element_type, n, precision, preconditioner="jacobi"):
+    """HBM bytes a request holds while admitted (geometry, moduli, vectors, preconditioner, plan), an upper
+    estimate."""
     nen, nq = (4, 4) if element_type == "quad4" else (9, 9)
     n_el = n * n
     side = n + 1 if nen == 4 else 2 * n + 1
     n_dof = 2 * side * side
     s = 8 if precision == "fp64" else 4
     per_qp = 2 * nen * 12 + 8 + 10 * s
-    return n_el * nq * per_qp + n_dof * (12 * s + 5 * 8) + n_el * nen * 8 + 64 * n_el + (1 << 20)
+    pre = n_dof * s // 2 if preconditioner == "block_jacobi" else 0     # the off-diagonal inverse entries
+    return n_el * nq * per_qp + n_dof * (12 * s + 5 * 8) + pre + n_el * nen * 8 + 64 * n_el + (1 << 20)
 
 
 # ------------------------------------------------------------------------------------------------------------
@@ -213,9 +215,12 @@ class Server:
     """Continuous-batching solver for a stream of requests on one GPU (see module docstring)."""
 
     def __init__(self, precision="fp64", max_slots=64, hbm_budget=64 << 30, pcg_chunk=8, rel_tol=1e-8,
-                 device=None, adjoint_rtol=1e-8, cg_max_iter=20000):
+                 device=None, adjoint_rtol=1e-8, cg_max_iter=20000, preconditioner="jacobi"):
         self.lib = _lib.load()
         self.device = torch.device(device or "cuda")
+        if preconditioner not in _lib.PRECOND:
+            raise ValueError(f"Unknown preconditioner '{preconditioner}'. Choose one of {tuple(_lib.PRECOND)}.")
+        self.preconditioner = preconditioner
         self.precision = precision
         self.prec_id = _lib.FP64 if precision == "fp64" else _lib.FP32
         self.max_slots = max_slots
@@ -271,12 +276,13 @@ class Server:
         params, theta, plan = prepared if prepared is not None else self.prepare(req)
         mesh = HipMesh(params, self.device, plan=plan)
         mp = params["mat_p"]
-        sys_ = HipSystem(mesh, theta, mp, params["f"], self.precision, uD=params["u_D"], sc=self.sc[slot])
+        sys_ = HipSystem(mesh, theta, mp, params["f"], self.precision, uD=params["u_D"], sc=self.sc[slot],
+                         preconditioner=self.preconditioner)
         sys_._ensure_moduli_buffers()
         self.table[slot * self.dsz:(slot + 1) * self.dsz].copy_(
             torch.frombuffer(bytearray(bytes(sys_.desc)), dtype=torch.uint8))
         job = Job(req, slot, mesh, sys_, params, theta)
-        job.bytes = footprint_bytes(req.element_type, req.n, self.precision)
+        job.bytes = footprint_bytes(req.element_type, req.n, self.precision, self.preconditioner)
         self.used_bytes += job.bytes
         pl = mesh.plan
         job.items = {"chunk": self._items(slot, pl.n_chunks), "fin": self._items(slot, mesh.n_finblk),
@@ -482,7 +488,7 @@ class Server:
                 prepare_arrived()
                 while i < len(pending) and (not realtime or pending[i].arrival <= now) and self.free:
                     r = pending[i]
-                    fb = footprint_bytes(r.element_type, r.n, self.precision)
+                    fb = footprint_bytes(r.element_type, r.n, self.precision, self.preconditioner)
                     if self.jobs and self.used_bytes + fb > self.hbm_budget:
                         break
                     fut = futures.get(i)
@@ -513,7 +519,8 @@ class Server:
             pool.shutdown(wait=True)
         wall = time.perf_counter() - t0
         lat = np.array([r["latency"] for r in records])
-        summary = dict(precision=self.precision, n_requests=len(records), wall_s=wall,
+        summary = dict(precision=self.precision, preconditioner=self.preconditioner, n_requests=len(records),
+                       wall_s=wall,
                        latency_from="arrival" if realtime else "admission",
                        solves_per_s=len(records) / wall, p50_latency_s=float(np.percentile(lat, 50)),
                        p99_latency_s=float(np.percentile(lat, 99)),
