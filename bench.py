@@ -116,6 +116,35 @@ def time_pcg_iteration(p, u, v, theta, precision, iters=400, chunk=50):
                 iterations=n)
 
 
+def time_postprocess(p, u, theta, steps=200):
+    """dfea_fields (element fields + nodal averages) and dfea_reactions on the same mesh and state.
+    Algorithmic bytes: per Gauss point the shape gradients and w det J (2 nen 8 + 8); per element theta,
+    int32 connectivity credit and, for the fields, 7 outputs; per node u read and, for the fields, 6 outputs."""
+    from difffea_amd.operators import make_system
+    dev = torch.device("cuda")
+    ud = u.to(dev)
+    s = make_system(ud, theta.to(dev), p)
+    m = s.mesh
+    bufs = s.fields_buffers()
+    r2 = torch.empty(2, dtype=torch.float64, device=dev)
+    out = {}
+    for name, fn, extra_el, extra_node in (("fields", lambda: s.fields(ud, bufs), 7 * 8, 6 * 8),
+                                            ("reactions", lambda: s.reactions(ud, r2), 0, 0)):
+        for _ in range(5):
+            fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(steps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        us = e0.elapsed_time(e1) * 1e3 / steps
+        B = m.nq * m.n_el * (2 * m.nen * 8 + 8) + m.n_el * (8 + 4 * m.nen + extra_el) + m.n_nodes * (16 + extra_node)
+        out[name] = dict(us_per_call=round(us, 3), gbs=round(B / (us * 1e-6) / 1e9, 2), bytes_per_call=B)
+    return out
+
+
 def cpu_baseline(p, u, v, theta, seconds=10.0):
     """The oracle's 'qp' restatement (gather, einsum, batched 4x4 matmul, einsum, index_add_), FP64."""
     from oracle import fem
@@ -254,6 +283,8 @@ def main():
     pcg32 = time_pcg_iteration(p, u, v, th, "mixed")
     print(f"# Q4 400x400 PCG iteration: fp64 {pcg64['us_per_iteration']:.2f} us, "
           f"fp32 {pcg32['us_per_iteration']:.2f} us", file=sys.stderr, flush=True)
+    post = time_postprocess(p, u, th)
+    print(f"# Q4 400x400 post-processing: {post}", file=sys.stderr, flush=True)
     p9, u9, v9, th9 = problem("quad9", 200)
     q9 = time_apply(p9, u9, v9, th9, "fp64", args.steps, args.warmup)
     print(f"# Q9 200x200 fp64 A(v): {q9['ms'] * 1e3:.2f} us, {q9['gbs']:.0f} GB/s", file=sys.stderr, flush=True)
@@ -321,7 +352,7 @@ def main():
                         "bytes_per_step": r32["bytes"]},
         "q9_200x200_fp64": {"us_per_product": round(q9["ms"] * 1e3, 3), "gbs": round(q9["gbs"], 2),
                             "bytes_per_step": q9["bytes"]},
-        "pcg_iteration_fp64": pcg64, "pcg_iteration_mixed": pcg32,
+        "pcg_iteration_fp64": pcg64, "pcg_iteration_mixed": pcg32, "postprocess_q4_400": post,
         "cpu_baseline": None if cpu is None else {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")},
         "solves": solves,
     }

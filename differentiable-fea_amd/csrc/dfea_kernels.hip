@@ -579,7 +579,10 @@ __global__ __launch_bounds__(FBS) void k_apply_fin(const dfea_desc* __restrict__
 }
 
 // ---- residual  R = m (R_int(u) - lf f)  (`operators.py:131-220`) -------------------------------------------
-template <int NEN, int NQ, int BS>
+// REACT: support reactions instead (`postprocess.py:121-137`): the chunk's sums of the internal force over its
+// fixed dofs (owned nodes, and its partial of each shared node: the total is linear in the partials) go to
+// red[b] (x) and red[n_chunks + b] (y); R is not written
+template <int NEN, int NQ, int BS, bool REACT = false>
 __global__ __launch_bounds__(BS) void k_residual(const dfea_desc* __restrict__ tab, const int* __restrict__ items,
                                                 const double* __restrict__ uin, double* __restrict__ rout,
                                                 double lf_arg) {
@@ -637,6 +640,22 @@ __global__ __launch_bounds__(BS) void k_residual(const dfea_desc* __restrict__ t
         }
     }
     colour_accumulate<double, NEN>(ch, t, l, fe, L.acc);
+    if constexpr (REACT) {
+        double rx = 0.0, ry = 0.0, unused = 0.0;
+        for (int k = t; k < ch.nk; k += BS) {
+            if (k >= ch.nown && !D.n_part) continue;     // halo plans: another chunk owns the node
+            const V2<double> acc = L.acc[k];
+            const unsigned char b = L.bits[k];
+            if (b & 1) rx += acc.x;
+            if (b & 2) ry += acc.y;
+        }
+        block_sum3<BS>(rx, ry, unused, L.red);
+        if (t == 0) {
+            D.red[it.b] = rx;
+            D.red[D.n_chunks + it.b] = ry;
+        }
+        return;
+    }
     V2<double>* part = reinterpret_cast<V2<double>*>(D.part);
     for (int k = t; k < ch.nk; k += BS) {
         const V2<double> acc = L.acc[k];
@@ -795,6 +814,160 @@ __global__ __launch_bounds__(FBS) void k_moduli_fin(const dfea_desc* __restrict_
                                                        : 0.0;
     const uint32_t g = D.sh_node[i];
     store_precond<CT>(D, (int)(g & DFEA_NODE_MASK), (unsigned char)(g >> 30), acc, axy);
+}
+
+// the sums of the reaction kernel's chunk partials, per solve: out[2 s], out[2 s + 1]
+__global__ __launch_bounds__(VBS) void k_reactions_final(const dfea_desc* __restrict__ tab, const int* __restrict__ items,
+                                                        double* __restrict__ out) {
+    __shared__ double s_red[16];
+    const Item it = load_item(items);
+    const dfea_desc& D = tab[it.s];
+    const double rx = sum_partials<VBS>(D.red, D.n_chunks, s_red);
+    const double ry = sum_partials<VBS>(D.red + D.n_chunks, D.n_chunks, s_red);
+    if (threadIdx.x == 0) {
+        out[2 * it.s] = rx;
+        out[2 * it.s + 1] = ry;
+    }
+}
+
+// ---- post-processing fields  (`postprocess.py:24-118`) ---------------------------------------------------------
+// Per element: the area sum_q w_q det J_q and the means over it (weights w_q det J_q) of J, of the Cauchy
+// stress sigma = P3 F3^T / J of the plane-strain embedding F3 = diag(F, 1) -- for the neo-Hookean energy
+// sigma = (mu (F F^T - I) + lam ln J I) / J in-plane and sigma_zz = lam ln J / J -- and of the energy density
+// psi = mu/2 (F:F - 2 - 2 ln J) + lam/2 (ln J)^2; the von Mises stress of the mean stress.  Written to
+// eout[k][original element id], k = area, J, sxx, syy, sxy, mises, psi.  Per node: the element values
+// J, sxx, syy, sxy, mises, psi summed over the elements at the node and divided by their number, written to
+// nout[k][node]; shared nodes leave four partial pairs per slot in work ([4][n_part] pairs) for k_fields_fin.
+template <int NEN>
+__device__ __forceinline__ void colour_accumulate4(const ChunkView& ch, int t, const int* l, const V2<double>* val,
+                                                   V2<double>* a0, V2<double>* a1, V2<double>* a2, V2<double>* a3) {
+    for (int c = 0; c < ch.ncol; ++c) {
+        if (in_colour(ch, t, c)) {
+#pragma unroll
+            for (int a = 0; a < NEN; ++a) {
+                const int k = l[a];
+                a0[k].x += val[0].x; a0[k].y += val[0].y;
+                a1[k].x += val[1].x; a1[k].y += val[1].y;
+                a2[k].x += val[2].x; a2[k].y += val[2].y;
+                a3[k].x += val[3].x; a3[k].y += val[3].y;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+template <int NEN, int NQ, int BS>
+__global__ __launch_bounds__(BS) void k_fields(const dfea_desc* __restrict__ tab, const int* __restrict__ items,
+                                              const double* __restrict__ uin, double* __restrict__ eout,
+                                              double* __restrict__ nout, double* __restrict__ work) {
+    __shared__ ChunkLDS<double> L;
+    __shared__ V2<double> acc1[MAXLN], acc2[MAXLN], acc3[MAXLN];
+    const Item it = load_item(items);
+    const dfea_desc& D = tab[it.s];
+    const ChunkView ch = chunk_view(D.chunks + it.b);
+    const int t = threadIdx.x;
+    const double* u = uin ? uin : static_cast<const double*>(D.vec[DFEA_U]);
+    for (int k = t; k < ch.nk; k += BS) {
+        const int n = (int)(D.lnode[ch.k0 + k] & DFEA_NODE_MASK);
+        L.v[k] = reinterpret_cast<const V2<double>*>(u)[n];
+        L.acc[k] = acc1[k] = acc2[k] = acc3[k] = V2<double>{0, 0};
+    }
+    __syncthreads();
+    int l[NEN];
+    int e;
+    const bool has_el = thread_element<NEN>(D, ch, t, e, l);
+    V2<double> val[4] = {};
+    if (has_el) {
+        const int ne = D.n_cols;
+        const double th = D.theta[e];
+        const double mu = D.mu0 * th, lam = D.lam0 * th;
+        V2<double> ue[NEN];
+#pragma unroll
+        for (int a = 0; a < NEN; ++a) ue[a] = L.v[l[a]];
+        double area = 0, mJ = 0, mxx = 0, myy = 0, mxy = 0, mzz = 0, mpsi = 0;
+        for (int q = 0; q < NQ; ++q) {
+            double F00 = 1, F01 = 0, F10 = 0, F11 = 1;
+#pragma unroll
+            for (int a = 0; a < NEN; ++a) {
+                const double d0 = D.dN64[(size_t)((q * NEN + a) * 2) * ne + e];
+                const double d1 = D.dN64[(size_t)((q * NEN + a) * 2 + 1) * ne + e];
+                F00 += ue[a].x * d0; F01 += ue[a].x * d1;
+                F10 += ue[a].y * d0; F11 += ue[a].y * d1;
+            }
+            const double J = F00 * F11 - F01 * F10;
+            const double lnJ = log(J);
+            const double w = D.w[(size_t)q * ne + e];
+            area += w;
+            mJ += w * J;
+            mxx += w * ((mu * (F00 * F00 + F01 * F01 - 1.0) + lam * lnJ) / J);
+            myy += w * ((mu * (F10 * F10 + F11 * F11 - 1.0) + lam * lnJ) / J);
+            mxy += w * (mu * (F00 * F10 + F01 * F11) / J);
+            mzz += w * (lam * lnJ / J);
+            mpsi += w * (0.5 * mu * (F00 * F00 + F01 * F01 + F10 * F10 + F11 * F11 - 2.0 - 2.0 * lnJ) +
+                         0.5 * lam * lnJ * lnJ);
+        }
+        mJ /= area; mxx /= area; myy /= area; mxy /= area; mzz /= area; mpsi /= area;
+        const double mises = sqrt(0.5 * ((mxx - myy) * (mxx - myy) + (myy - mzz) * (myy - mzz) +
+                                         (mzz - mxx) * (mzz - mxx)) + 3.0 * mxy * mxy);
+        if (t < ch.ne) {      // core elements (halo threads evaluate copies for their nodes only)
+            const int eo = D.perm[e];
+            const size_t ne0 = (size_t)D.n_el;
+            eout[eo] = area;
+            eout[ne0 + eo] = mJ;
+            eout[2 * ne0 + eo] = mxx;
+            eout[3 * ne0 + eo] = myy;
+            eout[4 * ne0 + eo] = mxy;
+            eout[5 * ne0 + eo] = mises;
+            eout[6 * ne0 + eo] = mpsi;
+        }
+        val[0] = V2<double>{mJ, mxx};
+        val[1] = V2<double>{myy, mxy};
+        val[2] = V2<double>{mises, mpsi};
+        val[3] = V2<double>{1.0, 0.0};
+    }
+    colour_accumulate4<NEN>(ch, t, l, val, L.acc, acc1, acc2, acc3);
+    const size_t N = (size_t)D.n_nodes;
+    V2<double>* wk = reinterpret_cast<V2<double>*>(work);
+    for (int k = t; k < ch.nk; k += BS) {
+        if (k < ch.nown) {
+            const int n = (int)(D.lnode[ch.k0 + k] & DFEA_NODE_MASK);
+            const double c = acc3[k].x;
+            nout[n] = L.acc[k].x / c;
+            nout[N + n] = L.acc[k].y / c;
+            nout[2 * N + n] = acc1[k].x / c;
+            nout[3 * N + n] = acc1[k].y / c;
+            nout[4 * N + n] = acc2[k].x / c;
+            nout[5 * N + n] = acc2[k].y / c;
+        } else if (D.n_part) {
+            const int slot = D.pslot[ch.pbase + (k - ch.nown)];
+            wk[slot] = L.acc[k];
+            wk[D.n_part + slot] = acc1[k];
+            wk[2 * D.n_part + slot] = acc2[k];
+            wk[3 * D.n_part + slot] = acc3[k];
+        }
+    }
+}
+
+__global__ __launch_bounds__(FBS) void k_fields_fin(const dfea_desc* __restrict__ tab, const int* __restrict__ items,
+                                                   double* __restrict__ nout, const double* __restrict__ work) {
+    const Item it = load_item(items);
+    const dfea_desc& D = tab[it.s];
+    const int i = it.b * FBS + threadIdx.x;
+    if (i >= D.n_shared) return;
+    const V2<double>* wk = reinterpret_cast<const V2<double>*>(work);
+    const int j0 = D.sh_off[i], m = D.sh_off[i + 1] - j0;
+    const V2<double> a0 = sum_slots<double>(wk, j0, m);
+    const V2<double> a1 = sum_slots<double>(wk + D.n_part, j0, m);
+    const V2<double> a2 = sum_slots<double>(wk + 2 * (size_t)D.n_part, j0, m);
+    const V2<double> a3 = sum_slots<double>(wk + 3 * (size_t)D.n_part, j0, m);
+    const size_t N = (size_t)D.n_nodes;
+    const int n = (int)(D.sh_node[i] & DFEA_NODE_MASK);
+    nout[n] = a0.x / a3.x;
+    nout[N + n] = a0.y / a3.x;
+    nout[2 * N + n] = a1.x / a3.x;
+    nout[3 * N + n] = a1.y / a3.x;
+    nout[4 * N + n] = a2.x / a3.x;
+    nout[5 * N + n] = a2.y / a3.x;
 }
 
 // ---- adjoint: g[e] = -sum (m lam)_e . R^e_int(u; mu0, lam0)  (`adjoint.py:75-106`) ----------------------------
@@ -1349,6 +1522,25 @@ struct Launch_residual {
     }
 };
 
+struct Launch_reactions {
+    const dfea_desc* t; const int32_t* it; int n; const int32_t* si; int ns; const double* u; double* out; void* st;
+    template <int NEN, int BS> int run() {
+        int rc = launch(k_residual<NEN, NEN, BS, true>, n, BS, st, "k_residual<react>", t, it, u, (double*)nullptr, 1.0);
+        if (rc) return rc;
+        return launch(k_reactions_final, ns, VBS, st, "k_reactions_final", t, si, out);
+    }
+};
+
+struct Launch_fields {
+    const dfea_desc* t; const int32_t* it; int n; const int32_t* fi; int nf; const double* u; double* eout;
+    double* nout; double* work; void* st;
+    template <int NEN, int BS> int run() {
+        int rc = launch(k_fields<NEN, NEN, BS>, n, BS, st, "k_fields", t, it, u, eout, nout, work);
+        if (rc) return rc;
+        return launch(k_fields_fin, nf, FBS, st, "k_fields_fin", t, fi, nout, (const double*)work);
+    }
+};
+
 struct Launch_apply {
     const dfea_desc* t; const int32_t* it; int n; const int32_t* fi; int nf; const void* v; void* y; int prec;
     void* st;
@@ -1442,6 +1634,18 @@ int dfea_residual(const dfea_desc* table, const int32_t* items, int n_items, con
                   int n_fin_items, const double* u, double* R, double lf, int nen, void* stream) {
     if (!table) return fail(DFEA_EINVAL, "dfea_residual: bad argument");
     return dispatch(nen, Launch_residual{table, items, n_items, fin_items, n_fin_items, u, R, lf, stream});
+}
+
+int dfea_reactions(const dfea_desc* table, const int32_t* items, int n_items, const int32_t* solve_items,
+                   int n_solves, const double* u, double* out, int nen, void* stream) {
+    if (!table || !out || n_solves < 1) return fail(DFEA_EINVAL, "dfea_reactions: bad argument");
+    return dispatch(nen, Launch_reactions{table, items, n_items, solve_items, n_solves, u, out, stream});
+}
+
+int dfea_fields(const dfea_desc* table, const int32_t* items, int n_items, const int32_t* fin_items,
+                int n_fin_items, const double* u, double* eout, double* nout, double* work, int nen, void* stream) {
+    if (!table || !eout || !nout || (n_fin_items > 0 && !work)) return fail(DFEA_EINVAL, "dfea_fields: bad argument");
+    return dispatch(nen, Launch_fields{table, items, n_items, fin_items, n_fin_items, u, eout, nout, work, stream});
 }
 
 int dfea_apply(const dfea_desc* table, const int32_t* items, int n_items, const int32_t* fin_items,

@@ -63,6 +63,8 @@ _SIGNATURES = {
     "dfea_chunk_threads": (_I, [_I]),
     "dfea_geometry": (_I, [_P, _P, _I, _I, _P]),
     "dfea_moduli": (_I, [_P, _P, _I, _P, _I, _P, _I, _I, _P]),
+    "dfea_reactions": (_I, [_P, _P, _I, _P, _I, _P, _P, _I, _P]),
+    "dfea_fields": (_I, [_P, _P, _I, _P, _I, _P, _P, _P, _P, _I, _P]),
     "dfea_residual": (_I, [_P, _P, _I, _P, _I, _P, _P, _D, _I, _P]),
     "dfea_apply": (_I, [_P, _P, _I, _P, _I, _P, _P, _I, _I, _P]),
     "dfea_dot": (_I, [_P, _P, _I, _P, _I, _P, _P, _P]),

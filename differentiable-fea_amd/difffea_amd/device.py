@@ -253,6 +253,33 @@ class HipSystem:
                                       _stream()), "dfea_axpy")
         return out
 
+    def reactions(self, u, out=None):
+        """(Rx, Ry): the internal force summed over the fixed dofs (device tensor of 2)."""
+        m = self.mesh
+        out = torch.empty(2, dtype=F64, device=m.device) if out is None else out
+        _lib.check(self.lib.dfea_reactions(_ptr(self.table), _ptr(m.items), m.plan.n_chunks, _ptr(m.solve_items), 1,
+                                           _ptr(u.to(F64).contiguous()), _ptr(out), m.kind, _stream()),
+                   "dfea_reactions")
+        return out
+
+    def fields_buffers(self):
+        """(eout, nout, work) for fields(); nout starts at zero (nodes that no element touches stay zero)."""
+        m = self.mesh
+        return (torch.empty(7, m.n_el, dtype=F64, device=m.device),
+                torch.zeros(6, m.n_nodes, dtype=F64, device=m.device),
+                torch.empty(max(1, 8 * m.plan.n_part), dtype=F64, device=m.device))
+
+    def fields(self, u, bufs=None):
+        """Element fields [7, n_el] (area, J, sxx, syy, sxy, mises, psi; original element order) and nodal
+        averages [6, N] (J, sxx, syy, sxy, mises, psi) of the state u; bufs from fields_buffers() may be
+        reused across calls."""
+        m = self.mesh
+        eout, nout, work = self.fields_buffers() if bufs is None else bufs
+        _lib.check(self.lib.dfea_fields(_ptr(self.table), _ptr(m.items), m.plan.n_chunks, _ptr(m.fin_items),
+                                        m.n_finblk, _ptr(u.to(F64).contiguous()), _ptr(eout), _ptr(nout),
+                                        _ptr(work), m.kind, _stream()), "dfea_fields")
+        return eout, nout
+
     def adjoint_grad(self, u, lam):
         m = self.mesh
         g = torch.empty(m.n_el, dtype=F64, device=m.device)

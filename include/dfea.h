@@ -161,6 +161,23 @@ int dfea_moduli(const dfea_desc* table, const int32_t* items, int n_items, const
 int dfea_residual(const dfea_desc* table, const int32_t* items, int n_items, const int32_t* fin_items,
                   int n_fin_items, const double* u, double* R, double lf, int nen, void* stream);
 
+/* ---- post-processing ---------------------------------------------------------------------------------- */
+/* Support reactions: out[2 s], out[2 s + 1] = the sums over the fixed dofs of the internal force R_int(u)
+ * (all-ones mask, no external load) for every solve s of solve_items (NULL: solve 0, n_solves = 1).  The chunk
+ * sums use the first 2 n_chunks doubles of desc.red (the PCG's partials): not while a PCG of the same solve
+ * runs.  u NULL -> vec[DFEA_U].  Replaces `src/difffea/postprocess.py:121-137` (support_reactions). */
+int dfea_reactions(const dfea_desc* table, const int32_t* items, int n_items, const int32_t* solve_items,
+                   int n_solves, const double* u, double* out, int nen, void* stream);
+/* Element and nodal fields of u (neo-Hookean, plane strain), items of one solve.
+ * eout [7][n_el], original element order: area, then the area-weighted means of J, Cauchy sxx, syy, sxy, the
+ * von Mises stress of the mean stress (with the plane-strain szz) and the energy density psi.
+ * nout [6][N]: J, sxx, syy, sxy, mises, psi summed over the elements at each node and divided by their number
+ * (nodes that no element touches are not written).  work: 8 * n_part doubles for the shared-node partials
+ * (NULL when the plan has no shared nodes).  u NULL -> vec[DFEA_U].
+ * Replaces `src/difffea/postprocess.py:24-118` (gauss_point_state, element_state, compute_fields). */
+int dfea_fields(const dfea_desc* table, const int32_t* items, int n_items, const int32_t* fin_items,
+                int n_fin_items, const double* u, double* eout, double* nout, double* work, int nen, void* stream);
+
 /* ---- tangent product -------------------------------------------------------------------------------- */
 /* y = m * J(u)(m v) + (1-m) v from the stored moduli (FP64 or FP32 per descriptor precision).
  * v/y NULL -> vec[DFEA_P] / vec[DFEA_W].  Deterministic (no atomics).

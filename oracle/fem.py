@@ -19,6 +19,10 @@ How each function is pinned (tests/test_oracle_*.py):
   * pcg                    -- dense solve
   * newton_krylov_solve    -- equilibrium, reactions balance the applied load, golden trajectory (G2)
   * adjoint                -- central finite differences; golden reference adjoint (G3)
+  * block-Jacobi blocks    -- the dense operator's nodal 2x2 diagonal blocks; PCG with them = dense solve
+  * fields / reactions     -- affine patches in closed form (Cauchy stress, energy, J, von Mises), element-by-
+                              element loops for the nodal averages, reactions = -(applied load) at equilibrium
+                              and = the fixed-dof sum of grad(total energy)
 """
 import torch
 from torch.func import grad, hessian, jvp, vjp, vmap
@@ -422,3 +426,65 @@ def compute_sensitivity(params, theta, loss_fn, n_load_steps=5, tol=1e-8, max_ne
     dL_du = grad(loss_fn)(u)
     lam, _ = solve_adjoint(u, theta, params, dL_du, rtol=adjoint_rtol, max_iter=adjoint_max_iter)
     return u, compute_implicit_vjp(u, theta, params, lam)
+
+
+# ---------------------------------------------------------------------------------------------------------
+# post-processing                                                   (postprocess.py:24-137, materials.py:37-73)
+# ---------------------------------------------------------------------------------------------------------
+def embed_plane_strain(F):
+    """F (2,2) -> F3 = [[F, 0], [0, 1]] (`materials.py:37-48`)."""
+    z, o = torch.zeros_like(F[0, 0]), torch.ones_like(F[0, 0])
+    return torch.stack([torch.stack([F[0, 0], F[0, 1], z]), torch.stack([F[1, 0], F[1, 1], z]),
+                        torch.stack([z, z, o])])
+
+
+def psi3_neo_hookean(F3, p):
+    """The 3-D neo-Hookean energy with det F3 = det(F2) F33 written out (`materials.py:51-73`)."""
+    mu, lam = p[0], p[1]
+    logJ = torch.log((F3[0, 0] * F3[1, 1] - F3[0, 1] * F3[1, 0]) * F3[2, 2])
+    return 0.5 * mu * (torch.sum(F3 * F3) - 3.0 - 2.0 * logJ) + 0.5 * lam * logJ ** 2
+
+
+def gauss_point_fields(F, p):
+    """[J, sxx, syy, sxy, szz, psi] at one Gauss point: P3 = grad psi3 (automatic differentiation),
+    sigma = P3 F3^T / J (`postprocess.py:24-48`)."""
+    F3 = embed_plane_strain(F)
+    P3 = grad(psi3_neo_hookean)(F3, p)
+    J = F[0, 0] * F[1, 1] - F[0, 1] * F[1, 0]
+    sig = P3 @ F3.T / J
+    return torch.stack([J, sig[0, 0], sig[1, 1], sig[0, 1], sig[2, 2], psi3_neo_hookean(F3, p)])
+
+
+def compute_fields(u, theta, params):
+    """Element means (weights w_q det J_q), von Mises of the mean stress, nodal averages over the elements
+    at each node, total strain energy and min J (`postprocess.py:51-118`)."""
+    conn = params["conn"]
+    N = params["nodes"].shape[0]
+    un = u.reshape(N, 2)
+    dN_dX, w_detJ = get_geometry(params)
+    F = torch.eye(2, dtype=u.dtype) + torch.einsum("eai,eqaj->eqij", un[conn], dN_dX)
+    pe = element_material_parameters(theta, params)
+    S = vmap(vmap(gauss_point_fields, in_dims=(0, None)))(F, pe)          # (n_el, n_q, 6)
+    area = w_detJ.sum(1)
+    J, sxx, syy, sxy, szz, psi = (torch.einsum("eq,eq->e", w_detJ, S[:, :, k]) / area for k in range(6))
+    mises = torch.sqrt(0.5 * ((sxx - syy) ** 2 + (syy - szz) ** 2 + (szz - sxx) ** 2) + 3.0 * sxy ** 2)
+    out = {"area": area, "J": J, "sxx": sxx, "syy": syy, "sxy": sxy, "mises": mises, "energy_density": psi,
+           "ux": un[:, 0], "uy": un[:, 1], "u_mag": torch.linalg.norm(un, dim=1),
+           "strain_energy": torch.sum(psi * area), "min_J": J.min()}
+    for k in ("ux", "uy", "u_mag"):
+        out[k + "_n"] = out[k]
+    flat = conn.reshape(-1)
+    count = torch.zeros(N, dtype=u.dtype).index_add_(0, flat, torch.ones(flat.numel(), dtype=u.dtype))
+    for k in ("J", "sxx", "syy", "sxy", "mises", "energy_density"):
+        s = torch.zeros(N, dtype=u.dtype).index_add_(0, flat, out[k][:, None].expand(-1, conn.shape[1]).reshape(-1))
+        out[k + "_n"] = s / count.clamp(min=1.0)
+    return out
+
+
+def support_reactions(u, theta, params):
+    """(Rx, Ry): the internal force (all-ones mask, no load) summed over the fixed dofs
+    (`postprocess.py:121-137`)."""
+    q = dict(params, m=torch.ones_like(params["m"]), f=torch.zeros_like(params["f"]))
+    R = global_residual(u, theta, q, 1.0).reshape(-1, 2)
+    fixed = params["fixed"].reshape(-1, 2)
+    return torch.where(fixed, R, torch.zeros_like(R)).sum(0)

@@ -77,3 +77,31 @@ def test_mixed_server_reaches_fp64_residual_tolerance():
         assert rel(r["u"], ref[r["rid"]]["u"]) < 1e-6
         if req.adjoint:
             assert rel(r["grad"], ref[r["rid"]]["grad"]) < 1e-4
+
+
+def test_block_jacobi_server_bitwise_solo_and_within_tolerance():
+    """The block-Jacobi server: every result the same bits alone and in the mix, the FP64 residual tolerance
+    met, and solutions / adjoint gradients agreeing with the point-Jacobi server's to their tolerances."""
+    from oracle import fem
+    from difffea_amd.serve import Server, request_problem
+    reqs = _stream()
+    recs, summ = Server("fp64", max_slots=16, preconditioner="block_jacobi").run(reqs, realtime=False,
+                                                                                 keep_solution=True)
+    assert summ["converged"] == len(reqs) and summ["preconditioner"] == "block_jacobi"
+    ref, _ = Server("fp64", max_slots=16).run(reqs, realtime=False, keep_solution=True)
+    ref = {r["rid"]: r for r in ref}
+    mix = {r["rid"]: r for r in recs}
+    for r in recs:
+        req = reqs[r["rid"]]
+        p, th = request_problem(req)
+        R = fem.global_residual(r["u"], th, fem.precompute_geometry(p), 1.0)
+        assert float(torch.norm(R)) < r["tol_abs"] * 1.0000001
+        assert rel(r["u"], ref[r["rid"]]["u"]) < 1e-6          # both within 1e-8 relative residual
+        if req.adjoint:
+            assert rel(r["grad"], ref[r["rid"]]["grad"]) < 1e-4   # adjoint solves to 1e-8 relative residual
+    for req in (reqs[2], reqs[4]):
+        solo, _ = Server("fp64", max_slots=2, preconditioner="block_jacobi").run([req], realtime=False,
+                                                                                 keep_solution=True)
+        assert torch.equal(solo[0]["u"], mix[req.rid]["u"])
+        if req.adjoint:
+            assert torch.equal(solo[0]["grad"], mix[req.rid]["grad"])

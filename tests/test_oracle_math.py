@@ -248,10 +248,102 @@ def test_solve_equilibrium_and_reactions_balance_load():
     assert info["converged"] and info["load_factor"] == 1.0
     assert float(torch.norm(fem.global_residual(u, th, p))) < 1e-9
     # reaction = internal force on the fixed dofs; it balances the applied load
-    p_all = dict(p, m=torch.ones_like(p["m"]), f=torch.zeros_like(p["f"]))
-    Rint = fem.global_residual(u, th, p_all).reshape(-1, 2)
-    reaction = (Rint * p["fixed"].reshape(-1, 2)).sum(0)
+    reaction = fem.support_reactions(u, th, p)
     assert torch.allclose(reaction + p["f"].reshape(-1, 2).sum(0), torch.zeros(2, dtype=DTYPE), atol=1e-7)
+    fields = fem.compute_fields(u, th, p)       # the reference's end-to-end check of the fields
+    assert float(fields["min_J"]) > 0 and torch.isfinite(fields["mises"]).all()
+
+
+# ---- post-processing -----------------------------------------------------------------------------------
+def _fields_closed(F, mu, lam):
+    """Neo-Hookean plane strain: J, sigma = (mu (F F^T - I) + lam lnJ I) / J, sigma_zz = lam lnJ / J, psi."""
+    J = F[0, 0] * F[1, 1] - F[0, 1] * F[1, 0]
+    lnJ = math.log(J)
+    B = F @ F.T
+    sxx = (mu * (B[0, 0] - 1) + lam * lnJ) / J
+    syy = (mu * (B[1, 1] - 1) + lam * lnJ) / J
+    sxy = mu * B[0, 1] / J
+    szz = lam * lnJ / J
+    psi = 0.5 * mu * (float((F * F).sum()) - 2 - 2 * lnJ) + 0.5 * lam * lnJ ** 2
+    mises = math.sqrt(0.5 * ((sxx - syy) ** 2 + (syy - szz) ** 2 + (szz - sxx) ** 2) + 3 * sxy ** 2)
+    return dict(J=float(J), sxx=float(sxx), syy=float(syy), sxy=float(sxy), szz=float(szz), psi=float(psi),
+                mises=float(mises))
+
+
+@pytest.mark.parametrize("et", ["quad4", "quad9"])
+def test_fields_of_an_affine_deformation_in_closed_form(et):
+    p = fem.precompute_geometry(unit_square(et, 3))
+    F0 = torch.tensor([[1.2, 0.1], [-0.05, 0.9]], dtype=DTYPE)
+    u = (p["nodes"] @ (F0 - torch.eye(2, dtype=DTYPE)).T).reshape(-1)
+    mu, lam = float(p["mat_p"][0]), float(p["mat_p"][1])
+    for scale in (1.0, 2.5):        # theta scales both Lame parameters
+        th = torch.full((p["conn"].shape[0],), scale, dtype=DTYPE)
+        out = fem.compute_fields(u, th, p)
+        c = _fields_closed(F0, scale * mu, scale * lam)
+        assert float(out["area"].sum()) == pytest.approx(1.0, abs=1e-13)
+        for k, ck in (("J", "J"), ("sxx", "sxx"), ("syy", "syy"), ("sxy", "sxy"), ("mises", "mises"),
+                      ("energy_density", "psi")):
+            assert torch.allclose(out[k], torch.full_like(out[k], c[ck]), rtol=0, atol=1e-12), k
+            assert torch.allclose(out[k + "_n"], torch.full_like(out[k + "_n"], c[ck]), rtol=0, atol=1e-12), k
+        assert float(out["strain_energy"]) == pytest.approx(c["psi"], abs=1e-12)
+        assert float(out["min_J"]) == pytest.approx(c["J"], abs=1e-13)
+    zero = fem.compute_fields(torch.zeros_like(u), th, p)
+    assert float(zero["mises"].abs().max()) < 1e-14 and float(zero["strain_energy"]) == 0.0
+    assert torch.allclose(zero["J"], torch.ones_like(zero["J"]), rtol=0, atol=1e-15)
+
+
+def test_fields_weighted_means_and_nodal_averages_by_explicit_loops():
+    p = fem.precompute_geometry(unit_square("quad9", 2))
+    th = 0.5 + torch.rand(p["conn"].shape[0], dtype=DTYPE, generator=torch.Generator().manual_seed(4))
+    u, _ = state(p, 9, scale=0.03)
+    out = fem.compute_fields(u, th, p)
+    conn, N = p["conn"], p["nodes"].shape[0]
+    ue = u.reshape(N, 2)[conn]
+    mu0, lam0 = float(p["mat_p"][0]), float(p["mat_p"][1])
+    elem = {k: [] for k in ("J", "sxx", "syy", "sxy", "mises", "energy_density")}
+    for e in range(conn.shape[0]):
+        w = p["w_detJ"][e]
+        acc = dict(J=0.0, sxx=0.0, syy=0.0, sxy=0.0, szz=0.0, psi=0.0)
+        for q in range(w.shape[0]):
+            F = torch.eye(2, dtype=DTYPE) + ue[e].T @ p["dN_dX"][e, q]
+            c = _fields_closed(F, mu0 * float(th[e]), lam0 * float(th[e]))
+            for k in acc:
+                acc[k] += float(w[q]) * c[k]
+        a = float(w.sum())
+        assert float(out["area"][e]) == pytest.approx(a, rel=1e-14)
+        m = {k: v / a for k, v in acc.items()}
+        m["mises"] = math.sqrt(0.5 * ((m["sxx"] - m["syy"]) ** 2 + (m["syy"] - m["szz"]) ** 2 +
+                                      (m["szz"] - m["sxx"]) ** 2) + 3 * m["sxy"] ** 2)
+        m["energy_density"] = m["psi"]
+        for k in elem:
+            elem[k].append(m[k])
+            assert float(out[k][e]) == pytest.approx(m[k], rel=1e-11, abs=1e-14), (k, e)
+    for n in range(N):
+        es = [e for e in range(conn.shape[0]) for a in range(conn.shape[1]) if int(conn[e, a]) == n]
+        for k in elem:
+            want = sum(elem[k][e] for e in es) / len(es)
+            assert float(out[k + "_n"][n]) == pytest.approx(want, rel=1e-11, abs=1e-14), (k, n)
+
+
+@pytest.mark.parametrize("et", ["quad4", "quad9"])
+def test_reactions_are_the_fixed_dof_sums_of_the_energy_gradient(et):
+    p = fem.precompute_geometry(unit_square(et, 3))
+    th = 0.5 + torch.rand(p["conn"].shape[0], dtype=DTYPE, generator=torch.Generator().manual_seed(5))
+    N = p["nodes"].shape[0]
+
+    def energy(u):
+        ue = u.reshape(N, 2)[p["conn"]]
+        F = torch.eye(2, dtype=DTYPE) + torch.einsum("eai,eqaj->eqij", ue, p["dN_dX"])
+        J = F[..., 0, 0] * F[..., 1, 1] - F[..., 0, 1] * F[..., 1, 0]
+        lnJ = torch.log(J)
+        mu, lam = p["mat_p"][0] * th[:, None], p["mat_p"][1] * th[:, None]
+        psi = 0.5 * mu * ((F * F).sum((-1, -2)) - 2 - 2 * lnJ) + 0.5 * lam * lnJ ** 2
+        return (p["w_detJ"] * psi).sum()
+
+    u, _ = state(p, 6, scale=0.04)
+    g = grad(energy)(u).reshape(N, 2)
+    want = (g * p["fixed"].reshape(N, 2)).sum(0)
+    assert torch.allclose(fem.support_reactions(u, th, p), want, atol=1e-12, rtol=1e-10)
 
 
 def test_displacement_controlled_solve_honours_the_prescribed_motion():
