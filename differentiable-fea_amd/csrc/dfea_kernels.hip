@@ -26,6 +26,7 @@
 #include <type_traits>
 
 #include "dfea.h"
+#include "dfea_material.h"
 
 namespace {
 
@@ -578,11 +579,28 @@ __global__ __launch_bounds__(FBS) void k_apply_fin(const dfea_desc* __restrict__
     }
 }
 
+// w P(F) of the element's material (the neo-Hookean closed form inline, the library through dfea_mat)
+template <bool LIB>
+__device__ __forceinline__ void piola_w(const dfea_mat::Mat& M, double F00, double F01, double F10, double F11,
+                                        double w, double* P) {
+    if (!LIB || M.id == DFEA_MAT_NEO_HOOKEAN) {
+        const Kin k = kinematics(F00, F01, F10, F11);
+        const double mu = M.p[0], b = M.p[1] * k.lnJ - mu;
+#pragma unroll
+        for (int m = 0; m < 4; ++m) P[m] = w * (mu * k.F[m] + b * k.G[m]);
+        return;
+    }
+    const double F[4] = {F00, F01, F10, F11};
+    dfea_mat::piola<LIB>(M, F, P);
+#pragma unroll
+    for (int m = 0; m < 4; ++m) P[m] *= w;
+}
+
 // ---- residual  R = m (R_int(u) - lf f)  (`operators.py:131-220`) -------------------------------------------
 // REACT: support reactions instead (`postprocess.py:121-137`): the chunk's sums of the internal force over its
 // fixed dofs (owned nodes, and its partial of each shared node: the total is linear in the partials) go to
 // red[b] (x) and red[n_chunks + b] (y); R is not written
-template <int NEN, int NQ, int BS, bool REACT = false>
+template <int NEN, int NQ, int BS, bool LIB, bool REACT = false>
 __global__ __launch_bounds__(BS) void k_residual(const dfea_desc* __restrict__ tab, const int* __restrict__ items,
                                                 const double* __restrict__ uin, double* __restrict__ rout,
                                                 double lf_arg) {
@@ -608,8 +626,7 @@ __global__ __launch_bounds__(BS) void k_residual(const dfea_desc* __restrict__ t
     V2<double> fe[NEN];
     if (has_el) {
         const int ne = D.n_cols;
-        const double th = D.theta[e];
-        const double mu = D.mu0 * th, lam = D.lam0 * th;
+        const dfea_mat::Mat M = dfea_mat::element_material<LIB>(D, D.theta[e]);
         V2<double> ue[NEN];
 #pragma unroll
         for (int a = 0; a < NEN; ++a) {
@@ -626,12 +643,9 @@ __global__ __launch_bounds__(BS) void k_residual(const dfea_desc* __restrict__ t
                 F00 += ue[a].x * d[a][0]; F01 += ue[a].x * d[a][1];
                 F10 += ue[a].y * d[a][0]; F11 += ue[a].y * d[a][1];
             }
-            const Kin k = kinematics(F00, F01, F10, F11);
             const double w = D.w[(size_t)q * ne + e];
-            const double b = lam * k.lnJ - mu;
             double P[4];
-#pragma unroll
-            for (int m = 0; m < 4; ++m) P[m] = w * (mu * k.F[m] + b * k.G[m]);
+            piola_w<LIB>(M, F00, F01, F10, F11, w, P);
 #pragma unroll
             for (int a = 0; a < NEN; ++a) {
                 fe[a].x += P[0] * d[a][0] + P[1] * d[a][1];
@@ -716,7 +730,7 @@ __device__ __forceinline__ void store_precond(const dfea_desc& D, int n, unsigne
     out[2 * n + 1] = (CT)d1;
 }
 
-template <int NEN, int NQ, int BS, typename CT>
+template <int NEN, int NQ, int BS, typename CT, bool LIB>
 __global__ __launch_bounds__(BS) void k_moduli(const dfea_desc* __restrict__ tab, const int* __restrict__ items,
                                               const double* __restrict__ uin) {
     __shared__ ChunkLDS<double> L;
@@ -741,8 +755,7 @@ __global__ __launch_bounds__(BS) void k_moduli(const dfea_desc* __restrict__ tab
     V2<double> fo[NEN];     // block Jacobi: {A_xy, 0} of the element's nodal blocks
     if (has_el) {
         const int ne = D.n_cols;
-        const double th = D.theta[e];
-        const double mu = D.mu0 * th, lam = D.lam0 * th;
+        const dfea_mat::Mat M = dfea_mat::element_material<LIB>(D, D.theta[e]);
         CT* Cout = std::is_same<CT, double>::value ? (CT*)D.C64 : (CT*)D.C32;
         V2<double> ue[NEN];
 #pragma unroll
@@ -761,9 +774,14 @@ __global__ __launch_bounds__(BS) void k_moduli(const dfea_desc* __restrict__ tab
                 F00 += ue[a].x * d[a][0]; F01 += ue[a].x * d[a][1];
                 F10 += ue[a].y * d[a][0]; F11 += ue[a].y * d[a][1];
             }
-            const Kin k = kinematics(F00, F01, F10, F11);
             double c[10];
-            moduli10(k, mu, lam, D.w[(size_t)q * ne + e], c);
+            const double w = D.w[(size_t)q * ne + e];
+            if (!LIB || M.id == DFEA_MAT_NEO_HOOKEAN) {
+                moduli10(kinematics(F00, F01, F10, F11), M.p[0], M.p[1], w, c);
+            } else {
+                const double F[4] = {F00, F01, F10, F11};
+                dfea_mat::moduli<LIB>(M, F, w, c);
+            }
 #pragma unroll
             for (int kk = 0; kk < 10; ++kk) Cout[(size_t)(kk * NQ + q) * ne + e] = (CT)c[kk];   // halo: copy column
             // K^e_{(a,i),(a,i)} += sum_{j,l} C[2i+j][2i+l] dN[a][j] dN[a][l]
@@ -856,7 +874,7 @@ __device__ __forceinline__ void colour_accumulate4(const ChunkView& ch, int t, c
     }
 }
 
-template <int NEN, int NQ, int BS>
+template <int NEN, int NQ, int BS, bool LIB>
 __global__ __launch_bounds__(BS) void k_fields(const dfea_desc* __restrict__ tab, const int* __restrict__ items,
                                               const double* __restrict__ uin, double* __restrict__ eout,
                                               double* __restrict__ nout, double* __restrict__ work) {
@@ -879,8 +897,7 @@ __global__ __launch_bounds__(BS) void k_fields(const dfea_desc* __restrict__ tab
     V2<double> val[4] = {};
     if (has_el) {
         const int ne = D.n_cols;
-        const double th = D.theta[e];
-        const double mu = D.mu0 * th, lam = D.lam0 * th;
+        const dfea_mat::Mat M = dfea_mat::element_material<LIB>(D, D.theta[e]);
         V2<double> ue[NEN];
 #pragma unroll
         for (int a = 0; a < NEN; ++a) ue[a] = L.v[l[a]];
@@ -895,16 +912,18 @@ __global__ __launch_bounds__(BS) void k_fields(const dfea_desc* __restrict__ tab
                 F10 += ue[a].y * d0; F11 += ue[a].y * d1;
             }
             const double J = F00 * F11 - F01 * F10;
-            const double lnJ = log(J);
             const double w = D.w[(size_t)q * ne + e];
+            const double F[4] = {F00, F01, F10, F11};
+            double P[4], P33, psi;
+            dfea_mat::stress_state<LIB>(M, F, P, P33, psi);
+            // sigma = P3 F3^T / J
             area += w;
             mJ += w * J;
-            mxx += w * ((mu * (F00 * F00 + F01 * F01 - 1.0) + lam * lnJ) / J);
-            myy += w * ((mu * (F10 * F10 + F11 * F11 - 1.0) + lam * lnJ) / J);
-            mxy += w * (mu * (F00 * F10 + F01 * F11) / J);
-            mzz += w * (lam * lnJ / J);
-            mpsi += w * (0.5 * mu * (F00 * F00 + F01 * F01 + F10 * F10 + F11 * F11 - 2.0 - 2.0 * lnJ) +
-                         0.5 * lam * lnJ * lnJ);
+            mxx += w * ((P[0] * F00 + P[1] * F01) / J);
+            myy += w * ((P[2] * F10 + P[3] * F11) / J);
+            mxy += w * ((P[0] * F10 + P[1] * F11) / J);
+            mzz += w * (P33 / J);
+            mpsi += w * psi;
         }
         mJ /= area; mxx /= area; myy /= area; mxy /= area; mzz /= area; mpsi /= area;
         const double mises = sqrt(0.5 * ((mxx - myy) * (mxx - myy) + (myy - mzz) * (myy - mzz) +
@@ -970,8 +989,8 @@ __global__ __launch_bounds__(FBS) void k_fields_fin(const dfea_desc* __restrict_
     nout[5 * N + n] = a2.y / a3.x;
 }
 
-// ---- adjoint: g[e] = -sum (m lam)_e . R^e_int(u; mu0, lam0)  (`adjoint.py:75-106`) ----------------------------
-template <int NEN, int NQ, int BS>
+// ---- adjoint: g[e] = -sum (m lam)_e . R^e_int(u; mat_p)  (`adjoint.py:75-106`) -------------------------------
+template <int NEN, int NQ, int BS, bool LIB>
 __global__ __launch_bounds__(BS) void k_adjoint(const dfea_desc* __restrict__ tab, const int* __restrict__ items,
                                                const double* __restrict__ u, const double* __restrict__ lam_in,
                                                double* __restrict__ gout) {
@@ -993,6 +1012,7 @@ __global__ __launch_bounds__(BS) void k_adjoint(const dfea_desc* __restrict__ ta
     if (t >= ch.ne) return;                       // per-element output: core elements only
     const int ne = D.n_cols;
     const int e = ch.e0 + t;
+    const dfea_mat::Mat M = dfea_mat::element_material<LIB>(D, 1.0);     // R^e_int is linear in theta_e
     V2<double> ue[NEN], le[NEN];
 #pragma unroll
     for (int a = 0; a < NEN; ++a) {
@@ -1011,12 +1031,8 @@ __global__ __launch_bounds__(BS) void k_adjoint(const dfea_desc* __restrict__ ta
             F00 += ue[a].x * d[a][0]; F01 += ue[a].x * d[a][1];
             F10 += ue[a].y * d[a][0]; F11 += ue[a].y * d[a][1];
         }
-        const Kin k = kinematics(F00, F01, F10, F11);
-        const double w = D.w[(size_t)q * ne + e];
-        const double b = D.lam0 * k.lnJ - D.mu0;
         double P[4];
-#pragma unroll
-        for (int m = 0; m < 4; ++m) P[m] = w * (D.mu0 * k.F[m] + b * k.G[m]);
+        piola_w<LIB>(M, F00, F01, F10, F11, D.w[(size_t)q * ne + e], P);
 #pragma unroll
         for (int a = 0; a < NEN; ++a)
             s += le[a].x * (P[0] * d[a][0] + P[1] * d[a][1]) + le[a].y * (P[2] * d[a][0] + P[3] * d[a][1]);
@@ -1456,24 +1472,32 @@ int launch(K kern, int grid, int block, void* stream, const char* name, A... arg
     return hip_status(name);
 }
 
-// element kind of a launch: nen, with DFEA_HALO set for halo plans
+// element kind of a launch: nen, with DFEA_HALO set for halo plans (DFEA_MATLIB selects the instantiations
+// with the whole material library and is not part of the kind)
 enum class Kind { Q4, Q4H, Q9, Bad };
 Kind kind_of(int k) {
+    k &= ~DFEA_MATLIB;
     if (k == 4) return Kind::Q4;
     if (k == (4 | DFEA_HALO)) return Kind::Q4H;
     if (k == 9) return Kind::Q9;
     return Kind::Bad;
 }
 
-// call f.template run<NEN, BS>() for the kind (one instantiation per element type and plan mode)
+// call f.template run<NEN, BS, LIB>() for the kind (one instantiation per element type, plan mode and, for
+// the kernels that evaluate the material, library or neo-Hookean-only code)
+template <bool LIB, typename F>
+int dispatch_lib(int kind, F f) {
+    switch (kind_of(kind)) {
+        case Kind::Q4: return f.template run<4, BS_Q4, LIB>();
+        case Kind::Q4H: return f.template run<4, BS_Q4H, LIB>();
+        case Kind::Q9: return f.template run<9, BS_Q9, LIB>();
+        default: return fail(DFEA_EINVAL, "unknown element kind (expected 4, 9 or 4 | DFEA_HALO, optionally | DFEA_MATLIB)");
+    }
+}
+
 template <typename F>
 int dispatch(int kind, F f) {
-    switch (kind_of(kind)) {
-        case Kind::Q4: return f.template run<4, BS_Q4>();
-        case Kind::Q4H: return f.template run<4, BS_Q4H>();
-        case Kind::Q9: return f.template run<9, BS_Q9>();
-        default: return fail(DFEA_EINVAL, "unknown element kind (expected 4, 9 or 4 | DFEA_HALO)");
-    }
+    return (kind & DFEA_MATLIB) ? dispatch_lib<true>(kind, f) : dispatch_lib<false>(kind, f);
 }
 
 // PCG kernels in pairs of parities 0, 1 (n_iter rounded up to even).  Quad9 chunks (64 elements, up to 289
@@ -1496,16 +1520,16 @@ int pcg_iter(const dfea_desc* table, const int32_t* items, int n_items, int n_it
 // launchers of the C ABI, one instantiation per element kind
 struct Launch_geometry {
     const dfea_desc* t; const int32_t* it; int n; void* st;
-    template <int NEN, int BS> int run() {
+    template <int NEN, int BS, bool LIB> int run() {
         return launch(k_geometry<NEN, NEN, BS>, n, BS, st, "k_geometry", t, it);
     }
 };
 
 struct Launch_moduli {
     const dfea_desc* t; const int32_t* it; int n; const int32_t* fi; int nf; const double* u; int prec; void* st;
-    template <int NEN, int BS> int run() {
-        int rc = prec == DFEA_FP64 ? launch(k_moduli<NEN, NEN, BS, double>, n, BS, st, "k_moduli", t, it, u)
-                                   : launch(k_moduli<NEN, NEN, BS, float>, n, BS, st, "k_moduli", t, it, u);
+    template <int NEN, int BS, bool LIB> int run() {
+        int rc = prec == DFEA_FP64 ? launch(k_moduli<NEN, NEN, BS, double, LIB>, n, BS, st, "k_moduli", t, it, u)
+                                   : launch(k_moduli<NEN, NEN, BS, float, LIB>, n, BS, st, "k_moduli", t, it, u);
         if (rc) return rc;
         return prec == DFEA_FP64 ? launch(k_moduli_fin<double>, nf, FBS, st, "k_moduli_fin", t, fi)
                                  : launch(k_moduli_fin<float>, nf, FBS, st, "k_moduli_fin", t, fi);
@@ -1515,8 +1539,8 @@ struct Launch_moduli {
 struct Launch_residual {
     const dfea_desc* t; const int32_t* it; int n; const int32_t* fi; int nf; const double* u; double* R;
     double lf; void* st;
-    template <int NEN, int BS> int run() {
-        int rc = launch(k_residual<NEN, NEN, BS>, n, BS, st, "k_residual", t, it, u, R, lf);
+    template <int NEN, int BS, bool LIB> int run() {
+        int rc = launch(k_residual<NEN, NEN, BS, LIB>, n, BS, st, "k_residual", t, it, u, R, lf);
         if (rc) return rc;
         return launch(k_residual_fin, nf, FBS, st, "k_residual_fin", t, fi, R, lf);
     }
@@ -1524,8 +1548,8 @@ struct Launch_residual {
 
 struct Launch_reactions {
     const dfea_desc* t; const int32_t* it; int n; const int32_t* si; int ns; const double* u; double* out; void* st;
-    template <int NEN, int BS> int run() {
-        int rc = launch(k_residual<NEN, NEN, BS, true>, n, BS, st, "k_residual<react>", t, it, u, (double*)nullptr, 1.0);
+    template <int NEN, int BS, bool LIB> int run() {
+        int rc = launch(k_residual<NEN, NEN, BS, LIB, true>, n, BS, st, "k_residual<react>", t, it, u, (double*)nullptr, 1.0);
         if (rc) return rc;
         return launch(k_reactions_final, ns, VBS, st, "k_reactions_final", t, si, out);
     }
@@ -1534,8 +1558,8 @@ struct Launch_reactions {
 struct Launch_fields {
     const dfea_desc* t; const int32_t* it; int n; const int32_t* fi; int nf; const double* u; double* eout;
     double* nout; double* work; void* st;
-    template <int NEN, int BS> int run() {
-        int rc = launch(k_fields<NEN, NEN, BS>, n, BS, st, "k_fields", t, it, u, eout, nout, work);
+    template <int NEN, int BS, bool LIB> int run() {
+        int rc = launch(k_fields<NEN, NEN, BS, LIB>, n, BS, st, "k_fields", t, it, u, eout, nout, work);
         if (rc) return rc;
         return launch(k_fields_fin, nf, FBS, st, "k_fields_fin", t, fi, nout, (const double*)work);
     }
@@ -1552,14 +1576,14 @@ struct Launch_apply {
         if (rc) return rc;
         return launch(k_apply_fin<T>, nf, FBS, st, "k_apply_fin", t, fi, vv, yy);
     }
-    template <int NEN, int BS> int run() {
+    template <int NEN, int BS, bool LIB> int run() {
         return prec == DFEA_FP64 ? go<double, NEN, BS>() : go<float, NEN, BS>();
     }
 };
 
 struct Launch_pcg_iterate {
     const dfea_desc* t; const int32_t* it; int n; int k; int prec; void* st;
-    template <int NEN, int BS> int run() {
+    template <int NEN, int BS, bool LIB> int run() {
         return prec == DFEA_FP64 ? pcg_iter<NEN, BS, double>(t, it, n, k, st)
                                  : pcg_iter<NEN, BS, float>(t, it, n, k, st);
     }
@@ -1567,8 +1591,8 @@ struct Launch_pcg_iterate {
 
 struct Launch_adjoint {
     const dfea_desc* t; const int32_t* it; int n; const double* u; const double* lam; double* g; void* st;
-    template <int NEN, int BS> int run() {
-        return launch(k_adjoint<NEN, NEN, BS>, n, BS, st, "k_adjoint", t, it, u, lam, g);
+    template <int NEN, int BS, bool LIB> int run() {
+        return launch(k_adjoint<NEN, NEN, BS, LIB>, n, BS, st, "k_adjoint", t, it, u, lam, g);
     }
 };
 
@@ -1594,12 +1618,13 @@ int dfea_desc_offset(int field) {
         (int)offsetof(dfea_desc, theta), (int)offsetof(dfea_desc, C64), (int)offsetof(dfea_desc, C32),
         (int)offsetof(dfea_desc, diag64), (int)offsetof(dfea_desc, diag32), (int)offsetof(dfea_desc, f),
         (int)offsetof(dfea_desc, part), (int)offsetof(dfea_desc, red), (int)offsetof(dfea_desc, sc),
-        (int)offsetof(dfea_desc, vec), (int)offsetof(dfea_desc, mu0), (int)offsetof(dfea_desc, lam0),
+        (int)offsetof(dfea_desc, vec), (int)offsetof(dfea_desc, mat_p),
         (int)offsetof(dfea_desc, n_chunks), (int)offsetof(dfea_desc, n_lnodes), (int)offsetof(dfea_desc, n_shared),
         (int)offsetof(dfea_desc, n_part), (int)offsetof(dfea_desc, n_el), (int)offsetof(dfea_desc, n_nodes),
         (int)offsetof(dfea_desc, nen), (int)offsetof(dfea_desc, nq), (int)offsetof(dfea_desc, precision),
         (int)offsetof(dfea_desc, n_finblk), (int)offsetof(dfea_desc, n_vblk), (int)offsetof(dfea_desc, n_halo),
-        (int)offsetof(dfea_desc, n_cols), (int)offsetof(dfea_desc, precond)};
+        (int)offsetof(dfea_desc, n_cols), (int)offsetof(dfea_desc, precond), (int)offsetof(dfea_desc, material),
+        (int)offsetof(dfea_desc, mat_scal)};
     if (field < 0 || field >= (int)(sizeof off / sizeof off[0])) return -1;
     return off[field];
 }

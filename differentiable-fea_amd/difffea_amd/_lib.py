@@ -18,6 +18,8 @@ PRECOND = {"jacobi": 0, "block_jacobi": 1}     # dfea_desc.precond
 MAXCOL = 16
 FIXED_X, FIXED_Y, NODE_MASK = 0x40000000, 0x80000000, 0x3FFFFFFF
 HALO = 0x100          # element-kind flag of halo plans
+MATLIB = 0x200        # element-kind flag: material kernels with the whole material library
+MAT_NP = 8            # dfea_desc.mat_p slots
 VEC = dict(x=0, r=1, w=2, p=3, s=4, b=5, u=6, R=7, uacc=8, uD=9, x2=10, r2=11, w2=12, p2=13, s2=14)
 NVEC = 15
 PCG_VECS = ("x", "r", "w", "p", "s", "b", "x2", "r2", "w2", "p2", "s2")   # in the solve's precision
@@ -38,12 +40,13 @@ class Desc(ctypes.Structure):
         ("conn", _P), ("perm", _P), ("nodes", _P), ("dNdxi", _P), ("gw", _P), ("nfix", _P),
         ("dN64", _P), ("dN32", _P), ("w", _P), ("theta", _P), ("C64", _P), ("C32", _P),
         ("diag64", _P), ("diag32", _P), ("f", _P), ("part", _P), ("red", _P), ("sc", _P),
-        ("vec", _P * NVEC), ("mu0", _D), ("lam0", _D),
+        ("vec", _P * NVEC), ("mat_p", _D * MAT_NP),
         ("n_chunks", ctypes.c_int32), ("n_lnodes", ctypes.c_int32), ("n_shared", ctypes.c_int32),
         ("n_part", ctypes.c_int32), ("n_el", ctypes.c_int32), ("n_nodes", ctypes.c_int32),
         ("nen", ctypes.c_int32), ("nq", ctypes.c_int32), ("precision", ctypes.c_int32),
         ("n_finblk", ctypes.c_int32), ("n_vblk", ctypes.c_int32), ("n_halo", ctypes.c_int32),
-        ("n_cols", ctypes.c_int32), ("precond", ctypes.c_int32),
+        ("n_cols", ctypes.c_int32), ("precond", ctypes.c_int32), ("material", ctypes.c_int32),
+        ("mat_scal", ctypes.c_int32),
     ]
 
 

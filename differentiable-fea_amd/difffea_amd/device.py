@@ -15,6 +15,7 @@ import numpy as np
 import torch
 
 from . import _lib
+from .materials import MaterialSpec
 from .elements import element_type_of, reference_gradient_table
 from .plan import MeshPlan
 
@@ -135,7 +136,9 @@ class HipMesh:
 class HipSystem:
     """One solve (mesh + theta + material + precision) with all its device buffers."""
 
-    def __init__(self, mesh, theta, mat_p, f, precision="fp64", uD=None, sc=None, preconditioner="jacobi"):
+    def __init__(self, mesh, theta, mat, f, precision="fp64", uD=None, sc=None, preconditioner="jacobi"):
+        """mat: a ``materials.MaterialSpec`` or, for neo-Hookean with both parameters theta-scaled, the pair
+        (mu0, lmbda0)."""
         self.mesh = m = mesh
         self.lib = mesh.lib
         dv = mesh.device
@@ -148,8 +151,13 @@ class HipSystem:
         self.prec_id = _lib.FP64 if precision == "fp64" else _lib.FP32
         self.vdt = F64 if precision == "fp64" else F32
         self.theta = mesh.to_internal(theta)
-        mp = torch.as_tensor(mat_p, dtype=F64).cpu()
-        self.mu0, self.lam0 = float(mp[0]), float(mp[1])
+        if not isinstance(mat, MaterialSpec):
+            mp = torch.as_tensor(mat, dtype=F64).cpu().reshape(-1)
+            if mp.numel() != 2:
+                raise ValueError("a bare parameter vector means neo-Hookean (mu0, lmbda0); pass a MaterialSpec")
+            mat = MaterialSpec(0, (float(mp[0]), float(mp[1])) + (0.0,) * (_lib.MAT_NP - 2), 3)
+        self.mat = mat
+        self.kind = m.kind | (_lib.MATLIB if mat.library else 0)   # element kind of the material kernels
         self.f = f.to(dv, F64).contiguous()
         nq, n_el, nd = m.nq, m.n_el, m.n_dof
         self.C = None                    # moduli and preconditioner: allocated by the first moduli() call
@@ -178,7 +186,9 @@ class HipSystem:
         d.f, d.part, d.red, d.sc = self.f.data_ptr(), self.part.data_ptr(), self.red.data_ptr(), self.sc.data_ptr()
         for k, i in _lib.VEC.items():
             d.vec[i] = self.vec[k].data_ptr()
-        d.mu0, d.lam0 = self.mu0, self.lam0
+        for k, v in enumerate(self.mat.p):
+            d.mat_p[k] = v
+        d.material, d.mat_scal = self.mat.mid, self.mat.scal
         d.precision = self.prec_id
         d.precond = _lib.PRECOND[self.preconditioner]
         return d
@@ -199,7 +209,7 @@ class HipSystem:
         m = self.mesh
         u = u.to(F64).contiguous()
         _lib.check(self.lib.dfea_moduli(_ptr(self.table), _ptr(m.items), m.plan.n_chunks, _ptr(m.fin_items),
-                                        m.n_finblk, _ptr(u), m.kind, self.prec_id, _stream()), "dfea_moduli")
+                                        m.n_finblk, _ptr(u), self.kind, self.prec_id, _stream()), "dfea_moduli")
 
     def precondition(self, r):
         """u = M r with the preconditioner of the last moduli() call (device tensors, the system's precision)."""
@@ -217,7 +227,7 @@ class HipSystem:
         u = u.to(F64).contiguous()
         out = torch.empty_like(u) if out is None else out
         _lib.check(self.lib.dfea_residual(_ptr(self.table), _ptr(m.items), m.plan.n_chunks, _ptr(m.fin_items),
-                                          m.n_finblk, _ptr(u), _ptr(out), float(lf), m.kind, _stream()),
+                                          m.n_finblk, _ptr(u), _ptr(out), float(lf), self.kind, _stream()),
                    "dfea_residual")
         return out
 
@@ -227,7 +237,7 @@ class HipSystem:
             v = v.to(self.vdt).contiguous()
         out = torch.empty_like(v) if out is None else out
         _lib.check(self.lib.dfea_apply(_ptr(self.table), _ptr(m.items), m.plan.n_chunks, _ptr(m.fin_items),
-                                       m.n_finblk, _ptr(v), _ptr(out), m.kind, self.prec_id, _stream()),
+                                       m.n_finblk, _ptr(v), _ptr(out), self.kind, self.prec_id, _stream()),
                    "dfea_apply")
         return out
 
@@ -258,7 +268,7 @@ class HipSystem:
         m = self.mesh
         out = torch.empty(2, dtype=F64, device=m.device) if out is None else out
         _lib.check(self.lib.dfea_reactions(_ptr(self.table), _ptr(m.items), m.plan.n_chunks, _ptr(m.solve_items), 1,
-                                           _ptr(u.to(F64).contiguous()), _ptr(out), m.kind, _stream()),
+                                           _ptr(u.to(F64).contiguous()), _ptr(out), self.kind, _stream()),
                    "dfea_reactions")
         return out
 
@@ -277,7 +287,7 @@ class HipSystem:
         eout, nout, work = self.fields_buffers() if bufs is None else bufs
         _lib.check(self.lib.dfea_fields(_ptr(self.table), _ptr(m.items), m.plan.n_chunks, _ptr(m.fin_items),
                                         m.n_finblk, _ptr(u.to(F64).contiguous()), _ptr(eout), _ptr(nout),
-                                        _ptr(work), m.kind, _stream()), "dfea_fields")
+                                        _ptr(work), self.kind, _stream()), "dfea_fields")
         return eout, nout
 
     def adjoint_grad(self, u, lam):
@@ -285,7 +295,7 @@ class HipSystem:
         g = torch.empty(m.n_el, dtype=F64, device=m.device)
         _lib.check(self.lib.dfea_adjoint_grad(_ptr(self.table), _ptr(m.items), m.plan.n_chunks,
                                               _ptr(u.to(F64).contiguous()), _ptr(lam.to(F64).contiguous()), _ptr(g),
-                                              m.kind, _stream()), "dfea_adjoint_grad")
+                                              self.kind, _stream()), "dfea_adjoint_grad")
         return g
 
     # -- fused PCG ---------------------------------------------------------------------------------------------
@@ -310,7 +320,7 @@ class HipSystem:
             m = self.mesh
             h = ctypes.c_void_p()
             _lib.check(self.lib.dfea_pcg_graph_create(
-                _ptr(self.table), _ptr(m.items), m.plan.n_chunks, int(n_iter), m.kind, self.prec_id,
+                _ptr(self.table), _ptr(m.items), m.plan.n_chunks, int(n_iter), self.kind, self.prec_id,
                 ctypes.byref(h)), "dfea_pcg_graph_create")
             self._graphs[n_iter] = g = h
         return g
@@ -322,7 +332,7 @@ class HipSystem:
         if self.use_graphs if graph is None else graph:
             _lib.check(self.lib.dfea_graph_launch(self._graph(int(n_iter)), _stream()), "dfea_graph_launch")
             return
-        _lib.check(self.lib.dfea_pcg_iterate(_ptr(self.table), _ptr(m.items), m.plan.n_chunks, int(n_iter), m.kind,
+        _lib.check(self.lib.dfea_pcg_iterate(_ptr(self.table), _ptr(m.items), m.plan.n_chunks, int(n_iter), self.kind,
                                              self.prec_id, _stream()), "dfea_pcg_iterate")
 
     def release_graphs(self):

@@ -23,6 +23,7 @@ import torch
 
 from . import _lib
 from .device import HipMesh, HipSystem
+from .materials import material_spec
 
 TANGENT_OPERATORS = ("auto", "hip")
 PRECONDITIONERS = ("jacobi", "block_jacobi")
@@ -32,15 +33,8 @@ _MESHES = {}
 
 
 def _check_material(params):
-    """Neo-Hookean with theta scaling both Lame parameters -> (mu0, lam0) tensor."""
-    if "material" in params:
-        if params["material"] != "neo_hookean":
-            raise ValueError(f"the hip kernels implement the neo_hookean material, not {params['material']!r}")
-        sc = params.get("mat_scalable")
-        if sc is not None and not bool(torch.as_tensor(sc).all()):
-            raise ValueError("the hip kernels scale both neo-Hookean parameters by theta")
-        return torch.as_tensor(params["mat_p"], dtype=torch.float64).cpu()
-    return torch.tensor([float(params["mu0"]), float(params["lmbda0"])], dtype=torch.float64)
+    """The material of ``params`` as the kernels take it (id, nominal parameters, theta-scaled entries)."""
+    return material_spec(params)
 
 
 def get_mesh(params, device=None):

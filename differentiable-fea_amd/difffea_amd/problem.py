@@ -3,10 +3,12 @@
 The ``params`` keys are the data contract of the reference's solver functions
 (`src/difffea/operators.py:19-34`, `src/difffea/problem.py:180-187`):
 
-    nodes (N,2) f64, conn (n_el,nen) i64, GP (n_q,2), GW (n_q,), material, mat_p (2,), mat_scalable (2,),
+    nodes (N,2) f64, conn (n_el,nen) i64, GP (n_q,2), GW (n_q,), material, mat_p (n_p,), mat_scalable (n_p,),
     f (2N,), m (2N,) {0,1}, fixed (2N,) bool, u_D (2N,), element_type
 
-Only the neo-Hookean material and the quad4 / quad9 elements are on the hot path.  Geometries:
+Materials: every entry of the library (``spec["material"]``, parameters by name in
+``spec["material_params"]``, neo-Hookean from ``E``, ``nu`` by default).  Elements: quad4 / quad9.
+Geometries:
 
 * ``"rectangle"``: ``geom_params={"width","height"}`` and either ``nx, ny`` or ``h`` (cells ~ size/h),
   as the reference's structured mesher (`src/difffea/problem.py:95-109`);
@@ -22,6 +24,7 @@ from . import mesh as _mesh
 from .boundary_conditions import (boundary_edges, dofs_from_node_mask, edge_traction_load,
                                   element_edges, select_boundary_nodes, traction_order)
 from .elements import element_quadrature
+from .materials import material_vector
 
 
 def lame_parameters(E, nu):
@@ -80,16 +83,19 @@ def build_problem(spec):
         f = edge_traction_load(nodes, edges, traction_order(et), load_nodes, (float(value[0]), float(value[1])))
 
     m = torch.where(fixed, 0.0, 1.0).to(torch.float64)
-    if "material_params" in spec and spec["material_params"] is not None:
-        mu0, l0 = float(spec["material_params"]["mu"]), float(spec["material_params"]["lmbda"])
-    else:
+    # material (`src/difffea/problem.py:172-178`): neo-Hookean from E, nu unless parameters are given
+    material = spec.get("material", "neo_hookean")
+    values = spec.get("material_params")
+    if values is None and material == "neo_hookean":
         mu0, l0 = lame_parameters(float(spec.get("E", 1.0)), float(spec.get("nu", 0.3)))
+        values = {"mu": mu0, "lmbda": l0}
+    mat_p, mat_scalable = material_vector(material, values)
     return {
         "nodes": nodes, "conn": conn,
         "GP": torch.from_numpy(GP), "GW": torch.from_numpy(GW),
-        "material": "neo_hookean",
-        "mat_p": torch.tensor([mu0, l0], dtype=torch.float64),
-        "mat_scalable": torch.tensor([True, True]),
+        "material": material,
+        "mat_p": mat_p,
+        "mat_scalable": mat_scalable,
         "f": f.reshape(-1).contiguous(), "m": m, "fixed": fixed, "u_D": u_D,
         "element_type": et,
         **({"cell_ij": cell_ij} if cell_ij is not None else {}),

@@ -28,6 +28,7 @@ import torch
 
 from . import _lib
 from .device import HipMesh, HipSystem, _ptr, _stream
+from .materials import material_spec
 from .plan import MeshPlan
 from .problem import build_problem
 
@@ -211,6 +212,11 @@ def _kinds(jobs):
     return sorted({j.mesh.kind for j in jobs})
 
 
+def _mat_kind(jobs, nen):
+    """The kind argument of the material kernels for a group: with the library flag when any job needs it."""
+    return nen | (_lib.MATLIB if any(j.sys.mat.library for j in jobs) else 0)
+
+
 class Server:
     """Continuous-batching solver for a stream of requests on one GPU (see module docstring)."""
 
@@ -275,9 +281,8 @@ class Server:
         slot = self.free.pop()
         params, theta, plan = prepared if prepared is not None else self.prepare(req)
         mesh = HipMesh(params, self.device, plan=plan)
-        mp = params["mat_p"]
-        sys_ = HipSystem(mesh, theta, mp, params["f"], self.precision, uD=params["u_D"], sc=self.sc[slot],
-                         preconditioner=self.preconditioner)
+        sys_ = HipSystem(mesh, theta, material_spec(params), params["f"], self.precision, uD=params["u_D"],
+                         sc=self.sc[slot], preconditioner=self.preconditioner)
         sys_._ensure_moduli_buffers()
         self.table[slot * self.dsz:(slot + 1) * self.dsz].copy_(
             torch.frombuffer(bytearray(bytes(sys_.desc)), dtype=torch.uint8))
@@ -368,7 +373,8 @@ class Server:
                 if not n:
                     continue
                 fi, nf = self._dev_items(sub, "fin")
-                _lib.check(lib.dfea_residual(T, _ptr(it), n, _ptr(fi), nf, None, None, -1.0, nen, st), "dfea_residual")
+                _lib.check(lib.dfea_residual(T, _ptr(it), n, _ptr(fi), nf, None, None, -1.0, _mat_kind(sub, nen), st),
+                           "dfea_residual")
             it, n = self._dev_items(js, "vec")
             si, ns = self._dev_items(js, "solve")
             _lib.check(lib.dfea_dot(T, _ptr(it), n, _ptr(si), ns, None, None, st), "dfea_dot")
@@ -379,7 +385,8 @@ class Server:
             it, n = self._dev_items(sub, "chunk")
             if n:
                 fi, nf = self._dev_items(sub, "fin")
-                _lib.check(lib.dfea_moduli(T, _ptr(it), n, _ptr(fi), nf, None, nen, self.prec_id, st), "dfea_moduli")
+                _lib.check(lib.dfea_moduli(T, _ptr(it), n, _ptr(fi), nf, None, _mat_kind(sub, nen), self.prec_id, st),
+                           "dfea_moduli")
         # PCG start (new solves) then a chunk of iterations for every running solve
         js = by.get("pcg", []) + by.get("adjoint", [])
         new = [j for j in js if not getattr(j, "pcg_running", False)]

@@ -1,6 +1,6 @@
 /*
  * dfea.h -- C ABI of libdfea_hip.so, the MI355X (gfx950) matrix-free Newton-Krylov hot path of
- * Differentiable-FEA (plane-strain compressible neo-Hookean, quad4 / quad9 elements).
+ * Differentiable-FEA (plane-strain hyperelastic material library, quad4 / quad9 elements).
  *
  * Conventions
  *   - Every entry point returns 0 on success, DFEA_EINVAL (-1) for an invalid argument, or
@@ -9,7 +9,8 @@
  *     synchronises the stream (callers read the scalar block, e.g. the PCG status, when they choose).
  *   - The caller allocates and owns every buffer.  Descriptors (dfea_desc) hold device pointers into those
  *     buffers; a descriptor table in device memory describes a batch of independent solves.
- *   - Element kind (`nen` arguments): 4 (quad4), 9 (quad9), or 4 | DFEA_HALO for quad4 halo plans.
+ *   - Element kind (`nen` arguments): 4 (quad4), 9 (quad9), or 4 | DFEA_HALO for quad4 halo plans;
+ *     | DFEA_MATLIB when a descriptor's material is not neo-Hookean.
  *   - Work items: an int32 pair {solve index, block index} per workgroup; for a batch, the concatenation
  *     over solves.  Any item table may be NULL for one solve: solve 0 (the table's first descriptor),
  *     blocks 0 .. n_items-1, which saves the workgroups one dependent load.
@@ -40,8 +41,19 @@ extern "C" {
 #define DFEA_FIXED_Y 0x80000000u
 #define DFEA_NODE_MASK 0x3fffffffu
 #define DFEA_HALO 0x100        /* element-kind flag: nen | DFEA_HALO selects the halo-plan kernels */
+#define DFEA_MATLIB 0x200      /* element-kind flag: the descriptors may use any DFEA_MAT_* material; without it
+                                  the material kernels compile only the neo-Hookean closed forms (fewer
+                                  registers) and give NaN for a descriptor of another material */
 
 enum { DFEA_FP64 = 0, DFEA_FP32 = 1 };
+/* dfea_desc.material: the energies of `src/difffea/materials.py:65-162`, parameters in that file's order
+ * (neo-Hookean [mu, lmbda]; Mooney-Rivlin [c10, c01, kappa]; Saint Venant-Kirchhoff [mu, lmbda]; Demiray
+ * [mu, gamma, lmbda]; fibre-reinforced HGO [mu, lmbda, k1, k2, beta_deg, d, second_family]) */
+enum {
+    DFEA_MAT_NEO_HOOKEAN = 0, DFEA_MAT_MOONEY_RIVLIN = 1, DFEA_MAT_SAINT_VENANT = 2, DFEA_MAT_DEMIRAY = 3,
+    DFEA_MAT_FIBER_REINFORCED = 4
+};
+#define DFEA_MAT_NP 8          /* parameter slots of dfea_desc.mat_p */
 /* dfea_desc.precond: point Jacobi (`operators.py:477-570`) or nodal block Jacobi (`operators.py:588-638`) */
 enum { DFEA_PRECOND_JACOBI = 0, DFEA_PRECOND_BLOCK = 1 };
 
@@ -118,12 +130,15 @@ typedef struct dfea_desc {
     double* sc;                 /* [DFEA_NSC] */
     void* vec[DFEA_NVEC];       /* PCG / Newton vectors; x,r,z,p,Ap,b in the solve's precision; u (evaluation
                                    point), R, u_acc (accepted state), u_D (prescribed values) FP64 */
-    double mu0, lam0;
+    double mat_p[DFEA_MAT_NP];  /* nominal material parameters; entry k is multiplied by theta_e when bit k of
+                                   mat_scal is set (`materials.py:239-246`) */
     int32_t n_chunks, n_lnodes, n_shared, n_part;
     int32_t n_el, n_nodes, nen, nq;
     int32_t precision, n_finblk, n_vblk, n_halo;
     int32_t n_cols;            /* plane length n_el + n_halo */
     int32_t precond;           /* DFEA_PRECOND_JACOBI or DFEA_PRECOND_BLOCK */
+    int32_t material;          /* DFEA_MAT_* */
+    int32_t mat_scal;          /* bit k: mat_p[k] is theta-scaled */
 } dfea_desc;
 
 /* ---- library ---------------------------------------------------------------------------------------- */
@@ -168,7 +183,7 @@ int dfea_residual(const dfea_desc* table, const int32_t* items, int n_items, con
  * runs.  u NULL -> vec[DFEA_U].  Replaces `src/difffea/postprocess.py:121-137` (support_reactions). */
 int dfea_reactions(const dfea_desc* table, const int32_t* items, int n_items, const int32_t* solve_items,
                    int n_solves, const double* u, double* out, int nen, void* stream);
-/* Element and nodal fields of u (neo-Hookean, plane strain), items of one solve.
+/* Element and nodal fields of u (plane strain, the descriptor's material), items of one solve.
  * eout [7][n_el], original element order: area, then the area-weighted means of J, Cauchy sxx, syy, sxy, the
  * von Mises stress of the mean stress (with the plane-strain szz) and the energy density psi.
  * nout [6][N]: J, sxx, syy, sxy, mises, psi summed over the elements at each node and divided by their number
@@ -232,8 +247,9 @@ int dfea_graph_launch(void* graph, void* stream);
 int dfea_graph_destroy(void* graph);
 
 /* ---- adjoint -------------------------------------------------------------------------------------------- */
-/* g[e] = -sum_{a,i} (m*lam)[2 conn(e,a)+i] * R^e_int[a,i](u; mu0, lam0), original element order.
- * dL/dtheta for neo-Hookean theta-scaling.  Replaces `adjoint.py:75-106` (compute_implicit_vjp). */
+/* g[e] = -sum_{a,i} (m*lam)[2 conn(e,a)+i] * R^e_int[a,i](u; mat_p), original element order: dL/dtheta,
+ * since every library energy is linear in its theta-scaled parameters (R^e_int(u; p(theta)) =
+ * theta_e R^e_int(u; mat_p)).  Replaces `adjoint.py:75-106` (compute_implicit_vjp). */
 int dfea_adjoint_grad(const dfea_desc* table, const int32_t* items, int n_items, const double* u,
                       const double* lam, double* g, int nen, void* stream);
 

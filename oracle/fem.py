@@ -10,6 +10,10 @@ It restates, in its own code, the reference's algorithms step for step (file:lin
 
 How each function is pinned (tests/test_oracle_*.py):
   * geometry               -- affine/rectangle closed forms, shape-gradient finite differences, mesh area
+  * library energies       -- stress-free reference, objectivity, plane-strain closed forms (Mooney-Rivlin
+                              I2 = J^2 + F:F, SVK through tr C^2 = (tr C)^2 - 2 det C), limits (Demiray
+                              gamma -> 0 and HGO k1 -> 0 are neo-Hookean), fibres slack in compression,
+                              P and C against central differences, R_int = grad(total energy)
   * psi / P / C            -- closed forms P = mu F + (lam lnJ - mu) F^-T and
                               C = mu d_ik d_jl + lam G_ij G_kl + (mu - lam lnJ) G_il G_kj, linear-elastic
                               tensor at F = I, objectivity, stress-free reference state
@@ -86,7 +90,7 @@ def get_geometry(params):
 
 
 # ---------------------------------------------------------------------------------------------------------
-# material                                                          (materials.py:37-73, 239-246)
+# material                                                          (materials.py:37-246)
 # ---------------------------------------------------------------------------------------------------------
 def psi_neo_hookean(F, p):
     """Plane-strain compressible neo-Hookean energy through the 3-D embedding F3 = [[F,0],[0,1]].
@@ -100,6 +104,82 @@ def psi_neo_hookean(F, p):
     return 0.5 * mu * (I1 - 3.0 - 2.0 * logJ) + 0.5 * lam * logJ ** 2
 
 
+def embed_plane_strain(F):
+    """F (2,2) -> F3 = [[F, 0], [0, 1]] (`materials.py:37-48`)."""
+    z, o = torch.zeros_like(F[0, 0]), torch.ones_like(F[0, 0])
+    return torch.stack([torch.stack([F[0, 0], F[0, 1], z]), torch.stack([F[1, 0], F[1, 1], z]),
+                        torch.stack([z, z, o])])
+
+
+def _det3(F3):
+    """det F3 of the block embedding, written out so second derivatives stay finite (`materials.py:51-54`)."""
+    return (F3[0, 0] * F3[1, 1] - F3[0, 1] * F3[1, 0]) * F3[2, 2]
+
+
+def psi3_neo_hookean(F3, p):
+    """mu/2 (I1 - 3 - 2 ln J) + lam/2 (ln J)^2, p = [mu, lmbda] (`materials.py:65-73`)."""
+    mu, lam = p[0], p[1]
+    logJ = torch.log(_det3(F3))
+    return 0.5 * mu * (torch.sum(F3 * F3) - 3.0 - 2.0 * logJ) + 0.5 * lam * logJ ** 2
+
+
+def psi3_mooney_rivlin(F3, p):
+    """c10 (J^-2/3 I1 - 3) + c01 (J^-4/3 I2 - 3) + kappa/2 (ln J)^2, I2 = (I1^2 - tr C^2)/2
+    (`materials.py:76-93`)."""
+    c10, c01, kappa = p[0], p[1], p[2]
+    C = F3.T @ F3
+    I1 = torch.trace(C)
+    I2 = 0.5 * (I1 ** 2 - torch.trace(C @ C))
+    J = _det3(F3)
+    return c10 * (J ** (-2.0 / 3.0) * I1 - 3.0) + c01 * (J ** (-4.0 / 3.0) * I2 - 3.0) + 0.5 * kappa * torch.log(J) ** 2
+
+
+def psi3_saint_venant(F3, p):
+    """lam/2 (tr E)^2 + mu tr(E^2), E = (C - I)/2 (`materials.py:96-104`)."""
+    mu, lam = p[0], p[1]
+    E = 0.5 * (F3.T @ F3 - torch.eye(3, dtype=F3.dtype))
+    return 0.5 * lam * torch.trace(E) ** 2 + mu * torch.sum(E * E)
+
+
+def psi3_demiray(F3, p):
+    """mu/(2 gamma) (exp(gamma (I1 - 3)) - 1) - mu ln J + lam/2 (ln J)^2 (`materials.py:107-119`)."""
+    mu, gamma, lam = p[0], p[1], p[2]
+    I1 = torch.sum(F3 * F3)
+    logJ = torch.log(_det3(F3))
+    return mu / (2.0 * gamma) * (torch.exp(gamma * (I1 - 3.0)) - 1.0) - mu * logJ + 0.5 * lam * logJ ** 2
+
+
+def psi3_fiber_reinforced(F3, p):
+    """Neo-Hookean matrix + fibre families a = (cos b, +-sin b, 0): k1/(2 k2) (exp(k2 <E>^2) - 1),
+    E = d I1 + (1 - 3d) a.C a - 1, <.> = max(., 0) (`materials.py:122-162`)."""
+    k1, k2, beta, d, second = p[2], p[3], torch.deg2rad(p[4]), p[5], p[6]
+    C = F3.T @ F3
+    I1 = torch.trace(C)
+    z = torch.zeros_like(beta)
+
+    def family(a):
+        E = d * I1 + (1.0 - 3.0 * d) * (a @ C @ a) - 1.0
+        return k1 / (2.0 * k2) * (torch.exp(k2 * torch.relu(E) ** 2) - 1.0)
+
+    a1 = torch.stack([torch.cos(beta), torch.sin(beta), z])
+    a2 = torch.stack([torch.cos(beta), -torch.sin(beta), z])
+    return psi3_neo_hookean(F3, p[:2]) + family(a1) + second * family(a2)
+
+
+PSI3 = {"neo_hookean": psi3_neo_hookean, "mooney_rivlin": psi3_mooney_rivlin, "saint_venant": psi3_saint_venant,
+        "demiray": psi3_demiray, "fiber_reinforced": psi3_fiber_reinforced}
+
+
+def material_psi(params):
+    """The plane-strain energy psi(F2, p) of the problem's material (`materials.py:199-205`); the neo-Hookean
+    one in its 2-D form above."""
+    name = params.get("material", "neo_hookean")
+    if name == "neo_hookean":
+        return psi_neo_hookean
+    psi3 = PSI3[name]
+    return lambda F, p: psi3(embed_plane_strain(F), p)
+
+
 def element_material_parameters(theta, params):
     """(n_el, 2): nominal [mu, lam] with the scalable entries (both, for neo-Hookean) times theta_e."""
     p0 = params["mat_p"].to(theta.dtype)
@@ -111,10 +191,10 @@ def element_material_parameters(theta, params):
 # ---------------------------------------------------------------------------------------------------------
 # residual                                                          (operators.py:131-220)
 # ---------------------------------------------------------------------------------------------------------
-def elem_residual_precomputed(ue, dN_dX, w_detJ, p):
+def elem_residual_precomputed(ue, dN_dX, w_detJ, p, psi=psi_neo_hookean):
     """R^e[a,i] = sum_q w_q detJ_q P_ij(F_q) dN_a/dX_j with P = grad psi (automatic differentiation)."""
     F = torch.eye(2, dtype=ue.dtype) + torch.einsum("ai,qaj->qij", ue, dN_dX)
-    P = vmap(grad(psi_neo_hookean), in_dims=(0, None))(F, p)
+    P = vmap(grad(psi), in_dims=(0, None))(F, p)
     return torch.einsum("q,qij,qaj->ai", w_detJ, P, dN_dX)
 
 
@@ -125,7 +205,8 @@ def global_residual(u, theta, params, load_factor=1.0):
     ue = u.reshape(N, 2)[conn]
     dN_dX, w_detJ = get_geometry(params)
     pe = element_material_parameters(theta, params)
-    Re = vmap(elem_residual_precomputed)(ue, dN_dX, w_detJ, pe)
+    psi = material_psi(params)
+    Re = vmap(lambda a, b, c, d: elem_residual_precomputed(a, b, c, d, psi))(ue, dN_dX, w_detJ, pe)
     Rint = torch.zeros(N, 2, dtype=u.dtype).index_add_(0, conn.reshape(-1), Re.reshape(-1, 2))
     return params["m"] * (Rint.reshape(-1) - load_factor * params["f"])
 
@@ -153,7 +234,7 @@ def qp_setup(u, theta, params):
     pe = element_material_parameters(theta, params)
     ue = u.reshape(N, 2)[conn]
     F = torch.eye(2, dtype=u.dtype) + torch.einsum("eai,eqaj->eqij", ue, dN_dX)
-    H = vmap(vmap(hessian(psi_neo_hookean), in_dims=(0, None)), in_dims=(0, 0))(F, pe)
+    H = vmap(vmap(hessian(material_psi(params)), in_dims=(0, None)), in_dims=(0, 0))(F, pe)
     return (H * w_detJ[:, :, None, None, None, None]).reshape(n_el, n_q, 4, 4), dN_dX
 
 
@@ -238,8 +319,10 @@ def compute_jacobi_diagonal(u, theta, params):
     dN_dX, w_detJ = get_geometry(params)
     basis = torch.eye(2 * nen, dtype=u.dtype).reshape(-1, nen, 2)
 
+    psi = material_psi(params)
+
     def entry(u_e, dN_e, w_e, p_e, e_vec):
-        _, Je = jvp(lambda w: elem_residual_precomputed(w, dN_e, w_e, p_e), (u_e,), (e_vec,))
+        _, Je = jvp(lambda w: elem_residual_precomputed(w, dN_e, w_e, p_e, psi), (u_e,), (e_vec,))
         return (Je * e_vec).sum()
 
     def one(u_e, dN_e, w_e, p_e):
@@ -270,8 +353,10 @@ def element_tangent_blocks(u, theta, params):
     dN_dX, w_detJ = get_geometry(params)
     units = torch.eye(2 * nen, dtype=u.dtype).reshape(-1, nen, 2)
 
+    psi = material_psi(params)
+
     def jacobian(u_e, dN_e, w_e, p_e):
-        res = lambda w: elem_residual_precomputed(w, dN_e, w_e, p_e)
+        res = lambda w: elem_residual_precomputed(w, dN_e, w_e, p_e, psi)
         cols = vmap(lambda e: jvp(res, (u_e,), (e,))[1])(units)      # [(b, j), a, i]
         cols = cols.reshape(nen, 2, nen, 2)                           # [b, j, a, i]
         return torch.diagonal(cols, dim1=0, dim2=2).permute(2, 1, 0)  # [a, i, j], a == b
@@ -431,28 +516,15 @@ def compute_sensitivity(params, theta, loss_fn, n_load_steps=5, tol=1e-8, max_ne
 # ---------------------------------------------------------------------------------------------------------
 # post-processing                                                   (postprocess.py:24-137, materials.py:37-73)
 # ---------------------------------------------------------------------------------------------------------
-def embed_plane_strain(F):
-    """F (2,2) -> F3 = [[F, 0], [0, 1]] (`materials.py:37-48`)."""
-    z, o = torch.zeros_like(F[0, 0]), torch.ones_like(F[0, 0])
-    return torch.stack([torch.stack([F[0, 0], F[0, 1], z]), torch.stack([F[1, 0], F[1, 1], z]),
-                        torch.stack([z, z, o])])
-
-
-def psi3_neo_hookean(F3, p):
-    """The 3-D neo-Hookean energy with det F3 = det(F2) F33 written out (`materials.py:51-73`)."""
-    mu, lam = p[0], p[1]
-    logJ = torch.log((F3[0, 0] * F3[1, 1] - F3[0, 1] * F3[1, 0]) * F3[2, 2])
-    return 0.5 * mu * (torch.sum(F3 * F3) - 3.0 - 2.0 * logJ) + 0.5 * lam * logJ ** 2
-
-
-def gauss_point_fields(F, p):
+def gauss_point_fields(F, p, psi3=None):
     """[J, sxx, syy, sxy, szz, psi] at one Gauss point: P3 = grad psi3 (automatic differentiation),
     sigma = P3 F3^T / J (`postprocess.py:24-48`)."""
+    psi3 = psi3 or psi3_neo_hookean
     F3 = embed_plane_strain(F)
-    P3 = grad(psi3_neo_hookean)(F3, p)
+    P3 = grad(psi3)(F3, p)
     J = F[0, 0] * F[1, 1] - F[0, 1] * F[1, 0]
     sig = P3 @ F3.T / J
-    return torch.stack([J, sig[0, 0], sig[1, 1], sig[0, 1], sig[2, 2], psi3_neo_hookean(F3, p)])
+    return torch.stack([J, sig[0, 0], sig[1, 1], sig[0, 1], sig[2, 2], psi3(F3, p)])
 
 
 def compute_fields(u, theta, params):
@@ -464,7 +536,8 @@ def compute_fields(u, theta, params):
     dN_dX, w_detJ = get_geometry(params)
     F = torch.eye(2, dtype=u.dtype) + torch.einsum("eai,eqaj->eqij", un[conn], dN_dX)
     pe = element_material_parameters(theta, params)
-    S = vmap(vmap(gauss_point_fields, in_dims=(0, None)))(F, pe)          # (n_el, n_q, 6)
+    psi3 = PSI3[params.get("material", "neo_hookean")]
+    S = vmap(vmap(lambda F_, p_: gauss_point_fields(F_, p_, psi3), in_dims=(0, None)))(F, pe)   # (n_el, n_q, 6)
     area = w_detJ.sum(1)
     J, sxx, syy, sxy, szz, psi = (torch.einsum("eq,eq->e", w_detJ, S[:, :, k]) / area for k in range(6))
     mises = torch.sqrt(0.5 * ((sxx - syy) ** 2 + (syy - szz) ** 2 + (szz - sxx) ** 2) + 3.0 * sxy ** 2)

@@ -12,10 +12,11 @@ def unit_square(element_type="quad4", n=8, mode="displacement", value=(0.1, None
     return build_problem(spec)
 
 
-def cantilever(element_type="quad4", h=0.5, load=(0.0, -40.0), E=2600.0, nu=0.3):
-    """The reference's small test problem: 2x1 rectangle, clamped left, consistent force on the right."""
+def cantilever(element_type="quad4", h=0.5, load=(0.0, -40.0), E=2600.0, nu=0.3, **kw):
+    """The reference's small test problem: 2x1 rectangle, clamped left, consistent force on the right
+    (other materials with their library defaults, as the reference's helper builds them)."""
     spec = dict(element_type=element_type, geometry="rectangle", geom_params=dict(width=2.0, height=1.0), h=h,
-                E=E, nu=nu, load_mode="force", load_value=load, fixed_side="left", load_side="right")
+                E=E, nu=nu, load_mode="force", load_value=load, fixed_side="left", load_side="right", **kw)
     return build_problem(spec)
 
 
