@@ -1,7 +1,7 @@
 // libdfea_hip: gfx950 kernels and the C ABI (include/dfea.h) of the matrix-free Newton-Krylov hot path.
 //
 // Execution model
-//   * Elements are grouped into chunks (<= CHUNK_Q4 / CHUNK_Q9 elements, <= MAXLN distinct nodes).  One
+//   * Elements are grouped into chunks (<= CHUNK_LIN / CHUNK_QUAD elements, <= MAXLN distinct nodes).  One
 //     workgroup handles one chunk: it gathers the chunk's nodal values into LDS, each thread evaluates one
 //     element from structure-of-arrays moduli / geometry planes (coalesced HBM reads), and the element
 //     vectors are summed into LDS node accumulators colour by colour (elements of one colour share no
@@ -30,15 +30,24 @@
 
 namespace {
 
-constexpr int CHUNK_Q4 = 256;     // core elements per chunk for quad4
-constexpr int CHUNK_Q9 = 64;      // core elements per chunk for quad9
-constexpr int BS_Q4 = 256;        // workgroup size for quad4 partials plans
+constexpr int CHUNK_LIN = 256;    // core elements per chunk for the linear elements (tri3, quad4)
+constexpr int CHUNK_QUAD = 64;    // core elements per chunk for the quadratic elements (tri6, quad8, quad9)
+constexpr int BS_LIN = 256;       // workgroup size for linear elements, partials plans
 constexpr int BS_Q4H = 320;       // workgroup size for quad4 halo plans: core + halo elements (16x16 + 33)
-constexpr int BS_Q9 = 64;         // workgroup size for quad9
-// workgroup size of the quad9 product and PCG kernels: Gauss-point groups of 64 threads (2 in FP64, where
-// registers limit the waves per SIMD, 3 in FP32; measured fastest on 200x200 and the 64x256 O-grid)
+constexpr int BS_QUAD = 64;       // workgroup size for quadratic elements
+// workgroup size of the quadratic elements' product and PCG kernels: Gauss-point groups of 64 threads (2 in
+// FP64, where registers limit the waves per SIMD, 3 in FP32; measured fastest for quad9 on 200x200 and on the
+// 64x256 O-grid)
 template <typename T>
-constexpr int bs_q9() { return std::is_same<T, double>::value ? 128 : 192; }
+constexpr int bs_groups() { return std::is_same<T, double>::value ? 128 : 192; }
+
+// element types by node count: tri3 (3), quad4 (4), tri6 (6), quad8 (8), quad9 (9)
+template <int NEN>
+constexpr bool quadratic() { return NEN > 4; }
+// Gauss points of each type's rule (`src/difffea/elements.py:38-57`): tri3 1 (centroid), quad4 2x2, tri6 6
+// (degree 4), quad8 and quad9 3x3
+template <int NEN>
+constexpr int nq_of() { return NEN == 3 ? 1 : NEN == 4 ? 4 : NEN == 6 ? 6 : 9; }
 // PCG launches of at most this many workgroups (one per CU or fewer: latency-bound) load the first node's
 // state before the scalar step
 constexpr int PCG_EARLY_ITEMS = 256;
@@ -340,7 +349,7 @@ __device__ __forceinline__ void colour_accumulate(const ChunkView& ch, int t, co
 // plans, whose workgroups are not a multiple of the chunk, always use G = 1.)
 template <int NEN, int BS>
 struct Groups {
-    static constexpr int CE = NEN == 4 ? CHUNK_Q4 : CHUNK_Q9;
+    static constexpr int CE = quadratic<NEN>() ? CHUNK_QUAD : CHUNK_LIN;
     static constexpr int G = BS % CE == 0 ? BS / CE : 1;
 };
 
@@ -1474,13 +1483,18 @@ int launch(K kern, int grid, int block, void* stream, const char* name, A... arg
 
 // element kind of a launch: nen, with DFEA_HALO set for halo plans (DFEA_MATLIB selects the instantiations
 // with the whole material library and is not part of the kind)
-enum class Kind { Q4, Q4H, Q9, Bad };
+enum class Kind { T3, Q4, Q4H, T6, Q8, Q9, Bad };
 Kind kind_of(int k) {
     k &= ~DFEA_MATLIB;
-    if (k == 4) return Kind::Q4;
-    if (k == (4 | DFEA_HALO)) return Kind::Q4H;
-    if (k == 9) return Kind::Q9;
-    return Kind::Bad;
+    switch (k) {
+        case 3: return Kind::T3;
+        case 4: return Kind::Q4;
+        case 4 | DFEA_HALO: return Kind::Q4H;
+        case 6: return Kind::T6;
+        case 8: return Kind::Q8;
+        case 9: return Kind::Q9;
+        default: return Kind::Bad;
+    }
 }
 
 // call f.template run<NEN, BS, LIB>() for the kind (one instantiation per element type, plan mode and, for
@@ -1488,10 +1502,15 @@ Kind kind_of(int k) {
 template <bool LIB, typename F>
 int dispatch_lib(int kind, F f) {
     switch (kind_of(kind)) {
-        case Kind::Q4: return f.template run<4, BS_Q4, LIB>();
+        case Kind::T3: return f.template run<3, BS_LIN, LIB>();
+        case Kind::Q4: return f.template run<4, BS_LIN, LIB>();
         case Kind::Q4H: return f.template run<4, BS_Q4H, LIB>();
-        case Kind::Q9: return f.template run<9, BS_Q9, LIB>();
-        default: return fail(DFEA_EINVAL, "unknown element kind (expected 4, 9 or 4 | DFEA_HALO, optionally | DFEA_MATLIB)");
+        case Kind::T6: return f.template run<6, BS_QUAD, LIB>();
+        case Kind::Q8: return f.template run<8, BS_QUAD, LIB>();
+        case Kind::Q9: return f.template run<9, BS_QUAD, LIB>();
+        default:
+            return fail(DFEA_EINVAL, "unknown element kind (expected 3, 4, 6, 8, 9 or 4 | DFEA_HALO, optionally "
+                                     "| DFEA_MATLIB)");
     }
 }
 
@@ -1500,16 +1519,16 @@ int dispatch(int kind, F f) {
     return (kind & DFEA_MATLIB) ? dispatch_lib<true>(kind, f) : dispatch_lib<false>(kind, f);
 }
 
-// PCG kernels in pairs of parities 0, 1 (n_iter rounded up to even).  Quad9 chunks (64 elements, up to 289
-// nodes) run with bs_q9 threads: Gauss-point groups per element, node updates on all lanes.
+// PCG kernels in pairs of parities 0, 1 (n_iter rounded up to even).  Quadratic-element chunks (64 elements,
+// up to 289 nodes) run with bs_groups threads: Gauss-point groups per element, node updates on all lanes.
 template <int NEN, int BS, typename T>
 int pcg_iter(const dfea_desc* table, const int32_t* items, int n_items, int n_iter, void* stream) {
-    constexpr int BSP = NEN == 9 ? bs_q9<T>() : BS;
+    constexpr int BSP = quadratic<NEN>() ? bs_groups<T>() : BS;
     const bool early = n_items <= PCG_EARLY_ITEMS;
     for (int i = 0; i < n_iter + (n_iter & 1); ++i) {
-        const int rc = early ? launch(k_pcg<NEN, NEN, BSP, T, 2, 1, true>, n_items, BSP, stream, "k_pcg", table, items,
+        const int rc = early ? launch(k_pcg<NEN, nq_of<NEN>(), BSP, T, 2, 1, true>, n_items, BSP, stream, "k_pcg", table, items,
                                       i & 1)
-                             : launch(k_pcg<NEN, NEN, BSP, T, 2, 1, false>, n_items, BSP, stream, "k_pcg", table,
+                             : launch(k_pcg<NEN, nq_of<NEN>(), BSP, T, 2, 1, false>, n_items, BSP, stream, "k_pcg", table,
                                       items, i & 1);
         if (rc) return rc;
     }
@@ -1521,15 +1540,15 @@ int pcg_iter(const dfea_desc* table, const int32_t* items, int n_items, int n_it
 struct Launch_geometry {
     const dfea_desc* t; const int32_t* it; int n; void* st;
     template <int NEN, int BS, bool LIB> int run() {
-        return launch(k_geometry<NEN, NEN, BS>, n, BS, st, "k_geometry", t, it);
+        return launch(k_geometry<NEN, nq_of<NEN>(), BS>, n, BS, st, "k_geometry", t, it);
     }
 };
 
 struct Launch_moduli {
     const dfea_desc* t; const int32_t* it; int n; const int32_t* fi; int nf; const double* u; int prec; void* st;
     template <int NEN, int BS, bool LIB> int run() {
-        int rc = prec == DFEA_FP64 ? launch(k_moduli<NEN, NEN, BS, double, LIB>, n, BS, st, "k_moduli", t, it, u)
-                                   : launch(k_moduli<NEN, NEN, BS, float, LIB>, n, BS, st, "k_moduli", t, it, u);
+        int rc = prec == DFEA_FP64 ? launch(k_moduli<NEN, nq_of<NEN>(), BS, double, LIB>, n, BS, st, "k_moduli", t, it, u)
+                                   : launch(k_moduli<NEN, nq_of<NEN>(), BS, float, LIB>, n, BS, st, "k_moduli", t, it, u);
         if (rc) return rc;
         return prec == DFEA_FP64 ? launch(k_moduli_fin<double>, nf, FBS, st, "k_moduli_fin", t, fi)
                                  : launch(k_moduli_fin<float>, nf, FBS, st, "k_moduli_fin", t, fi);
@@ -1540,7 +1559,7 @@ struct Launch_residual {
     const dfea_desc* t; const int32_t* it; int n; const int32_t* fi; int nf; const double* u; double* R;
     double lf; void* st;
     template <int NEN, int BS, bool LIB> int run() {
-        int rc = launch(k_residual<NEN, NEN, BS, LIB>, n, BS, st, "k_residual", t, it, u, R, lf);
+        int rc = launch(k_residual<NEN, nq_of<NEN>(), BS, LIB>, n, BS, st, "k_residual", t, it, u, R, lf);
         if (rc) return rc;
         return launch(k_residual_fin, nf, FBS, st, "k_residual_fin", t, fi, R, lf);
     }
@@ -1549,7 +1568,7 @@ struct Launch_residual {
 struct Launch_reactions {
     const dfea_desc* t; const int32_t* it; int n; const int32_t* si; int ns; const double* u; double* out; void* st;
     template <int NEN, int BS, bool LIB> int run() {
-        int rc = launch(k_residual<NEN, NEN, BS, LIB, true>, n, BS, st, "k_residual<react>", t, it, u, (double*)nullptr, 1.0);
+        int rc = launch(k_residual<NEN, nq_of<NEN>(), BS, LIB, true>, n, BS, st, "k_residual<react>", t, it, u, (double*)nullptr, 1.0);
         if (rc) return rc;
         return launch(k_reactions_final, ns, VBS, st, "k_reactions_final", t, si, out);
     }
@@ -1559,7 +1578,7 @@ struct Launch_fields {
     const dfea_desc* t; const int32_t* it; int n; const int32_t* fi; int nf; const double* u; double* eout;
     double* nout; double* work; void* st;
     template <int NEN, int BS, bool LIB> int run() {
-        int rc = launch(k_fields<NEN, NEN, BS, LIB>, n, BS, st, "k_fields", t, it, u, eout, nout, work);
+        int rc = launch(k_fields<NEN, nq_of<NEN>(), BS, LIB>, n, BS, st, "k_fields", t, it, u, eout, nout, work);
         if (rc) return rc;
         return launch(k_fields_fin, nf, FBS, st, "k_fields_fin", t, fi, nout, (const double*)work);
     }
@@ -1569,10 +1588,10 @@ struct Launch_apply {
     const dfea_desc* t; const int32_t* it; int n; const int32_t* fi; int nf; const void* v; void* y; int prec;
     void* st;
     template <typename T, int NEN, int BS0> int go() {
-        constexpr int BS = NEN == 9 ? bs_q9<T>() : BS0;
+        constexpr int BS = quadratic<NEN>() ? bs_groups<T>() : BS0;
         const T* vv = static_cast<const T*>(v);
         T* yy = static_cast<T*>(y);
-        int rc = launch(k_apply<NEN, NEN, BS, T>, n, BS, st, "k_apply", t, it, vv, yy);
+        int rc = launch(k_apply<NEN, nq_of<NEN>(), BS, T>, n, BS, st, "k_apply", t, it, vv, yy);
         if (rc) return rc;
         return launch(k_apply_fin<T>, nf, FBS, st, "k_apply_fin", t, fi, vv, yy);
     }
@@ -1592,7 +1611,7 @@ struct Launch_pcg_iterate {
 struct Launch_adjoint {
     const dfea_desc* t; const int32_t* it; int n; const double* u; const double* lam; double* g; void* st;
     template <int NEN, int BS, bool LIB> int run() {
-        return launch(k_adjoint<NEN, NEN, BS, LIB>, n, BS, st, "k_adjoint", t, it, u, lam, g);
+        return launch(k_adjoint<NEN, nq_of<NEN>(), BS, LIB>, n, BS, st, "k_adjoint", t, it, u, lam, g);
     }
 };
 
@@ -1632,13 +1651,19 @@ int dfea_desc_offset(int field) {
 int dfea_red_size(int n_chunks, int n_finblk, int n_vblk) { return 6 * n_chunks + n_finblk + 3 * n_vblk; }
 int dfea_vblk(int n_dof) { return (n_dof + VBLK - 1) / VBLK; }
 int dfea_finblk(int n_shared) { return (n_shared + FBS - 1) / FBS; }
-int dfea_chunk_elems(int nen) { return (nen & ~DFEA_HALO) == 4 ? CHUNK_Q4 : nen == 9 ? CHUNK_Q9 : -1; }
+int dfea_chunk_elems(int nen) {
+    switch (kind_of(nen)) {
+        case Kind::T3: case Kind::Q4: case Kind::Q4H: return CHUNK_LIN;
+        case Kind::T6: case Kind::Q8: case Kind::Q9: return CHUNK_QUAD;
+        default: return -1;
+    }
+}
 int dfea_chunk_nodes(void) { return MAXLN; }
 int dfea_chunk_threads(int nen) {
     switch (kind_of(nen)) {
-        case Kind::Q4: return BS_Q4;
+        case Kind::T3: case Kind::Q4: return BS_LIN;
         case Kind::Q4H: return BS_Q4H;
-        case Kind::Q9: return BS_Q9;
+        case Kind::T6: case Kind::Q8: case Kind::Q9: return BS_QUAD;
         default: return -1;
     }
 }

@@ -16,7 +16,7 @@ import torch
 
 from . import _lib
 from .materials import MaterialSpec
-from .elements import element_type_of, reference_gradient_table
+from .elements import NEN, default_nq, element_type_of, reference_gradient_table
 from .plan import MeshPlan
 
 F64, F32 = torch.float64, torch.float32
@@ -51,9 +51,10 @@ class HipMesh:
         GP = params["GP"].detach().to("cpu", F64).numpy()
         GW = params["GW"].detach().to("cpu", F64).numpy()
         self.nq = GP.shape[0]
-        if (self.nen, self.nq) not in ((4, 4), (9, 9)):
-            raise ValueError(f"the hip kernels support quad4 with 2x2 and quad9 with 3x3 Gauss points, "
-                             f"got nen={self.nen}, n_q={self.nq}")
+        if self.nen != NEN[self.element_type] or self.nq != default_nq(self.element_type):
+            raise ValueError(f"the hip kernels take {self.element_type} elements with {NEN[self.element_type]} "
+                             f"nodes and a {default_nq(self.element_type)}-point rule, got nen={self.nen}, "
+                             f"n_q={self.nq}")
         cij = params.get("cell_ij")
         # plan: a MeshPlan of exactly these tables, built beforehand (e.g. while the GPU was busy)
         self.plan = p = plan if plan is not None else MeshPlan(

@@ -1,9 +1,12 @@
 """Structured meshes for the hot-path workloads (host-side set-up, CPU).
 
-* ``create_2d_quad_mesh`` / ``create_2d_quad9_mesh``: rectangle [0,Lx]x[0,Ly] with the reference's node
-  numbering (x-major node ids ``i*(ny+1)+j``, elements ``i*ny+j``, CCW corners; Q9 on the stride-2 grid,
-  corners -> mid-sides -> centre), `src/difffea/mesh.py:19-185`.  Coordinates come from ``torch.linspace``
-  exactly as in the reference so that node coordinates agree bit for bit.
+* ``create_2d_quad_mesh`` / ``create_2d_quad9_mesh`` / ``create_2d_quad8_mesh`` / ``create_2d_tri3_mesh`` /
+  ``create_2d_tri6_mesh``: rectangle [0,Lx]x[0,Ly] with the reference's node numbering (x-major node ids
+  ``i*(ny+1)+j``, elements ``i*ny+j``, CCW corners; Q9 on the stride-2 grid, corners -> mid-sides -> centre;
+  Q8 the same grid without the cell centres, renumbered consecutively; triangles split every cell along the
+  diagonal from its top-left to its bottom-right corner, all lower-left triangles first, then all upper-right
+  ones), `src/difffea/mesh.py:19-421`.  Coordinates come from ``torch.linspace`` exactly as in the reference
+  so that node coordinates agree bit for bit.
 * ``create_plate_hole_ogrid_q9``: square [-w/2,w/2]^2 with a centred circular hole as a structured O-grid
   of Q9 elements, linear blending between the circle and the square boundary along each ray.  This mesh
   has no counterpart in the reference (which meshes curved geometries with Gmsh).
@@ -42,13 +45,49 @@ def create_2d_quad9_mesh(Lx, Ly, nx, ny, dtype=torch.float64):
     return nodes, conn
 
 
+def create_2d_quad8_mesh(Lx, Ly, nx, ny, dtype=torch.float64):
+    """Serendipity Q8 mesh: the Q9 node grid without the cell centres (odd, odd grid points), nodes renumbered
+    consecutively in grid order; element nodes = corners (CCW), then mid-sides bottom, right, top, left."""
+    nodes9, conn9 = create_2d_quad9_mesh(Lx, Ly, nx, ny, dtype=dtype)
+    gnx, gny = 2 * nx, 2 * ny
+    keep = torch.ones(gnx + 1, gny + 1, dtype=torch.bool)
+    keep[1:gnx:2, 1:gny:2] = False
+    keep = keep.flatten()
+    new_id = torch.full((keep.numel(),), -1, dtype=torch.int64)
+    new_id[keep] = torch.arange(int(keep.sum()))
+    return nodes9[keep], new_id[conn9[:, :8]]
+
+
+def create_2d_tri3_mesh(Lx, Ly, nx, ny, dtype=torch.float64):
+    """Tri3 mesh: each Q4 cell (n0, n1, n2, n3) CCW from bottom-left is cut along n3-n1 into the lower-left
+    triangle (n0, n1, n3) and the upper-right triangle (n1, n2, n3), both CCW; all lower-left triangles (in cell
+    order) come first, then all upper-right ones."""
+    nodes, q = create_2d_quad_mesh(Lx, Ly, nx, ny, dtype=dtype)
+    lower = q[:, [0, 1, 3]]
+    upper = q[:, [1, 2, 3]]
+    return nodes, torch.cat([lower, upper], dim=0)
+
+
+def create_2d_tri6_mesh(Lx, Ly, nx, ny, dtype=torch.float64):
+    """Tri6 mesh: each Q9 cell cut along its top-left / bottom-right diagonal, whose mid-point is the cell
+    centre.  Lower-left triangle: corners (1, 2, 4), mid-sides (5, 9, 8); upper-right: corners (2, 3, 4),
+    mid-sides (6, 7, 9) in the Q9 cell's 1-based node numbers; lower-left triangles first."""
+    nodes, q = create_2d_quad9_mesh(Lx, Ly, nx, ny, dtype=dtype)
+    lower = q[:, [0, 1, 3, 4, 8, 7]]
+    upper = q[:, [1, 2, 3, 5, 6, 8]]
+    return nodes, torch.cat([lower, upper], dim=0)
+
+
+_BUILDERS = {"quad4": create_2d_quad_mesh, "quad8": create_2d_quad8_mesh, "quad9": create_2d_quad9_mesh,
+             "tri3": create_2d_tri3_mesh, "tri6": create_2d_tri6_mesh}
+
+
 def create_rectangle_mesh(element_type, Lx, Ly, nx, ny, dtype=torch.float64):
-    """Dispatcher over the hot-path element types (`src/difffea/mesh.py:424-438`)."""
-    if element_type == "quad4":
-        return create_2d_quad_mesh(Lx, Ly, nx, ny, dtype=dtype)
-    if element_type == "quad9":
-        return create_2d_quad9_mesh(Lx, Ly, nx, ny, dtype=dtype)
-    raise ValueError(f"Unknown element type '{element_type}'. Choose one of ['quad4', 'quad9'].")
+    """Structured mesh of [0,Lx]x[0,Ly] with nx x ny cells of any element type; triangle meshes have 2 nx ny
+    elements (`src/difffea/mesh.py:424-438`)."""
+    if element_type not in _BUILDERS:
+        raise ValueError(f"Unknown element type '{element_type}'. Choose one of {list(_BUILDERS)}.")
+    return _BUILDERS[element_type](Lx, Ly, nx, ny, dtype=dtype)
 
 
 def create_plate_hole_ogrid_q9(width=1.0, radius=0.2, n_rad=64, n_circ=256, dtype=torch.float64):

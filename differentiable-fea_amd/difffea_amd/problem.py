@@ -7,7 +7,8 @@ The ``params`` keys are the data contract of the reference's solver functions
     f (2N,), m (2N,) {0,1}, fixed (2N,) bool, u_D (2N,), element_type
 
 Materials: every entry of the library (``spec["material"]``, parameters by name in
-``spec["material_params"]``, neo-Hookean from ``E``, ``nu`` by default).  Elements: quad4 / quad9.
+``spec["material_params"]``, neo-Hookean from ``E``, ``nu`` by default).  Elements: tri3, quad4, tri6,
+quad8, quad9.
 Geometries:
 
 * ``"rectangle"``: ``geom_params={"width","height"}`` and either ``nx, ny`` or ``h`` (cells ~ size/h),

@@ -1,6 +1,6 @@
 """Served solve stream with continuous batching on one GPU.
 
-Requests (meshes of uneven size, quad4 / quad9, own material and load, optionally a compliance gradient)
+Requests (meshes of uneven size, any element type, own material and load, optionally a compliance gradient)
 arrive over time.  The server admits them as slots and HBM budget free up, keeps every solve's state on
 the device, and advances all admitted solves together in *rounds*: in one round every solve performs its
 next device operation (lift, residual + norm, moduli + diagonal, PCG start, a chunk of PCG iterations,
@@ -28,6 +28,7 @@ import torch
 
 from . import _lib
 from .device import HipMesh, HipSystem, _ptr, _stream
+from .elements import FAMILY, NEN, ORDER, default_nq
 from .materials import material_spec
 from .plan import MeshPlan
 from .problem import build_problem
@@ -111,10 +112,10 @@ def reference_residual_norm(sys_, params, device):
 def footprint_bytes(element_type, n, precision, preconditioner="jacobi"):
     """HBM bytes a request holds while admitted (geometry, moduli, vectors, preconditioner, plan), an upper
     estimate."""
-    nen, nq = (4, 4) if element_type == "quad4" else (9, 9)
-    n_el = n * n
-    side = n + 1 if nen == 4 else 2 * n + 1
-    n_dof = 2 * side * side
+    nen, nq = NEN[element_type], default_nq(element_type)
+    n_el = (2 if FAMILY[element_type] == "tri" else 1) * n * n
+    side = n + 1 if ORDER[element_type] == 1 else 2 * n + 1
+    n_dof = 2 * (side * side - (n * n if element_type == "quad8" else 0))
     s = 8 if precision == "fp64" else 4
     per_qp = 2 * nen * 12 + 8 + 10 * s
     pre = n_dof * s // 2 if preconditioner == "block_jacobi" else 0     # the off-diagonal inverse entries

@@ -1,6 +1,6 @@
 /*
  * dfea.h -- C ABI of libdfea_hip.so, the MI355X (gfx950) matrix-free Newton-Krylov hot path of
- * Differentiable-FEA (plane-strain hyperelastic material library, quad4 / quad9 elements).
+ * Differentiable-FEA (plane-strain hyperelastic material library; tri3, quad4, tri6, quad8 and quad9 elements).
  *
  * Conventions
  *   - Every entry point returns 0 on success, DFEA_EINVAL (-1) for an invalid argument, or
@@ -9,8 +9,10 @@
  *     synchronises the stream (callers read the scalar block, e.g. the PCG status, when they choose).
  *   - The caller allocates and owns every buffer.  Descriptors (dfea_desc) hold device pointers into those
  *     buffers; a descriptor table in device memory describes a batch of independent solves.
- *   - Element kind (`nen` arguments): 4 (quad4), 9 (quad9), or 4 | DFEA_HALO for quad4 halo plans;
- *     | DFEA_MATLIB when a descriptor's material is not neo-Hookean.
+ *   - Element kind (`nen` arguments): the nodes per element, 3 (tri3, 1 Gauss point), 4 (quad4, 2x2),
+ *     6 (tri6, 6-point degree-4 rule), 8 (quad8, 3x3) or 9 (quad9, 3x3), with nq of the descriptors equal to
+ *     that rule's size; 4 | DFEA_HALO for quad4 halo plans; | DFEA_MATLIB when a descriptor's material is
+ *     not neo-Hookean.
  *   - Work items: an int32 pair {solve index, block index} per workgroup; for a batch, the concatenation
  *     over solves.  Any item table may be NULL for one solve: solve 0 (the table's first descriptor),
  *     blocks 0 .. n_items-1, which saves the workgroups one dependent load.

@@ -51,7 +51,31 @@ def shape_fn_quad9(xi, eta):
                         a[1] * b[0], a[2] * b[1], a[1] * b[2], a[0] * b[1], a[1] * b[1]])
 
 
-SHAPE_FN = {"quad4": shape_fn_quad4, "quad9": shape_fn_quad9}
+def shape_fn_quad8(xi, eta):
+    """Serendipity Q8: corners with the (xi_a xi + eta_a eta - 1) factor, then mid-sides bottom, right, top,
+    left (shape_functions_2d.py:100-129)."""
+    corners = [(-1.0, -1.0), (1.0, -1.0), (1.0, 1.0), (-1.0, 1.0)]
+    N = [0.25 * (1 + a * xi) * (1 + b * eta) * (a * xi + b * eta - 1) for a, b in corners]
+    N += [0.5 * (1 - xi ** 2) * (1 - eta), 0.5 * (1 + xi) * (1 - eta ** 2),
+          0.5 * (1 - xi ** 2) * (1 + eta), 0.5 * (1 - xi) * (1 - eta ** 2)]
+    return torch.stack(N)
+
+
+def shape_fn_tri3(xi, eta):
+    """Linear triangle: the barycentric coordinates (1 - xi - eta, xi, eta) (shape_functions_2d.py:177-195)."""
+    return torch.stack([1 - xi - eta, xi, eta])
+
+
+def shape_fn_tri6(xi, eta):
+    """Quadratic triangle: corners L(2L - 1), mid-sides of edges 1-2, 2-3, 3-1 as 4 L_i L_j
+    (shape_functions_2d.py:198-230)."""
+    L1, L2, L3 = 1 - xi - eta, xi, eta
+    return torch.stack([L1 * (2 * L1 - 1), L2 * (2 * L2 - 1), L3 * (2 * L3 - 1), 4 * L1 * L2, 4 * L2 * L3,
+                        4 * L3 * L1])
+
+
+SHAPE_FN = {"quad4": shape_fn_quad4, "quad8": shape_fn_quad8, "quad9": shape_fn_quad9,
+            "tri3": shape_fn_tri3, "tri6": shape_fn_tri6}
 
 
 def gauss_rule_2d(n):
@@ -61,6 +85,38 @@ def gauss_rule_2d(n):
     p, w = torch.tensor(p, dtype=DT), torch.tensor(w, dtype=DT)
     GP = torch.stack([p.repeat(n), p.repeat_interleave(n)], dim=1)
     return GP, torch.outer(w, w).reshape(-1)
+
+
+def gauss_rule_triangle(degree):
+    """Rule on the unit triangle exact to total degree ``degree`` (gauss_quadratures.py:100-160): centroid (1),
+    three interior points (2), the two-orbit 6-point rule with the reference's 15-digit constants (3-4), else
+    the collapsed rule xi = s, eta = t (1 - s) from Gauss-Legendre points on [0, 1], s-major.  Weights sum to
+    1/2."""
+    import math
+    import numpy as np
+    if degree <= 1:
+        pts, wts = [[1 / 3, 1 / 3]], [0.5]
+    elif degree == 2:
+        pts, wts = [[1 / 6, 1 / 6], [2 / 3, 1 / 6], [1 / 6, 2 / 3]], [1 / 6] * 3
+    elif degree <= 4:
+        pts, wts = [], []
+        for a, w in ((0.445948490915965, 0.223381589678011), (0.091576213509771, 0.109951743655322)):
+            pts += [[a, a], [1 - 2 * a, a], [a, 1 - 2 * a]]
+            wts += [w / 2] * 3
+    else:
+        n = math.ceil((degree + 2) / 2)
+        g, w = np.polynomial.legendre.leggauss(n)
+        g, w = (g + 1) / 2, w / 2
+        pts = [[gs, gt * (1 - gs)] for gs in g for gt in g]
+        wts = [ws * wt * (1 - gs) for gs, ws in zip(g, w) for wt in w]
+    return torch.tensor(pts, dtype=DT), torch.tensor(wts, dtype=DT)
+
+
+def element_rule(element_type):
+    """Default rule of an element type (elements.py:38-93): 2x2 quad4, 3x3 quad8/quad9, degree 1 tri3,
+    degree 4 tri6."""
+    return {"quad4": lambda: gauss_rule_2d(2), "quad8": lambda: gauss_rule_2d(3), "quad9": lambda: gauss_rule_2d(3),
+            "tri3": lambda: gauss_rule_triangle(1), "tri6": lambda: gauss_rule_triangle(4)}[element_type]()
 
 
 def reference_gradients(element_type, GP):

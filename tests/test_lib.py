@@ -33,5 +33,7 @@ def test_descriptor_layout_matches_ctypes_mirror():
 def test_size_queries():
     lib = _lib.load()
     assert lib.dfea_vblk(1024) == 1 and lib.dfea_vblk(1025) == 2
-    assert lib.dfea_chunk_elems(4) > 0 and lib.dfea_chunk_elems(9) > 0 and lib.dfea_chunk_elems(3) == -1
+    assert [lib.dfea_chunk_elems(n) for n in (3, 4, 6, 8, 9)] == [256, 256, 64, 64, 64]
+    assert [lib.dfea_chunk_threads(n) for n in (3, 4, 6, 8, 9)] == [256, 256, 64, 64, 64]
+    assert lib.dfea_chunk_elems(5) == -1 and lib.dfea_chunk_threads(7) == -1
     assert lib.dfea_red_size(3, 2, 1) == 6 * 3 + 2 + 3

@@ -28,6 +28,10 @@ MESHES = {
     "q4_cant": lambda: cantilever("quad4", h=0.05),
     "q9_ogrid": lambda: _ogrid(),
     "q9_ogrid_generic": lambda: {k: v for k, v in _ogrid().items() if k != "cell_ij"},
+    "t3_30": lambda: unit_square("tri3", 30),
+    "t6_13": lambda: unit_square("tri6", 13),
+    "q8_17": lambda: unit_square("quad8", 17),
+    "t6_cant": lambda: cantilever("tri6", h=0.1),
 }
 
 

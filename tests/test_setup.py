@@ -13,21 +13,35 @@ from difffea_amd.boundary_conditions import (boundary_edges, dofs_from_node_mask
 from difffea_amd.mesh import create_plate_hole_ogrid_q9, create_rectangle_mesh
 from difffea_amd.problem import build_problem
 
-ETS = ["quad4", "quad9"]
+ETS = ["quad4", "quad8", "quad9", "tri3", "tri6"]
+# nodes per element and element count per cell of the structured meshes
+NEN = {"quad4": 4, "quad8": 8, "quad9": 9, "tri3": 3, "tri6": 6}
+PER_CELL = {"quad4": 1, "quad8": 1, "quad9": 1, "tri3": 2, "tri6": 2}
+
+
+def _n_nodes(et, nx, ny):
+    if et in ("quad4", "tri3"):
+        return (nx + 1) * (ny + 1)
+    return (2 * nx + 1) * (2 * ny + 1) - (nx * ny if et == "quad8" else 0)
 
 
 @pytest.mark.parametrize("et", ETS)
 def test_mesh_sizes_and_counter_clockwise_elements(et):
     nodes, conn = create_rectangle_mesh(et, 2.0, 1.0, 3, 2)
-    side = (4, 3) if et == "quad4" else (7, 5)
-    assert nodes.shape == (side[0] * side[1], 2) and conn.shape == (6, 4 if et == "quad4" else 9)
-    X = nodes[conn[:, :4]].numpy()                       # corners, counter-clockwise
+    nc = 3 if et.startswith("tri") else 4
+    assert nodes.shape == (_n_nodes(et, 3, 2), 2) and conn.shape == (6 * PER_CELL[et], NEN[et])
+    assert torch.equal(torch.unique(conn), torch.arange(nodes.shape[0]))    # every node used, ids consecutive
+    X = nodes[conn[:, :nc]].numpy()                      # corners, counter-clockwise
     area = 0.5 * np.sum(X[:, :, 0] * np.roll(X[:, :, 1], -1, 1) - np.roll(X[:, :, 0], -1, 1) * X[:, :, 1], axis=1)
-    assert np.allclose(area, 2.0 / 6)                    # every cell positive, total = the rectangle
-    if et == "quad9":                                    # mid-side and centre nodes at the midpoints
-        mids = nodes[conn[:, 4:8]].numpy()
+    assert np.allclose(area, 2.0 / (6 * PER_CELL[et]))   # every element positive, total = the rectangle
+    if NEN[et] > nc:                                     # mid-side nodes at the edge midpoints, in edge order
+        mids = nodes[conn[:, nc:2 * nc]].numpy()
         assert np.allclose(mids, 0.5 * (X + np.roll(X, -1, 1)))
+    if et == "quad9":
         assert np.allclose(nodes[conn[:, 8]].numpy(), X.mean(axis=1))
+    if et.startswith("tri"):                             # lower-left triangles first: (n0, n1, n3) of each cell
+        qn, q = create_rectangle_mesh("quad4", 2.0, 1.0, 3, 2)
+        assert np.allclose(X[:6], qn[q[:, [0, 1, 3]]].numpy()) and np.allclose(X[6:], qn[q[:, [1, 2, 3]]].numpy())
 
 
 @pytest.mark.parametrize("et", ETS)
@@ -35,6 +49,7 @@ def test_boundary_edges_lie_on_the_boundary(et):
     nodes, conn = create_rectangle_mesh(et, 1.0, 1.0, 3, 2)
     edges = boundary_edges(conn, element_edges(et))
     assert edges.shape[0] == 2 * (3 + 2)                 # one boundary edge per cell side on the perimeter
+    assert edges.shape[1] == (2 if et in ("quad4", "tri3") else 3)
     xy = nodes[edges].numpy()
     on = (xy[..., 0] < 1e-12) | (xy[..., 0] > 1 - 1e-12) | (xy[..., 1] < 1e-12) | (xy[..., 1] > 1 - 1e-12)
     assert on.all()
@@ -51,7 +66,8 @@ def test_consistent_load_total_force_and_support(et):
 
 
 def test_edge_load_split_linear_and_quadratic():
-    for et, expect in (("quad4", [3.0, 3.0]), ("quad9", [1.0, 1.0, 4.0])):
+    for et, expect in (("quad4", [3.0, 3.0]), ("quad9", [1.0, 1.0, 4.0]), ("quad8", [1.0, 1.0, 4.0]),
+                       ("tri3", [3.0, 3.0]), ("tri6", [1.0, 1.0, 4.0])):
         nodes, conn = create_rectangle_mesh(et, 1.0, 1.0, 1, 1)
         edges = boundary_edges(conn, element_edges(et))
         mask = select_boundary_nodes(nodes, "right")
