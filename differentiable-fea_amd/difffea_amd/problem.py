@@ -15,6 +15,11 @@ Geometries:
   as the reference's structured mesher (`src/difffea/problem.py:95-109`);
 * ``"plate_hole_ogrid"``: ``geom_params={"width","radius","n_rad","n_circ"}``, quad9 only.
 
+Meshers (``spec["mesher"]``): ``"structured"`` (default; the geometries above), ``"unstructured"`` (Delaunay
+mesher of ``mesh.create_unstructured_plate_hole`` for ``"rectangle"`` and ``"plate_hole"``
+(``geom_params={"width","height","radius"}``), edge length ``h``, seed ``mesh_seed``) and ``"npz"`` (a mesh
+file in the Gmsh exchange format, ``spec["mesh_file"]``, cleaned and made counter-clockwise on load).
+
 Supports and loads follow `src/difffea/problem.py:115-194`: clamped side, prescribed displacement
 (``load_mode="displacement"``, components may be ``None``) or a consistent total edge force
 (``load_mode="force"``).
@@ -39,6 +44,23 @@ def build_mesh(spec):
     et = spec["element_type"]
     geom = spec.get("geometry", "rectangle")
     gp = spec.get("geom_params", {})
+    mesher = spec.get("mesher", "structured")
+    if mesher == "npz":
+        nodes, conn = _mesh.load_mesh_npz(spec["mesh_file"], et)
+        return nodes, conn, None
+    if mesher == "unstructured":
+        if geom not in ("rectangle", "plate_hole"):
+            raise ValueError("the unstructured mesher knows the geometries 'rectangle' and 'plate_hole'")
+        W, H = float(gp.get("width", 1.0)), float(gp.get("height", 1.0))
+        r = float(gp.get("radius", 0.2)) if geom == "plate_hole" else 0.0
+        nodes, conn = _mesh.create_unstructured_plate_hole(et, W, H, r, float(spec.get("h", 0.05)),
+                                                           int(spec.get("mesh_seed", 0)))
+        return nodes, conn, None
+    if mesher == "gmsh":
+        raise ValueError("Gmsh is not part of this build: mesh with Gmsh elsewhere and load its .npz output "
+                         "with mesher='npz', or use mesher='structured' / 'unstructured'")
+    if mesher != "structured":
+        raise ValueError(f"Unknown mesher '{mesher}'. Choose 'structured', 'unstructured' or 'npz'.")
     if geom == "rectangle":
         W, H = float(gp.get("width", 1.0)), float(gp.get("height", 1.0))
         if "nx" in spec:

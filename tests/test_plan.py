@@ -22,6 +22,13 @@ def _ogrid(n_rad=6, n_circ=16):
                               E=1.0, nu=0.3, load_mode="force", load_value=(0.1, 0.0)))
 
 
+def _unstructured(et, h):
+    """Delaunay plate with a hole (irregular node valence, no cell structure)."""
+    return build_problem(dict(element_type=et, mesher="unstructured", geometry="plate_hole",
+                              geom_params=dict(width=1.0, height=1.0, radius=0.2), h=h, E=1.0, nu=0.3,
+                              load_mode="force", load_value=(0.1, 0.0)))
+
+
 MESHES = {
     "q4_37": lambda: unit_square("quad4", 37),
     "q9_19": lambda: unit_square("quad9", 19),
@@ -32,6 +39,9 @@ MESHES = {
     "t6_13": lambda: unit_square("tri6", 13),
     "q8_17": lambda: unit_square("quad8", 17),
     "t6_cant": lambda: cantilever("tri6", h=0.1),
+    "q4_unstructured": lambda: _unstructured("quad4", 0.03),
+    "t3_unstructured": lambda: _unstructured("tri3", 0.03),
+    "q9_unstructured": lambda: _unstructured("quad9", 0.06),
 }
 
 
