@@ -145,6 +145,37 @@ def time_postprocess(p, u, theta, steps=200):
     return out
 
 
+def element_products(steps=200):
+    """FP64 A(v) of the other element types and of unstructured meshes at about the headline's size:
+    structured tri3 283x283 (160,178 triangles), tri6 141x141 (39,762), quad8 200x200 (40,000), and Delaunay
+    plates with a hole r = 0.2 (irregular node valence): quad4 h = 0.0025 (209,811 quadrilaterals) and tri3
+    h = 0.0025 (279,801 triangles).  State: 5 % stretch + N(0,(1e-3 h)^2) noise; B_alg as the headline."""
+    from difffea_amd.problem import build_problem
+    out = {}
+    cases = (("tri3_283", dict(element_type="tri3", nx=283, ny=283), 1 / 283),
+             ("tri6_141", dict(element_type="tri6", nx=141, ny=141), 1 / 141),
+             ("quad8_200", dict(element_type="quad8", nx=200, ny=200), 1 / 200),
+             ("quad4_unstructured_h0.0025", dict(element_type="quad4", mesher="unstructured", geometry="plate_hole",
+                                                 h=0.0025), 0.0025),
+             ("tri3_unstructured_h0.0025", dict(element_type="tri3", mesher="unstructured", geometry="plate_hole",
+                                                h=0.0025), 0.0025))
+    for name, spec, h in cases:
+        p = build_problem(dict(spec, E=1.0, nu=0.3, load_mode="displacement", load_value=(0.1, None),
+                               fixed_side="left", load_side="right"))
+        g = torch.Generator().manual_seed(0)
+        X = p["nodes"]
+        u = torch.zeros_like(X)
+        u[:, 0] = 0.05 * X[:, 0]
+        u = u.reshape(-1) + 1e-3 * h * torch.randn(u.numel(), dtype=torch.float64, generator=g)
+        v = torch.randn(u.numel(), dtype=torch.float64, generator=torch.Generator().manual_seed(1))
+        th = torch.ones(p["conn"].shape[0], dtype=torch.float64)
+        r = time_apply(p, u, v, th, "fp64", steps, 10)
+        out[name] = dict(n_elements=int(p["conn"].shape[0]), n_dofs=int(u.numel()), us_per_product=round(r["ms"] * 1e3, 3),
+                         gbs=round(r["gbs"], 2), bytes_per_step=r["bytes"])
+        print(f"# {name} fp64 A(v): {out[name]}", file=sys.stderr, flush=True)
+    return out
+
+
 def cpu_baseline(p, u, v, theta, seconds=10.0):
     """The oracle's 'qp' restatement (gather, einsum, batched 4x4 matmul, einsum, index_add_), FP64."""
     from oracle import fem
@@ -289,6 +320,8 @@ def main():
     q9 = time_apply(p9, u9, v9, th9, "fp64", args.steps, args.warmup)
     print(f"# Q9 200x200 fp64 A(v): {q9['ms'] * 1e3:.2f} us, {q9['gbs']:.0f} GB/s", file=sys.stderr, flush=True)
 
+    elems = element_products()
+
     solves = None
     if not args.no_solve:
         solves = {}
@@ -353,6 +386,7 @@ def main():
         "q9_200x200_fp64": {"us_per_product": round(q9["ms"] * 1e3, 3), "gbs": round(q9["gbs"], 2),
                             "bytes_per_step": q9["bytes"]},
         "pcg_iteration_fp64": pcg64, "pcg_iteration_mixed": pcg32, "postprocess_q4_400": post,
+        "element_products_fp64": elems,
         "cpu_baseline": None if cpu is None else {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")},
         "solves": solves,
     }
