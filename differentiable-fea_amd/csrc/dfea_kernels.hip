@@ -376,6 +376,23 @@ __device__ __forceinline__ bool chunk_element(const dfea_desc& D, const ChunkVie
 
 // NT: nontemporal loads (the element data is streamed once per product; measured faster for the plain
 // product, slower inside the PCG kernel, whose node updates share the cache)
+// Loads through global-address-space pointers (global_load rather than flat_load: no LDS-aperture check, and
+// the wait counts of loads and LDS operations stay apart).  gload: element i of an array through a 32-bit byte
+// offset from a uniform base, the scalar-base + one-VGPR-offset form (the host keeps n_el and n_nodes below
+// 2^28, so i * sizeof(T) fits).  gload2: node pair i of a V2 array as one 2-wide vector load.
+template <typename T> using gptr = const __attribute__((address_space(1))) T*;
+template <bool NT, typename T>
+__device__ __forceinline__ T gload(const T* base, uint32_t i) {
+    gptr<T> p = (gptr<T>)((gptr<char>)(gptr<T>)base + i * (uint32_t)sizeof(T));
+    if constexpr (NT) return __builtin_nontemporal_load(p);
+    else return *p;
+}
+template <typename T>
+__device__ __forceinline__ V2<T> gload2(const V2<T>* base, uint32_t i) {
+    typedef T vec2 __attribute__((ext_vector_type(2)));
+    const vec2 w = *((gptr<vec2>)(gptr<V2<T>>)base + i);
+    return V2<T>{w.x, w.y};
+}
 template <bool NT, typename T>
 __device__ __forceinline__ T stream_load(const T* p) {
     if (NT) return __builtin_nontemporal_load(p);
@@ -385,13 +402,14 @@ __device__ __forceinline__ T stream_load(const T* p) {
 template <int NEN, int NQ, typename T, bool NT = false>
 __device__ __forceinline__ void load_gauss_point(const T* dN, const T* C, int ne, int e, int q, T (&d)[NEN][2],
                                                  T (&c)[10]) {
+    // plane base pointers are uniform (scalar registers); the only per-thread term is the element column e
 #pragma unroll
     for (int a = 0; a < NEN; ++a) {
-        d[a][0] = stream_load<NT>(&dN[(size_t)((q * NEN + a) * 2) * ne + e]);
-        d[a][1] = stream_load<NT>(&dN[(size_t)((q * NEN + a) * 2 + 1) * ne + e]);
+        d[a][0] = gload<NT>(dN + (size_t)((q * NEN + a) * 2) * ne, (uint32_t)e);
+        d[a][1] = gload<NT>(dN + (size_t)((q * NEN + a) * 2 + 1) * ne, (uint32_t)e);
     }
 #pragma unroll
-    for (int k = 0; k < 10; ++k) c[k] = stream_load<NT>(&C[(size_t)(k * NQ + q) * ne + e]);
+    for (int k = 0; k < 10; ++k) c[k] = gload<NT>(C + (size_t)(k * NQ + q) * ne, (uint32_t)e);
 }
 
 // the first PF Gauss points of the thread's group
@@ -416,7 +434,7 @@ struct GroupLDS {
 
 // fe = K_e (m v_e) from the gathered LDS values (the group's Gauss points, the groups added in order), then
 // the colour-ordered sum into L.acc.  d / c hold the group's first PF Gauss points on entry.
-template <int NEN, int NQ, int BS, int PF, typename T, bool NT = false>
+template <int NEN, int NQ, int BS, int PF, typename T, bool NT = false, bool AHEAD = false>
 __device__ __forceinline__ void element_product(const ChunkView& ch, int t, bool has_el, const int* l, const T* dN,
                                                 const T* C, int ne, int e,
                                                 T (&d)[GaussSplit<NQ, Groups<NEN, BS>::G>::QG][NEN][2],
@@ -441,7 +459,13 @@ __device__ __forceinline__ void element_product(const ChunkView& ch, int t, bool
         for (int j = 0; j < QG; ++j) {
             const int q = q0 + j;
             if (q >= q1) break;
-            if (j >= PF) load_gauss_point<NEN, NQ, T, NT>(dN, C, ne, e, q, d[j], c[j]);
+            // AHEAD: the Gauss point PF ahead is requested before this one is used, so PF points stay in flight
+            // (costs registers; used where they are plentiful)
+            if (AHEAD) {
+                if (j + PF < QG && q + PF < q1) load_gauss_point<NEN, NQ, T, NT>(dN, C, ne, e, q + PF, d[j + PF], c[j + PF]);
+            } else if (j >= PF) {
+                load_gauss_point<NEN, NQ, T, NT>(dN, C, ne, e, q, d[j], c[j]);
+            }
             T gr[4] = {0, 0, 0, 0};    // gr[2i+k] = sum_a ve[a][i] dN[a][k]
 #pragma unroll
             for (int a = 0; a < NEN; ++a) {
@@ -502,40 +526,80 @@ __global__ __launch_bounds__(BS, MINW) void k_apply(const dfea_desc* __restrict_
     const T* dN = std::is_same<T, double>::value ? (const T*)D.dN64 : (const T*)D.dN32;
     const T* C = std::is_same<T, double>::value ? (const T*)D.C64 : (const T*)D.C32;
 
-    // 0. element, first PF Gauss points
+    // 0. element (local node indices).  Partials plans: tail threads of the last chunk read the last element
+    // and drop the result, so every load below is unconditional (no branch splits the loads in flight).
+    constexpr int CE = Groups<NEN, BS>::CE;
     int l[NEN];
     int e;
-    const bool has_el = chunk_element<NEN, BS>(D, ch, it.b, t, e, l);
+    bool has_el;
+    if constexpr (BS % CE == 0) {
+        e = it.b * CE + t % CE;
+        has_el = e < D.n_el;
+        if (!has_el) e = D.n_el - 1;
+    } else {
+        has_el = thread_element<NEN>(D, ch, t, e, l);
+    }
     constexpr int QG = GaussSplit<NQ, Groups<NEN, BS>::G>::QG;
     T d[QG][NEN][2];
     T c[QG][10];
-    if (has_el) prefetch_gauss_points<NEN, NQ, BS, PF, T, true>(dN, C, ne, e, t, d, c);
 
-    // 1. gather
-    for (int k = t; k < ch.nk; k += BS) {
-        const uint32_t g = D.lnode[ch.k0 + k];
-        L.v[k] = reinterpret_cast<const V2<T>*>(v)[(int)(g & DFEA_NODE_MASK)];
-        L.acc[k] = V2<T>{0, 0};
-        L.bits[k] = (unsigned char)(g >> 30);
+    // 1. gather: node ids, then the nodal values; the element data of the first PF Gauss points is requested
+    // behind them, so the gather's loads return first
+    constexpr int NG = (MAXLN + BS - 1) / BS;
+    uint32_t gid[NG];
+    V2<T> gv[NG];
+#pragma unroll
+    for (int i = 0; i < NG; ++i) {
+        const int k = t + i * BS;
+        gid[i] = gload<false>(D.lnode, (uint32_t)(ch.k0 + (k < ch.nk ? k : ch.nk - 1)));
     }
+#pragma unroll
+    for (int i = 0; i < NG; ++i) gv[i] = gload2<T>(reinterpret_cast<const V2<T>*>(v), gid[i] & DFEA_NODE_MASK);
+    // partial slot of every shared entry the thread writes at the end (loaded now, used after the product)
+    int slot[NG];
+#pragma unroll
+    for (int i = 0; i < NG; ++i) {
+        const int k = t + i * BS;
+        const int ent = k >= ch.nown && k < ch.nk ? ch.pbase + (k - ch.nown) : 0;
+        slot[i] = D.n_part ? gload<false>(D.pslot, (uint32_t)ent) : 0;
+    }
+    if constexpr (BS % CE == 0) {
+#pragma unroll
+        for (int a = 0; a < NEN; ++a) l[a] = gload<false>(D.lelem, (uint32_t)(a * D.n_el + e));
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    prefetch_gauss_points<NEN, NQ, BS, PF, T, true>(dN, C, ne, e, t, d, c);
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int i = 0; i < NG; ++i) {
+        const int k = t + i * BS;
+        if (k < ch.nk) {
+            L.v[k] = gv[i];
+            L.acc[k] = V2<T>{0, 0};
+            L.bits[k] = (unsigned char)(gid[i] >> 30);
+        }
+    }
+    __builtin_amdgcn_sched_barrier(0);
     __syncthreads();
+    __builtin_amdgcn_sched_barrier(0);
 
     // 2. element products, colour-ordered sums
-    element_product<NEN, NQ, BS, PF, T, true>(ch, t, has_el, l, dN, C, ne, e, d, c, L, GL);
+    element_product<NEN, NQ, BS, PF, T, true, NEN <= 4>(ch, t, has_el, l, dN, C, ne, e, d, c, L, GL);
 
     // 3. owned nodes: final value; shared nodes (partials plans): partial
     V2<T>* part = reinterpret_cast<V2<T>*>(D.part);
-    for (int k = t; k < ch.nk; k += BS) {
+#pragma unroll
+    for (int i = 0; i < NG; ++i) {
+        const int k = t + i * BS;
         if (k < ch.nown) {
             const V2<T> acc = L.acc[k];
             const V2<T> raw = L.v[k];
-            const unsigned char b = L.bits[k];
             V2<T> out;
-            out.x = (b & 1) ? raw.x : acc.x;
-            out.y = (b & 2) ? raw.y : acc.y;
-            reinterpret_cast<V2<T>*>(y)[(int)(D.lnode[ch.k0 + k] & DFEA_NODE_MASK)] = out;
-        } else if (D.n_part) {
-            part[D.pslot[ch.pbase + (k - ch.nown)]] = L.acc[k];
+            out.x = (gid[i] & DFEA_FIXED_X) ? raw.x : acc.x;
+            out.y = (gid[i] & DFEA_FIXED_Y) ? raw.y : acc.y;
+            reinterpret_cast<V2<T>*>(y)[(int)(gid[i] & DFEA_NODE_MASK)] = out;
+        } else if (k < ch.nk && D.n_part) {
+            part[slot[i]] = L.acc[k];
         }
     }
 }

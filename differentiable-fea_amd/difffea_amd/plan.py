@@ -155,8 +155,9 @@ class MeshPlan:
             raise ValueError(f"plan mode must be 'auto', 'halo' or 'partials', got {mode!r}")
         if E <= 0:
             raise ValueError(f"no chunk size for elements with {nen} nodes")
-        if N >= (1 << 30):
-            raise ValueError("meshes with 2^30 or more nodes are not supported")
+        if N >= (1 << 28) or n_el >= (1 << 28):
+            raise ValueError("meshes with 2^28 or more nodes or elements are not supported (the kernels address "
+                             "element planes and node arrays through 32-bit byte offsets)")
         self.n_el, self.nen, self.n_nodes = n_el, nen, N
 
         # 1. order
