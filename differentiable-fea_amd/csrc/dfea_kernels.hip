@@ -1302,42 +1302,35 @@ __device__ __forceinline__ V2<T> precond_apply(V2<T> r, V2<T> d, T o, bool blk) 
     return V2<T>{r.x / d.x, r.y / d.y};
 }
 
+// the state of local node kk whose node id g, slot range rg and slot sl (shared entries) are already loaded
 template <typename T>
-__device__ __forceinline__ NodeState<T> load_node_state(const dfea_desc& D, const ChunkView& ch, int kk, bool upd,
-                                                        const PcgSet<T>& Rd, const V2<T>* dg, const T* dgo,
-                                                        const V2<T>* part_rd, const int2* prg) {
+__device__ __forceinline__ NodeState<T> node_state(const dfea_desc& D, const ChunkView& ch, int kk, uint32_t g,
+                                                   int2 rg, int sl, bool upd, const PcgSet<T>& Rd, const V2<T>* dg,
+                                                   const T* dgo, const V2<T>* part_rd) {
     NodeState<T> ns;
-    const uint32_t g = D.lnode[ch.k0 + kk];
     ns.n = (int)(g & DFEA_NODE_MASK);
     ns.b = (unsigned char)(g >> 30);
     const bool shared = kk >= ch.nown && D.n_part;
-    ns.writer = kk < ch.nown;
-    int2 ss = make_int2(0, 0);
-    if (shared) {                 // entry tables, loaded together with the node id
-        const int ent = ch.pbase + (kk - ch.nown);
-        ss = prg[ent];
-        ns.writer = ss.x == D.pslot[ent];
-    }
-    ns.d = dg[ns.n];
-    ns.o = dgo ? dgo[ns.n] : (T)0;
-    ns.r = Rd.r[ns.n];
+    ns.writer = shared ? rg.x == sl : kk < ch.nown;
+    ns.d = gload2<T>(dg, (uint32_t)ns.n);
+    ns.o = dgo ? gload<false>(dgo, (uint32_t)ns.n) : (T)0;
+    ns.r = gload2<T>(Rd.r, (uint32_t)ns.n);
     if (upd) {
-        if (!shared) {
-            ns.w = Rd.w[ns.n];
-        } else {
-            ns.w = sum_slots<T>(part_rd, ss.x, ss.y);
-        }
-        ns.x = Rd.x[ns.n];
-        ns.p = Rd.p[ns.n];
-        ns.s = Rd.s[ns.n];
+        ns.w = shared ? sum_slots<T>(part_rd, rg.x, rg.y) : gload2<T>(Rd.w, (uint32_t)ns.n);
+        ns.x = gload2<T>(Rd.x, (uint32_t)ns.n);
+        ns.p = gload2<T>(Rd.p, (uint32_t)ns.n);
+        ns.s = gload2<T>(Rd.s, (uint32_t)ns.n);
     }
     return ns;
 }
 
 // one PCG iteration (see above); par = launch parity; MINW: minimum waves per SIMD for the register budget.
-// EARLY: the state of the thread's first node is loaded before the scalar step, so its loads overlap the
-// reduction; it costs registers (waves per SIMD), which matters only when the launch has more workgroups
-// than the GPU has CUs.
+// Loads return in the order they are issued (waiting for one waits for those before it, not for those after),
+// so the kernel issues everything that does not depend on the scalars first, in the order it is needed: the
+// previous kernel's chunk partials (scalar step), the node ids and slot tables of the thread's nodes, the
+// element's local node indices, the first PF Gauss points.  EARLY: the state of the thread's first node is also
+// loaded before the scalar step, so its loads overlap the reduction; it costs registers (waves per SIMD),
+// which matters only when the launch has more workgroups than the GPU has CUs.
 template <int NEN, int NQ, int BS, typename T, int PF = 2, int MINW = 1, bool EARLY = false>
 __global__ __launch_bounds__(BS, MINW) void k_pcg(const dfea_desc* __restrict__ tab, const int* __restrict__ items, int par) {
     static_assert(3 * (BS / 64) <= 16, "block_sum3 needs 3 slots per wave in ChunkLDS::red");
@@ -1365,26 +1358,79 @@ __global__ __launch_bounds__(BS, MINW) void k_pcg(const dfea_desc* __restrict__ 
     const V2<T>* part_rd = reinterpret_cast<const V2<T>*>(D.part) + (size_t)(1 - par) * D.n_part;
     V2<T>* part_wr = reinterpret_cast<V2<T>*>(D.part) + (size_t)par * D.n_part;
     const int2* prg = reinterpret_cast<const int2*>(D.prange);
+    constexpr int CE = Groups<NEN, BS>::CE;
+    constexpr int NG = (MAXLN + BS - 1) / BS;     // local nodes per thread, at most
+    constexpr int NRD = 4;                        // chunk partials per thread loaded ahead of the scalar step
 
-    // element and its first Gauss points (independent of the scalars)
+    // 1a. the previous kernel's chunk partials {r.u, w.u, r.r} (index clamped: extra threads read chunk 0 and
+    // drop it)
+    const double* rd = pcg_red(D, 1 - par);
+    double pr[NRD][3];
+    if (k >= 0.0) {
+#pragma unroll
+        for (int j = 0; j < NRD; ++j) {
+            const int i = t + j * BS;
+            const uint32_t ic = (uint32_t)(i < D.n_chunks ? i : 0);
+#pragma unroll
+            for (int m = 0; m < 3; ++m) pr[j][m] = gload<false>(rd, 3u * ic + m);
+        }
+    }
+    // 1b. node ids, and for shared entries the slot range of the node and the chunk's own slot
+    uint32_t gid[NG];
+    int2 rng[NG];
+    int slt[NG];
+#pragma unroll
+    for (int i = 0; i < NG; ++i) {
+        const int kk = t + i * BS;
+        const int kc = kk < ch.nk ? kk : ch.nk - 1;
+        gid[i] = gload<false>(D.lnode, (uint32_t)(ch.k0 + kc));
+        rng[i] = make_int2(0, 0);
+        slt[i] = 0;
+        if (D.n_part) {
+            const uint32_t ent = (uint32_t)(kc >= ch.nown ? ch.pbase + (kc - ch.nown) : 0);
+            const V2<int> r2 = gload2<int>(reinterpret_cast<const V2<int>*>(prg), ent);
+            rng[i] = make_int2(r2.x, r2.y);
+            slt[i] = gload<false>(D.pslot, ent);
+        }
+    }
+    // 1c. element (partials plans: tail threads of the last chunk read the last element and drop it) and its
+    // first PF Gauss points
     int l[NEN];
     int e;
-    const bool has_el = chunk_element<NEN, BS>(D, ch, it.b, t, e, l);
+    bool has_el;
+    if constexpr (BS % CE == 0) {
+        e = it.b * CE + t % CE;
+        has_el = e < D.n_el;
+        if (!has_el) e = D.n_el - 1;
+#pragma unroll
+        for (int a = 0; a < NEN; ++a) l[a] = gload<false>(D.lelem, (uint32_t)(a * D.n_el + e));
+    } else {
+        has_el = thread_element<NEN>(D, ch, t, e, l);
+    }
     constexpr int QG = GaussSplit<NQ, Groups<NEN, BS>::G>::QG;
     T d[QG][NEN][2];
     T c[QG][10];
-    if (has_el) prefetch_gauss_points<NEN, NQ, BS, PF, T>(dN, C, ne, e, t, d, c);
+    __builtin_amdgcn_sched_barrier(0);
+    prefetch_gauss_points<NEN, NQ, BS, PF, T>(dN, C, ne, e, t, d, c);
+    __builtin_amdgcn_sched_barrier(0);
 
-    // the first node's state (independent of the scalars)
+    // 1d. the first node's state (independent of the scalars)
     NodeState<T> ns0;
-    if (EARLY && t < ch.nk) ns0 = load_node_state<T>(D, ch, t, k >= 0.0, Rd, dg, dgo, part_rd, prg);
+    if (EARLY && t < ch.nk) ns0 = node_state<T>(D, ch, t, gid[0], rng[0], slt[0], k >= 0.0, Rd, dg, dgo, part_rd);
 
-    // scalar step
+    // 2. scalar step
     double alpha = 0.0, beta = 0.0, gam = 0.0, den = 0.0, rr = 0.0;
     if (k >= 0.0) {
-        const double* rd = pcg_red(D, 1 - par);
         double a0 = 0.0, a1 = 0.0, a2 = 0.0;
-        for (int i = t; i < D.n_chunks; i += BS) {
+#pragma unroll
+        for (int j = 0; j < NRD; ++j) {
+            if (t + j * BS < D.n_chunks) {
+                a0 += pr[j][0];
+                a1 += pr[j][1];
+                a2 += pr[j][2];
+            }
+        }
+        for (int i = t + NRD * BS; i < D.n_chunks; i += BS) {
             a0 += rd[3 * i];
             a1 += rd[3 * i + 1];
             a2 += rd[3 * i + 2];
@@ -1440,11 +1486,15 @@ __global__ __launch_bounds__(BS, MINW) void k_pcg(const dfea_desc* __restrict__ 
         }
     }
 
-    // update of the chunk's nodes, u_{k+1} into LDS
+    // 3. update of the chunk's nodes, u_{k+1} into LDS
     double g_part = 0.0, r_part = 0.0;
-    for (int kk = t; kk < ch.nk; kk += BS) {
-        const bool first = EARLY && kk == t;
-        NodeState<T> ns = first ? ns0 : load_node_state<T>(D, ch, kk, k >= 0.0, Rd, dg, dgo, part_rd, prg);
+#pragma unroll
+    for (int i = 0; i < NG; ++i) {
+        const int kk = t + i * BS;
+        if (kk >= ch.nk) break;
+        NodeState<T> ns = (EARLY && i == 0) ? ns0
+                                            : node_state<T>(D, ch, kk, gid[i], rng[i], slt[i], k >= 0.0, Rd, dg, dgo,
+                                                            part_rd);
         const int n = ns.n;
         const unsigned char b = ns.b;
         V2<T> rn = ns.r;
@@ -1481,11 +1531,14 @@ __global__ __launch_bounds__(BS, MINW) void k_pcg(const dfea_desc* __restrict__ 
     }
     __syncthreads();
 
-    // w_{k+1} = A u_{k+1}
+    // 4. w_{k+1} = A u_{k+1}
     element_product<NEN, NQ, BS, PF, T>(ch, t, has_el, l, dN, C, ne, e, d, c, L, GL);
 
     double d_part = 0.0;
-    for (int kk = t; kk < ch.nk; kk += BS) {
+#pragma unroll
+    for (int i = 0; i < NG; ++i) {
+        const int kk = t + i * BS;
+        if (kk >= ch.nk) break;
         const V2<T> u = L.v[kk];
         const V2<T> acc = L.acc[kk];
         const unsigned char b = L.bits[kk];
@@ -1493,16 +1546,15 @@ __global__ __launch_bounds__(BS, MINW) void k_pcg(const dfea_desc* __restrict__ 
             V2<T> out;
             out.x = (b & 1) ? u.x : acc.x;
             out.y = (b & 2) ? u.y : acc.y;
-            Wr.w[(int)(D.lnode[ch.k0 + kk] & DFEA_NODE_MASK)] = out;
+            Wr.w[(int)(gid[i] & DFEA_NODE_MASK)] = out;
             d_part += (double)u.x * (double)out.x + (double)u.y * (double)out.y;
         } else if (D.n_part) {
-            const int ent = ch.pbase + (kk - ch.nown);
-            const int slot = D.pslot[ent];
+            const int slot = slt[i];
             part_wr[slot] = acc;
             if (!(b & 1)) d_part += (double)u.x * (double)acc.x;
             if (!(b & 2)) d_part += (double)u.y * (double)acc.y;
             if (b) {     // fixed components (w = u there) once, by the chunk holding the node's first slot
-                if (slot == prg[ent].x) {
+                if (slot == rng[i].x) {
                     if (b & 1) d_part += (double)u.x * (double)u.x;
                     if (b & 2) d_part += (double)u.y * (double)u.y;
                 }
