@@ -51,6 +51,9 @@ constexpr int nq_of() { return NEN == 3 ? 1 : NEN == 4 ? 4 : NEN == 6 ? 6 : 9; }
 // PCG launches of at most this many workgroups (one per CU or fewer: latency-bound) load the first node's
 // state before the scalar step
 constexpr int PCG_EARLY_ITEMS = 256;
+// product launches of at most this many workgroups request the Gauss points one step ahead (linear elements):
+// that variant holds 3 workgroups of 256 threads per CU, the other 4
+constexpr int APPLY_AHEAD_ITEMS = 3 * 256;
 constexpr int MAXLN = 640;        // local nodes per chunk (LDS capacity)
 constexpr int VBS = 256;          // workgroup size of the vector kernels
 constexpr int VPT = 4;            // dofs per thread in the vector kernels
@@ -511,7 +514,7 @@ __device__ __forceinline__ void element_product(const ChunkView& ch, int t, bool
 
 // PF: Gauss points whose data is loaded before the gather barrier; MINW: minimum waves per SIMD the
 // register allocation must allow (1 = no constraint)
-template <int NEN, int NQ, int BS, typename T, int PF = 2, int MINW = 1>
+template <int NEN, int NQ, int BS, typename T, int PF = 2, int MINW = 1, bool AHEAD = false>
 __global__ __launch_bounds__(BS, MINW) void k_apply(const dfea_desc* __restrict__ tab, const int* __restrict__ items,
                                                    const T* __restrict__ vin, T* __restrict__ vout) {
     __shared__ ChunkLDS<T> L;
@@ -584,7 +587,7 @@ __global__ __launch_bounds__(BS, MINW) void k_apply(const dfea_desc* __restrict_
     __builtin_amdgcn_sched_barrier(0);
 
     // 2. element products, colour-ordered sums
-    element_product<NEN, NQ, BS, PF, T, true, NEN <= 4>(ch, t, has_el, l, dN, C, ne, e, d, c, L, GL);
+    element_product<NEN, NQ, BS, PF, T, true, AHEAD>(ch, t, has_el, l, dN, C, ne, e, d, c, L, GL);
 
     // 3. owned nodes: final value; shared nodes (partials plans): partial
     V2<T>* part = reinterpret_cast<V2<T>*>(D.part);
@@ -1707,7 +1710,11 @@ struct Launch_apply {
         constexpr int BS = quadratic<NEN>() ? bs_groups<T>() : BS0;
         const T* vv = static_cast<const T*>(v);
         T* yy = static_cast<T*>(y);
-        int rc = launch(k_apply<NEN, nq_of<NEN>(), BS, T>, n, BS, st, "k_apply", t, it, vv, yy);
+        // linear elements: Gauss points requested one step ahead while the launch fits in the workgroup slots
+        // of the (costlier in registers) ahead variant
+        const bool ahead = NEN <= 4 && n <= APPLY_AHEAD_ITEMS;
+        int rc = ahead ? launch(k_apply<NEN, nq_of<NEN>(), BS, T, 2, 1, NEN <= 4>, n, BS, st, "k_apply", t, it, vv, yy)
+                       : launch(k_apply<NEN, nq_of<NEN>(), BS, T>, n, BS, st, "k_apply", t, it, vv, yy);
         if (rc) return rc;
         return launch(k_apply_fin<T>, nf, FBS, st, "k_apply_fin", t, fi, vv, yy);
     }
