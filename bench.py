@@ -84,6 +84,32 @@ def time_apply(p, u, v, theta, precision, steps, warmup):
     return dict(ms=ms, gbs=B / (ms * 1e-3) / 1e9, bytes=B, products_per_s=1e3 / ms, y=y)
 
 
+def time_apply_cold(p, u, v, theta, precision, steps=100, flush_mb=512):
+    """A(v) with the caches cold: before every product a 512 MB buffer (twice the 256 MB Infinity Cache) is
+    rewritten, then the product alone is timed with HIP events on the kernels' stream; median over steps."""
+    from difffea_amd.operators import make_tangent_operator
+    dev = torch.device("cuda")
+    A = make_tangent_operator("hip", u.to(dev), theta.to(dev), p, precision=precision)
+    vd = v.to(dev, torch.float64 if precision == "fp64" else torch.float32)
+    y = torch.empty_like(vd)
+    sysm = A.system
+    junk = torch.empty(flush_mb * (1 << 20) // 4, dtype=torch.float32, device=dev)
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    for i in range(steps + 3):
+        junk.fill_(float(i))
+        if i >= 3:
+            ev[i - 3][0].record()
+        sysm.apply(vd, y)
+        if i >= 3:
+            ev[i - 3][1].record()
+    torch.cuda.synchronize()
+    ms = statistics.median(a.elapsed_time(b) for a, b in ev)
+    m = sysm.mesh
+    B = b_alg(m.n_el, m.nen, m.nq, m.n_dof, 8 if precision == "fp64" else 4)
+    return dict(us_per_product=round(ms * 1e3, 3), gbs=round(B / (ms * 1e-3) / 1e9, 2), flush_mb=flush_mb,
+                steps=steps)
+
+
 def time_pcg_iteration(p, u, v, theta, precision, iters=400, chunk=50):
     """One fused PCG iteration (one kernel: scalar step, update, next product) on the same mesh and state,
     replayed from HIP graphs of `chunk` kernels; rtol = 0 keeps the solve running.  Algorithmic bytes per
@@ -310,6 +336,10 @@ def main():
     print(f"# Q4 400x400 fp64 A(v): {r64['ms'] * 1e3:.2f} us, {r64['gbs']:.0f} GB/s", file=sys.stderr, flush=True)
     r32 = time_apply(p, u, v, th, "mixed", args.steps, args.warmup)
     print(f"# Q4 400x400 fp32 A(v): {r32['ms'] * 1e3:.2f} us, {r32['gbs']:.0f} GB/s", file=sys.stderr, flush=True)
+    cold64 = time_apply_cold(p, u, v, th, "fp64")
+    cold32 = time_apply_cold(p, u, v, th, "mixed")
+    print(f"# Q4 400x400 A(v), caches flushed before each product: fp64 {cold64}, fp32 {cold32}", file=sys.stderr,
+          flush=True)
     pcg64 = time_pcg_iteration(p, u, v, th, "fp64")
     pcg32 = time_pcg_iteration(p, u, v, th, "mixed")
     print(f"# Q4 400x400 PCG iteration: fp64 {pcg64['us_per_iteration']:.2f} us, "
@@ -383,6 +413,7 @@ def main():
                              "profiles/apply_q4_400_fp64_pmc.md"},
         "fp32_moduli": {"us_per_product": round(r32["ms"] * 1e3, 3), "gbs": round(r32["gbs"], 2),
                         "bytes_per_step": r32["bytes"]},
+        "cold_cache_fp64": cold64, "cold_cache_fp32": cold32,
         "q9_200x200_fp64": {"us_per_product": round(q9["ms"] * 1e3, 3), "gbs": round(q9["gbs"], 2),
                             "bytes_per_step": q9["bytes"]},
         "pcg_iteration_fp64": pcg64, "pcg_iteration_mixed": pcg32, "postprocess_q4_400": post,
