@@ -266,8 +266,11 @@ def solve_config3(precision, preconditioner="jacobi"):
                    converged=info["converged"], tol_abs=tol)
 
 
-def sequential_fp64(n_requests, seed=5, side=(32, 512)):
-    """The plain FP64 GPU solver: the same requests solved one after another with newton_krylov_solve."""
+def sequential_fp64(n_requests, seed=5, side=(32, 512), rate=None):
+    """The plain FP64 GPU solver: the same requests solved one after another with newton_krylov_solve.
+    With ``rate``, also the latency from arrival of a first-in-first-out queue in front of it (Poisson arrivals
+    of that rate, the stream's seed): request i starts at max(arrival_i, end of request i-1) and takes the
+    service time measured for it here (the solver serves one request at a time, so the queue is exact)."""
     import numpy as np
     from difffea_amd.serve import make_stream, reference_residual_norm, request_problem
     from difffea_amd.newton_krylov import HipNewton
@@ -290,9 +293,15 @@ def sequential_fp64(n_requests, seed=5, side=(32, 512)):
             print(f"# plain FP64 solver: {len(lat)} of {len(reqs)} requests, {time.perf_counter() - t0:.1f} s",
                   file=sys.stderr, flush=True)
     wall = time.perf_counter() - t0
-    return dict(precision="fp64", mode="sequential newton_krylov_solve", n_requests=len(reqs), wall_s=wall,
-                solves_per_s=len(reqs) / wall, p50_latency_s=float(np.percentile(lat, 50)),
-                p99_latency_s=float(np.percentile(lat, 99)))
+    out = dict(precision="fp64", mode="sequential newton_krylov_solve", n_requests=len(reqs), wall_s=wall,
+               solves_per_s=len(reqs) / wall, p50_latency_s=float(np.percentile(lat, 50)),
+               p99_latency_s=float(np.percentile(lat, 99)), latency_from="start of the request's solve")
+    if rate is not None:
+        from difffea_amd.serve import fifo_latencies
+        qlat = fifo_latencies([r.arrival for r in make_stream(n_requests, rate=rate, seed=seed, side=side)], lat)
+        out.update(arrival_rate_per_s=rate, p50_latency_from_arrival_s=float(np.percentile(qlat, 50)),
+                   p99_latency_from_arrival_s=float(np.percentile(qlat, 99)))
+    return out
 
 
 def serve_stream(n_requests, precision, rate=None, seed=5, side=(32, 512), preconditioner="jacobi"):

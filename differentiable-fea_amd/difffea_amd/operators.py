@@ -13,6 +13,9 @@ Mirrors `src/difffea/operators.py`:
   (`operators.py:588-638`).  ``"ilu"`` needs the assembled sparse matrix on the host and is not offered.
 * ``compute_jacobi_diagonal(u, theta, params)``                (`operators.py:477-556`)
 * ``global_residual(u, theta, params, load_factor)``           (`operators.py:159-220`)
+* ``precompute_geometry(params)`` / ``get_geometry(params)``    (`operators.py:85-128`)
+  ``dN_dX`` (n_el, n_q, nen, 2) and ``w_detJ`` (n_el, n_q) from the device geometry kernel, in the
+  original element order, on the device of ``params["nodes"]``.
 
 Every closure keeps its own device buffers (moduli, diagonal), so closures built at different states stay
 valid side by side, as the reference's functional closures do.
@@ -35,6 +38,31 @@ _MESHES = {}
 def _check_material(params):
     """The material of ``params`` as the kernels take it (id, nominal parameters, theta-scaled entries)."""
     return material_spec(params)
+
+
+def precompute_geometry(params, device=None):
+    """``params`` plus the geometry cache ``dN_dX`` (n_el, n_q, nen, 2) = dN_a/dX_j and ``w_detJ`` (n_el, n_q)
+    = w_q det J, computed by the geometry kernel (`src/difffea/operators.py:85-120`) and returned in the
+    original element order on the device of ``params["nodes"]``."""
+    mesh = get_mesh(params, device)
+    n_el, nq, nen = mesh.n_el, mesh.nq, mesh.nen
+    dN = mesh.dN64[:, :n_el].reshape(nq, nen, 2, n_el).permute(3, 0, 1, 2)     # internal element order
+    w = mesh.w[:, :n_el].t()
+    dN_dX = torch.empty_like(dN)
+    w_detJ = torch.empty_like(w)
+    dN_dX[mesh.perm] = dN
+    w_detJ[mesh.perm] = w
+    dev = params["nodes"].device
+    return {**params, "dN_dX": dN_dX.contiguous().to(dev), "w_detJ": w_detJ.contiguous().to(dev)}
+
+
+def get_geometry(params, device=None):
+    """(dN_dX, w_detJ) of ``params``: the cached entries when present, else the geometry kernel's
+    (`src/difffea/operators.py:123-128`)."""
+    if "dN_dX" in params and "w_detJ" in params:
+        return params["dN_dX"], params["w_detJ"]
+    g = precompute_geometry(params, device)
+    return g["dN_dX"], g["w_detJ"]
 
 
 def get_mesh(params, device=None):

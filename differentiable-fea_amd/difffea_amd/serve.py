@@ -71,6 +71,17 @@ def make_stream(n_requests=2000, rate=50.0, seed=5, side=(32, 512), q4_fraction=
     return out
 
 
+def fifo_latencies(arrivals, service_times):
+    """Latency from arrival of each request in a first-in-first-out queue in front of a server that handles
+    one request at a time: request i starts at max(arrival_i, end of request i-1) and ends service_times[i]
+    later.  ``arrivals`` must be sorted."""
+    end, out = 0.0, []
+    for a, d in zip(arrivals, service_times):
+        end = max(a, end) + d
+        out.append(end - a)
+    return out
+
+
 def request_problem(req):
     """params + theta of a request (unit square, left edge clamped, right edge loaded)."""
     if req.adjoint:

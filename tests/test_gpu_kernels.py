@@ -206,3 +206,17 @@ def test_apply_properties_at_the_largest_served_size():
     assert torch.equal(Aa[fixed], a[fixed])
     A32 = make_tangent_operator("hip", u.to(DEV), th.to(DEV), p, precision="mixed")
     assert rel(A32(a.float()), Aa) < 1e-5
+
+
+@pytest.mark.parametrize("et,n", [("quad4", 37), ("quad9", 19), ("tri6", 13)])
+def test_precompute_geometry_matches_oracle(et, n):
+    """dN_dX and w_detJ of the geometry kernel, read back in the original element order, against the
+    oracle's Jacobian inverse (`operators.py:85-120`); get_geometry returns cached entries unchanged."""
+    from difffea_amd.operators import get_geometry, precompute_geometry
+    p = unit_square(et, n)
+    ref = fem.precompute_geometry(p)
+    g = precompute_geometry(p)
+    assert g["dN_dX"].shape == ref["dN_dX"].shape and g["w_detJ"].shape == ref["w_detJ"].shape
+    assert rel(g["dN_dX"], ref["dN_dX"]) < 1e-14 and rel(g["w_detJ"], ref["w_detJ"]) < 1e-14
+    dN, w = get_geometry(g)
+    assert dN is g["dN_dX"] and w is g["w_detJ"]

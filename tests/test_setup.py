@@ -110,3 +110,11 @@ def test_ogrid_hole_and_outer_square():
     X = nodes[conn[:, :4]].numpy()
     area = 0.5 * np.sum(X[:, :, 0] * np.roll(X[:, :, 1], -1, 1) - np.roll(X[:, :, 0], -1, 1) * X[:, :, 1], axis=1)
     assert (area > 0).all()
+
+
+def test_fifo_queue_latencies():
+    """The queue in front of a one-request-at-a-time solver: idle server, back-to-back arrivals, a burst."""
+    from difffea_amd.serve import fifo_latencies
+    assert fifo_latencies([0.0, 10.0], [1.0, 2.0]) == [1.0, 2.0]              # no waiting
+    assert fifo_latencies([0.0, 0.5, 0.5], [1.0, 1.0, 1.0]) == [1.0, 1.5, 2.5]  # each waits for the one before
+    assert fifo_latencies([], []) == []
