@@ -332,6 +332,8 @@ def main():
                     help="requests of the served stream (the full workload is 2000)")
     ap.add_argument("--serve-side", type=int, nargs=2, default=(32, 256),
                     help="side range of the served meshes (the full workload is 32 512)")
+    ap.add_argument("--config1-only", action="store_true",
+                    help="only the 100x100 Q4 solves (FP64 and mixed), one JSON line; for profiling the solve")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (use --gpus 1)", file=sys.stderr)
@@ -339,6 +341,10 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py needs a ROCm GPU", file=sys.stderr)
         sys.exit(2)
+
+    if args.config1_only:
+        print(json.dumps({prec: solve_config1(prec)[1] for prec in ("fp64", "mixed")}))
+        return
 
     p, u, v, th = problem("quad4", 400)
     r64 = time_apply(p, u, v, th, "fp64", args.steps, args.warmup)

@@ -217,6 +217,7 @@ def test_precompute_geometry_matches_oracle(et, n):
     ref = fem.precompute_geometry(p)
     g = precompute_geometry(p)
     assert g["dN_dX"].shape == ref["dN_dX"].shape and g["w_detJ"].shape == ref["w_detJ"].shape
-    assert rel(g["dN_dX"], ref["dN_dX"]) < 1e-14 and rel(g["w_detJ"], ref["w_detJ"]) < 1e-14
+    # closed-form 2x2 inverse and determinant against LU: a few units in the last place
+    assert rel(g["dN_dX"], ref["dN_dX"]) < 1e-13 and rel(g["w_detJ"], ref["w_detJ"]) < 1e-13
     dN, w = get_geometry(g)
     assert dN is g["dN_dX"] and w is g["w_detJ"]
