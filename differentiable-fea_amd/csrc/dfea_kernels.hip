@@ -428,6 +428,76 @@ __device__ __forceinline__ void prefetch_gauss_points(const T* dN, const T* C, i
         if (q0 + j < q1) load_gauss_point<NEN, NQ, T, NT>(dN, C, ne, e, q0 + j, d[j], c[j]);
 }
 
+// LDS of the product and PCG kernels on partials plans.  acc holds the chunk's element vectors (slot
+// e_local * NEN + a) instead of node accumulators; slot / offs hold the chunk's inverse map (node k's element
+// vectors are slots slot[offs[k] .. offs[k+1])), so every node sums its own in one pass after one barrier.
+constexpr int SUMSLOTS = 1024;    // element-vector slots of a chunk, at most (256 quad4 elements x 4 nodes)
+template <typename T>
+struct ProductLDS {
+    V2<T> v[MAXLN];
+    V2<T> acc[SUMSLOTS];
+    unsigned char bits[MAXLN];
+    uint16_t slot[SUMSLOTS];
+    int offs[MAXLN + 1];
+    double red[16];
+};
+
+// the inverse-map tables of a chunk, requested into registers early (the loads return while other work is in
+// flight) and stored into LDS before the gather barrier; indices clamped into the chunk's ranges
+template <int NEN, int BS>
+struct InvRegs {
+    static constexpr int CE = Groups<NEN, BS>::CE;
+    static constexpr int NSL = (CE * NEN + BS - 1) / BS;     // slots per thread
+    static constexpr int NOF = (MAXLN + 1 + BS - 1) / BS;    // node offsets per thread
+    uint16_t sl[NSL];
+    int of[NOF];
+};
+
+template <int NEN, int BS>
+__device__ __forceinline__ void inv_load(const dfea_desc& D, const ChunkView& ch, int t, InvRegs<NEN, BS>& R) {
+    const int n = ch.ne * NEN;
+    const uint32_t base = (uint32_t)(ch.e0 * NEN);
+#pragma unroll
+    for (int i = 0; i < InvRegs<NEN, BS>::NSL; ++i) {
+        const int j = t + i * BS;
+        R.sl[i] = gload<false>(D.inv_slot, base + (uint32_t)(j < n ? j : n - 1));
+    }
+#pragma unroll
+    for (int i = 0; i < InvRegs<NEN, BS>::NOF; ++i) {
+        const int k = t + i * BS;
+        R.of[i] = gload<false>(D.inv_off, (uint32_t)(ch.k0 + (k <= ch.nk ? k : ch.nk))) - (int)base;
+    }
+}
+
+template <int NEN, int BS, typename T>
+__device__ __forceinline__ void inv_store(const ChunkView& ch, int t, const InvRegs<NEN, BS>& R, ProductLDS<T>& L) {
+    const int n = ch.ne * NEN;
+#pragma unroll
+    for (int i = 0; i < InvRegs<NEN, BS>::NSL; ++i) {
+        const int j = t + i * BS;
+        if (j < n) L.slot[j] = R.sl[i];
+    }
+#pragma unroll
+    for (int i = 0; i < InvRegs<NEN, BS>::NOF; ++i) {
+        const int k = t + i * BS;
+        if (k <= ch.nk) L.offs[k] = R.of[i];
+    }
+}
+
+// the element vectors of local node k summed in slot (= colour) order: the same additions, in the same
+// order, as the colour-by-colour accumulation into a zeroed accumulator
+template <typename T>
+__device__ __forceinline__ V2<T> node_sum(const ProductLDS<T>& L, int k) {
+    V2<T> s{0, 0};
+    const int j1 = L.offs[k + 1];
+    for (int j = L.offs[k]; j < j1; ++j) {
+        const V2<T> f = L.acc[L.slot[j]];
+        s.x += f.x;
+        s.y += f.y;
+    }
+    return s;
+}
+
 // LDS for the element vectors of groups 1..G-1
 template <int NEN, int BS, typename T>
 struct GroupLDS {
@@ -437,12 +507,12 @@ struct GroupLDS {
 
 // fe = K_e (m v_e) from the gathered LDS values (the group's Gauss points, the groups added in order), then
 // the colour-ordered sum into L.acc.  d / c hold the group's first PF Gauss points on entry.
-template <int NEN, int NQ, int BS, int PF, typename T, bool NT = false, bool AHEAD = false>
+template <int NEN, int NQ, int BS, int PF, typename T, bool NT = false, bool AHEAD = false, typename LDS>
 __device__ __forceinline__ void element_product(const ChunkView& ch, int t, bool has_el, const int* l, const T* dN,
                                                 const T* C, int ne, int e,
                                                 T (&d)[GaussSplit<NQ, Groups<NEN, BS>::G>::QG][NEN][2],
                                                 T (&c)[GaussSplit<NQ, Groups<NEN, BS>::G>::QG][10],
-                                                ChunkLDS<T>& L, GroupLDS<NEN, BS, T>& GL) {
+                                                LDS& L, GroupLDS<NEN, BS, T>& GL) {
     constexpr int G = Groups<NEN, BS>::G, CE = Groups<NEN, BS>::CE, QG = GaussSplit<NQ, G>::QG;
     const int g = G > 1 ? t / CE : 0;
     const int q0 = g * NQ / G, q1 = (g + 1) * NQ / G;
@@ -509,7 +579,16 @@ __device__ __forceinline__ void element_product(const ChunkView& ch, int t, bool
             }
         }
     }
-    colour_accumulate<T, NEN>(ch, t, l, fe, L.acc);
+    if constexpr (std::is_same<LDS, ProductLDS<T>>::value) {
+        // inverse-map sums: element vectors into their slots, one barrier, then every node sums its own
+        if (g == 0 && has_el) {
+#pragma unroll
+            for (int a = 0; a < NEN; ++a) L.acc[(t % CE) * NEN + a] = fe[a];
+        }
+        __syncthreads();
+    } else {
+        colour_accumulate<T, NEN>(ch, t, l, fe, L.acc);
+    }
 }
 
 // PF: Gauss points whose data is loaded before the gather barrier; MINW: minimum waves per SIMD the
@@ -517,7 +596,9 @@ __device__ __forceinline__ void element_product(const ChunkView& ch, int t, bool
 template <int NEN, int NQ, int BS, typename T, int PF = 2, int MINW = 1, bool AHEAD = false>
 __global__ __launch_bounds__(BS, MINW) void k_apply(const dfea_desc* __restrict__ tab, const int* __restrict__ items,
                                                    const T* __restrict__ vin, T* __restrict__ vout) {
-    __shared__ ChunkLDS<T> L;
+    // partials plans (chunks of exactly CE elements) sum nodes through the inverse map, halo plans by colour
+    constexpr bool INV = BS % Groups<NEN, BS>::CE == 0;
+    __shared__ std::conditional_t<INV, ProductLDS<T>, ChunkLDS<T>> L;
     __shared__ GroupLDS<NEN, BS, T> GL;
     const Item it = load_item(items);
     const dfea_desc& D = tab[it.s];
@@ -566,9 +647,11 @@ __global__ __launch_bounds__(BS, MINW) void k_apply(const dfea_desc* __restrict_
         const int ent = k >= ch.nown && k < ch.nk ? ch.pbase + (k - ch.nown) : 0;
         slot[i] = D.n_part ? gload<false>(D.pslot, (uint32_t)ent) : 0;
     }
-    if constexpr (BS % CE == 0) {
+    InvRegs<NEN, BS> inv;
+    if constexpr (INV) {
 #pragma unroll
         for (int a = 0; a < NEN; ++a) l[a] = gload<false>(D.lelem, (uint32_t)(a * D.n_el + e));
+        inv_load<NEN, BS>(D, ch, t, inv);
     }
     __builtin_amdgcn_sched_barrier(0);
     prefetch_gauss_points<NEN, NQ, BS, PF, T, true>(dN, C, ne, e, t, d, c);
@@ -578,15 +661,16 @@ __global__ __launch_bounds__(BS, MINW) void k_apply(const dfea_desc* __restrict_
         const int k = t + i * BS;
         if (k < ch.nk) {
             L.v[k] = gv[i];
-            L.acc[k] = V2<T>{0, 0};
+            if constexpr (!INV) L.acc[k] = V2<T>{0, 0};
             L.bits[k] = (unsigned char)(gid[i] >> 30);
         }
     }
+    if constexpr (INV) inv_store<NEN, BS, T>(ch, t, inv, L);
     __builtin_amdgcn_sched_barrier(0);
     __syncthreads();
     __builtin_amdgcn_sched_barrier(0);
 
-    // 2. element products, colour-ordered sums
+    // 2. element products; node sums in colour order (inverse map or colour passes)
     element_product<NEN, NQ, BS, PF, T, true, AHEAD>(ch, t, has_el, l, dN, C, ne, e, d, c, L, GL);
 
     // 3. owned nodes: final value; shared nodes (partials plans): partial
@@ -594,15 +678,18 @@ __global__ __launch_bounds__(BS, MINW) void k_apply(const dfea_desc* __restrict_
 #pragma unroll
     for (int i = 0; i < NG; ++i) {
         const int k = t + i * BS;
+        if (k >= ch.nk) break;
+        V2<T> acc;
+        if constexpr (INV) acc = node_sum<T>(L, k);
+        else acc = L.acc[k];
         if (k < ch.nown) {
-            const V2<T> acc = L.acc[k];
             const V2<T> raw = L.v[k];
             V2<T> out;
             out.x = (gid[i] & DFEA_FIXED_X) ? raw.x : acc.x;
             out.y = (gid[i] & DFEA_FIXED_Y) ? raw.y : acc.y;
             reinterpret_cast<V2<T>*>(y)[(int)(gid[i] & DFEA_NODE_MASK)] = out;
-        } else if (k < ch.nk && D.n_part) {
-            part[slot[i]] = L.acc[k];
+        } else if (D.n_part) {
+            part[slot[i]] = acc;
         }
     }
 }
@@ -1337,7 +1424,9 @@ __device__ __forceinline__ NodeState<T> node_state(const dfea_desc& D, const Chu
 template <int NEN, int NQ, int BS, typename T, int PF = 2, int MINW = 1, bool EARLY = false>
 __global__ __launch_bounds__(BS, MINW) void k_pcg(const dfea_desc* __restrict__ tab, const int* __restrict__ items, int par) {
     static_assert(3 * (BS / 64) <= 16, "block_sum3 needs 3 slots per wave in ChunkLDS::red");
-    __shared__ ChunkLDS<T> L;
+    // partials plans (chunks of exactly CE elements) sum nodes through the inverse map, halo plans by colour
+    constexpr bool INV = BS % Groups<NEN, BS>::CE == 0;
+    __shared__ std::conditional_t<INV, ProductLDS<T>, ChunkLDS<T>> L;
     __shared__ GroupLDS<NEN, BS, T> GL;
     const Item it = load_item(items);
     const dfea_desc& D = tab[it.s];
@@ -1363,7 +1452,7 @@ __global__ __launch_bounds__(BS, MINW) void k_pcg(const dfea_desc* __restrict__ 
     const int2* prg = reinterpret_cast<const int2*>(D.prange);
     constexpr int CE = Groups<NEN, BS>::CE;
     constexpr int NG = (MAXLN + BS - 1) / BS;     // local nodes per thread, at most
-    constexpr int NRD = 4;                        // chunk partials per thread loaded ahead of the scalar step
+    constexpr int NRD = 3;                        // chunk partials per thread loaded ahead of the scalar step
 
     // 1a. the previous kernel's chunk partials {r.u, w.u, r.r} (index clamped: extra threads read chunk 0 and
     // drop it)
@@ -1401,12 +1490,14 @@ __global__ __launch_bounds__(BS, MINW) void k_pcg(const dfea_desc* __restrict__ 
     int l[NEN];
     int e;
     bool has_el;
+    InvRegs<NEN, BS> inv;
     if constexpr (BS % CE == 0) {
         e = it.b * CE + t % CE;
         has_el = e < D.n_el;
         if (!has_el) e = D.n_el - 1;
 #pragma unroll
         for (int a = 0; a < NEN; ++a) l[a] = gload<false>(D.lelem, (uint32_t)(a * D.n_el + e));
+        inv_load<NEN, BS>(D, ch, t, inv);
     } else {
         has_el = thread_element<NEN>(D, ch, t, e, l);
     }
@@ -1529,9 +1620,10 @@ __global__ __launch_bounds__(BS, MINW) void k_pcg(const dfea_desc* __restrict__ 
             r_part += (double)rn.x * (double)rn.x + (double)rn.y * (double)rn.y;
         }
         L.v[kk] = un;
-        L.acc[kk] = V2<T>{0, 0};
+        if constexpr (!INV) L.acc[kk] = V2<T>{0, 0};
         L.bits[kk] = b;
     }
+    if constexpr (INV) inv_store<NEN, BS, T>(ch, t, inv, L);
     __syncthreads();
 
     // 4. w_{k+1} = A u_{k+1}
@@ -1543,7 +1635,9 @@ __global__ __launch_bounds__(BS, MINW) void k_pcg(const dfea_desc* __restrict__ 
         const int kk = t + i * BS;
         if (kk >= ch.nk) break;
         const V2<T> u = L.v[kk];
-        const V2<T> acc = L.acc[kk];
+        V2<T> acc;
+        if constexpr (INV) acc = node_sum<T>(L, kk);
+        else acc = L.acc[kk];
         const unsigned char b = L.bits[kk];
         if (kk < ch.nown) {
             V2<T> out;
@@ -1647,7 +1741,7 @@ int pcg_iter(const dfea_desc* table, const int32_t* items, int n_items, int n_it
     for (int i = 0; i < n_iter + (n_iter & 1); ++i) {
         const int rc = early ? launch(k_pcg<NEN, nq_of<NEN>(), BSP, T, 2, 1, true>, n_items, BSP, stream, "k_pcg", table, items,
                                       i & 1)
-                             : launch(k_pcg<NEN, nq_of<NEN>(), BSP, T, 2, 1, false>, n_items, BSP, stream, "k_pcg", table,
+                             : launch(k_pcg<NEN, nq_of<NEN>(), BSP, T, 2, (NEN <= 4 ? 3 : 1), false>, n_items, BSP, stream, "k_pcg", table,
                                       items, i & 1);
         if (rc) return rc;
     }
@@ -1753,7 +1847,7 @@ int dfea_desc_offset(int field) {
     static const int off[] = {
         (int)offsetof(dfea_desc, chunks), (int)offsetof(dfea_desc, lnode), (int)offsetof(dfea_desc, lelem),
         (int)offsetof(dfea_desc, sh_node), (int)offsetof(dfea_desc, sh_off), (int)offsetof(dfea_desc, pslot), (int)offsetof(dfea_desc, hel), (int)offsetof(dfea_desc, hlelem),
-        (int)offsetof(dfea_desc, prange),
+        (int)offsetof(dfea_desc, prange), (int)offsetof(dfea_desc, inv_off), (int)offsetof(dfea_desc, inv_slot),
         (int)offsetof(dfea_desc, conn), (int)offsetof(dfea_desc, perm), (int)offsetof(dfea_desc, nodes),
         (int)offsetof(dfea_desc, dNdxi), (int)offsetof(dfea_desc, gw), (int)offsetof(dfea_desc, nfix),
         (int)offsetof(dfea_desc, dN64), (int)offsetof(dfea_desc, dN32), (int)offsetof(dfea_desc, w),

@@ -37,6 +37,7 @@ class Desc(ctypes.Structure):
     """Mirror of ``dfea_desc`` (field order = declaration order in include/dfea.h)."""
     _fields_ = [
         ("chunks", _P), ("lnode", _P), ("lelem", _P), ("sh_node", _P), ("sh_off", _P), ("pslot", _P), ("hel", _P), ("hlelem", _P), ("prange", _P),
+        ("inv_off", _P), ("inv_slot", _P),
         ("conn", _P), ("perm", _P), ("nodes", _P), ("dNdxi", _P), ("gw", _P), ("nfix", _P),
         ("dN64", _P), ("dN32", _P), ("w", _P), ("theta", _P), ("C64", _P), ("C32", _P),
         ("diag64", _P), ("diag32", _P), ("f", _P), ("part", _P), ("red", _P), ("sc", _P),

@@ -80,6 +80,9 @@ class HipMesh:
         self.t_pslot = _dev_int32(p.pslot if len(p.pslot) else np.zeros(1, np.int32), dv)
         self.t_hel = _dev_int32(p.hel if len(p.hel) else np.zeros(1, np.int32), dv)
         self.t_prange = _dev_int32(p.prange if p.n_part else np.zeros((1, 2), np.int32), dv)
+        inv_off, inv_slot = p.inverse_map() if p.mode == "partials" else (np.zeros(1, np.int32), np.zeros(1, np.uint16))
+        self.t_inv_off = _dev_int32(inv_off, dv)
+        self.t_inv_slot = torch.from_numpy(inv_slot.view(np.int16)).to(dv)
         self.t_hlelem = torch.from_numpy(np.ascontiguousarray(p.hlelem).view(np.int16).reshape(-1) if p.hlelem.size
                                          else np.zeros(1, np.int16)).to(dv)
         self.t_conn = _dev_int32(p.conn_int, dv)
@@ -115,6 +118,7 @@ class HipMesh:
         d.sh_node, d.sh_off, d.pslot = self.t_sh_node.data_ptr(), self.t_sh_off.data_ptr(), self.t_pslot.data_ptr()
         d.hel, d.hlelem, d.n_halo = self.t_hel.data_ptr(), self.t_hlelem.data_ptr(), len(p.hel)
         d.prange = self.t_prange.data_ptr()
+        d.inv_off, d.inv_slot = self.t_inv_off.data_ptr(), self.t_inv_slot.data_ptr()
         d.n_cols = self.n_cols
         d.conn, d.perm, d.nodes = self.t_conn.data_ptr(), self.t_perm.data_ptr(), self.t_nodes.data_ptr()
         d.dNdxi, d.gw, d.nfix = self.t_dNdxi.data_ptr(), self.t_gw.data_ptr(), self.t_nfix.data_ptr()

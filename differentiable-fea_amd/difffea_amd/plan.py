@@ -369,6 +369,24 @@ class MeshPlan:
         out[nodes, 1] = np.diff(self.sh_off)
         return out
 
+    def inverse_map(self):
+        """Node-to-element map of the product's node sums (partials plans): ``inv_off`` [n_lnodes + 1] int32 and
+        ``inv_slot`` [n_el * nen] uint16.  The element vectors of local node entry i (global entry index
+        k0 + k) are the chunk-local slots inv_slot[inv_off[i] .. inv_off[i+1]), slot = e_local * nen + a
+        (element e_local of the chunk, its node a), sorted by slot, i.e. in element order, which inside a chunk
+        is colour order: the sums come out in the same order as colour-by-colour accumulation.  A chunk's
+        slots occupy inv_slot[e0 * nen .. (e0 + ne) * nen)."""
+        ch = self.chunks
+        nen = self.nen
+        chunk_of_e = np.repeat(np.arange(self.n_chunks), ch[:, 1])
+        e_local = np.arange(self.n_el) - ch[chunk_of_e, 0]
+        entry = (ch[chunk_of_e, 2][None, :] + self.lelem.astype(np.int64)).T.reshape(-1)   # (e, a) -> entry
+        slot = (e_local[:, None] * nen + np.arange(nen)[None, :]).reshape(-1)
+        order = np.lexsort((slot, entry))
+        inv_slot = slot[order].astype(np.uint16)
+        inv_off = np.searchsorted(entry[order], np.arange(self.n_lnodes + 1)).astype(np.int32)
+        return inv_off, inv_slot
+
     @property
     def prange(self):
         """[n_part][2] int32: first slot and number of slots of the shared node of every partials entry

@@ -111,6 +111,10 @@ typedef struct dfea_desc {
     const uint16_t* hlelem;     /* [nen][n_halo] halo mode: local node index of halo entry h, node a */
     const int32_t* prange;      /* [n_part][2] {first slot, number of slots} of the shared node of each (chunk,
                                    shared node) entry, chunk order (entry tables load without the node id) */
+    const int32_t* inv_off;     /* [n_lnodes + 1] partials plans: the element vectors of local node entry i are the
+                                   chunk-local slots inv_slot[inv_off[i] .. inv_off[i+1]) */
+    const uint16_t* inv_slot;   /* [n_el * nen] slot e_local * nen + a, sorted by slot per node entry (element =
+                                   colour order); chunk c's slots occupy [e0 * nen, (e0 + ne) * nen) */
     const int32_t* conn;        /* [nen][n_el] node ids, internal element order */
     const int32_t* perm;        /* [n_el] original element id of internal element e */
     const double* nodes;        /* [N][2] reference coordinates */

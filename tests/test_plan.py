@@ -56,6 +56,7 @@ def _emulate_apply(plan, C, dN, v, dots=None):
     part = np.zeros((plan.n_part, 2))
     vv = v.reshape(N, 2)
     shmap = plan.shmap
+    inv = plan.inverse_map() if plan.mode == "partials" else None
     for c in range(plan.n_chunks):
         e0, ne, k0, nk, nown, pbase, ncol, h0, nh = plan.chunks[c, :9]
         ln = plan.lnode[k0:k0 + nk]
@@ -80,6 +81,18 @@ def _emulate_apply(plan, C, dN, v, dots=None):
             idx = loc[thr].reshape(-1)
             assert len(np.unique(idx)) == len(idx), "two elements of one colour share a node"
             acc[idx] += fe[thr].reshape(-1, 2)
+        if inv is not None:
+            # the kernels' node sums through the inverse map: each node adds its element vectors in slot order,
+            # which must be the colour order above, bit for bit
+            inv_off, inv_slot = inv
+            base = e0 * plan.nen
+            fe_flat = fe[:ne].reshape(-1, 2)
+            acc_inv = np.zeros((nk, 2))
+            for k in range(nk):
+                for j in range(inv_off[k0 + k], inv_off[k0 + k + 1]):
+                    acc_inv[k] += fe_flat[inv_slot[j]]
+                    assert inv_slot[j] // plan.nen < ne and j - base < ne * plan.nen
+            assert np.array_equal(acc_inv, acc), "inverse-map sums differ from the colour-ordered sums"
         out = np.where(np.stack([bits & 1, bits & 2], 1) > 0, raw, acc)
         y[nodes[:nown]] = out[:nown]
         slots = plan.pslot[pbase:pbase + nk - nown]
