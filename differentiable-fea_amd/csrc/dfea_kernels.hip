@@ -3,9 +3,11 @@
 // Execution model
 //   * Elements are grouped into chunks (<= CHUNK_LIN / CHUNK_QUAD elements, <= MAXLN distinct nodes).  One
 //     workgroup handles one chunk: it gathers the chunk's nodal values into LDS, each thread evaluates one
-//     element from structure-of-arrays moduli / geometry planes (coalesced HBM reads), and the element
-//     vectors are summed into LDS node accumulators colour by colour (elements of one colour share no
-//     node), so the summation order is fixed: no atomics, bit-reproducible results.
+//     element from structure-of-arrays moduli / geometry planes (coalesced HBM reads), and every node sums
+//     the element vectors that touch it in a fixed order (elements are sorted by colour inside a chunk): the
+//     product and PCG kernels through the chunk's inverse map (node -> element-vector slots, one barrier),
+//     the other kernels colour by colour into LDS accumulators (elements of one colour share no node).  No
+//     atomics, bit-reproducible results.
 //   * Partials plans (the default): nodes used by several chunks receive one partial per chunk, stored
 //     contiguously per node in chunk order, and the next kernel sums them in that order.  Halo plans
 //     (quad4, optional): every node is owned by one chunk, and a chunk also evaluates the neighbouring
