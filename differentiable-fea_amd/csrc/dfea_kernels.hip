@@ -582,6 +582,7 @@ __device__ __forceinline__ void element_product(const ChunkView& ch, int t, bool
         }
     }
     if constexpr (std::is_same<LDS, ProductLDS<T>>::value) {
+        static_assert(CE * NEN <= SUMSLOTS, "element-vector slots exceed ProductLDS");
         // inverse-map sums: element vectors into their slots, one barrier, then every node sums its own
         if (g == 0 && has_el) {
 #pragma unroll
