@@ -48,6 +48,25 @@ def _check(element_type):
         raise ValueError(f"Unknown element type '{element_type}'. Choose one of {list(ELEMENT_TYPES)}.")
 
 
+def element_names():
+    """Names of the element types (`src/difffea/elements.py:60-62`)."""
+    return list(ELEMENT_TYPES)
+
+
+def get_element(name):
+    """The library entry of an element type: n_nodes, family, order, n_corners, edges, gauss
+    (`src/difffea/elements.py:65-70`); unknown names raise ``ValueError``."""
+    _check(name)
+    return dict(n_nodes=NEN[name], family=FAMILY[name], order=ORDER[name],
+                n_corners=3 if FAMILY[name] == "tri" else 4, edges=list(EDGES[name]), gauss=GAUSS[name])
+
+
+def reference_node_coordinates(name):
+    """(xi, eta) of the nodes on the reference element, (nen, 2) (`src/difffea/elements.py:103-118`)."""
+    _check(name)
+    return REFERENCE_NODES[name].copy()
+
+
 def element_type_of(nen):
     """Element type from the number of nodes per element (the node counts of the five types differ)."""
     for name, n in NEN.items():

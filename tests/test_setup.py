@@ -118,3 +118,19 @@ def test_fifo_queue_latencies():
     assert fifo_latencies([0.0, 10.0], [1.0, 2.0]) == [1.0, 2.0]              # no waiting
     assert fifo_latencies([0.0, 0.5, 0.5], [1.0, 1.0, 1.0]) == [1.0, 1.5, 2.5]  # each waits for the one before
     assert fifo_latencies([], []) == []
+
+
+def test_element_library_entries():
+    """The library rows of the reference (`elements.py:38-57`): node counts, families, orders, corner counts,
+    edge tables, default rules; reference node coordinates are where the shape functions are Kronecker."""
+    from difffea_amd import elements
+    assert elements.element_names() == ["quad4", "quad8", "quad9", "tri3", "tri6"]
+    for name in elements.element_names():
+        e = elements.get_element(name)
+        X = elements.reference_node_coordinates(name)
+        assert X.shape == (e["n_nodes"], 2)
+        assert np.allclose(elements.shape_values(name, X[:, 0], X[:, 1]), np.eye(e["n_nodes"]), atol=1e-15)
+        assert len(e["edges"]) == e["n_corners"] and all(len(ed) == (2 if e["order"] == 1 else 3) for ed in e["edges"])
+    assert elements.get_element("tri6")["gauss"] == ("triangle", 4)
+    with pytest.raises(ValueError):
+        elements.get_element("hex8")
