@@ -37,3 +37,17 @@ def test_size_queries():
     assert [lib.dfea_chunk_threads(n) for n in (3, 4, 6, 8, 9)] == [256, 256, 64, 64, 64]
     assert lib.dfea_chunk_elems(5) == -1 and lib.dfea_chunk_threads(7) == -1
     assert lib.dfea_red_size(3, 2, 1) == 6 * 3 + 2 + 3
+
+
+def test_missing_library_fails_loudly():
+    """Without the HIP library the path raises (no CPU fallback): a fresh process pointed at a missing file."""
+    import subprocess
+    import sys
+    code = ("import sys; sys.path[:0] = [sys.argv[1]]\n"
+            "from difffea_amd import _lib\n"
+            "try:\n    _lib.load()\nexcept _lib.DfeaError as e:\n    print('raised:', e)\n")
+    env = dict(os.environ, DFEA_LIB=os.path.join(ROOT, "no_such_dir", "libdfea_hip.so"))
+    r = subprocess.run([sys.executable, "-c", code, os.path.join(ROOT, "differentiable-fea_amd")], env=env,
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "raised:" in r.stdout and "is not built" in r.stdout
