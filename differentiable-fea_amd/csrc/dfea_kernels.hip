@@ -595,8 +595,9 @@ __device__ __forceinline__ void element_product(const ChunkView& ch, int t, bool
 }
 
 // PF: Gauss points whose data is loaded before the gather barrier; MINW: minimum waves per SIMD the
-// register allocation must allow (1 = no constraint)
-template <int NEN, int NQ, int BS, typename T, int PF = 2, int MINW = 1, bool AHEAD = false>
+// register allocation must allow (1 = no constraint); EARLY (partials plans): those PF Gauss points are
+// requested before the gather rather than behind it
+template <int NEN, int NQ, int BS, typename T, int PF = 2, int MINW = 1, bool AHEAD = false, bool EARLY = false>
 __global__ __launch_bounds__(BS, MINW) void k_apply(const dfea_desc* __restrict__ tab, const int* __restrict__ items,
                                                    const T* __restrict__ vin, T* __restrict__ vout) {
     // partials plans (chunks of exactly CE elements) sum nodes through the inverse map, halo plans by colour
@@ -629,9 +630,14 @@ __global__ __launch_bounds__(BS, MINW) void k_apply(const dfea_desc* __restrict_
     constexpr int QG = GaussSplit<NQ, Groups<NEN, BS>::G>::QG;
     T d[QG][NEN][2];
     T c[QG][10];
+    if constexpr (INV && EARLY) {
+        // the bulk stream starts right after the descriptor; the gather below then returns behind it
+        prefetch_gauss_points<NEN, NQ, BS, PF, T, true>(dN, C, ne, e, t, d, c);
+        __builtin_amdgcn_sched_barrier(0);
+    }
 
     // 1. gather: node ids, then the nodal values; the element data of the first PF Gauss points is requested
-    // behind them, so the gather's loads return first
+    // behind them (unless EARLY), so the gather's loads return first
     constexpr int NG = (MAXLN + BS - 1) / BS;
     uint32_t gid[NG];
     V2<T> gv[NG];
@@ -657,7 +663,7 @@ __global__ __launch_bounds__(BS, MINW) void k_apply(const dfea_desc* __restrict_
         inv_load<NEN, BS>(D, ch, t, inv);
     }
     __builtin_amdgcn_sched_barrier(0);
-    prefetch_gauss_points<NEN, NQ, BS, PF, T, true>(dN, C, ne, e, t, d, c);
+    if constexpr (!(INV && EARLY)) prefetch_gauss_points<NEN, NQ, BS, PF, T, true>(dN, C, ne, e, t, d, c);
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int i = 0; i < NG; ++i) {
@@ -1808,10 +1814,12 @@ struct Launch_apply {
         const T* vv = static_cast<const T*>(v);
         T* yy = static_cast<T*>(y);
         // linear elements: Gauss points requested one step ahead while the launch fits in the workgroup slots
-        // of the (costlier in registers) ahead variant
+        // of the (costlier in registers) ahead variant; larger quad4 launches start the element stream before
+        // the gather instead
         const bool ahead = NEN <= 4 && n <= APPLY_AHEAD_ITEMS;
         int rc = ahead ? launch(k_apply<NEN, nq_of<NEN>(), BS, T, 2, 1, NEN <= 4>, n, BS, st, "k_apply", t, it, vv, yy)
-                       : launch(k_apply<NEN, nq_of<NEN>(), BS, T>, n, BS, st, "k_apply", t, it, vv, yy);
+                       : launch(k_apply<NEN, nq_of<NEN>(), BS, T, 2, 1, false, NEN == 4>, n, BS, st, "k_apply", t,
+                                it, vv, yy);
         if (rc) return rc;
         return launch(k_apply_fin<T>, nf, FBS, st, "k_apply_fin", t, fi, vv, yy);
     }

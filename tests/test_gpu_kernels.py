@@ -110,9 +110,11 @@ def test_adjoint_gradient_kernel_matches_oracle_vjp(et, n):
     assert rel(g, ref) < 1e-11
 
 
-@pytest.mark.parametrize("et,n", [("quad4", 400), ("quad9", 200)])
+@pytest.mark.parametrize("et,n", [("quad4", 400), ("quad9", 200), ("quad4", 566)])
 def test_apply_at_full_size(et, n):
-    """The bench workloads: 400x400 Q4 and 200x200 Q9 with the stretched, noisy state."""
+    """The bench workloads: 400x400 Q4 and 200x200 Q9 with the stretched, noisy state; 566x566 Q4 is a launch of
+    1,252 chunks, beyond the workgroup slots of the issue-ahead variant (the element stream starts before the
+    gather there)."""
     from difffea_amd.operators import make_tangent_operator
     p = fem.precompute_geometry(unit_square(et, n))
     u = stretch_state(p, 1.0 / n, seed=0)
