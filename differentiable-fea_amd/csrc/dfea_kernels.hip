@@ -595,9 +595,10 @@ __device__ __forceinline__ void element_product(const ChunkView& ch, int t, bool
 }
 
 // PF: Gauss points whose data is loaded before the gather barrier; MINW: minimum waves per SIMD the
-// register allocation must allow (1 = no constraint); EARLY (partials plans): those PF Gauss points are
+// register allocation must allow (1 = no constraint); STREAM_FIRST (partials plans): those PF Gauss points are
 // requested before the gather rather than behind it
-template <int NEN, int NQ, int BS, typename T, int PF = 2, int MINW = 1, bool AHEAD = false, bool EARLY = false>
+template <int NEN, int NQ, int BS, typename T, int PF = 2, int MINW = 1, bool AHEAD = false,
+          bool STREAM_FIRST = false>
 __global__ __launch_bounds__(BS, MINW) void k_apply(const dfea_desc* __restrict__ tab, const int* __restrict__ items,
                                                    const T* __restrict__ vin, T* __restrict__ vout) {
     // partials plans (chunks of exactly CE elements) sum nodes through the inverse map, halo plans by colour
@@ -630,14 +631,14 @@ __global__ __launch_bounds__(BS, MINW) void k_apply(const dfea_desc* __restrict_
     constexpr int QG = GaussSplit<NQ, Groups<NEN, BS>::G>::QG;
     T d[QG][NEN][2];
     T c[QG][10];
-    if constexpr (INV && EARLY) {
+    if constexpr (INV && STREAM_FIRST) {
         // the bulk stream starts right after the descriptor; the gather below then returns behind it
         prefetch_gauss_points<NEN, NQ, BS, PF, T, true>(dN, C, ne, e, t, d, c);
         __builtin_amdgcn_sched_barrier(0);
     }
 
     // 1. gather: node ids, then the nodal values; the element data of the first PF Gauss points is requested
-    // behind them (unless EARLY), so the gather's loads return first
+    // behind them (unless STREAM_FIRST), so the gather's loads return first
     constexpr int NG = (MAXLN + BS - 1) / BS;
     uint32_t gid[NG];
     V2<T> gv[NG];
@@ -663,7 +664,7 @@ __global__ __launch_bounds__(BS, MINW) void k_apply(const dfea_desc* __restrict_
         inv_load<NEN, BS>(D, ch, t, inv);
     }
     __builtin_amdgcn_sched_barrier(0);
-    if constexpr (!(INV && EARLY)) prefetch_gauss_points<NEN, NQ, BS, PF, T, true>(dN, C, ne, e, t, d, c);
+    if constexpr (!(INV && STREAM_FIRST)) prefetch_gauss_points<NEN, NQ, BS, PF, T, true>(dN, C, ne, e, t, d, c);
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int i = 0; i < NG; ++i) {
