@@ -202,6 +202,19 @@ def element_products(steps=200):
     return out
 
 
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo), or the platform's processor string."""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+    return platform.processor() or "unknown"
+
+
 def cpu_baseline(p, u, v, theta, seconds=10.0):
     """The oracle's 'qp' restatement (gather, einsum, batched 4x4 matmul, einsum, index_add_), FP64."""
     from oracle import fem
@@ -222,6 +235,7 @@ def cpu_baseline(p, u, v, theta, seconds=10.0):
     n_el, nen = p["conn"].shape
     B = b_alg(n_el, nen, p["dN_dX"].shape[1], u.numel(), 8)
     return dict(value=B / med / 1e9, unit="GB/s", cores=torch.get_num_threads(), kind="port",
+                cpu_model=cpu_model(), os_cpu_count=os.cpu_count(),
                 sample=f"400x400 Q4 FP64 A(v), oracle qp restatement, median of {len(times)} products "
                        f"({med * 1e3:.2f} ms each) after 2 warm-ups",
                 ms_per_product=med * 1e3)

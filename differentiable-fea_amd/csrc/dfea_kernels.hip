@@ -196,6 +196,18 @@ __device__ __forceinline__ constexpr int uidx(int m, int n) {
     return m <= n ? (m * 4 - (m * (m - 1)) / 2 + (n - m)) : (n * 4 - (n * (n - 1)) / 2 + (m - n));
 }
 
+// value of a constrained dof of A(v) = m Jv + (1 - m) v (`operators.py:380,410`): 0 * Jv + v, which is v for
+// finite Jv and NaN where an inverted element made the moduli NaN, as in the reference
+template <typename T>
+__device__ __forceinline__ T pivot(T jv, T v) { return (T)0 * jv + v; }
+
+// R = m (R_int - lf f) (`operators.py:213-220`): constrained dofs are 0 * (R_int - lf f), NaN where an
+// inverted element made R_int NaN, as in the reference
+__device__ __forceinline__ double masked(bool fixed, double r) { return fixed ? 0.0 * r : r; }
+
+// the clamped Jacobi entry of a free dof (`operators.py:555`): max(a, 1e-6) with NaN kept, as torch.clamp does
+__device__ __forceinline__ double clamp_diag(double a) { return a < 1e-6 ? 1e-6 : a; }
+
 // the scalar fields of a chunk record; the colour table is read through a pointer (a local copy of the
 // record indexed by a loop variable would be placed in scratch memory)
 struct ChunkView {
@@ -695,8 +707,8 @@ __global__ __launch_bounds__(BS, MINW) void k_apply(const dfea_desc* __restrict_
         if (k < ch.nown) {
             const V2<T> raw = L.v[k];
             V2<T> out;
-            out.x = (gid[i] & DFEA_FIXED_X) ? raw.x : acc.x;
-            out.y = (gid[i] & DFEA_FIXED_Y) ? raw.y : acc.y;
+            out.x = (gid[i] & DFEA_FIXED_X) ? pivot(acc.x, raw.x) : acc.x;
+            out.y = (gid[i] & DFEA_FIXED_Y) ? pivot(acc.y, raw.y) : acc.y;
             reinterpret_cast<V2<T>*>(y)[(int)(gid[i] & DFEA_NODE_MASK)] = out;
         } else if (D.n_part) {
             part[slot[i]] = acc;
@@ -746,8 +758,8 @@ __global__ __launch_bounds__(FBS) void k_apply_fin(const dfea_desc* __restrict__
         const int n = (int)(g & DFEA_NODE_MASK);
         const V2<T> raw = reinterpret_cast<const V2<T>*>(v)[n];
         V2<T> out;
-        out.x = (g & DFEA_FIXED_X) ? raw.x : acc.x;
-        out.y = (g & DFEA_FIXED_Y) ? raw.y : acc.y;
+        out.x = (g & DFEA_FIXED_X) ? pivot(acc.x, raw.x) : acc.x;
+        out.y = (g & DFEA_FIXED_Y) ? pivot(acc.y, raw.y) : acc.y;
         reinterpret_cast<V2<T>*>(y)[n] = out;
     }
 }
@@ -850,8 +862,8 @@ __global__ __launch_bounds__(BS) void k_residual(const dfea_desc* __restrict__ t
             const int n = (int)(D.lnode[ch.k0 + k] & DFEA_NODE_MASK);
             const unsigned char b = L.bits[k];
             V2<double> out;
-            out.x = (b & 1) ? 0.0 : acc.x - lf * D.f[2 * n];
-            out.y = (b & 2) ? 0.0 : acc.y - lf * D.f[2 * n + 1];
+            out.x = masked(b & 1, acc.x - lf * D.f[2 * n]);
+            out.y = masked(b & 2, acc.y - lf * D.f[2 * n + 1]);
             reinterpret_cast<V2<double>*>(R)[n] = out;
         } else if (D.n_part) {
             part[D.pslot[ch.pbase + (k - ch.nown)]] = acc;
@@ -873,8 +885,8 @@ __global__ __launch_bounds__(FBS) void k_residual_fin(const dfea_desc* __restric
     const uint32_t g = D.sh_node[i];
     const int n = (int)(g & DFEA_NODE_MASK);
     V2<double> out;
-    out.x = (g & DFEA_FIXED_X) ? 0.0 : acc.x - lf * D.f[2 * n];
-    out.y = (g & DFEA_FIXED_Y) ? 0.0 : acc.y - lf * D.f[2 * n + 1];
+    out.x = masked(g & DFEA_FIXED_X, acc.x - lf * D.f[2 * n]);
+    out.y = masked(g & DFEA_FIXED_Y, acc.y - lf * D.f[2 * n + 1]);
     reinterpret_cast<V2<double>*>(R)[n] = out;
 }
 
@@ -896,8 +908,8 @@ __device__ __forceinline__ void store_precond(const dfea_desc& D, int n, unsigne
         d1 = b00 / det;
         out[2 * D.n_nodes + n] = (CT)(-b01 / det);
     } else {
-        d0 = (b & 1) ? 1.0 : fmax(a.x, 1e-6);
-        d1 = (b & 2) ? 1.0 : fmax(a.y, 1e-6);
+        d0 = (b & 1) ? 1.0 : clamp_diag(a.x);
+        d1 = (b & 2) ? 1.0 : clamp_diag(a.y);
     }
     out[2 * n] = (CT)d0;
     out[2 * n + 1] = (CT)d1;
@@ -1553,7 +1565,7 @@ __global__ __launch_bounds__(BS, MINW) void k_pcg(const dfea_desc* __restrict__ 
         } else {
             beta = k >= 1.0 ? gam / S[PS_GAM] : 0.0;
             den = k >= 1.0 ? delta - beta * gam / S[PS_ALPHA] : delta;
-            if (!(den > 0.0)) {
+            if (den <= 0.0) {      // `newton_krylov.py:70-71`: NaN runs on, as in the reference
                 st = DFEA_PCG_CURVATURE;
                 iters = k;
             } else {

@@ -64,10 +64,13 @@ class HipMesh:
             raise ValueError("the plan was built for another mesh")
         # element kind of every launch: nen, flagged for halo plans
         self.kind = self.nen | (_lib.HALO if p.mode == "halo" else 0)
-        if p.validate() > self.lib.dfea_chunk_threads(self.kind):
+        inv_off, inv_slot = p.inverse_map() if p.mode == "partials" else (np.zeros(1, np.int32), np.zeros(1, np.uint16))
+        if p.validate(self.lib.dfea_chunk_nodes(), (inv_off, inv_slot) if p.mode == "partials" else None) > \
+                self.lib.dfea_chunk_threads(self.kind):
             raise ValueError("chunk larger than the kernel's workgroup")
         E = self.lib.dfea_chunk_elems(self.nen)
-        if p.mode == "partials" and not np.array_equal(p.chunks[:, 0], np.arange(p.n_chunks) * E):
+        if p.mode == "partials" and (not np.array_equal(p.chunks[:, 0], np.arange(p.n_chunks) * E) or
+                                     p.chunks[:, 1].max() > E):
             raise ValueError(f"partials plans must cut the element order into chunks of {E} elements")
         self.n_el, self.n_nodes = p.n_el, p.n_nodes
         self.n_dof = 2 * p.n_nodes
@@ -80,7 +83,6 @@ class HipMesh:
         self.t_pslot = _dev_int32(p.pslot if len(p.pslot) else np.zeros(1, np.int32), dv)
         self.t_hel = _dev_int32(p.hel if len(p.hel) else np.zeros(1, np.int32), dv)
         self.t_prange = _dev_int32(p.prange if p.n_part else np.zeros((1, 2), np.int32), dv)
-        inv_off, inv_slot = p.inverse_map() if p.mode == "partials" else (np.zeros(1, np.int32), np.zeros(1, np.uint16))
         self.t_inv_off = _dev_int32(inv_off, dv)
         self.t_inv_slot = torch.from_numpy(inv_slot.view(np.int16)).to(dv)
         self.t_hlelem = torch.from_numpy(np.ascontiguousarray(p.hlelem).view(np.int16).reshape(-1) if p.hlelem.size

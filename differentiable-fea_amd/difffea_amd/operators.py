@@ -65,21 +65,38 @@ def get_geometry(params, device=None):
     return g["dN_dX"], g["w_detJ"]
 
 
+def _mesh_tensors(params):
+    return params["nodes"], params["conn"], params["fixed"]
+
+
 def get_mesh(params, device=None):
-    """Device plan + geometry of ``params`` (built once, cached while the mesh tensors are alive)."""
+    """Device plan + geometry of ``params``: built once and reused while the mesh tensors (nodes, conn, fixed)
+    are alive and unmodified (an in-place edit bumps a tensor's version and the mesh is rebuilt, as the
+    reference recomputes from the tensors on every call); an entry is dropped as soon as one of its tensors
+    is freed, so the device memory of meshes the caller no longer holds is released."""
     device = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
     if device.type != "cuda":
         raise ValueError("the hip variant runs on a GPU: move the state to a cuda (ROCm) device")
-    key = (id(params["nodes"]), id(params["conn"]), id(params["fixed"]), str(device))
+    ts = _mesh_tensors(params)
+    key = (tuple(id(t) for t in ts), str(device))
+    versions = tuple(t._version for t in ts)
     ent = _MESHES.get(key)
     if ent is not None:
-        refs, mesh = ent
-        if all(r() is not None for r in refs):
+        refs, vers, mesh = ent
+        if vers == versions and all(r() is t for r, t in zip(refs, ts)):
             return mesh
+        del _MESHES[key]
     with torch.cuda.device(device):
         mesh = HipMesh(params, device)
-    _MESHES[key] = ([weakref.ref(params["nodes"]), weakref.ref(params["conn"]), weakref.ref(params["fixed"])], mesh)
+    # the callback runs when a tensor is freed (before its id can be reused), so a key never outlives them
+    refs = [weakref.ref(t, lambda _r, k=key: _MESHES.pop(k, None)) for t in ts]
+    _MESHES[key] = (refs, versions, mesh)
     return mesh
+
+
+def cached_meshes():
+    """Number of device meshes held by the cache (for tests and diagnostics)."""
+    return len(_MESHES)
 
 
 def make_system(u, theta, params, precision="fp64", preconditioner="jacobi"):

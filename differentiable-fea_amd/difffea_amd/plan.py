@@ -135,6 +135,12 @@ def jones_plassmann(conn, seed=0):
     return colour
 
 
+
+def _require(ok, what):
+    if not ok:
+        raise ValueError(f"invalid mesh plan: {what}")
+
+
 class MeshPlan:
     """Integer tables of one mesh + Dirichlet mask (numpy arrays, see module docstring)."""
 
@@ -394,39 +400,65 @@ class MeshPlan:
         s = np.searchsorted(self.sh_off, self.pslot, side="right") - 1      # shared index of each entry
         return np.stack([self.sh_off[s], self.sh_off[s + 1] - self.sh_off[s]], axis=1).astype(np.int32)
 
-    def validate(self):
-        """Host-side bounds checks of every index the kernels follow (run before any launch).
+    def validate(self, max_nodes=None, inv=None):
+        """Host-side bounds checks of every index the kernels follow (run before any launch); raises ValueError
+        on the first violation.  max_nodes: the kernels' local-node capacity per chunk (LDS), checked when
+        given; inv: the (inv_off, inv_slot) tables of a partials plan, checked when given.
         Returns the largest number of threads a chunk needs (core + halo elements)."""
         ch = self.chunks
-        assert ch[:, 0].min() >= 0 and (ch[:, 0] + ch[:, 1]).max() <= self.n_el
-        assert (ch[:, 2] + ch[:, 3]).max() <= self.n_lnodes and (ch[:, 4] <= ch[:, 3]).all()
+        nen = self.nen
+        _require(ch[:, 0].min() >= 0 and (ch[:, 0] + ch[:, 1]).max() <= self.n_el, "chunk element range")
+        _require((ch[:, 2] + ch[:, 3]).max() <= self.n_lnodes and (ch[:, 4] <= ch[:, 3]).all(), "chunk node range")
+        _require(ch[:, 3].min() >= 1, "chunk without nodes")
+        if max_nodes is not None:
+            _require(int(ch[:, 3].max()) <= max_nodes,
+                     f"a chunk touches {int(ch[:, 3].max())} nodes, more than the kernels' {max_nodes}")
         if self.mode == "partials":
-            assert (ch[:, 5] + ch[:, 3] - ch[:, 4]).max() <= max(self.n_part, 0)
-            assert (ch[:, 8] == 0).all()
+            _require((ch[:, 5] + ch[:, 3] - ch[:, 4]).max() <= max(self.n_part, 0), "chunk partials range")
+            _require((ch[:, 8] == 0).all(), "halo elements in a partials plan")
         else:
-            assert self.n_part == 0 and ch[:, 4].sum() == self.n_nodes
-            assert (ch[:, 7] + ch[:, 8]).max() <= len(self.hel)
+            _require(self.n_part == 0 and ch[:, 4].sum() == self.n_nodes, "halo plan ownership")
+            _require((ch[:, 7] + ch[:, 8]).max() <= len(self.hel), "halo range")
             if len(self.hel):
-                assert self.hel.min() >= 0 and self.hel.max() < self.n_el
+                _require(self.hel.min() >= 0 and self.hel.max() < self.n_el, "halo element id")
                 ch_of_h = np.repeat(np.arange(self.n_chunks), ch[:, 8])
-                assert (self.hlelem.astype(np.int64) < ch[ch_of_h, 3][None, :]).all()
+                _require((self.hlelem.astype(np.int64) < ch[ch_of_h, 3][None, :]).all(), "halo local node index")
         # every element's local indices fall inside its chunk's node range
         chunk_of_e = np.repeat(np.arange(self.n_chunks), ch[:, 1])
-        assert (self.lelem.astype(np.int64) < ch[chunk_of_e, 3][None, :]).all()
+        _require((self.lelem.astype(np.int64) < ch[chunk_of_e, 3][None, :]).all(), "element local node index")
         # colour ranges tile the core and halo threads
-        for c in range(self.n_chunks):
-            nc = ch[c, 6]
-            assert 1 <= nc <= _lib.MAXCOL
-            assert ch[c, _lib.COL_OFF] == 0 and ch[c, _lib.COL_OFF + nc] == ch[c, 1]
-            assert ch[c, _lib.HCOL_OFF] == 0 and ch[c, _lib.HCOL_OFF + nc] == ch[c, 8]
-        assert (self.lnode & _lib.NODE_MASK).max() < self.n_nodes
+        nc = ch[:, 6]
+        _require(((nc >= 1) & (nc <= _lib.MAXCOL)).all(), "colour count")
+        rows = np.arange(self.n_chunks)
+        _require((ch[:, _lib.COL_OFF] == 0).all() and (ch[rows, _lib.COL_OFF + nc] == ch[:, 1]).all(), "colour ranges")
+        _require((ch[:, _lib.HCOL_OFF] == 0).all() and (ch[rows, _lib.HCOL_OFF + nc] == ch[:, 8]).all(),
+                 "halo colour ranges")
+        for c in range(_lib.MAXCOL):      # colour ranges non-decreasing up to the colour count
+            live = c < nc
+            _require((ch[live, _lib.COL_OFF + c] <= ch[live, _lib.COL_OFF + c + 1]).all(), "colour order")
+        _require((self.lnode & _lib.NODE_MASK).max() < self.n_nodes, "local node id")
         if self.n_shared:
-            assert self.sh_off[0] == 0 and self.sh_off[-1] == len(self.pslot) == self.n_part
-            assert np.array_equal(np.sort(self.pslot), np.arange(self.n_part))
-            assert (self.sh_node & _lib.NODE_MASK).max() < self.n_nodes
-            assert len(np.unique(self.sh_node & _lib.NODE_MASK)) == self.n_shared
-            assert np.array_equal(self.sh_node >> 30, self.nfix[self.sh_node & _lib.NODE_MASK])
-        assert self.conn_int.min() >= 0 and self.conn_int.max() < self.n_nodes
+            _require(self.sh_off[0] == 0 and self.sh_off[-1] == len(self.pslot) == self.n_part, "shared slot ranges")
+            _require((np.diff(self.sh_off) >= 1).all(), "shared node without slots")
+            _require(np.array_equal(np.sort(self.pslot), np.arange(self.n_part)), "partial slots")
+            _require((self.sh_node & _lib.NODE_MASK).max() < self.n_nodes, "shared node id")
+            _require(len(np.unique(self.sh_node & _lib.NODE_MASK)) == self.n_shared, "shared node repeated")
+            _require(np.array_equal(self.sh_node >> 30, self.nfix[self.sh_node & _lib.NODE_MASK]), "shared fixed bits")
+            pr = self.prange
+            _require(pr[:, 0].min() >= 0 and (pr[:, 1] >= 1).all() and (pr[:, 0] + pr[:, 1]).max() <= self.n_part,
+                     "partials entry slot range")
+            _require(((self.pslot >= pr[:, 0]) & (self.pslot < pr[:, 0] + pr[:, 1])).all(),
+                     "a chunk's slot outside its node's range")
+        if inv is not None:
+            inv_off, inv_slot = inv
+            _require(len(inv_off) == self.n_lnodes + 1 and len(inv_slot) == self.n_el * nen, "inverse map size")
+            _require(inv_off[0] == 0 and (np.diff(inv_off) >= 0).all(), "inverse map offsets not monotone")
+            _require((inv_off[ch[:, 2]] == ch[:, 0] * nen).all() and
+                     (inv_off[ch[:, 2] + ch[:, 3]] == (ch[:, 0] + ch[:, 1]) * nen).all(),
+                     "inverse map offsets outside their chunk")
+            ent_chunk = np.repeat(np.arange(self.n_chunks), ch[:, 1] * nen)
+            _require((inv_slot.astype(np.int64) < ch[ent_chunk, 1] * nen).all(), "inverse map slot")
+        _require(self.conn_int.min() >= 0 and self.conn_int.max() < self.n_nodes, "connectivity")
         return int((ch[:, 1] + ch[:, 8]).max())
 
     def stats(self):

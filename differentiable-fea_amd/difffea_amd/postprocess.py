@@ -10,7 +10,8 @@
   the internal force (all-ones mask, no external load) summed over the constrained dofs
   (``dfea_reactions``: per-chunk sums in a fixed order, no atomics).
 
-Neo-Hookean with theta scaling both parameters, quad4 and quad9, as the rest of the ``hip`` variant.
+Every element type (tri3, quad4, tri6, quad8, quad9) and every library material (neo-Hookean, Mooney-Rivlin,
+Saint Venant-Kirchhoff, Demiray, HGO fibres), as the rest of the ``hip`` variant.
 """
 import torch
 

@@ -28,12 +28,13 @@ import torch
 
 from . import _lib
 from .device import HipMesh, HipSystem, _ptr, _stream
-from .elements import FAMILY, NEN, ORDER, default_nq
+from .elements import FAMILY, NEN, ORDER, default_nq, element_type_of
 from .materials import material_spec
 from .plan import MeshPlan
 from .problem import build_problem
 
 MIXED_RTOL_FLOOR = 1e-6
+HBM_SHARE = 0.8          # share of the free device memory the server admits requests into by default
 
 
 # ------------------------------------------------------------------------------------------------------------
@@ -51,6 +52,13 @@ class Request:
     n_load_steps: int = 3
     adjoint: bool = False          # compliance gradient w.r.t. per-element rho (force-loaded request)
     seed: int = 0
+    # a caller's own problem instead of the unit square above: the reference's params dictionary and theta
+    # (element_type, n, mu, lam and stretch are then unused)
+    params: dict = None
+    theta: torch.Tensor = None
+    tol: float = None              # absolute tolerance on ||R|| (None: the server's relative tolerance)
+    max_newton: int = 25
+    max_cutbacks: int = 8
 
 
 def make_stream(n_requests=2000, rate=50.0, seed=5, side=(32, 512), q4_fraction=0.7, adjoint_fraction=0.2):
@@ -83,7 +91,11 @@ def fifo_latencies(arrivals, service_times):
 
 
 def request_problem(req):
-    """params + theta of a request (unit square, left edge clamped, right edge loaded)."""
+    """params + theta of a request (its own problem, or the unit square, left edge clamped, right edge
+    loaded)."""
+    if req.params is not None:
+        theta = req.theta if req.theta is not None else torch.ones(req.params["conn"].shape[0], dtype=torch.float64)
+        return req.params, theta.detach().to("cpu", torch.float64)
     if req.adjoint:
         # dead traction on the right edge: total force = stretch * E (uniaxial estimate), so strains match
         E = req.mu * (3 * req.lam + 2 * req.mu) / (req.lam + req.mu)
@@ -120,17 +132,28 @@ def reference_residual_norm(sys_, params, device):
     return float(torch.norm(R)) / LIFT_PROBE
 
 
-def footprint_bytes(element_type, n, precision, preconditioner="jacobi"):
+def footprint_bytes(element_type, n, precision, preconditioner="jacobi", n_el=None, n_dof=None):
     """HBM bytes a request holds while admitted (geometry, moduli, vectors, preconditioner, plan), an upper
-    estimate."""
+    estimate; n_el / n_dof: the sizes of a caller's own mesh (else those of the n x n unit square)."""
     nen, nq = NEN[element_type], default_nq(element_type)
-    n_el = (2 if FAMILY[element_type] == "tri" else 1) * n * n
-    side = n + 1 if ORDER[element_type] == 1 else 2 * n + 1
-    n_dof = 2 * (side * side - (n * n if element_type == "quad8" else 0))
+    if n_el is None:
+        n_el = (2 if FAMILY[element_type] == "tri" else 1) * n * n
+        side = n + 1 if ORDER[element_type] == 1 else 2 * n + 1
+        n_dof = 2 * (side * side - (n * n if element_type == "quad8" else 0))
     s = 8 if precision == "fp64" else 4
     per_qp = 2 * nen * 12 + 8 + 10 * s
     pre = n_dof * s // 2 if preconditioner == "block_jacobi" else 0     # the off-diagonal inverse entries
     return n_el * nq * per_qp + n_dof * (12 * s + 5 * 8) + pre + n_el * nen * 8 + 64 * n_el + (1 << 20)
+
+
+def request_footprint(req, precision, preconditioner="jacobi"):
+    """footprint_bytes of a request."""
+    if req.params is not None:
+        conn = req.params["conn"]
+        et = req.params.get("element_type") or element_type_of(conn.shape[1])
+        return footprint_bytes(et, 0, precision, preconditioner, n_el=conn.shape[0],
+                               n_dof=2 * req.params["nodes"].shape[0])
+    return footprint_bytes(req.element_type, req.n, precision, preconditioner)
 
 
 # ------------------------------------------------------------------------------------------------------------
@@ -155,14 +178,19 @@ def solve_ops(job, tol, max_newton=25, max_cutbacks=8, cg_max_iter=500):
     """The reference's Newton-Krylov state machine, as a generator of device operations.
 
     yields ("lift", lf) -> None, ("resnorm", lf) -> ||R||, ("moduli",) -> None,
-    ("pcg", rtol, max_iter, neg_res) -> iterations, ("trial", alpha) -> None, ("accept",) -> None
+    ("pcg", rtol, max_iter, neg_res) -> iterations, ("trial", alpha) -> None, ("accept",) -> None,
+    ("restore",) -> None.  Returns the reference's ``info`` entries (`newton_krylov.py:226-264`: converged,
+    load_factor, history of per-iteration records, total_newton, total_cg, n_cutbacks) plus the last
+    residual norm.
     """
-    info = {"total_newton": 0, "total_cg": 0, "n_cutbacks": 0, "converged": False, "residual": float("nan")}
+    info = {"total_newton": 0, "total_cg": 0, "n_cutbacks": 0, "converged": False, "residual": float("nan"),
+            "load_factor": 0.0, "history": []}
     mixed = job.sys.precision == "mixed"
     d_nom = 1.0 / job.req.n_load_steps
-    d, lf, in_row = d_nom, 0.0, 0
+    d, lf, in_row, step = d_nom, 0.0, 0, 0
     while lf < 1.0 - 1e-12:
         lf_try = min(1.0, lf + d)
+        step += 1
         # ---- newton_at_load ----
         yield ("lift", lf_try)                    # vec[u] = where(fixed, lf u_D, u_acc)
         yield ("accept",)                         # the lifted state is the Newton start (u_acc = vec[u])
@@ -191,11 +219,13 @@ def solve_ops(job, tol, max_newton=25, max_cutbacks=8, cg_max_iter=500):
                     rn, accepted = rc, True
                     break
                 alpha *= 0.5
+            info["history"].append(dict(step=step, load_factor=lf_try, iteration=it, residual=float(rn),
+                                        cg_iters=int(k), alpha=alpha))
             if not accepted:
                 break
         if ok:
             lf = lf_try
-            info["last_good"] = lf
+            info["load_factor"] = lf
             in_row = 0
             d = min(d_nom, 1.5 * d)
             # keep the converged state as the base of the next load step (u_acc already holds it)
@@ -204,11 +234,14 @@ def solve_ops(job, tol, max_newton=25, max_cutbacks=8, cg_max_iter=500):
             info["n_cutbacks"] += 1
             if in_row > max_cutbacks:
                 info["residual"] = rn
+                info["load_factor"] = lf
+                yield ("restore",)                # the result is the last converged state
                 return info
             d *= 0.5
             yield ("restore",)                    # u_acc = last converged state
     info["converged"] = True
     info["residual"] = rn
+    info["load_factor"] = 1.0
     if job.req.adjoint:
         yield ("moduli",)                         # tangent at u* (vec[u] holds the accepted state)
         k = yield ("adjoint",)                    # A lam = m f (compliance: dL/du = f); then the gradient kernel
@@ -232,7 +265,7 @@ def _mat_kind(jobs, nen):
 class Server:
     """Continuous-batching solver for a stream of requests on one GPU (see module docstring)."""
 
-    def __init__(self, precision="fp64", max_slots=64, hbm_budget=64 << 30, pcg_chunk=8, rel_tol=1e-8,
+    def __init__(self, precision="fp64", max_slots=64, hbm_budget=None, pcg_chunk=8, rel_tol=1e-8,
                  device=None, adjoint_rtol=1e-8, cg_max_iter=20000, preconditioner="jacobi"):
         self.lib = _lib.load()
         self.device = torch.device(device or "cuda")
@@ -242,6 +275,11 @@ class Server:
         self.precision = precision
         self.prec_id = _lib.FP64 if precision == "fp64" else _lib.FP32
         self.max_slots = max_slots
+        if hbm_budget is None:
+            # admission budget: a share of the device memory free when the server starts (the rest stays for
+            # the caching allocator's slack and other users of the device)
+            free, _ = torch.cuda.mem_get_info(self.device)
+            hbm_budget = int(HBM_SHARE * free)
         self.hbm_budget = hbm_budget
         self.pcg_chunk = pcg_chunk
         self.rel_tol = rel_tol
@@ -299,17 +337,18 @@ class Server:
         self.table[slot * self.dsz:(slot + 1) * self.dsz].copy_(
             torch.frombuffer(bytearray(bytes(sys_.desc)), dtype=torch.uint8))
         job = Job(req, slot, mesh, sys_, params, theta)
-        job.bytes = footprint_bytes(req.element_type, req.n, self.precision, self.preconditioner)
+        job.bytes = request_footprint(req, self.precision, self.preconditioner)
         self.used_bytes += job.bytes
         pl = mesh.plan
         job.items = {"chunk": self._items(slot, pl.n_chunks), "fin": self._items(slot, mesh.n_finblk),
                      "vec": self._items(slot, mesh.n_vblk), "solve": self._items(slot, 1)}
-        job.tol = self.rel_tol * reference_residual_norm(sys_, params, self.device)
+        job.tol = req.tol if req.tol is not None else self.rel_tol * reference_residual_norm(sys_, params, self.device)
         if req.adjoint:              # right-hand side of the compliance adjoint, m f
             job.bvec = (params["m"] * params["f"]).to(self.device, sys_.vdt)
         self._item_cache.clear()
         job.saved = torch.zeros_like(sys_.vec["uacc"])
-        job.gen = solve_ops(job, job.tol, cg_max_iter=self.cg_max_iter)
+        job.gen = solve_ops(job, job.tol, max_newton=req.max_newton, max_cutbacks=req.max_cutbacks,
+                            cg_max_iter=self.cg_max_iter)
         job.op = next(job.gen)
         job.t_admit = time.perf_counter()
         self.jobs[slot] = job
@@ -492,6 +531,7 @@ class Server:
         pool = ThreadPoolExecutor(max_workers=self.prep_workers)
         t0 = time.perf_counter()
         i = 0
+        peak_jobs = peak_bytes = 0
 
         def prepare_arrived():
             # start the host-only admission work of the next arrived requests on the worker thread
@@ -507,7 +547,7 @@ class Server:
                 prepare_arrived()
                 while i < len(pending) and (not realtime or pending[i].arrival <= now) and self.free:
                     r = pending[i]
-                    fb = footprint_bytes(r.element_type, r.n, self.precision, self.preconditioner)
+                    fb = request_footprint(r, self.precision, self.preconditioner)
                     if self.jobs and self.used_bytes + fb > self.hbm_budget:
                         break
                     fut = futures.get(i)
@@ -515,6 +555,8 @@ class Server:
                         break                      # admitted once its host set-up is done; the GPU keeps working
                     job = self._admit(r, futures.pop(i).result() if fut is not None else None)
                     job.t_arrival = t0 + r.arrival if realtime else job.t_admit
+                    peak_jobs = max(peak_jobs, len(self.jobs))
+                    peak_bytes = max(peak_bytes, self.used_bytes)
                     i += 1
                     prepare_arrived()
                 if not self.jobs:
@@ -543,5 +585,6 @@ class Server:
                        latency_from="arrival" if realtime else "admission",
                        solves_per_s=len(records) / wall, p50_latency_s=float(np.percentile(lat, 50)),
                        p99_latency_s=float(np.percentile(lat, 99)),
-                       converged=int(sum(r["converged"] for r in records)))
+                       converged=int(sum(r["converged"] for r in records)),
+                       hbm_budget=self.hbm_budget, peak_admitted=peak_jobs, peak_admitted_bytes=peak_bytes)
         return records, summary

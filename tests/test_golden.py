@@ -1,53 +1,50 @@
 """The oracle against vectors written by the reference itself (tests/golden/make_golden.py):
 G2 = Newton-Krylov trajectories, G3 = the reference's torch.func implicit-function adjoint."""
-import json
-import os
-
 import numpy as np
 import pytest
 import torch
 
 from oracle import fem
-from difffea_amd.problem import build_problem
-
-GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
-
-
-def _load(name):
-    z = np.load(os.path.join(GOLD, name + ".npz"))          # allow_pickle=False (default)
-    return z, json.loads(str(z["spec"]))
+from helpers import GOLDEN_SOLVES, check_solution, check_trajectory, golden_max_newton, golden_problem, \
+    history_array, load_golden as _load
 
 
 def _params(spec):
-    sp = dict(spec)
-    sp.pop("mesher", None)
-    return fem.precompute_geometry(build_problem(sp))
+    return fem.precompute_geometry(golden_problem(spec))
 
 
-@pytest.mark.parametrize("name", ["g2_q4_disp_20", "g2_q9_force_8", "g2_q4_force_16"])
+@pytest.mark.parametrize("name", GOLDEN_SOLVES)
 def test_oracle_reproduces_reference_newton_trajectory(name):
     z, spec = _load(name)
     p = _params(spec)
     th = torch.ones(p["conn"].shape[0], dtype=torch.float64)
-    u, info = fem.newton_krylov_solve(p, th, int(z["n_load_steps"]), float(z["tol"]))
+    records = []
+    u, info = fem.newton_krylov_solve(p, th, int(z["n_load_steps"]), float(z["tol"]), golden_max_newton(z),
+                                      callback=records.append)
     assert info["converged"] == bool(z["converged"])
-    ref_u = torch.from_numpy(z["u"])
-    assert float((u - ref_u).abs().max() / ref_u.abs().max()) < 1e-10
-    hist = np.array([[r["step"], r["load_factor"], r["iteration"], r["residual"], r["cg_iters"], r["alpha"]]
-                     for r in info["history"]])
-    ref = z["history"]
-    assert hist.shape == ref.shape
-    assert np.array_equal(hist[:, [0, 2, 5]], ref[:, [0, 2, 5]])      # step, iteration, line-search alpha
-    assert np.allclose(hist[:, 1], ref[:, 1], rtol=0, atol=1e-15)      # load factors
-    # Loose inner solves (eta up to 0.5) amplify round-off: on the quad9 case merely switching the oracle's
-    # contraction order (qp -> qp_ew) moves the first Newton residual by 0.3 %.  Round-off differences with
-    # the reference (geometry, load integration) therefore move residuals by a few per cent and late CG
-    # counts by a few iterations; Newton counts, line-search steps and the solution stay identical.
-    assert np.all(np.abs(hist[:, 4] - ref[:, 4]) <= np.maximum(2, 0.06 * ref[:, 4]))
-    big = ref[:, 3] > 1e-6 * ref[:, 3].max()
-    assert np.allclose(hist[big, 3], ref[big, 3], rtol=5e-2)
+    assert info["n_cutbacks"] == int(z["n_cutbacks"])
+    if "load_factor" in z.files:
+        assert info["load_factor"] == float(z["load_factor"])
+        assert np.array_equal([lf for lf, _ in info["snapshots"]], z["snapshot_lf"])
+    assert records == info["history"]
+    check_solution(u, z)
+    check_trajectory(history_array(info), z)
     assert info["total_newton"] == int(z["total_newton"])
     assert abs(info["total_cg"] - int(z["total_cg"])) <= 0.02 * int(z["total_cg"])
+
+
+def test_cutback_goldens_exercise_the_branches():
+    """The fixtures hold what they are for: NaN residuals and a NaN-driven cut-back, cut-backs after
+    max_newton with backtracking steps, and a give-up at load factor 0."""
+    z, _ = _load("g2_q4_nan_cutback_4")
+    h = z["history"]
+    assert int(z["n_cutbacks"]) >= 1 and np.isnan(h[:, 3]).any() and bool(z["converged"])
+    assert np.all(h[np.isnan(h[:, 3]), 4] == 500)          # NaN PCG runs to the cap, as the reference's does
+    z, _ = _load("g2_q4_maxnewton_cutback")
+    h = z["history"]
+    assert int(z["n_cutbacks"]) >= 1 and (h[:, 5] < 1).any() and bool(z["converged"])
+    z, _ = _load("g2_q4_giveup")
+    assert not bool(z["converged"]) and float(z["load_factor"]) == 0.0 and int(z["n_cutbacks"]) == 9
 
 
 def test_oracle_adjoint_matches_reference_adjoint():

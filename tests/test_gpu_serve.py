@@ -4,7 +4,7 @@ import numpy as np
 import pytest
 import torch
 
-from helpers import rel
+from helpers import check_solution, check_trajectory, golden_max_newton, golden_problem, history_array, load_golden, rel
 
 pytestmark = pytest.mark.gpu
 
@@ -105,3 +105,44 @@ def test_block_jacobi_server_bitwise_solo_and_within_tolerance():
         assert torch.equal(solo[0]["u"], mix[req.rid]["u"])
         if req.adjoint:
             assert torch.equal(solo[0]["grad"], mix[req.rid]["grad"])
+
+
+@pytest.mark.parametrize("name", ["g2_q4_nan_cutback_4", "g2_q4_maxnewton_cutback", "g2_q4_giveup"])
+def test_server_reproduces_reference_cutbacks(name):
+    """A request with the caller's own problem, tolerance and Newton limits follows the reference's
+    trajectory through cut-backs (the state of the last converged level restored, `newton_krylov.py:247-260`)
+    and the give-up exit (result = the last converged state), alone and bitwise the same in a mix."""
+    from difffea_amd.serve import Request, Server
+    z, spec = load_golden(name)
+    p = golden_problem(spec)
+    req = Request(100, 0.0, spec["element_type"], 0, 0.0, 0.0, 0.0, n_load_steps=int(z["n_load_steps"]), params=p,
+                  tol=float(z["tol"]), max_newton=golden_max_newton(z))
+    solo, _ = Server("fp64", max_slots=4, cg_max_iter=500).run([req], realtime=False, keep_solution=True)
+    r = solo[0]
+    assert r["converged"] == bool(z["converged"])
+    assert r["n_cutbacks"] == int(z["n_cutbacks"])
+    assert r["total_newton"] == int(z["total_newton"])
+    assert r["load_factor"] == float(z["load_factor"])
+    check_solution(r["u"], z)
+    check_trajectory(history_array(r), z)
+    mix, _ = Server("fp64", max_slots=16, cg_max_iter=500).run(_stream() + [req], realtime=False, keep_solution=True)
+    m = {x["rid"]: x for x in mix}[100]
+    assert torch.equal(m["u"], r["u"])
+    assert np.array_equal(history_array(m), history_array(r), equal_nan=True)
+
+
+def test_hbm_footprint_admission_binds_and_keeps_results():
+    """With an HBM budget that holds about two of the requests, the server admits by footprint (never more
+    bytes than the budget once more than one solve runs), and every result is the same bits as with room
+    for all of them."""
+    from difffea_amd.serve import Server, request_footprint
+    reqs = _stream()
+    big = max(request_footprint(r, "fp64") for r in reqs)
+    tight, ts = Server("fp64", max_slots=16, hbm_budget=int(2.2 * big)).run(reqs, realtime=False, keep_solution=True)
+    roomy, rs = Server("fp64", max_slots=16).run(reqs, realtime=False, keep_solution=True)
+    assert ts["converged"] == rs["converged"] == len(reqs)
+    assert ts["peak_admitted_bytes"] <= ts["hbm_budget"] and 2 <= ts["peak_admitted"] < rs["peak_admitted"]
+    assert rs["hbm_budget"] > 0
+    roomy = {r["rid"]: r for r in roomy}
+    for r in tight:
+        assert torch.equal(r["u"], roomy[r["rid"]]["u"])

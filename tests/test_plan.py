@@ -195,3 +195,50 @@ def test_plan_rejects_oversized_chunks():
     p = unit_square("quad4", 40)
     with pytest.raises(ValueError):
         _plan(p, max_nodes=100)
+
+
+def _corrupt_cases():
+    """(name, mutate(plan, inv) -> None): each breaks one index the kernels follow."""
+    def set_(attr, idx, val):
+        def f(pl, inv):
+            a = getattr(pl, attr)
+            a[idx] = val
+        return f
+
+    def inv_off_(idx, val):
+        def f(pl, inv):
+            inv[0][idx] = val
+        return f
+
+    def inv_slot_(idx, val):
+        def f(pl, inv):
+            inv[1][idx] = val
+        return f
+    return [
+        ("chunk element range", set_("chunks", (0, 1), 10 ** 6)),
+        ("chunk node count over the LDS capacity", set_("chunks", (0, 3), 700)),
+        ("element local node index", set_("lelem", (0, 0), 999)),
+        ("local node id", set_("lnode", 0, 0x3FFFFFF0)),
+        ("partial slot", set_("pslot", 0, 10 ** 6)),
+        ("shared slot ranges", set_("sh_off", -1, 10 ** 6)),
+        ("connectivity", set_("conn_int", (0, 0), -1)),
+        ("inverse map offsets", inv_off_(3, 10 ** 6)),
+        ("inverse map slot", inv_slot_(5, 4000)),
+    ]
+
+
+@pytest.mark.parametrize("case", range(9))
+def test_validate_rejects_corrupt_tables(case):
+    """MeshPlan.validate raises ValueError (not assert: it must survive python -O) for every corrupted
+    table the kernels would follow without checks."""
+    import copy
+    p = unit_square("quad4", 40)
+    base = MeshPlan(p["nodes"].numpy(), p["conn"].numpy(), p["fixed"].numpy())
+    inv0 = base.inverse_map()
+    base.validate(640, inv0)
+    name, mutate = _corrupt_cases()[case]
+    pl = copy.deepcopy(base)
+    inv = (inv0[0].copy(), inv0[1].copy())
+    mutate(pl, inv)
+    with pytest.raises(ValueError):
+        pl.validate(640, inv)
