@@ -25,6 +25,7 @@
 #include <cstdio>
 #include <cstring>
 #include <cstddef>
+#include <mutex>
 #include <type_traits>
 
 #include "dfea.h"
@@ -612,7 +613,8 @@ __device__ __forceinline__ void element_product(const ChunkView& ch, int t, bool
 template <int NEN, int NQ, int BS, typename T, int PF = 2, int MINW = 1, bool AHEAD = false,
           bool STREAM_FIRST = false>
 __global__ __launch_bounds__(BS, MINW) void k_apply(const dfea_desc* __restrict__ tab, const int* __restrict__ items,
-                                                   const T* __restrict__ vin, T* __restrict__ vout) {
+                                                   const T* __restrict__ vin, T* __restrict__ vout,
+                                                   V2<T>* __restrict__ part_arg, int part_cap) {
     // partials plans (chunks of exactly CE elements) sum nodes through the inverse map, halo plans by colour
     constexpr bool INV = BS % Groups<NEN, BS>::CE == 0;
     __shared__ std::conditional_t<INV, ProductLDS<T>, ChunkLDS<T>> L;
@@ -695,8 +697,9 @@ __global__ __launch_bounds__(BS, MINW) void k_apply(const dfea_desc* __restrict_
     // 2. element products; node sums in colour order (inverse map or colour passes)
     element_product<NEN, NQ, BS, PF, T, true, AHEAD>(ch, t, has_el, l, dN, C, ne, e, d, c, L, GL);
 
-    // 3. owned nodes: final value; shared nodes (partials plans): partial
-    V2<T>* part = reinterpret_cast<V2<T>*>(D.part);
+    // 3. owned nodes: final value; shared nodes (partials plans): partial (into part_arg when given: the
+    // caller's buffer of part_cap entries, every slot checked against it)
+    V2<T>* part = part_arg ? part_arg : reinterpret_cast<V2<T>*>(D.part);
 #pragma unroll
     for (int i = 0; i < NG; ++i) {
         const int k = t + i * BS;
@@ -710,7 +713,7 @@ __global__ __launch_bounds__(BS, MINW) void k_apply(const dfea_desc* __restrict_
             out.x = (gid[i] & DFEA_FIXED_X) ? pivot(acc.x, raw.x) : acc.x;
             out.y = (gid[i] & DFEA_FIXED_Y) ? pivot(acc.y, raw.y) : acc.y;
             reinterpret_cast<V2<T>*>(y)[(int)(gid[i] & DFEA_NODE_MASK)] = out;
-        } else if (D.n_part) {
+        } else if (D.n_part && slot[i] < part_cap) {
             part[slot[i]] = acc;
         }
     }
@@ -744,16 +747,19 @@ __device__ __forceinline__ V2<T> sum_slots(const V2<T>* part, int j0, int cnt) {
 // shared nodes of the tangent product (partials plans): partials summed in slot order
 template <typename T>
 __global__ __launch_bounds__(FBS) void k_apply_fin(const dfea_desc* __restrict__ tab, const int* __restrict__ items,
-                                                  const T* __restrict__ vin, T* __restrict__ vout) {
+                                                  const T* __restrict__ vin, T* __restrict__ vout,
+                                                  const V2<T>* __restrict__ part_arg, int part_cap) {
     const Item it = load_item(items);
     const dfea_desc& D = tab[it.s];
     const int i = it.b * FBS + threadIdx.x;
     const T* v = vin ? vin : static_cast<const T*>(D.vec[DFEA_P]);
     T* y = vout ? vout : static_cast<T*>(D.vec[DFEA_W]);
     if (i < D.n_shared) {
-        const V2<T>* part = reinterpret_cast<const V2<T>*>(D.part);
+        const V2<T>* part = part_arg ? part_arg : reinterpret_cast<const V2<T>*>(D.part);
         const int j0 = D.sh_off[i];
-        const V2<T> acc = sum_slots<T>(part, j0, D.sh_off[i + 1] - j0);
+        const int j1 = D.sh_off[i + 1];
+        if (j1 > part_cap) return;
+        const V2<T> acc = sum_slots<T>(part, j0, j1 - j0);
         const uint32_t g = D.sh_node[i];
         const int n = (int)(g & DFEA_NODE_MASK);
         const V2<T> raw = reinterpret_cast<const V2<T>*>(v)[n];
@@ -1822,6 +1828,8 @@ struct Launch_fields {
 struct Launch_apply {
     const dfea_desc* t; const int32_t* it; int n; const int32_t* fi; int nf; const void* v; void* y; int prec;
     void* st;
+    void* part = nullptr;        // shared-node partials buffer (nullptr: the descriptor's part)
+    int part_cap = 0x7fffffff;   // its capacity in entries
     template <typename T, int NEN, int BS0> int go() {
         constexpr int BS = quadratic<NEN>() ? bs_groups<T>() : BS0;
         const T* vv = static_cast<const T*>(v);
@@ -1830,11 +1838,13 @@ struct Launch_apply {
         // of the (costlier in registers) ahead variant; larger quad4 launches start the element stream before
         // the gather instead
         const bool ahead = NEN <= 4 && n <= APPLY_AHEAD_ITEMS;
-        int rc = ahead ? launch(k_apply<NEN, nq_of<NEN>(), BS, T, 2, 1, NEN <= 4>, n, BS, st, "k_apply", t, it, vv, yy)
+        V2<T>* pp = static_cast<V2<T>*>(part);
+        int rc = ahead ? launch(k_apply<NEN, nq_of<NEN>(), BS, T, 2, 1, NEN <= 4>, n, BS, st, "k_apply", t, it, vv, yy,
+                                pp, part_cap)
                        : launch(k_apply<NEN, nq_of<NEN>(), BS, T, 2, 1, false, NEN == 4>, n, BS, st, "k_apply", t,
-                                it, vv, yy);
+                                it, vv, yy, pp, part_cap);
         if (rc) return rc;
-        return launch(k_apply_fin<T>, nf, FBS, st, "k_apply_fin", t, fi, vv, yy);
+        return launch(k_apply_fin<T>, nf, FBS, st, "k_apply_fin", t, fi, vv, yy, (const V2<T>*)pp, part_cap);
     }
     template <int NEN, int BS, bool LIB> int run() {
         return prec == DFEA_FP64 ? go<double, NEN, BS>() : go<float, NEN, BS>();
@@ -1855,6 +1865,40 @@ struct Launch_adjoint {
         return launch(k_adjoint<NEN, nq_of<NEN>(), BS, LIB>, n, BS, st, "k_adjoint", t, it, u, lam, g);
     }
 };
+
+}  // namespace
+
+// ======================================================================================================
+// scratch of the no-allocation product: the shared-node partials of dfea_apply_noalloc live in this array,
+// one copy per device (each GPU the library runs on holds it until the process ends).  Host code never uses
+// the array's name as an address: the device address of the current device's copy comes from
+// hipGetSymbolAddress, once per device, and is kept in g_scratch[].
+// ======================================================================================================
+static __device__ __align__(256) unsigned char scratch[DFEA_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void* g_scratch[MAX_DEVICES] = {};
+
+// device address of the current device's scratch (looked up on the first call on that device)
+int scratch_address(void** out) {
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return fail(-1000 - (int)e, "dfea_apply_noalloc: hipGetDevice failed");
+    if (dev < 0 || dev >= MAX_DEVICES) return fail(DFEA_EINVAL, "dfea_apply_noalloc: device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(-1000 - (int)e, "dfea_apply_noalloc: hipGetSymbolAddress(scratch) failed");
+        g_scratch[dev] = p;
+    }
+    *out = g_scratch[dev];
+    return 0;
+}
 
 }  // namespace
 
@@ -1944,6 +1988,28 @@ int dfea_apply(const dfea_desc* table, const int32_t* items, int n_items, const 
     if (!table || (precision != DFEA_FP64 && precision != DFEA_FP32))
         return fail(DFEA_EINVAL, "dfea_apply: bad argument");
     return dispatch(nen, Launch_apply{table, items, n_items, fin_items, n_fin_items, v, y, precision, stream});
+}
+
+int dfea_apply_noalloc(const dfea_desc* table, int n_chunks, int n_fin_blocks, int n_part, const void* v, void* y,
+                       int nen, int precision, void* stream) {
+    if (!table || !v || !y || n_chunks < 0 || n_fin_blocks < 0 || n_part < 0 ||
+        (precision != DFEA_FP64 && precision != DFEA_FP32))
+        return fail(DFEA_EINVAL, "dfea_apply_noalloc: bad argument");
+    const size_t entry = precision == DFEA_FP64 ? 2 * sizeof(double) : 2 * sizeof(float);
+    if ((size_t)n_part * entry > (size_t)DFEA_NOALLOC_SCRATCH_BYTES) {
+        snprintf(g_err, sizeof g_err,
+                 "dfea_apply_noalloc: %d shared-node partials need %zu bytes, more than the %d-byte scratch "
+                 "(use dfea_apply with the descriptor's partials buffer)",
+                 n_part, (size_t)n_part * entry, DFEA_NOALLOC_SCRATCH_BYTES);
+        return DFEA_EINVAL;
+    }
+    void* part = nullptr;
+    const int rc = scratch_address(&part);
+    if (rc) return rc;
+    Launch_apply la{table, nullptr, n_chunks, nullptr, n_fin_blocks, v, y, precision, stream};
+    la.part = part;
+    la.part_cap = n_part;
+    return dispatch(nen, la);
 }
 
 int dfea_dot(const dfea_desc* table, const int32_t* items, int n_items, const int32_t* solve_items, int n_solves,

@@ -20,6 +20,7 @@ FIXED_X, FIXED_Y, NODE_MASK = 0x40000000, 0x80000000, 0x3FFFFFFF
 HALO = 0x100          # element-kind flag of halo plans
 MATLIB = 0x200        # element-kind flag: material kernels with the whole material library
 MAT_NP = 8            # dfea_desc.mat_p slots
+NOALLOC_SCRATCH_BYTES = 8 << 20   # DFEA_NOALLOC_SCRATCH_BYTES: static device scratch of dfea_apply_noalloc
 VEC = dict(x=0, r=1, w=2, p=3, s=4, b=5, u=6, R=7, uacc=8, uD=9, x2=10, r2=11, w2=12, p2=13, s2=14)
 NVEC = 15
 PCG_VECS = ("x", "r", "w", "p", "s", "b", "x2", "r2", "w2", "p2", "s2")   # in the solve's precision
@@ -71,6 +72,7 @@ _SIGNATURES = {
     "dfea_fields": (_I, [_P, _P, _I, _P, _I, _P, _P, _P, _P, _I, _P]),
     "dfea_residual": (_I, [_P, _P, _I, _P, _I, _P, _P, _D, _I, _P]),
     "dfea_apply": (_I, [_P, _P, _I, _P, _I, _P, _P, _I, _I, _P]),
+    "dfea_apply_noalloc": (_I, [_P, _I, _I, _I, _P, _P, _I, _I, _P]),
     "dfea_dot": (_I, [_P, _P, _I, _P, _I, _P, _P, _P]),
     "dfea_axpy": (_I, [_P, _P, _I, _I, _P, _P, _P, _P, _P, _I, _P]),
     "dfea_pcg_init": (_I, [_P, _P, _I, _P, _I, _I, _I, _P]),

@@ -39,6 +39,7 @@ extern "C" {
 
 #define DFEA_EINVAL (-1)
 #define DFEA_MAXCOL 16         /* element colours per chunk */
+#define DFEA_NOALLOC_SCRATCH_BYTES (8 << 20)   /* static device scratch of dfea_apply_noalloc, per GPU */
 #define DFEA_FIXED_X 0x40000000u
 #define DFEA_FIXED_Y 0x80000000u
 #define DFEA_NODE_MASK 0x3fffffffu
@@ -205,6 +206,21 @@ int dfea_fields(const dfea_desc* table, const int32_t* items, int n_items, const
  * Replaces the closure of `operators.py:345-412` (make_A_operator_qp / _qp_ew). */
 int dfea_apply(const dfea_desc* table, const int32_t* items, int n_items, const int32_t* fin_items,
                int n_fin_items, const void* v, void* y, int nen, int precision, void* stream);
+
+/* The same product for ONE solve without any device allocation (no counterpart in the reference: it is for
+ * callers that cannot allocate -- code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The shared-node partials that dfea_apply keeps in the descriptor's part buffer go instead to
+ * a static device array of DFEA_NOALLOC_SCRATCH_BYTES per GPU (held until the process ends; its address is
+ * looked up once per device, with that device current).  table: one descriptor (the solve); n_chunks and
+ * n_fin_blocks: the work-item counts dfea_apply takes for that solve alone (items and fin_items NULL);
+ * n_part: the plan's number of shared-node partials (dfea_desc.n_part), at most
+ * DFEA_NOALLOC_SCRATCH_BYTES / 16 in FP64 and / 8 in FP32, else DFEA_EINVAL and nothing is launched.
+ * v and y must not be NULL.  The current device must be the stream's.  No hipMalloc, hipFree or
+ * allocating library call.  Every partial is written before it is read, so nothing needs clearing
+ * between calls; calls that overlap on one GPU (other streams, other host threads) share the array and
+ * race. */
+int dfea_apply_noalloc(const dfea_desc* table, int n_chunks, int n_fin_blocks, int n_part, const void* v, void* y,
+                       int nen, int precision, void* stream);
 
 /* ---- deterministic reductions ----------------------------------------------------------------------- */
 /* sc[DFEA_SC_NORM] = sum(x*y) over the solve's dofs (x==y gives the squared norm); FP64 vectors,

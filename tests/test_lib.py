@@ -51,3 +51,16 @@ def test_missing_library_fails_loudly():
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     assert "raised:" in r.stdout and "is not built" in r.stdout
+
+
+def test_noalloc_product_refuses_bad_sizes_before_any_launch():
+    """The no-allocation product checks its sizes on the host and returns DFEA_EINVAL before touching the
+    device (the table pointer is never dereferenced on the host)."""
+    lib = _lib.load()
+    fake = ctypes.c_void_p(256)
+    too_many = _lib.NOALLOC_SCRATCH_BYTES // 16 + 1
+    assert lib.dfea_apply_noalloc(fake, 1, 1, too_many, fake, fake, 4, _lib.FP64, None) == -1
+    assert b"scratch" in lib.dfea_last_error()
+    assert lib.dfea_apply_noalloc(fake, 1, 1, 2 * too_many, fake, fake, 4, _lib.FP32, None) == -1
+    assert lib.dfea_apply_noalloc(fake, 1, 1, 10, None, fake, 4, _lib.FP64, None) == -1
+    assert lib.dfea_apply_noalloc(fake, 1, 1, 10, fake, fake, 4, 7, None) == -1
