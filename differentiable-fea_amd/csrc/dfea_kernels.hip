@@ -1446,9 +1446,11 @@ __device__ __forceinline__ NodeState<T> node_state(const dfea_desc& D, const Chu
 // Loads return in the order they are issued (waiting for one waits for those before it, not for those after),
 // so the kernel issues everything that does not depend on the scalars first, in the order it is needed: the
 // previous kernel's chunk partials (scalar step), the node ids and slot tables of the thread's nodes, the
-// element's local node indices, the first PF Gauss points.  EARLY: the state of the thread's first node is also
-// loaded before the scalar step, so its loads overlap the reduction; it costs registers (waves per SIMD),
-// which matters only when the launch has more workgroups than the GPU has CUs.
+// element's local node indices, the first PF Gauss points.  EARLY: the state of every node of the thread is also
+// loaded before the scalar step, so its loads overlap the reduction (and the callers pick PF = all the group's
+// Gauss points); it costs registers (waves per SIMD), which matters only when the launch has more workgroups
+// than the GPU has CUs.  The scalars the step reads (tolerance, cap, the previous iteration's gamma and
+// alpha) are requested at the start, before the first barrier.
 template <int NEN, int NQ, int BS, typename T, int PF = 2, int MINW = 1, bool EARLY = false>
 __global__ __launch_bounds__(BS, MINW) void k_pcg(const dfea_desc* __restrict__ tab, const int* __restrict__ items, int par) {
     static_assert(3 * (BS / 64) <= 16, "block_sum3 needs 3 slots per wave in ChunkLDS::red");
@@ -1462,6 +1464,9 @@ __global__ __launch_bounds__(BS, MINW) void k_pcg(const dfea_desc* __restrict__ 
     const double* S = sc + (par ? DFEA_SC_S1 : DFEA_SC_S0);
     if (S[PS_ST] != DFEA_PCG_RUNNING) return;
     const double k = S[PS_K];
+    // the step's scalars, requested now (a load behind the reduction's barrier would wait a round trip there)
+    const double s_rtol = sc[DFEA_SC_RTOL], s_bb = sc[DFEA_SC_BB], s_maxit = sc[DFEA_SC_MAXIT];
+    const double s_gam = S[PS_GAM], s_alpha = S[PS_ALPHA];
     const ChunkView ch = chunk_view(D.chunks + it.b);
     const int t = threadIdx.x;
     const bool lead = it.b == 0 && t == 0;
@@ -1536,9 +1541,17 @@ __global__ __launch_bounds__(BS, MINW) void k_pcg(const dfea_desc* __restrict__ 
     prefetch_gauss_points<NEN, NQ, BS, PF, T>(dN, C, ne, e, t, d, c);
     __builtin_amdgcn_sched_barrier(0);
 
-    // 1d. the first node's state (independent of the scalars)
-    NodeState<T> ns0;
-    if (EARLY && t < ch.nk) ns0 = node_state<T>(D, ch, t, gid[0], rng[0], slt[0], k >= 0.0, Rd, dg, dgo, part_rd);
+    // 1d. the node states (independent of the scalars)
+    // EARLY: the first NGE node slots of the thread (chunks of up to 384 nodes: every structured chunk)
+    constexpr int NGE = (384 + BS - 1) / BS < NG ? (384 + BS - 1) / BS : NG;
+    NodeState<T> nse[EARLY ? NGE : 1];
+    if constexpr (EARLY) {
+#pragma unroll
+        for (int i = 0; i < NGE; ++i) {
+            const int kk = t + i * BS;
+            if (kk < ch.nk) nse[i] = node_state<T>(D, ch, kk, gid[i], rng[i], slt[i], k >= 0.0, Rd, dg, dgo, part_rd);
+        }
+    }
 
     // 2. scalar step
     double alpha = 0.0, beta = 0.0, gam = 0.0, den = 0.0, rr = 0.0;
@@ -1562,15 +1575,15 @@ __global__ __launch_bounds__(BS, MINW) void k_pcg(const dfea_desc* __restrict__ 
         const double delta = a1;
         rr = a2;
         double st = DFEA_PCG_RUNNING, iters = 0.0;
-        if (k >= 1.0 && sqrt(rr) <= sc[DFEA_SC_RTOL] * sqrt(sc[DFEA_SC_BB])) {
+        if (k >= 1.0 && sqrt(rr) <= s_rtol * sqrt(s_bb)) {
             st = DFEA_PCG_CONVERGED;
             iters = k;
-        } else if (k >= sc[DFEA_SC_MAXIT]) {
+        } else if (k >= s_maxit) {
             st = DFEA_PCG_MAXITER;
-            iters = sc[DFEA_SC_MAXIT];
+            iters = s_maxit;
         } else {
-            beta = k >= 1.0 ? gam / S[PS_GAM] : 0.0;
-            den = k >= 1.0 ? delta - beta * gam / S[PS_ALPHA] : delta;
+            beta = k >= 1.0 ? gam / s_gam : 0.0;
+            den = k >= 1.0 ? delta - beta * gam / s_alpha : delta;
             if (den <= 0.0) {      // `newton_krylov.py:70-71`: NaN runs on, as in the reference
                 st = DFEA_PCG_CURVATURE;
                 iters = k;
@@ -1614,9 +1627,9 @@ __global__ __launch_bounds__(BS, MINW) void k_pcg(const dfea_desc* __restrict__ 
     for (int i = 0; i < NG; ++i) {
         const int kk = t + i * BS;
         if (kk >= ch.nk) break;
-        NodeState<T> ns = (EARLY && i == 0) ? ns0
-                                            : node_state<T>(D, ch, kk, gid[i], rng[i], slt[i], k >= 0.0, Rd, dg, dgo,
-                                                            part_rd);
+        NodeState<T> ns;
+        if (EARLY && i < NGE) ns = nse[EARLY ? (i < NGE ? i : 0) : 0];
+        else ns = node_state<T>(D, ch, kk, gid[i], rng[i], slt[i], k >= 0.0, Rd, dg, dgo, part_rd);
         const int n = ns.n;
         const unsigned char b = ns.b;
         V2<T> rn = ns.r;
@@ -1767,8 +1780,11 @@ int pcg_iter(const dfea_desc* table, const int32_t* items, int n_items, int n_it
     constexpr int BSP = quadratic<NEN>() ? bs_groups<T>() : BS;
     const bool early = n_items <= PCG_EARLY_ITEMS;
     for (int i = 0; i < n_iter + (n_iter & 1); ++i) {
-        const int rc = early ? launch(k_pcg<NEN, nq_of<NEN>(), BSP, T, 2, 1, true>, n_items, BSP, stream, "k_pcg", table, items,
-                                      i & 1)
+        // small launches (one workgroup per CU or fewer): every node state and, for linear elements, every Gauss
+        // point requested before the scalar step
+        constexpr int PF_EARLY = NEN <= 4 && BSP % CHUNK_LIN == 0 ? nq_of<NEN>() : 2;
+        const int rc = early ? launch(k_pcg<NEN, nq_of<NEN>(), BSP, T, PF_EARLY, 1, true>, n_items, BSP, stream, "k_pcg",
+                                      table, items, i & 1)
                              : launch(k_pcg<NEN, nq_of<NEN>(), BSP, T, 2, (NEN <= 4 ? 3 : 1), false>, n_items, BSP, stream, "k_pcg", table,
                                       items, i & 1);
         if (rc) return rc;

@@ -13,6 +13,9 @@ HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
     # packed FP32 instructions stay off (they have given silently wrong results on these boxes)
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # the first kernel arguments arrive in SGPRs (no kernarg-segment load before the descriptor load): measured
+    # 20.9 -> 20.3 us per 400x400 FP64 product, 11.9 -> 11.3 us FP32
+    "-mllvm", "-amdgpu-kernarg-preload-count=8",
 ]
 
 
