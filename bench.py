@@ -280,58 +280,187 @@ def solve_config3(precision, preconditioner="jacobi"):
                    converged=info["converged"], tol_abs=tol)
 
 
-def sequential_fp64(n_requests, seed=5, side=(32, 512), rate=None):
-    """The plain FP64 GPU solver: the same requests solved one after another with newton_krylov_solve.
-    With ``rate``, also the latency from arrival of a first-in-first-out queue in front of it (Poisson arrivals
-    of that rate, the stream's seed): request i starts at max(arrival_i, end of request i-1) and takes the
-    service time measured for it here (the solver serves one request at a time, so the queue is exact)."""
-    import numpy as np
-    from difffea_amd.serve import make_stream, reference_residual_norm, request_problem
-    from difffea_amd.newton_krylov import HipNewton
-    reqs = make_stream(n_requests, rate=1e9, seed=seed, side=side)
-    lat = []
-    t0 = time.perf_counter()
-    for r in reqs:
-        t1 = time.perf_counter()
-        p, th = request_problem(r)
-        solver = HipNewton(p, th.cuda())
-        ref = reference_residual_norm(solver.sys, p, solver.device)
-        u, info = solver.solve(r.n_load_steps, 1e-8 * ref, cg_max_iter=20000)
-        if r.adjoint:
-            from difffea_amd.adjoint import solve_adjoint, compute_implicit_vjp
-            lam, _ = solve_adjoint(solver, u, p["f"].cuda(), rtol=1e-8, max_iter=20000)
-            compute_implicit_vjp(solver, u, lam)
-        torch.cuda.synchronize()
-        lat.append(time.perf_counter() - t1)
-        if len(lat) % 32 == 0:
-            print(f"# plain FP64 solver: {len(lat)} of {len(reqs)} requests, {time.perf_counter() - t0:.1f} s",
-                  file=sys.stderr, flush=True)
-    wall = time.perf_counter() - t0
-    out = dict(precision="fp64", mode="sequential newton_krylov_solve", n_requests=len(reqs), wall_s=wall,
-               solves_per_s=len(reqs) / wall, p50_latency_s=float(np.percentile(lat, 50)),
-               p99_latency_s=float(np.percentile(lat, 99)), latency_from="start of the request's solve")
-    if rate is not None:
-        from difffea_amd.serve import fifo_latencies
-        qlat = fifo_latencies([r.arrival for r in make_stream(n_requests, rate=rate, seed=seed, side=side)], lat)
-        out.update(arrival_rate_per_s=rate, p50_latency_from_arrival_s=float(np.percentile(qlat, 50)),
-                   p99_latency_from_arrival_s=float(np.percentile(qlat, 99)))
+def time_events(fn, steps=200, warmup=5):
+    """Mean time of fn() in us (HIP events on torch's current stream, the kernels' stream)."""
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) * 1e3 / steps
+
+
+def newton_step_kernels(p, u, theta):
+    """The per-Newton-step kernels on the 400x400 Q4 mesh, FP64 state: the moduli + Jacobi diagonal kernel
+    (FP64 and FP32 moduli), the residual and the adjoint-gradient kernel, with their algorithmic bytes
+    (per Gauss point: shape gradients 2 nen 8 B + w det J 8 B; per element: int32 connectivity 4 nen B and
+    theta 8 B where read; per dof: the FP64 vectors read and written; the moduli kernel writes 10 moduli per
+    Gauss point and the diagonal per dof in its precision; the gradient kernel writes 8 B per element)."""
+    from difffea_amd.operators import make_system
+    dev = torch.device("cuda")
+    ud = u.to(dev)
+    lam = torch.randn(u.numel(), dtype=torch.float64, generator=torch.Generator().manual_seed(6)).to(dev)
+    out = {}
+    for prec, sC in (("fp64", 8), ("mixed", 4)):
+        s = make_system(ud, theta.to(dev), p, prec)
+        m = s.mesh
+        nqp, nd = m.n_el * m.nq, m.n_dof
+        t = time_events(lambda: s.moduli(ud))
+        B = nqp * (2 * m.nen * 8 + 8 + 10 * sC) + m.n_el * (4 * m.nen + 8) + nd * (8 + sC)
+        out["moduli_" + prec] = dict(us_per_call=round(t, 3), bytes_per_call=B, gbs=round(B / t / 1e3, 1))
+    s = make_system(ud, theta.to(dev), p, "fp64")
+    m = s.mesh
+    nqp, nd = m.n_el * m.nq, m.n_dof
+    R = torch.empty_like(ud)
+    t = time_events(lambda: s.residual(ud, out=R, lf=1.0))
+    B = nqp * (2 * m.nen * 8 + 8) + m.n_el * (4 * m.nen + 8) + nd * 3 * 8
+    out["residual"] = dict(us_per_call=round(t, 3), bytes_per_call=B, gbs=round(B / t / 1e3, 1))
+    t = time_events(lambda: s.adjoint_grad(ud, lam))
+    B = nqp * (2 * m.nen * 8 + 8) + m.n_el * (4 * m.nen + 8) + nd * 2 * 8
+    out["adjoint_grad"] = dict(us_per_call=round(t, 3), bytes_per_call=B, gbs=round(B / t / 1e3, 1))
     return out
 
 
-def serve_stream(n_requests, precision, rate=None, seed=5, side=(32, 512), preconditioner="jacobi"):
-    from difffea_amd.serve import Server, make_stream
-    reqs = make_stream(n_requests, rate=rate or 1e9, seed=seed, side=side)
-    srv = Server(precision, max_slots=64, preconditioner=preconditioner)
+def solve_config4(precision):
+    """Adjoint sensitivity at 256x256 Q4 (65,536 elements): rho_e ~ U(0.1, 1) (seed 3) scaling mu and lambda,
+    left edge clamped, dead vertical load on the right edge with total 0.05 mean(rho) E H (E = 1, H = 1),
+    compliance f.u; forward Newton solve to 1e-10 ||f||, adjoint PCG to 1e-10 relative (mixed: refined against
+    an FP64 tangent to the same tolerance), then the gradient kernel.  Times in seconds, HIP-synchronised."""
+    from difffea_amd.adjoint import compute_implicit_vjp, solve_adjoint
+    from difffea_amd.newton_krylov import HipNewton
+    from difffea_amd.problem import build_problem
+    n = 256
+    rho = 0.1 + 0.9 * torch.rand(n * n, dtype=torch.float64, generator=torch.Generator().manual_seed(3))
+    F = 0.05 * float(rho.mean())
+    p = build_problem(dict(element_type="quad4", nx=n, ny=n, E=1.0, nu=0.3, load_mode="force",
+                           load_value=(0.0, -F)))
+    dev = torch.device("cuda")
+    f = p["f"].to(dev)
+    tol = 1e-10 * float(torch.norm(p["f"]))
+    res = {}
+    for rep in range(2):                          # the first run warms up (graphs, allocator)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        solver = HipNewton(p, rho.to(dev), precision)
+        u, info = solver.solve(1, tol, cg_max_iter=20000)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        lam, its = solve_adjoint(solver, u, f, rtol=1e-10, max_iter=20000)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        g = compute_implicit_vjp(solver, u, lam)
+        torch.cuda.synchronize()
+        t3 = time.perf_counter()
+        res = dict(forward_s=round(t1 - t0, 4), adjoint_solve_s=round(t2 - t1, 4), gradient_s=round(t3 - t2, 5),
+                   total_s=round(t3 - t0, 4), gradients_per_s=round(1.0 / (t3 - t0), 3), converged=info["converged"],
+                   newton=info["total_newton"], cg=info["total_cg"], adjoint_cg=int(its))
+    res["grad_kernel_us"] = round(time_events(lambda: solver.sys.adjoint_grad(u, lam)), 3)
+    return g, res
 
-    def progress(done, total):
-        if done % 32 == 0:
-            print(f"# served {precision} {preconditioner}: {done} of {total} requests", file=sys.stderr,
-                  flush=True)
 
-    recs, summ = srv.run(reqs, realtime=rate is not None, progress=progress)
-    summ["n_newton"] = int(sum(r["total_newton"] for r in recs))
-    summ["n_cg"] = int(sum(r["total_cg"] for r in recs))
-    return summ
+def served_comparison(n_requests, side=(32, 512), seed=5, preconditioner="jacobi"):
+    """The served request stream (config-5 distribution: sides log-uniform in ``side`` for Q4 and Q9 alike,
+    70 % Q4, mu ~ U(0.2, 0.6), lambda/mu ~ U(1, 3), stretch ~ U(1 %, 15 %) in 3 load steps, 20 % with the
+    compliance gradient; seed 5), every request to 1e-8 relative residual, in four runs on the same requests:
+
+    * plain: the FP64 GPU solver (``HipNewton``, the single-solve API) one request after another;
+    * fp64 / mixed: the continuous-batching server, all requests queued at t = 0 (capacity);
+    * mixed_poisson: the mixed server with Poisson arrivals at 0.8x the plain solver's capacity (latency from
+      arrival), beside the plain solver behind a first-in-first-out queue with the same arrivals (latency from
+      its measured service times).
+
+    The host-only set-up of every request (problem, mesh plan) is done once beforehand and given to all runs,
+    and its time is reported apart, so the ratios compare GPU work: ``mixed_vs_fp64_served`` is the precision
+    effect, ``fp64_served_vs_plain`` the batching effect, ``mixed_served_vs_plain`` both.  Each mixed result is
+    compared with the FP64 server's (max-norm relative error of u, and of the gradient where asked)."""
+    import numpy as np
+    from difffea_amd.adjoint import compute_implicit_vjp, solve_adjoint
+    from difffea_amd.device import HipMesh
+    from difffea_amd.newton_krylov import HipNewton
+    from difffea_amd.serve import Server, fifo_latencies, make_stream, reference_residual_norm
+    dev = torch.device("cuda")
+    reqs = make_stream(n_requests, rate=1e9, seed=seed, side=side)
+    t0 = time.perf_counter()
+    prepared = {r.rid: Server.prepare(r) for r in reqs}
+    setup_s = time.perf_counter() - t0
+    print(f"# served stream: {len(reqs)} requests, host set-up {setup_s:.1f} s", file=sys.stderr, flush=True)
+
+    def summary(lat, wall, n, **kw):
+        return dict(n_requests=n, wall_s=round(wall, 3), solves_per_s=round(n / wall, 3),
+                    p50_latency_s=round(float(np.percentile(lat, 50)), 4),
+                    p99_latency_s=round(float(np.percentile(lat, 99)), 4), **kw)
+
+    # plain FP64 solver, one request after another (mesh upload + geometry and the solve timed)
+    lat, plain_u = [], {}
+    t0 = time.perf_counter()
+    for r in reqs:
+        t1 = time.perf_counter()
+        p, th, plan = prepared[r.rid]
+        solver = HipNewton(p, th.to(dev), mesh=HipMesh(p, dev, plan=plan), preconditioner=preconditioner)
+        u, info = solver.solve(r.n_load_steps, 1e-8 * reference_residual_norm(solver.sys, p, dev), cg_max_iter=20000)
+        if r.adjoint:
+            lam, _ = solve_adjoint(solver, u, p["f"].to(dev), rtol=1e-8, max_iter=20000)
+            compute_implicit_vjp(solver, u, lam)
+        torch.cuda.synchronize()
+        lat.append(time.perf_counter() - t1)
+        plain_u[r.rid] = u.cpu()
+        del solver
+        if len(lat) % 16 == 0:
+            print(f"# plain FP64 solver: {len(lat)} of {len(reqs)} requests, {time.perf_counter() - t0:.1f} s",
+                  file=sys.stderr, flush=True)
+    plain = summary(lat, time.perf_counter() - t0, len(reqs), latency_from="start of the request's solve",
+                    mode="HipNewton, one request after another")
+    service = list(lat)
+
+    def serve(precision, realtime, rate=None):
+        rs = make_stream(n_requests, rate=rate or 1e9, seed=seed, side=side)
+        srv = Server(precision, max_slots=64, preconditioner=preconditioner)
+        recs, summ = srv.run(rs, realtime=realtime, keep_solution=True, prepared=prepared,
+                             progress=lambda d, t: print(f"# served {precision}: {d} of {t}", file=sys.stderr,
+                                                         flush=True) if d % 16 == 0 else None)
+        summ["n_newton"] = int(sum(x["total_newton"] for x in recs))
+        summ["n_cg"] = int(sum(x["total_cg"] for x in recs))
+        return {x["rid"]: x for x in recs}, summ
+
+    r64, sat64 = serve("fp64", False)
+    r32, sat32 = serve("mixed", False)
+    err_u = max(float((r32[i]["u"] - r64[i]["u"]).abs().max() / r64[i]["u"].abs().max()) for i in r64)
+    err_g = [float((r32[i]["grad"] - r64[i]["grad"]).abs().max() / r64[i]["grad"].abs().max())
+             for i in r64 if "grad" in r64[i]]
+    plain_vs_fp64 = max(float((plain_u[i] - r64[i]["u"]).abs().max() / r64[i]["u"].abs().max()) for i in r64)
+    rate = 0.8 * plain["solves_per_s"]
+    _, poi32 = serve("mixed", True, rate)
+    arrivals = [r.arrival for r in make_stream(n_requests, rate=rate, seed=seed, side=side)]
+    q = fifo_latencies(arrivals, service)
+    keys = ("wall_s", "solves_per_s", "p50_latency_s", "p99_latency_s", "latency_from", "converged", "n_newton",
+            "n_cg", "peak_admitted")
+    return {
+        "workload": f"{len(reqs)} requests, sides log-uniform in [{side[0]}, {side[1]}] (Q4 and Q9), 70 % Q4, "
+                    f"20 % with the compliance gradient, seed {seed}, 1e-8 relative residual",
+        "n_q9": int(sum(r.element_type == "quad9" for r in reqs)), "max_dofs": int(max(
+            prepared[r.rid][0]["nodes"].shape[0] for r in reqs) * 2),
+        "host_setup_s": round(setup_s, 2), "host_setup_s_per_request": round(setup_s / len(reqs), 4),
+        "plain_fp64": plain,
+        "served_fp64": {k: sat64[k] for k in keys if k in sat64},
+        "served_mixed": {k: sat32[k] for k in keys if k in sat32},
+        "mixed_vs_fp64_served": round(sat32["solves_per_s"] / sat64["solves_per_s"], 3),
+        "fp64_served_vs_plain": round(sat64["solves_per_s"] / plain["solves_per_s"], 3),
+        "mixed_served_vs_plain": round(sat32["solves_per_s"] / plain["solves_per_s"], 3),
+        "mixed_vs_plain_incl_host_setup": round(sat32["solves_per_s"] * (plain["wall_s"] + setup_s)
+                                                / len(reqs), 3),
+        "mixed_vs_fp64_max_rel_error_u": err_u,
+        "mixed_vs_fp64_max_rel_error_grad": max(err_g) if err_g else None,
+        "plain_vs_served_fp64_max_rel_error_u": plain_vs_fp64,
+        "poisson_rate_per_s": round(rate, 3),
+        "served_mixed_poisson": {k: poi32[k] for k in keys if k in poi32},
+        "plain_fp64_fifo_poisson": dict(p50_latency_s=round(float(np.percentile(q, 50)), 4),
+                                        p99_latency_s=round(float(np.percentile(q, 99)), 4),
+                                        latency_from="arrival (service times of the plain run)"),
+    }
 
 
 def main():
@@ -344,8 +473,9 @@ def main():
     ap.add_argument("--no-solve", action="store_true", help="skip the complete-solve workloads")
     ap.add_argument("--serve-requests", type=int, default=64,
                     help="requests of the served stream (the full workload is 2000)")
-    ap.add_argument("--serve-side", type=int, nargs=2, default=(32, 256),
-                    help="side range of the served meshes (the full workload is 32 512)")
+    ap.add_argument("--serve-side", type=int, nargs=2, default=(32, 512),
+                    help="side range of the served meshes (the configured workload: 32 512)")
+    ap.add_argument("--served-only", action="store_true", help="only the served-stream comparison, one JSON line")
     ap.add_argument("--config1-only", action="store_true",
                     help="only the 100x100 Q4 solves (FP64 and mixed), one JSON line; for profiling the solve")
     args = ap.parse_args()
@@ -358,6 +488,9 @@ def main():
 
     if args.config1_only:
         print(json.dumps({prec: solve_config1(prec)[1] for prec in ("fp64", "mixed")}))
+        return
+    if args.served_only:
+        print(json.dumps(served_comparison(args.serve_requests, tuple(args.serve_side))))
         return
 
     p, u, v, th = problem("quad4", 400)
@@ -375,6 +508,8 @@ def main():
           f"fp32 {pcg32['us_per_iteration']:.2f} us", file=sys.stderr, flush=True)
     post = time_postprocess(p, u, th)
     print(f"# Q4 400x400 post-processing: {post}", file=sys.stderr, flush=True)
+    steps = newton_step_kernels(p, u, th)
+    print(f"# Q4 400x400 Newton-step kernels: {steps}", file=sys.stderr, flush=True)
     p9, u9, v9, th9 = problem("quad9", 200)
     q9 = time_apply(p9, u9, v9, th9, "fp64", args.steps, args.warmup)
     print(f"# Q9 200x200 fp64 A(v): {q9['ms'] * 1e3:.2f} us, {q9['gbs']:.0f} GB/s", file=sys.stderr, flush=True)
@@ -400,25 +535,13 @@ def main():
         solves["c3_block_vs_jacobi_rel"] = float((vb - v64).abs().max() / v64.abs().max())
         print(f"# config-3 solves: {solves['c3_fp64']} {solves['c3_mixed']} {solves['c3_fp64_block_jacobi']} "
               f"{solves['c3_mixed_block_jacobi']}", file=sys.stderr, flush=True)
+        g64, solves["c4_fp64"] = solve_config4("fp64")
+        g32, solves["c4_mixed"] = solve_config4("mixed")
+        solves["c4_mixed_vs_fp64_grad_rel"] = float((g32 - g64).abs().max() / g64.abs().max())
+        print(f"# config-4 adjoint: {solves['c4_fp64']} {solves['c4_mixed']} "
+              f"{solves['c4_mixed_vs_fp64_grad_rel']:.2e}", file=sys.stderr, flush=True)
         if args.serve_requests > 0:
-            sat64 = serve_stream(args.serve_requests, "fp64", side=tuple(args.serve_side))
-            print(f"# served fp64 (saturated): {sat64}", file=sys.stderr, flush=True)
-            rate = 0.8 * sat64["solves_per_s"]
-            seq = sequential_fp64(args.serve_requests, side=tuple(args.serve_side))
-            print(f"# plain FP64 GPU solver (sequential): {seq}", file=sys.stderr, flush=True)
-            sat32 = serve_stream(args.serve_requests, "mixed", side=tuple(args.serve_side))
-            print(f"# served mixed (saturated): {sat32}", file=sys.stderr, flush=True)
-            sat32b = serve_stream(args.serve_requests, "mixed", side=tuple(args.serve_side),
-                                  preconditioner="block_jacobi")
-            print(f"# served mixed block-Jacobi (saturated): {sat32b}", file=sys.stderr, flush=True)
-            srv = {"saturated_fp64": sat64, "saturated_mixed": sat32, "saturated_mixed_block_jacobi": sat32b,
-                   "plain_fp64_sequential": seq,
-                   "saturated_mixed_vs_plain_fp64": round(sat32["solves_per_s"] / seq["solves_per_s"], 3),
-                   "saturated_mixed_block_jacobi_vs_plain_fp64": round(sat32b["solves_per_s"] / seq["solves_per_s"],
-                                                                       3),
-                   "fp64": serve_stream(args.serve_requests, "fp64", rate, side=tuple(args.serve_side)),
-                   "mixed": serve_stream(args.serve_requests, "mixed", rate, side=tuple(args.serve_side)),
-                   "arrival_rate_per_s": rate}
+            srv = served_comparison(args.serve_requests, tuple(args.serve_side))
             print(f"# served: {srv}", file=sys.stderr, flush=True)
             solves["served"] = srv
 
@@ -446,6 +569,7 @@ def main():
         "q9_200x200_fp64": {"us_per_product": round(q9["ms"] * 1e3, 3), "gbs": round(q9["gbs"], 2),
                             "bytes_per_step": q9["bytes"]},
         "pcg_iteration_fp64": pcg64, "pcg_iteration_mixed": pcg32, "postprocess_q4_400": post,
+        "newton_step_kernels_q4_400": steps,
         "element_products_fp64": elems,
         "cpu_baseline": None if cpu is None else {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")},
         "solves": solves,

@@ -37,7 +37,8 @@ constexpr int CHUNK_LIN = 256;    // core elements per chunk for the linear elem
 constexpr int CHUNK_QUAD = 64;    // core elements per chunk for the quadratic elements (tri6, quad8, quad9)
 constexpr int BS_LIN = 256;       // workgroup size for linear elements, partials plans
 constexpr int BS_Q4H = 320;       // workgroup size for quad4 halo plans: core + halo elements (16x16 + 33)
-constexpr int BS_QUAD = 64;       // workgroup size for quadratic elements
+constexpr int BS_QUAD = 64;       // workgroup size for quadratic elements (and small-chunk quad4 plans)
+constexpr int BS_Q4S = 256;       // product and PCG workgroups of small-chunk quad4 plans: 4 threads per element
 // workgroup size of the quadratic elements' product and PCG kernels: Gauss-point groups of 64 threads (2 in
 // FP64, where registers limit the waves per SIMD, 3 in FP32; measured fastest for quad9 on 200x200 and on the
 // 64x256 O-grid)
@@ -365,9 +366,14 @@ __device__ __forceinline__ void colour_accumulate(const ChunkView& ch, int t, co
 // element G threads, thread t taking element t mod CE and the Gauss points [g NQ / G, (g+1) NQ / G) of group
 // g = t / CE; the groups' element vectors are added in group order.  (G = 1: one thread per element; halo
 // plans, whose workgroups are not a multiple of the chunk, always use G = 1.)
-template <int NEN, int BS>
+// chunk elements of a kernel instantiation: CHUNK_LIN / CHUNK_QUAD by element type, CHUNK_QUAD for the quad4
+// small-chunk plans (DFEA_SMALL)
+template <int NEN>
+constexpr int default_ce() { return quadratic<NEN>() ? CHUNK_QUAD : CHUNK_LIN; }
+
+template <int NEN, int BS, int CEX = default_ce<NEN>()>
 struct Groups {
-    static constexpr int CE = quadratic<NEN>() ? CHUNK_QUAD : CHUNK_LIN;
+    static constexpr int CE = CEX;
     static constexpr int G = BS % CE == 0 ? BS / CE : 1;
 };
 
@@ -431,11 +437,11 @@ __device__ __forceinline__ void load_gauss_point(const T* dN, const T* C, int ne
 }
 
 // the first PF Gauss points of the thread's group
-template <int NEN, int NQ, int BS, int PF, typename T, bool NT = false>
+template <int NEN, int NQ, int BS, int PF, typename T, bool NT = false, int CEX = default_ce<NEN>()>
 __device__ __forceinline__ void prefetch_gauss_points(const T* dN, const T* C, int ne, int e, int t,
-                                                      T (&d)[GaussSplit<NQ, Groups<NEN, BS>::G>::QG][NEN][2],
-                                                      T (&c)[GaussSplit<NQ, Groups<NEN, BS>::G>::QG][10]) {
-    constexpr int G = Groups<NEN, BS>::G, CE = Groups<NEN, BS>::CE, QG = GaussSplit<NQ, G>::QG;
+                                                      T (&d)[GaussSplit<NQ, Groups<NEN, BS, CEX>::G>::QG][NEN][2],
+                                                      T (&c)[GaussSplit<NQ, Groups<NEN, BS, CEX>::G>::QG][10]) {
+    constexpr int G = Groups<NEN, BS, CEX>::G, CE = Groups<NEN, BS, CEX>::CE, QG = GaussSplit<NQ, G>::QG;
     const int g = G > 1 ? t / CE : 0;
     const int q0 = g * NQ / G, q1 = (g + 1) * NQ / G;
 #pragma unroll
@@ -459,41 +465,42 @@ struct ProductLDS {
 
 // the inverse-map tables of a chunk, requested into registers early (the loads return while other work is in
 // flight) and stored into LDS before the gather barrier; indices clamped into the chunk's ranges
-template <int NEN, int BS>
+template <int NEN, int BS, int CEX = default_ce<NEN>()>
 struct InvRegs {
-    static constexpr int CE = Groups<NEN, BS>::CE;
+    static constexpr int CE = Groups<NEN, BS, CEX>::CE;
     static constexpr int NSL = (CE * NEN + BS - 1) / BS;     // slots per thread
     static constexpr int NOF = (MAXLN + 1 + BS - 1) / BS;    // node offsets per thread
     uint16_t sl[NSL];
     int of[NOF];
 };
 
-template <int NEN, int BS>
-__device__ __forceinline__ void inv_load(const dfea_desc& D, const ChunkView& ch, int t, InvRegs<NEN, BS>& R) {
+template <int NEN, int BS, int CEX = default_ce<NEN>()>
+__device__ __forceinline__ void inv_load(const dfea_desc& D, const ChunkView& ch, int t, InvRegs<NEN, BS, CEX>& R) {
     const int n = ch.ne * NEN;
     const uint32_t base = (uint32_t)(ch.e0 * NEN);
 #pragma unroll
-    for (int i = 0; i < InvRegs<NEN, BS>::NSL; ++i) {
+    for (int i = 0; i < InvRegs<NEN, BS, CEX>::NSL; ++i) {
         const int j = t + i * BS;
         R.sl[i] = gload<false>(D.inv_slot, base + (uint32_t)(j < n ? j : n - 1));
     }
 #pragma unroll
-    for (int i = 0; i < InvRegs<NEN, BS>::NOF; ++i) {
+    for (int i = 0; i < InvRegs<NEN, BS, CEX>::NOF; ++i) {
         const int k = t + i * BS;
         R.of[i] = gload<false>(D.inv_off, (uint32_t)(ch.k0 + (k <= ch.nk ? k : ch.nk))) - (int)base;
     }
 }
 
-template <int NEN, int BS, typename T>
-__device__ __forceinline__ void inv_store(const ChunkView& ch, int t, const InvRegs<NEN, BS>& R, ProductLDS<T>& L) {
+template <int NEN, int BS, typename T, int CEX = default_ce<NEN>()>
+__device__ __forceinline__ void inv_store(const ChunkView& ch, int t, const InvRegs<NEN, BS, CEX>& R,
+                                          ProductLDS<T>& L) {
     const int n = ch.ne * NEN;
 #pragma unroll
-    for (int i = 0; i < InvRegs<NEN, BS>::NSL; ++i) {
+    for (int i = 0; i < InvRegs<NEN, BS, CEX>::NSL; ++i) {
         const int j = t + i * BS;
         if (j < n) L.slot[j] = R.sl[i];
     }
 #pragma unroll
-    for (int i = 0; i < InvRegs<NEN, BS>::NOF; ++i) {
+    for (int i = 0; i < InvRegs<NEN, BS, CEX>::NOF; ++i) {
         const int k = t + i * BS;
         if (k <= ch.nk) L.offs[k] = R.of[i];
     }
@@ -514,21 +521,22 @@ __device__ __forceinline__ V2<T> node_sum(const ProductLDS<T>& L, int k) {
 }
 
 // LDS for the element vectors of groups 1..G-1
-template <int NEN, int BS, typename T>
+template <int NEN, int BS, typename T, int CEX = default_ce<NEN>()>
 struct GroupLDS {
-    static constexpr int G = Groups<NEN, BS>::G, CE = Groups<NEN, BS>::CE;
+    static constexpr int G = Groups<NEN, BS, CEX>::G, CE = Groups<NEN, BS, CEX>::CE;
     V2<T> fe[G > 1 ? G - 1 : 1][G > 1 ? CE : 1][NEN];
 };
 
 // fe = K_e (m v_e) from the gathered LDS values (the group's Gauss points, the groups added in order), then
 // the colour-ordered sum into L.acc.  d / c hold the group's first PF Gauss points on entry.
-template <int NEN, int NQ, int BS, int PF, typename T, bool NT = false, bool AHEAD = false, typename LDS>
+template <int NEN, int NQ, int BS, int PF, typename T, bool NT = false, bool AHEAD = false,
+          int CEX = default_ce<NEN>(), typename LDS>
 __device__ __forceinline__ void element_product(const ChunkView& ch, int t, bool has_el, const int* l, const T* dN,
                                                 const T* C, int ne, int e,
-                                                T (&d)[GaussSplit<NQ, Groups<NEN, BS>::G>::QG][NEN][2],
-                                                T (&c)[GaussSplit<NQ, Groups<NEN, BS>::G>::QG][10],
-                                                LDS& L, GroupLDS<NEN, BS, T>& GL) {
-    constexpr int G = Groups<NEN, BS>::G, CE = Groups<NEN, BS>::CE, QG = GaussSplit<NQ, G>::QG;
+                                                T (&d)[GaussSplit<NQ, Groups<NEN, BS, CEX>::G>::QG][NEN][2],
+                                                T (&c)[GaussSplit<NQ, Groups<NEN, BS, CEX>::G>::QG][10],
+                                                LDS& L, GroupLDS<NEN, BS, T, CEX>& GL) {
+    constexpr int G = Groups<NEN, BS, CEX>::G, CE = Groups<NEN, BS, CEX>::CE, QG = GaussSplit<NQ, G>::QG;
     const int g = G > 1 ? t / CE : 0;
     const int q0 = g * NQ / G, q1 = (g + 1) * NQ / G;
     V2<T> fe[NEN];
@@ -611,14 +619,14 @@ __device__ __forceinline__ void element_product(const ChunkView& ch, int t, bool
 // register allocation must allow (1 = no constraint); STREAM_FIRST (partials plans): those PF Gauss points are
 // requested before the gather rather than behind it
 template <int NEN, int NQ, int BS, typename T, int PF = 2, int MINW = 1, bool AHEAD = false,
-          bool STREAM_FIRST = false>
+          bool STREAM_FIRST = false, int CEX = default_ce<NEN>()>
 __global__ __launch_bounds__(BS, MINW) void k_apply(const dfea_desc* __restrict__ tab, const int* __restrict__ items,
                                                    const T* __restrict__ vin, T* __restrict__ vout,
                                                    V2<T>* __restrict__ part_arg, int part_cap) {
     // partials plans (chunks of exactly CE elements) sum nodes through the inverse map, halo plans by colour
-    constexpr bool INV = BS % Groups<NEN, BS>::CE == 0;
+    constexpr bool INV = BS % Groups<NEN, BS, CEX>::CE == 0;
     __shared__ std::conditional_t<INV, ProductLDS<T>, ChunkLDS<T>> L;
-    __shared__ GroupLDS<NEN, BS, T> GL;
+    __shared__ GroupLDS<NEN, BS, T, CEX> GL;
     const Item it = load_item(items);
     const dfea_desc& D = tab[it.s];
     const ChunkView ch = chunk_view(D.chunks + it.b);
@@ -631,7 +639,7 @@ __global__ __launch_bounds__(BS, MINW) void k_apply(const dfea_desc* __restrict_
 
     // 0. element (local node indices).  Partials plans: tail threads of the last chunk read the last element
     // and drop the result, so every load below is unconditional (no branch splits the loads in flight).
-    constexpr int CE = Groups<NEN, BS>::CE;
+    constexpr int CE = Groups<NEN, BS, CEX>::CE;
     int l[NEN];
     int e;
     bool has_el;
@@ -642,12 +650,12 @@ __global__ __launch_bounds__(BS, MINW) void k_apply(const dfea_desc* __restrict_
     } else {
         has_el = thread_element<NEN>(D, ch, t, e, l);
     }
-    constexpr int QG = GaussSplit<NQ, Groups<NEN, BS>::G>::QG;
+    constexpr int QG = GaussSplit<NQ, Groups<NEN, BS, CEX>::G>::QG;
     T d[QG][NEN][2];
     T c[QG][10];
     if constexpr (INV && STREAM_FIRST) {
         // the bulk stream starts right after the descriptor; the gather below then returns behind it
-        prefetch_gauss_points<NEN, NQ, BS, PF, T, true>(dN, C, ne, e, t, d, c);
+        prefetch_gauss_points<NEN, NQ, BS, PF, T, true, CEX>(dN, C, ne, e, t, d, c);
         __builtin_amdgcn_sched_barrier(0);
     }
 
@@ -671,14 +679,14 @@ __global__ __launch_bounds__(BS, MINW) void k_apply(const dfea_desc* __restrict_
         const int ent = k >= ch.nown && k < ch.nk ? ch.pbase + (k - ch.nown) : 0;
         slot[i] = D.n_part ? gload<false>(D.pslot, (uint32_t)ent) : 0;
     }
-    InvRegs<NEN, BS> inv;
+    InvRegs<NEN, BS, CEX> inv;
     if constexpr (INV) {
 #pragma unroll
         for (int a = 0; a < NEN; ++a) l[a] = gload<false>(D.lelem, (uint32_t)(a * D.n_el + e));
-        inv_load<NEN, BS>(D, ch, t, inv);
+        inv_load<NEN, BS, CEX>(D, ch, t, inv);
     }
     __builtin_amdgcn_sched_barrier(0);
-    if constexpr (!(INV && STREAM_FIRST)) prefetch_gauss_points<NEN, NQ, BS, PF, T, true>(dN, C, ne, e, t, d, c);
+    if constexpr (!(INV && STREAM_FIRST)) prefetch_gauss_points<NEN, NQ, BS, PF, T, true, CEX>(dN, C, ne, e, t, d, c);
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int i = 0; i < NG; ++i) {
@@ -689,13 +697,13 @@ __global__ __launch_bounds__(BS, MINW) void k_apply(const dfea_desc* __restrict_
             L.bits[k] = (unsigned char)(gid[i] >> 30);
         }
     }
-    if constexpr (INV) inv_store<NEN, BS, T>(ch, t, inv, L);
+    if constexpr (INV) inv_store<NEN, BS, T, CEX>(ch, t, inv, L);
     __builtin_amdgcn_sched_barrier(0);
     __syncthreads();
     __builtin_amdgcn_sched_barrier(0);
 
     // 2. element products; node sums in colour order (inverse map or colour passes)
-    element_product<NEN, NQ, BS, PF, T, true, AHEAD>(ch, t, has_el, l, dN, C, ne, e, d, c, L, GL);
+    element_product<NEN, NQ, BS, PF, T, true, AHEAD, CEX>(ch, t, has_el, l, dN, C, ne, e, d, c, L, GL);
 
     // 3. owned nodes: final value; shared nodes (partials plans): partial (into part_arg when given: the
     // caller's buffer of part_cap entries, every slot checked against it)
@@ -1451,13 +1459,14 @@ __device__ __forceinline__ NodeState<T> node_state(const dfea_desc& D, const Chu
 // Gauss points); it costs registers (waves per SIMD), which matters only when the launch has more workgroups
 // than the GPU has CUs.  The scalars the step reads (tolerance, cap, the previous iteration's gamma and
 // alpha) are requested at the start, before the first barrier.
-template <int NEN, int NQ, int BS, typename T, int PF = 2, int MINW = 1, bool EARLY = false>
+template <int NEN, int NQ, int BS, typename T, int PF = 2, int MINW = 1, bool EARLY = false,
+          int CEX = default_ce<NEN>()>
 __global__ __launch_bounds__(BS, MINW) void k_pcg(const dfea_desc* __restrict__ tab, const int* __restrict__ items, int par) {
     static_assert(3 * (BS / 64) <= 16, "block_sum3 needs 3 slots per wave in ChunkLDS::red");
     // partials plans (chunks of exactly CE elements) sum nodes through the inverse map, halo plans by colour
-    constexpr bool INV = BS % Groups<NEN, BS>::CE == 0;
+    constexpr bool INV = BS % Groups<NEN, BS, CEX>::CE == 0;
     __shared__ std::conditional_t<INV, ProductLDS<T>, ChunkLDS<T>> L;
-    __shared__ GroupLDS<NEN, BS, T> GL;
+    __shared__ GroupLDS<NEN, BS, T, CEX> GL;
     const Item it = load_item(items);
     const dfea_desc& D = tab[it.s];
     double* sc = D.sc;
@@ -1483,7 +1492,7 @@ __global__ __launch_bounds__(BS, MINW) void k_pcg(const dfea_desc* __restrict__ 
     const V2<T>* part_rd = reinterpret_cast<const V2<T>*>(D.part) + (size_t)(1 - par) * D.n_part;
     V2<T>* part_wr = reinterpret_cast<V2<T>*>(D.part) + (size_t)par * D.n_part;
     const int2* prg = reinterpret_cast<const int2*>(D.prange);
-    constexpr int CE = Groups<NEN, BS>::CE;
+    constexpr int CE = Groups<NEN, BS, CEX>::CE;
     constexpr int NG = (MAXLN + BS - 1) / BS;     // local nodes per thread, at most
     constexpr int NRD = 3;                        // chunk partials per thread loaded ahead of the scalar step
 
@@ -1523,22 +1532,22 @@ __global__ __launch_bounds__(BS, MINW) void k_pcg(const dfea_desc* __restrict__ 
     int l[NEN];
     int e;
     bool has_el;
-    InvRegs<NEN, BS> inv;
+    InvRegs<NEN, BS, CEX> inv;
     if constexpr (BS % CE == 0) {
         e = it.b * CE + t % CE;
         has_el = e < D.n_el;
         if (!has_el) e = D.n_el - 1;
 #pragma unroll
         for (int a = 0; a < NEN; ++a) l[a] = gload<false>(D.lelem, (uint32_t)(a * D.n_el + e));
-        inv_load<NEN, BS>(D, ch, t, inv);
+        inv_load<NEN, BS, CEX>(D, ch, t, inv);
     } else {
         has_el = thread_element<NEN>(D, ch, t, e, l);
     }
-    constexpr int QG = GaussSplit<NQ, Groups<NEN, BS>::G>::QG;
+    constexpr int QG = GaussSplit<NQ, Groups<NEN, BS, CEX>::G>::QG;
     T d[QG][NEN][2];
     T c[QG][10];
     __builtin_amdgcn_sched_barrier(0);
-    prefetch_gauss_points<NEN, NQ, BS, PF, T>(dN, C, ne, e, t, d, c);
+    prefetch_gauss_points<NEN, NQ, BS, PF, T, false, CEX>(dN, C, ne, e, t, d, c);
     __builtin_amdgcn_sched_barrier(0);
 
     // 1d. the node states (independent of the scalars)
@@ -1664,11 +1673,11 @@ __global__ __launch_bounds__(BS, MINW) void k_pcg(const dfea_desc* __restrict__ 
         if constexpr (!INV) L.acc[kk] = V2<T>{0, 0};
         L.bits[kk] = b;
     }
-    if constexpr (INV) inv_store<NEN, BS, T>(ch, t, inv, L);
+    if constexpr (INV) inv_store<NEN, BS, T, CEX>(ch, t, inv, L);
     __syncthreads();
 
     // 4. w_{k+1} = A u_{k+1}
-    element_product<NEN, NQ, BS, PF, T>(ch, t, has_el, l, dN, C, ne, e, d, c, L, GL);
+    element_product<NEN, NQ, BS, PF, T, false, false, CEX>(ch, t, has_el, l, dN, C, ne, e, d, c, L, GL);
 
     double d_part = 0.0;
 #pragma unroll
@@ -1737,13 +1746,14 @@ int launch(K kern, int grid, int block, void* stream, const char* name, A... arg
 
 // element kind of a launch: nen, with DFEA_HALO set for halo plans (DFEA_MATLIB selects the instantiations
 // with the whole material library and is not part of the kind)
-enum class Kind { T3, Q4, Q4H, T6, Q8, Q9, Bad };
+enum class Kind { T3, Q4, Q4H, Q4S, T6, Q8, Q9, Bad };
 Kind kind_of(int k) {
     k &= ~DFEA_MATLIB;
     switch (k) {
         case 3: return Kind::T3;
         case 4: return Kind::Q4;
         case 4 | DFEA_HALO: return Kind::Q4H;
+        case 4 | DFEA_SMALL: return Kind::Q4S;
         case 6: return Kind::T6;
         case 8: return Kind::Q8;
         case 9: return Kind::Q9;
@@ -1759,12 +1769,13 @@ int dispatch_lib(int kind, F f) {
         case Kind::T3: return f.template run<3, BS_LIN, LIB>();
         case Kind::Q4: return f.template run<4, BS_LIN, LIB>();
         case Kind::Q4H: return f.template run<4, BS_Q4H, LIB>();
+        case Kind::Q4S: return f.template run<4, BS_QUAD, LIB>();
         case Kind::T6: return f.template run<6, BS_QUAD, LIB>();
         case Kind::Q8: return f.template run<8, BS_QUAD, LIB>();
         case Kind::Q9: return f.template run<9, BS_QUAD, LIB>();
         default:
-            return fail(DFEA_EINVAL, "unknown element kind (expected 3, 4, 6, 8, 9 or 4 | DFEA_HALO, optionally "
-                                     "| DFEA_MATLIB)");
+            return fail(DFEA_EINVAL, "unknown element kind (expected 3, 4, 6, 8, 9, 4 | DFEA_HALO or 4 | DFEA_SMALL, "
+                                     "optionally | DFEA_MATLIB)");
     }
 }
 
@@ -1777,16 +1788,20 @@ int dispatch(int kind, F f) {
 // up to 289 nodes) run with bs_groups threads: Gauss-point groups per element, node updates on all lanes.
 template <int NEN, int BS, typename T>
 int pcg_iter(const dfea_desc* table, const int32_t* items, int n_items, int n_iter, void* stream) {
-    constexpr int BSP = quadratic<NEN>() ? bs_groups<T>() : BS;
+    // small-chunk quad4 plans (dispatched with the quadratic elements' chunk workgroup): 64-element chunks,
+    // 4 threads per element (one Gauss point each)
+    constexpr bool SMALL = NEN == 4 && BS == BS_QUAD;
+    constexpr int CEX = SMALL ? CHUNK_QUAD : default_ce<NEN>();
+    constexpr int BSP = quadratic<NEN>() ? bs_groups<T>() : SMALL ? BS_Q4S : BS;
     const bool early = n_items <= PCG_EARLY_ITEMS;
     for (int i = 0; i < n_iter + (n_iter & 1); ++i) {
         // small launches (one workgroup per CU or fewer): every node state and, for linear elements, every Gauss
         // point requested before the scalar step
         constexpr int PF_EARLY = NEN <= 4 && BSP % CHUNK_LIN == 0 ? nq_of<NEN>() : 2;
-        const int rc = early ? launch(k_pcg<NEN, nq_of<NEN>(), BSP, T, PF_EARLY, 1, true>, n_items, BSP, stream, "k_pcg",
-                                      table, items, i & 1)
-                             : launch(k_pcg<NEN, nq_of<NEN>(), BSP, T, 2, (NEN <= 4 ? 3 : 1), false>, n_items, BSP, stream, "k_pcg", table,
-                                      items, i & 1);
+        const int rc = early ? launch(k_pcg<NEN, nq_of<NEN>(), BSP, T, PF_EARLY, 1, true, CEX>, n_items, BSP, stream,
+                                      "k_pcg", table, items, i & 1)
+                             : launch(k_pcg<NEN, nq_of<NEN>(), BSP, T, 2, (NEN <= 4 && !SMALL ? 3 : 1), false, CEX>,
+                                      n_items, BSP, stream, "k_pcg", table, items, i & 1);
         if (rc) return rc;
     }
     return 0;
@@ -1847,18 +1862,20 @@ struct Launch_apply {
     void* part = nullptr;        // shared-node partials buffer (nullptr: the descriptor's part)
     int part_cap = 0x7fffffff;   // its capacity in entries
     template <typename T, int NEN, int BS0> int go() {
-        constexpr int BS = quadratic<NEN>() ? bs_groups<T>() : BS0;
+        constexpr bool SMALL = NEN == 4 && BS0 == BS_QUAD;     // small-chunk quad4 plans (see pcg_iter)
+        constexpr int CEX = SMALL ? CHUNK_QUAD : default_ce<NEN>();
+        constexpr int BS = quadratic<NEN>() ? bs_groups<T>() : SMALL ? BS_Q4S : BS0;
         const T* vv = static_cast<const T*>(v);
         T* yy = static_cast<T*>(y);
         // linear elements: Gauss points requested one step ahead while the launch fits in the workgroup slots
         // of the (costlier in registers) ahead variant; larger quad4 launches start the element stream before
         // the gather instead
-        const bool ahead = NEN <= 4 && n <= APPLY_AHEAD_ITEMS;
+        const bool ahead = NEN <= 4 && !SMALL && n <= APPLY_AHEAD_ITEMS;
         V2<T>* pp = static_cast<V2<T>*>(part);
-        int rc = ahead ? launch(k_apply<NEN, nq_of<NEN>(), BS, T, 2, 1, NEN <= 4>, n, BS, st, "k_apply", t, it, vv, yy,
-                                pp, part_cap)
-                       : launch(k_apply<NEN, nq_of<NEN>(), BS, T, 2, 1, false, NEN == 4>, n, BS, st, "k_apply", t,
-                                it, vv, yy, pp, part_cap);
+        int rc = ahead ? launch(k_apply<NEN, nq_of<NEN>(), BS, T, 2, 1, NEN <= 4 && !SMALL, false, CEX>, n, BS, st,
+                                "k_apply", t, it, vv, yy, pp, part_cap)
+                       : launch(k_apply<NEN, nq_of<NEN>(), BS, T, 2, 1, false, NEN == 4 && !SMALL, CEX>, n, BS, st,
+                                "k_apply", t, it, vv, yy, pp, part_cap);
         if (rc) return rc;
         return launch(k_apply_fin<T>, nf, FBS, st, "k_apply_fin", t, fi, vv, yy, (const V2<T>*)pp, part_cap);
     }
@@ -1955,7 +1972,7 @@ int dfea_finblk(int n_shared) { return (n_shared + FBS - 1) / FBS; }
 int dfea_chunk_elems(int nen) {
     switch (kind_of(nen)) {
         case Kind::T3: case Kind::Q4: case Kind::Q4H: return CHUNK_LIN;
-        case Kind::T6: case Kind::Q8: case Kind::Q9: return CHUNK_QUAD;
+        case Kind::Q4S: case Kind::T6: case Kind::Q8: case Kind::Q9: return CHUNK_QUAD;
         default: return -1;
     }
 }
@@ -1964,7 +1981,7 @@ int dfea_chunk_threads(int nen) {
     switch (kind_of(nen)) {
         case Kind::T3: case Kind::Q4: return BS_LIN;
         case Kind::Q4H: return BS_Q4H;
-        case Kind::T6: case Kind::Q8: case Kind::Q9: return BS_QUAD;
+        case Kind::Q4S: case Kind::T6: case Kind::Q8: case Kind::Q9: return BS_QUAD;
         default: return -1;
     }
 }

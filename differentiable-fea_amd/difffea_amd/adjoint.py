@@ -2,7 +2,8 @@
 
 For R(u*, theta) = 0 and a scalar L(u*):   A^T lam = m * dL/du,   dL/dtheta = -lam^T dR/dtheta.
 
-The tangent is symmetric, so the adjoint solve reuses the fused PCG of the forward tangent at u*.  For the
+The tangent is symmetric, so the adjoint solve reuses the fused PCG of the forward tangent at u* (in mixed
+precision with iterative refinement against an FP64 tangent, so it meets the same tolerance as in FP64).  For the
 neo-Hookean material with theta scaling both Lame parameters, R^e_int is linear in theta_e and
 
     dL/dtheta_e = -sum_{a,i} (m lam)[2 conn(e,a)+i] R^e_int[a,i](u*; mu0, lam0)
@@ -15,20 +16,51 @@ adjoint solve + the hand-written gradient kernel, so ``loss(u(theta)).backward()
 import torch
 from torch.func import grad
 
+from .device import HipSystem
 from .newton_krylov import HipNewton, MIXED_RTOL_FLOOR
 
 
+MAX_REFINE = 12       # refinement steps of the mixed-precision adjoint, at most
+
+
 def solve_adjoint(solver, u_star, dL_du, rtol=1e-6, max_iter=500):
-    """PCG on A lam = m * dL/du at u* (A symmetric = A^T); returns (lam, iterations)."""
+    """PCG on A lam = m * dL/du at u* (A symmetric = A^T); returns (lam, iterations).
+
+    FP64: one PCG to ``rtol``.  Mixed precision: iterative refinement to the same FP64 relative residual --
+    the correction solves A d = r run in FP32 (moduli, product, vectors; FP64 dot products) to the FP32 floor,
+    lam accumulates in FP64, and r = b - A lam comes from an FP64 tangent at u* (its own moduli), so the
+    adjoint meets ``rtol`` as the FP64 solve does."""
     s = solver.sys
     s.moduli(u_star)
     m = solver.params["m"].to(u_star.device, torch.float64)
-    s.vec["b"].copy_((m * dL_du).to(s.vdt))
-    if s.precision == "mixed":
-        rtol = max(rtol, MIXED_RTOL_FLOOR)
-    s.pcg_start(rtol=rtol, max_iter=max_iter, neg_res=False)
-    its, _ = s.pcg_run(max_iter)
-    return s.vec["x"].to(torch.float64).clone(), its
+    b = (m * dL_du.to(torch.float64)).contiguous()
+    if s.precision != "mixed":
+        s.vec["b"].copy_(b)
+        s.pcg_start(rtol=rtol, max_iter=max_iter, neg_res=False)
+        its, _ = s.pcg_run(max_iter)
+        return s.vec["x"].to(torch.float64).clone(), its
+    s64 = HipSystem(s.mesh, solver.theta, s.mat, s.f, "fp64", preconditioner=s.preconditioner)
+    s64.moduli(u_star)
+    return refine(s, s64, b, rtol, max_iter)
+
+
+def refine(s, s64, b, rtol, max_iter):
+    """Mixed-precision iterative refinement of A lam = b: FP32 PCG corrections on s (moduli already set),
+    residuals b - A lam with the FP64 tangent s64; returns (lam FP64, total inner iterations)."""
+    nb = float(torch.norm(b))
+    lam = torch.zeros_like(b)
+    r = b
+    its = 0
+    for _ in range(MAX_REFINE):
+        s.vec["b"].copy_(r.to(s.vdt))
+        s.pcg_start(rtol=max(rtol, MIXED_RTOL_FLOOR), max_iter=max_iter, neg_res=False)
+        k, _ = s.pcg_run(max_iter)
+        its += k
+        lam += s.vec["x"].to(torch.float64)
+        r = b - s64.apply(lam)
+        if float(torch.norm(r)) <= rtol * nb:
+            break
+    return lam, its
 
 
 def compute_implicit_vjp(solver, u_star, lam):

@@ -17,7 +17,7 @@ import torch
 from . import _lib
 from .materials import MaterialSpec
 from .elements import NEN, default_nq, element_type_of, reference_gradient_table
-from .plan import MeshPlan
+from .plan import MeshPlan, chunking_for
 
 F64, F32 = torch.float64, torch.float32
 
@@ -37,7 +37,9 @@ def _dev_int32(a, device):
 class HipMesh:
     """Mesh plan + geometry on the device.  ``params``: the reference's params contract."""
 
-    def __init__(self, params, device=None, plan_mode="auto", plan=None):
+    def __init__(self, params, device=None, plan_mode="auto", plan=None, chunking="auto"):
+        """plan_mode: "auto" / "partials" / "halo" (quad4); chunking: "auto" / "small" / "large" -- the chunk
+        size of partials plans (plan.chunking_for); plan: a MeshPlan of these tables built beforehand."""
         self.lib = _lib.load()
         device = torch.device(device or "cuda")
         if device.type != "cuda":
@@ -57,18 +59,20 @@ class HipMesh:
                              f"n_q={self.nq}")
         cij = params.get("cell_ij")
         # plan: a MeshPlan of exactly these tables, built beforehand (e.g. while the GPU was busy)
+        ce, _ = chunking_for(self.nen, conn.shape[0], chunking) if plan_mode != "halo" else (None, 0)
         self.plan = p = plan if plan is not None else MeshPlan(
-            nodes.numpy(), conn.numpy(), fixed.numpy(), mode=plan_mode,
+            nodes.numpy(), conn.numpy(), fixed.numpy(), mode=plan_mode, chunk_elems=ce,
             cell_ij=None if cij is None else cij.detach().cpu().numpy())
         if p.n_el != conn.shape[0] or p.n_nodes != nodes.shape[0]:
             raise ValueError("the plan was built for another mesh")
-        # element kind of every launch: nen, flagged for halo plans
-        self.kind = self.nen | (_lib.HALO if p.mode == "halo" else 0)
+        # element kind of every launch: nen, flagged for halo plans and for small-chunk quad4 plans
+        small = p.mode == "partials" and self.nen == 4 and p.chunk_elems == self.lib.dfea_chunk_elems(4 | _lib.SMALL)
+        self.kind = self.nen | (_lib.HALO if p.mode == "halo" else 0) | (_lib.SMALL if small else 0)
         inv_off, inv_slot = p.inverse_map() if p.mode == "partials" else (np.zeros(1, np.int32), np.zeros(1, np.uint16))
         if p.validate(self.lib.dfea_chunk_nodes(), (inv_off, inv_slot) if p.mode == "partials" else None) > \
                 self.lib.dfea_chunk_threads(self.kind):
             raise ValueError("chunk larger than the kernel's workgroup")
-        E = self.lib.dfea_chunk_elems(self.nen)
+        E = self.lib.dfea_chunk_elems(self.kind)
         if p.mode == "partials" and (not np.array_equal(p.chunks[:, 0], np.arange(p.n_chunks) * E) or
                                      p.chunks[:, 1].max() > E):
             raise ValueError(f"partials plans must cut the element order into chunks of {E} elements")

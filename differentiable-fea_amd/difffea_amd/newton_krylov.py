@@ -55,12 +55,14 @@ def pcg(A_fn, b, Minv_fn, rtol=1e-5, max_iter=500):
 class HipNewton:
     """The Newton-Krylov state machine of one solve on one ``HipSystem`` (device-resident)."""
 
-    def __init__(self, params, theta, precision="fp64", device=None, preconditioner="jacobi"):
+    def __init__(self, params, theta, precision="fp64", device=None, preconditioner="jacobi", mesh=None):
+        """mesh: a ``HipMesh`` of exactly these params built beforehand (else the cached one of params)."""
         if precision not in PRECISIONS:
             raise ValueError(f"Unknown precision '{precision}'. Choose one of {PRECISIONS}.")
         device = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
         self.params = params
-        self.mesh = get_mesh(params, device)
+        self.theta = theta.detach()
+        self.mesh = mesh if mesh is not None else get_mesh(params, device)
         self.sys = HipSystem(self.mesh, theta.detach(), _check_material(params), params["f"], precision,
                              preconditioner=preconditioner)
         self.device = self.mesh.device

@@ -136,6 +136,21 @@ def jones_plassmann(conn, seed=0):
 
 
 
+# quad4 meshes of at most this many elements use small-chunk plans (64-element chunks, DFEA_SMALL kernels): with
+# 256-element chunks a 100x100 mesh keeps 40 of the 256 CUs busy.  Measured on MI355X (PCG iteration, FP64):
+# 100x100 11.4 -> 8.7 us, 150x150 11.7 -> 10.6 us, 200x200 12.1 -> 12.9 us (slower), hence the bound
+SMALL_QUAD4_MAX_ELEMS = 24000
+
+
+def chunking_for(nen, n_el, chunking="auto"):
+    """(chunk_elems or None for the element type's default, element-kind flag) of a mesh: "auto" picks
+    small-chunk plans for quad4 meshes of at most SMALL_QUAD4_MAX_ELEMS elements; "small" / "large" force."""
+    if chunking not in ("auto", "small", "large"):
+        raise ValueError(f"chunking must be 'auto', 'small' or 'large', got {chunking!r}")
+    small = nen == 4 and (chunking == "small" or (chunking == "auto" and n_el <= SMALL_QUAD4_MAX_ELEMS))
+    return (_lib.load().dfea_chunk_elems(4 | _lib.SMALL), _lib.SMALL) if small else (None, 0)
+
+
 def _require(ok, what):
     if not ok:
         raise ValueError(f"invalid mesh plan: {what}")
@@ -165,6 +180,7 @@ class MeshPlan:
             raise ValueError("meshes with 2^28 or more nodes or elements are not supported (the kernels address "
                              "element planes and node arrays through 32-bit byte offsets)")
         self.n_el, self.nen, self.n_nodes = n_el, nen, N
+        self.chunk_elems = E
 
         # 1. order
         if cell_ij is not None:                 # logical cell indices supplied by the mesh generator

@@ -30,8 +30,9 @@ from . import _lib
 from .device import HipMesh, HipSystem, _ptr, _stream
 from .elements import FAMILY, NEN, ORDER, default_nq, element_type_of
 from .materials import material_spec
-from .plan import MeshPlan
+from .plan import MeshPlan, chunking_for
 from .problem import build_problem
+from .adjoint import MAX_REFINE
 
 MIXED_RTOL_FLOOR = 1e-6
 HBM_SHARE = 0.8          # share of the free device memory the server admits requests into by default
@@ -69,9 +70,7 @@ def make_stream(n_requests=2000, rate=50.0, seed=5, side=(32, 512), q4_fraction=
     out = []
     for i in range(n_requests):
         et = "quad4" if rng.random() < q4_fraction else "quad9"
-        n = int(round(math.exp(rng.uniform(math.log(side[0]), math.log(side[1])))))
-        if et == "quad9":
-            n = max(side[0] // 2, n // 2)          # same node count per side as a Q4 mesh of side n
+        n = int(round(math.exp(rng.uniform(math.log(side[0]), math.log(side[1])))))   # cells per side, Q4 and Q9
         mu = rng.uniform(0.2, 0.6)
         lam = mu * rng.uniform(1.0, 3.0)
         out.append(Request(i, float(t[i]), et, n, mu, lam, rng.uniform(0.01, 0.15),
@@ -147,13 +146,21 @@ def footprint_bytes(element_type, n, precision, preconditioner="jacobi", n_el=No
 
 
 def request_footprint(req, precision, preconditioner="jacobi"):
-    """footprint_bytes of a request."""
+    """footprint_bytes of a request (plus, for a mixed-precision adjoint, its FP64 refinement tangent)."""
     if req.params is not None:
         conn = req.params["conn"]
         et = req.params.get("element_type") or element_type_of(conn.shape[1])
-        return footprint_bytes(et, 0, precision, preconditioner, n_el=conn.shape[0],
-                               n_dof=2 * req.params["nodes"].shape[0])
-    return footprint_bytes(req.element_type, req.n, precision, preconditioner)
+        n_el, n_dof = conn.shape[0], 2 * req.params["nodes"].shape[0]
+        b = footprint_bytes(et, 0, precision, preconditioner, n_el=n_el, n_dof=n_dof)
+    else:
+        et = req.element_type
+        b = footprint_bytes(et, req.n, precision, preconditioner)
+        n_el = (2 if FAMILY[et] == "tri" else 1) * req.n * req.n
+        side = req.n + 1 if ORDER[et] == 1 else 2 * req.n + 1
+        n_dof = 2 * side * side
+    if req.adjoint and precision == "mixed":
+        b += n_el * default_nq(et) * 10 * 8 + n_dof * 16 * 8
+    return b
 
 
 # ------------------------------------------------------------------------------------------------------------
@@ -174,12 +181,14 @@ class Job:
     items: dict = field(default_factory=dict)
 
 
-def solve_ops(job, tol, max_newton=25, max_cutbacks=8, cg_max_iter=500):
+def solve_ops(job, tol, max_newton=25, max_cutbacks=8, cg_max_iter=500, adjoint_rtol=1e-8):
     """The reference's Newton-Krylov state machine, as a generator of device operations.
 
     yields ("lift", lf) -> None, ("resnorm", lf) -> ||R||, ("moduli",) -> None,
     ("pcg", rtol, max_iter, neg_res) -> iterations, ("trial", alpha) -> None, ("accept",) -> None,
-    ("restore",) -> None.  Returns the reference's ``info`` entries (`newton_krylov.py:226-264`: converged,
+    ("restore",) -> None, ("adjoint",) -> iterations (PCG on the adjoint right-hand side or, in mixed precision,
+    on the refinement residual), ("adjoint_residual",) -> ||m f - A lam|| / ||m f|| (FP64 tangent),
+    ("adjoint_grad",) -> None.  Returns the reference's ``info`` entries (`newton_krylov.py:226-264`: converged,
     load_factor, history of per-iteration records, total_newton, total_cg, n_cutbacks) plus the last
     residual norm.
     """
@@ -244,14 +253,36 @@ def solve_ops(job, tol, max_newton=25, max_cutbacks=8, cg_max_iter=500):
     info["load_factor"] = 1.0
     if job.req.adjoint:
         yield ("moduli",)                         # tangent at u* (vec[u] holds the accepted state)
-        k = yield ("adjoint",)                    # A lam = m f (compliance: dL/du = f); then the gradient kernel
+        k = yield ("adjoint",)                    # A lam = m f (compliance: dL/du = f)
         info["adjoint_cg"] = k
+        if mixed:
+            # iterative refinement to the FP64 tolerance: residual m f - A lam with an FP64 tangent, FP32
+            # corrections A d = r
+            for _ in range(MAX_REFINE):
+                rel = yield ("adjoint_residual",)
+                if not rel > adjoint_rtol:
+                    break
+                info["adjoint_cg"] += yield ("adjoint",)
+        yield ("adjoint_grad",)                   # the hand-written gradient kernel
     return info
 
 
 # ------------------------------------------------------------------------------------------------------------
 # the server
 # ------------------------------------------------------------------------------------------------------------
+class _Done:
+    """A finished future holding a value (host set-up the caller did beforehand)."""
+
+    def __init__(self, value):
+        self.value = value
+
+    def done(self):
+        return True
+
+    def result(self):
+        return self.value
+
+
 def _kinds(jobs):
     """Element kinds (nen plus plan flags) present among the jobs, in a fixed order."""
     return sorted({j.mesh.kind for j in jobs})
@@ -323,7 +354,9 @@ class Server:
         runs on a worker thread while the GPU works."""
         params, theta = request_problem(req)
         cij = params.get("cell_ij")
-        plan = MeshPlan(params["nodes"].numpy(), params["conn"].numpy(), params["fixed"].numpy(),
+        conn = params["conn"].numpy()
+        plan = MeshPlan(params["nodes"].numpy(), conn, params["fixed"].numpy(),
+                        chunk_elems=chunking_for(conn.shape[1], conn.shape[0])[0],
                         cell_ij=None if cij is None else cij.numpy())
         return params, theta, plan
 
@@ -344,11 +377,14 @@ class Server:
                      "vec": self._items(slot, mesh.n_vblk), "solve": self._items(slot, 1)}
         job.tol = req.tol if req.tol is not None else self.rel_tol * reference_residual_norm(sys_, params, self.device)
         if req.adjoint:              # right-hand side of the compliance adjoint, m f
-            job.bvec = (params["m"] * params["f"]).to(self.device, sys_.vdt)
+            job.bvec64 = (params["m"] * params["f"]).to(self.device, torch.float64).contiguous()
+            job.pcg_rhs = job.bvec64.to(sys_.vdt)
+            job.lam = None
+            job.sys64 = None
         self._item_cache.clear()
         job.saved = torch.zeros_like(sys_.vec["uacc"])
         job.gen = solve_ops(job, job.tol, max_newton=req.max_newton, max_cutbacks=req.max_cutbacks,
-                            cg_max_iter=self.cg_max_iter)
+                            cg_max_iter=self.cg_max_iter, adjoint_rtol=self.adjoint_rtol)
         job.op = next(job.gen)
         job.t_admit = time.perf_counter()
         self.jobs[slot] = job
@@ -453,9 +489,9 @@ class Server:
                 sub = [j for j in new if (j.op[0] == "pcg") == neg]
                 if not sub:
                     continue
-                if not neg:                      # adjoint right-hand side m f (compliance), and the moduli at u*
+                if not neg:                      # adjoint right-hand side (m f, or a refinement residual)
                     for j in sub:
-                        j.sys.vec["b"].copy_(j.bvec)
+                        j.sys.vec["b"].copy_(j.pcg_rhs)
                 it, n = self._dev_items(sub, "vec")
                 si, ns = self._dev_items(sub, "solve")
                 _lib.check(lib.dfea_pcg_init(T, _ptr(it), n, _ptr(si), ns, int(neg), self.prec_id, st), "dfea_pcg_init")
@@ -492,6 +528,20 @@ class Server:
                     joins.append(done)
             for done in joins:
                 main.wait_event(done)
+        # adjoint refinement residuals (mixed precision) and gradients: per job, on the main stream
+        for j in by.get("adjoint_residual", []):
+            x = j.sys.vec["x"].to(torch.float64)
+            j.lam = x.clone() if j.lam is None else j.lam + x
+            if j.sys64 is None:                  # the FP64 tangent at u*, for the refinement residuals
+                j.sys64 = HipSystem(j.mesh, j.theta, j.sys.mat, j.sys.f, "fp64", preconditioner=self.preconditioner)
+                j.sys64.moduli(j.sys.vec["uacc"])
+            r = j.bvec64 - j.sys64.apply(j.lam)
+            j.pcg_rhs = r.to(j.sys.vdt)
+            j.adj_rel = float(torch.norm(r)) / max(float(torch.norm(j.bvec64)), 1e-300)
+        for j in by.get("adjoint_grad", []):
+            lam = j.lam if j.lam is not None else j.sys.vec["x"].to(torch.float64)
+            j.result["grad"] = j.sys.adjoint_grad(j.sys.vec["uacc"], lam)
+            j.sys64 = None
         if host_work is not None:
             host_work()
         # one host sync: status, iterations, norms
@@ -505,9 +555,8 @@ class Server:
                     continue
                 j.pcg_running = False
                 val = int(sc[j.slot, _lib.SC["iters"]])
-                if kind == "adjoint":
-                    lam = j.sys.vec["x"].to(torch.float64)
-                    j.result["grad"] = j.sys.adjoint_grad(j.sys.vec["uacc"], lam)
+            elif kind == "adjoint_residual":
+                val = j.adj_rel
             elif kind == "resnorm":
                 val = math.sqrt(sc[j.slot, _lib.SC["norm"]])
             else:
@@ -522,9 +571,10 @@ class Server:
         return finished
 
     # -- driver ------------------------------------------------------------------------------------------------
-    def run(self, requests, realtime=True, keep_solution=False, progress=None):
+    def run(self, requests, realtime=True, keep_solution=False, progress=None, prepared=None):
         """Serve ``requests`` (sorted by arrival).  Returns the per-request records and a summary.
-        ``progress``: optional callable(n_done, n_total), called as requests complete."""
+        ``progress``: optional callable(n_done, n_total), called as requests complete.  ``prepared``: optional
+        dict rid -> the ``prepare(req)`` result, host set-up done beforehand (else worker threads do it)."""
         pending = sorted(requests, key=lambda r: r.arrival)
         records = []
         futures = {}                 # request index -> host set-up running on the worker thread
@@ -538,7 +588,8 @@ class Server:
             now, k = time.perf_counter() - t0, i
             while k < len(pending) and k < i + self.prep_ahead and (not realtime or pending[k].arrival <= now):
                 if k not in futures:
-                    futures[k] = pool.submit(self.prepare, pending[k])
+                    pre = prepared.get(pending[k].rid) if prepared else None
+                    futures[k] = pool.submit(self.prepare, pending[k]) if pre is None else _Done(pre)
                 k += 1
 
         try:

@@ -44,6 +44,8 @@ extern "C" {
 #define DFEA_FIXED_Y 0x80000000u
 #define DFEA_NODE_MASK 0x3fffffffu
 #define DFEA_HALO 0x100        /* element-kind flag: nen | DFEA_HALO selects the halo-plan kernels */
+#define DFEA_SMALL 0x400       /* element-kind flag: 4 | DFEA_SMALL selects the quad4 kernels of small-chunk plans
+                                  (64-element chunks; the product and PCG give each element 4 threads) */
 #define DFEA_MATLIB 0x200      /* element-kind flag: the descriptors may use any DFEA_MAT_* material; without it
                                   the material kernels compile only the neo-Hookean closed forms (fewer
                                   registers) and give NaN for a descriptor of another material */

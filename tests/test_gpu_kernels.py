@@ -260,3 +260,36 @@ def test_inverted_elements_give_the_reference_nans(prec):
     x, k, st = s.pcg_solve((-Rref).to(DEV).to(s.vdt), rtol=0.5, max_iter=30)
     assert (st, k, k_ref) == (3, 30, 30)
     assert torch.isnan(x).any()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_small_and_large_chunk_plans_agree_with_oracle(precision):
+    """The same quad4 mesh through small-chunk plans (64-element chunks, 4 threads per element in the product
+    and PCG) and the default 256-element plans: product, residual, diagonal and PCG against the oracle and
+    against each other."""
+    from difffea_amd import _lib
+    from difffea_amd.device import HipMesh, HipSystem
+    from difffea_amd.materials import material_spec
+    p, u, v, th = _setup("quad4", 60, seed=11, nonuniform=True, amp=0.02)     # a positive definite tangent
+    tol = 1e-11 if precision == "fp64" else 1e-5
+    ref = fem.make_A_operator_qp(u, th, p)(v)
+    b = v * p["m"]
+    x_ref, k_ref = fem.pcg(fem.make_A_operator_qp(u, th, p), b, fem.make_jacobi_preconditioner(u, th, p),
+                           rtol=1e-10, max_iter=3000)
+    assert float(torch.norm(b - fem.make_A_operator_qp(u, th, p)(x_ref))) < 1e-9 * float(torch.norm(b))
+    out = {}
+    for ch in ("small", "large"):
+        m = HipMesh(p, DEV, chunking=ch)
+        assert bool(m.kind & _lib.SMALL) == (ch == "small")
+        s = HipSystem(m, th, material_spec(p), p["f"], precision)
+        s.moduli(u.to(DEV))
+        y = s.apply(v.to(DEV).to(s.vdt))
+        assert rel(y, ref) < tol
+        assert rel(s.residual(u.to(DEV)), fem.global_residual(u, th, p)) < 1e-11
+        assert rel(s.diag, fem.compute_jacobi_diagonal(u, th, p)) < (1e-11 if precision == "fp64" else 1e-6)
+        x, k, st = s.pcg_solve(b.to(DEV).to(s.vdt), rtol=1e-10 if precision == "fp64" else 1e-6, max_iter=3000)
+        assert st == _lib.PCG_CONVERGED
+        assert rel(x, x_ref) < (1e-8 if precision == "fp64" else 1e-4)
+        out[ch] = (y.double().cpu(), k)
+    assert rel(out["small"][0], out["large"][0]) < tol
+    assert abs(out["small"][1] - out["large"][1]) <= max(2, 0.02 * out["large"][1])

@@ -76,7 +76,8 @@ def test_mixed_server_reaches_fp64_residual_tolerance():
         assert float(torch.norm(R)) < r["tol_abs"] * 1.0000001
         assert rel(r["u"], ref[r["rid"]]["u"]) < 1e-6
         if req.adjoint:
-            assert rel(r["grad"], ref[r["rid"]]["grad"]) < 1e-4
+            # the mixed adjoint is refined against an FP64 tangent to the FP64 server's 1e-8 relative residual
+            assert rel(r["grad"], ref[r["rid"]]["grad"]) < 1e-6
 
 
 def test_block_jacobi_server_bitwise_solo_and_within_tolerance():

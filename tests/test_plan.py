@@ -122,11 +122,17 @@ def _emulate_apply(plan, C, dN, v, dots=None):
     return y.reshape(-1)
 
 
-@pytest.mark.parametrize("mode", ["halo", "partials"])
+@pytest.mark.parametrize("mode", ["halo", "partials", "small"])
 @pytest.mark.parametrize("name", list(MESHES))
 def test_plan_tables_and_emulated_assembly_match_oracle(name, mode):
     p = fem.precompute_geometry(MESHES[name]())
-    plan = _plan(p, mode=mode)
+    if mode == "small":                 # small-chunk plans: quad4 in 64-element chunks
+        if p["conn"].shape[1] != 4:
+            pytest.skip("small-chunk plans are for quad4")
+        plan = _plan(p, chunk_elems=64)
+        assert plan.chunk_elems == 64 and plan.chunks[:, 1].max() <= 64
+    else:
+        plan = _plan(p, mode=mode)
     if mode == "halo" and plan.mode != "halo":
         pytest.skip("halo chunks would exceed the workgroup for this element type")
     plan.validate()
@@ -242,3 +248,16 @@ def test_validate_rejects_corrupt_tables(case):
     mutate(pl, inv)
     with pytest.raises(ValueError):
         pl.validate(640, inv)
+
+
+def test_chunking_choice():
+    """quad4 meshes up to SMALL_QUAD4_MAX_ELEMS elements get 64-element chunks (the DFEA_SMALL kernels),
+    larger ones and other element types the element type's default."""
+    from difffea_amd.plan import SMALL_QUAD4_MAX_ELEMS, chunking_for
+    assert chunking_for(4, 10000) == (64, _lib.SMALL) and chunking_for(4, SMALL_QUAD4_MAX_ELEMS)[1] == _lib.SMALL
+    assert chunking_for(4, SMALL_QUAD4_MAX_ELEMS + 1) == (None, 0)
+    assert chunking_for(4, 10 ** 6, "small") == (64, _lib.SMALL)
+    assert chunking_for(4, 100, "large") == (None, 0)
+    assert chunking_for(9, 100) == (None, 0)
+    with pytest.raises(ValueError):
+        chunking_for(4, 100, "tiny")
