@@ -58,7 +58,9 @@ constexpr int PCG_EARLY_ITEMS = 256;
 // product launches of at most this many workgroups request the Gauss points one step ahead (linear elements):
 // that variant holds 3 workgroups of 256 threads per CU, the other 4
 constexpr int APPLY_AHEAD_ITEMS = 3 * 256;
-constexpr int MAXLN = 640;        // local nodes per chunk (LDS capacity)
+// local nodes per chunk (LDS capacity): the planner's chunks stay far below it (structured quad4 289, quad9 289-305,
+// Delaunay quad4 335, quad9 333 measured); the LDS it saves lets more workgroups share a CU
+constexpr int MAXLN = 448;
 constexpr int VBS = 256;          // workgroup size of the vector kernels
 constexpr int VPT = 4;            // dofs per thread in the vector kernels
 constexpr int VBLK = VBS * VPT;   // dofs per vector block
@@ -452,13 +454,14 @@ __device__ __forceinline__ void prefetch_gauss_points(const T* dN, const T* C, i
 // LDS of the product and PCG kernels on partials plans.  acc holds the chunk's element vectors (slot
 // e_local * NEN + a) instead of node accumulators; slot / offs hold the chunk's inverse map (node k's element
 // vectors are slots slot[offs[k] .. offs[k+1])), so every node sums its own in one pass after one barrier.
-constexpr int SUMSLOTS = 1024;    // element-vector slots of a chunk, at most (256 quad4 elements x 4 nodes)
-template <typename T>
+// SLOTS: element-vector slots of a chunk (chunk elements x nodes per element)
+template <typename T, int SLOTS>
 struct ProductLDS {
+    static constexpr bool kInverse = true;
     V2<T> v[MAXLN];
-    V2<T> acc[SUMSLOTS];
+    V2<T> acc[SLOTS];
     unsigned char bits[MAXLN];
-    uint16_t slot[SUMSLOTS];
+    uint16_t slot[SLOTS];
     int offs[MAXLN + 1];
     double red[16];
 };
@@ -492,7 +495,7 @@ __device__ __forceinline__ void inv_load(const dfea_desc& D, const ChunkView& ch
 
 template <int NEN, int BS, typename T, int CEX = default_ce<NEN>()>
 __device__ __forceinline__ void inv_store(const ChunkView& ch, int t, const InvRegs<NEN, BS, CEX>& R,
-                                          ProductLDS<T>& L) {
+                                          ProductLDS<T, CEX * NEN>& L) {
     const int n = ch.ne * NEN;
 #pragma unroll
     for (int i = 0; i < InvRegs<NEN, BS, CEX>::NSL; ++i) {
@@ -508,8 +511,8 @@ __device__ __forceinline__ void inv_store(const ChunkView& ch, int t, const InvR
 
 // the element vectors of local node k summed in slot (= colour) order: the same additions, in the same
 // order, as the colour-by-colour accumulation into a zeroed accumulator
-template <typename T>
-__device__ __forceinline__ V2<T> node_sum(const ProductLDS<T>& L, int k) {
+template <typename T, int SLOTS>
+__device__ __forceinline__ V2<T> node_sum(const ProductLDS<T, SLOTS>& L, int k) {
     V2<T> s{0, 0};
     const int j1 = L.offs[k + 1];
     for (int j = L.offs[k]; j < j1; ++j) {
@@ -602,8 +605,7 @@ __device__ __forceinline__ void element_product(const ChunkView& ch, int t, bool
             }
         }
     }
-    if constexpr (std::is_same<LDS, ProductLDS<T>>::value) {
-        static_assert(CE * NEN <= SUMSLOTS, "element-vector slots exceed ProductLDS");
+    if constexpr (std::is_same<LDS, ProductLDS<T, CE * NEN>>::value) {
         // inverse-map sums: element vectors into their slots, one barrier, then every node sums its own
         if (g == 0 && has_el) {
 #pragma unroll
@@ -625,7 +627,7 @@ __global__ __launch_bounds__(BS, MINW) void k_apply(const dfea_desc* __restrict_
                                                    V2<T>* __restrict__ part_arg, int part_cap) {
     // partials plans (chunks of exactly CE elements) sum nodes through the inverse map, halo plans by colour
     constexpr bool INV = BS % Groups<NEN, BS, CEX>::CE == 0;
-    __shared__ std::conditional_t<INV, ProductLDS<T>, ChunkLDS<T>> L;
+    __shared__ std::conditional_t<INV, ProductLDS<T, CEX * NEN>, ChunkLDS<T>> L;
     __shared__ GroupLDS<NEN, BS, T, CEX> GL;
     const Item it = load_item(items);
     const dfea_desc& D = tab[it.s];
@@ -1465,7 +1467,7 @@ __global__ __launch_bounds__(BS, MINW) void k_pcg(const dfea_desc* __restrict__ 
     static_assert(3 * (BS / 64) <= 16, "block_sum3 needs 3 slots per wave in ChunkLDS::red");
     // partials plans (chunks of exactly CE elements) sum nodes through the inverse map, halo plans by colour
     constexpr bool INV = BS % Groups<NEN, BS, CEX>::CE == 0;
-    __shared__ std::conditional_t<INV, ProductLDS<T>, ChunkLDS<T>> L;
+    __shared__ std::conditional_t<INV, ProductLDS<T, CEX * NEN>, ChunkLDS<T>> L;
     __shared__ GroupLDS<NEN, BS, T, CEX> GL;
     const Item it = load_item(items);
     const dfea_desc& D = tab[it.s];

@@ -33,8 +33,8 @@ from .materials import material_spec
 from .plan import MeshPlan, chunking_for
 from .problem import build_problem
 from .adjoint import MAX_REFINE
+from .newton_krylov import MIXED_RTOL_FLOOR
 
-MIXED_RTOL_FLOOR = 1e-6
 HBM_SHARE = 0.8          # share of the free device memory the server admits requests into by default
 
 
