@@ -55,6 +55,7 @@ constexpr int nq_of() { return NEN == 3 ? 1 : NEN == 4 ? 4 : NEN == 6 ? 6 : 9; }
 // PCG launches of at most this many workgroups (one per CU or fewer: latency-bound) load the first node's
 // state before the scalar step
 constexpr int PCG_EARLY_ITEMS = 256;
+
 // product launches of at most this many workgroups request the Gauss points one step ahead (linear elements):
 // that variant holds 3 workgroups of 256 threads per CU, the other 4
 constexpr int APPLY_AHEAD_ITEMS = 3 * 256;
@@ -263,9 +264,18 @@ __device__ __forceinline__ bool in_colour(const ChunkView& ch, int t, int c) {
 }
 
 // reduction partials (dfea_desc.red): [0, 6 n_chunks) the PCG kernel's chunk partials (two parities x
-// {r.u, w.u, r.r} per chunk), then n_finblk unused, then 3 n_vblk for the vector kernels
+// {r.u, w.u, r.r} per chunk), then their totals (two parities x 3, large solves: see k_pcg_reduce), then n_finblk
+// unused, then 3 n_vblk for the vector kernels
 __device__ __forceinline__ double* pcg_red(const dfea_desc& D, int par) { return D.red + 3 * par * D.n_chunks; }
-__device__ __forceinline__ double* vec_red(const dfea_desc& D) { return D.red + 6 * D.n_chunks + D.n_finblk; }
+__device__ __forceinline__ double* pcg_tot(const dfea_desc& D, int par) { return D.red + 6 * D.n_chunks + 3 * par; }
+__device__ __forceinline__ double* vec_red(const dfea_desc& D) { return D.red + 6 * D.n_chunks + 6 + D.n_finblk; }
+
+// solves of more chunks than this have their PCG chunk partials summed by k_pcg_reduce between iterations
+// (one workgroup per solve), instead of by every workgroup of the next iteration.  Measured PCG iteration, FP64 /
+// FP32, summed in every workgroup -> presummed: 500x500 Q9 (4,096 chunks) 186.2 / 114.6 -> 173.4 / 100.5 us,
+// 800x800 Q4 (2,500) 121.1 / 55.6 -> 121.6 / 54.3 us, 566x566 Q4 (1,252) 45.5 / 29.1 -> 48.5 / 31.5 us (slower)
+constexpr int PCG_PRESUM_CHUNKS = 2048;
+constexpr int PCG_REDUCE_GRID = 64;      // workgroups of k_pcg_reduce (each scans its share of the items)
 
 // work item of this workgroup; items == NULL: one solve (descriptor 0), block = workgroup index
 struct Item { int s, b; };
@@ -1501,8 +1511,15 @@ __global__ __launch_bounds__(BS, MINW) void k_pcg(const dfea_desc* __restrict__ 
     // 1a. the previous kernel's chunk partials {r.u, w.u, r.r} (index clamped: extra threads read chunk 0 and
     // drop it)
     const double* rd = pcg_red(D, 1 - par);
+    const bool presum = D.n_chunks > PCG_PRESUM_CHUNKS;      // the totals come from k_pcg_reduce
     double pr[NRD][3];
-    if (k >= 0.0) {
+    double tot[3] = {0.0, 0.0, 0.0};
+    if (k >= 0.0 && presum) {
+        const double* tt = pcg_tot(D, 1 - par);
+        tot[0] = tt[0];
+        tot[1] = tt[1];
+        tot[2] = tt[2];
+    } else if (k >= 0.0) {
 #pragma unroll
         for (int j = 0; j < NRD; ++j) {
             const int i = t + j * BS;
@@ -1568,20 +1585,26 @@ __global__ __launch_bounds__(BS, MINW) void k_pcg(const dfea_desc* __restrict__ 
     double alpha = 0.0, beta = 0.0, gam = 0.0, den = 0.0, rr = 0.0;
     if (k >= 0.0) {
         double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+        if (presum) {
+            a0 = tot[0];
+            a1 = tot[1];
+            a2 = tot[2];
+        } else {
 #pragma unroll
-        for (int j = 0; j < NRD; ++j) {
-            if (t + j * BS < D.n_chunks) {
-                a0 += pr[j][0];
-                a1 += pr[j][1];
-                a2 += pr[j][2];
+            for (int j = 0; j < NRD; ++j) {
+                if (t + j * BS < D.n_chunks) {
+                    a0 += pr[j][0];
+                    a1 += pr[j][1];
+                    a2 += pr[j][2];
+                }
             }
+            for (int i = t + NRD * BS; i < D.n_chunks; i += BS) {
+                a0 += rd[3 * i];
+                a1 += rd[3 * i + 1];
+                a2 += rd[3 * i + 2];
+            }
+            block_sum3<BS, true>(a0, a1, a2, L.red);     // L.red is next written after the gather barrier
         }
-        for (int i = t + NRD * BS; i < D.n_chunks; i += BS) {
-            a0 += rd[3 * i];
-            a1 += rd[3 * i + 1];
-            a2 += rd[3 * i + 2];
-        }
-        block_sum3<BS, true>(a0, a1, a2, L.red);     // L.red is next written after the gather barrier
         gam = a0;
         const double delta = a1;
         rr = a2;
@@ -1734,6 +1757,34 @@ __global__ __launch_bounds__(BS, MINW) void k_pcg(const dfea_desc* __restrict__ 
     }
 }
 
+// the totals {r.u, w.u, r.r} of the chunk partials a PCG kernel of parity par wrote, for every solve of more than
+// PCG_PRESUM_CHUNKS chunks: one workgroup per solve (the solves are found as their block-0 items), the same fixed
+// order of additions every time (thread-strided sums, then the workgroup tree)
+__global__ __launch_bounds__(VBS) void k_pcg_reduce(const dfea_desc* __restrict__ tab, const int* __restrict__ items,
+                                                   int n_items, int par) {
+    __shared__ double s_red[3 * (VBS / 64)];
+    for (int j = blockIdx.x; j < n_items; j += gridDim.x) {
+        const int sidx = items ? items[2 * j] : 0, b = items ? items[2 * j + 1] : j;
+        if (b != 0) continue;
+        const dfea_desc& D = tab[sidx];
+        if (D.n_chunks <= PCG_PRESUM_CHUNKS) continue;
+        const double* rd = pcg_red(D, par);
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+        for (int i = threadIdx.x; i < D.n_chunks; i += VBS) {
+            a0 += rd[3 * i];
+            a1 += rd[3 * i + 1];
+            a2 += rd[3 * i + 2];
+        }
+        block_sum3<VBS>(a0, a1, a2, s_red);
+        if (threadIdx.x == 0) {
+            double* tt = pcg_tot(D, par);
+            tt[0] = a0;
+            tt[1] = a1;
+            tt[2] = a2;
+        }
+    }
+}
+
 // ======================================================================================================
 // host launch helpers
 // ======================================================================================================
@@ -1805,6 +1856,11 @@ int pcg_iter(const dfea_desc* table, const int32_t* items, int n_items, int n_it
                              : launch(k_pcg<NEN, nq_of<NEN>(), BSP, T, 2, (NEN <= 4 && !SMALL ? 3 : 1), false, CEX>,
                                       n_items, BSP, stream, "k_pcg", table, items, i & 1);
         if (rc) return rc;
+        if (n_items > PCG_PRESUM_CHUNKS) {      // a solve of the launch may have more chunks: sum its partials
+            const int grid = n_items < PCG_REDUCE_GRID ? n_items : PCG_REDUCE_GRID;
+            const int rr = launch(k_pcg_reduce, grid, VBS, stream, "k_pcg_reduce", table, items, n_items, i & 1);
+            if (rr) return rr;
+        }
     }
     return 0;
 }
@@ -1968,7 +2024,7 @@ int dfea_desc_offset(int field) {
     return off[field];
 }
 
-int dfea_red_size(int n_chunks, int n_finblk, int n_vblk) { return 6 * n_chunks + n_finblk + 3 * n_vblk; }
+int dfea_red_size(int n_chunks, int n_finblk, int n_vblk) { return 6 * n_chunks + 6 + n_finblk + 3 * n_vblk; }
 int dfea_vblk(int n_dof) { return (n_dof + VBLK - 1) / VBLK; }
 int dfea_finblk(int n_shared) { return (n_shared + FBS - 1) / FBS; }
 int dfea_chunk_elems(int nen) {

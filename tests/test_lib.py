@@ -36,7 +36,7 @@ def test_size_queries():
     assert [lib.dfea_chunk_elems(n) for n in (3, 4, 6, 8, 9)] == [256, 256, 64, 64, 64]
     assert [lib.dfea_chunk_threads(n) for n in (3, 4, 6, 8, 9)] == [256, 256, 64, 64, 64]
     assert lib.dfea_chunk_elems(5) == -1 and lib.dfea_chunk_threads(7) == -1
-    assert lib.dfea_red_size(3, 2, 1) == 6 * 3 + 2 + 3
+    assert lib.dfea_red_size(3, 2, 1) == 6 * 3 + 6 + 2 + 3
 
 
 def test_missing_library_fails_loudly():
