@@ -476,6 +476,8 @@ def main():
     ap.add_argument("--serve-side", type=int, nargs=2, default=(32, 512),
                     help="side range of the served meshes (the configured workload: 32 512)")
     ap.add_argument("--served-only", action="store_true", help="only the served-stream comparison, one JSON line")
+    ap.add_argument("--config4-only", action="store_true",
+                    help="only the 256x256 Q4 adjoint sensitivity (FP64 and mixed), one JSON line; for profiling it")
     ap.add_argument("--config1-only", action="store_true",
                     help="only the 100x100 Q4 solves (FP64 and mixed), one JSON line; for profiling the solve")
     args = ap.parse_args()
@@ -488,6 +490,9 @@ def main():
 
     if args.config1_only:
         print(json.dumps({prec: solve_config1(prec)[1] for prec in ("fp64", "mixed")}))
+        return
+    if args.config4_only:
+        print(json.dumps({prec: solve_config4(prec)[1] for prec in ("fp64", "mixed")}))
         return
     if args.served_only:
         print(json.dumps(served_comparison(args.serve_requests, tuple(args.serve_side))))

@@ -109,17 +109,44 @@ __device__ double block_sum(double v, double* s_red) {
     return t;
 }
 
-// three workgroup sums at once, each in the order block_sum uses.  ALL: every thread computes the totals
-// (same order, same bits) after one barrier, and s_red must not be written again before a later barrier;
-// otherwise the totals are valid on thread 0 and a second barrier frees s_red.  s_red holds 3 * BS / 64
-// doubles.
+// v of the lane that DPP control CTRL selects (both 32-bit halves moved; every lane has a source)
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(u & 0xffffffffu), CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(u >> 32), CTRL, 0xf, 0xf, false);
+    return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+
+__device__ __forceinline__ double readlane_f64(double v, int lane) {
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const int lo = __builtin_amdgcn_readlane((int)(u & 0xffffffffu), lane);
+    const int hi = __builtin_amdgcn_readlane((int)(u >> 32), lane);
+    return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+
+// sum over the 64 lanes of a wave with the same bits in every lane: in each row of 16 lanes, the quad
+// butterflies (DPP quad_perm lane ^ 1, lane ^ 2), then row_mirror (quad q with quad 3 - q) and
+// row_half_mirror (quad 0 with 1, 2 with 3) -- each step adds a pair of lanes that swap partners, so both hold
+// the same sum (addition commutes exactly) -- then the four row sums, read from lanes 0, 16, 32 and 48, are
+// added in that fixed order.  Register moves only (no LDS round trips as in a shuffle).
+__device__ __forceinline__ double wave_sum(double v) {
+    v += dpp_f64<0xb1>(v);      // quad_perm [1, 0, 3, 2]
+    v += dpp_f64<0x4e>(v);      // quad_perm [2, 3, 0, 1]
+    v += dpp_f64<0x140>(v);     // row_mirror
+    v += dpp_f64<0x141>(v);     // row_half_mirror
+    return (readlane_f64(v, 0) + readlane_f64(v, 16)) + (readlane_f64(v, 32) + readlane_f64(v, 48));
+}
+
+// three workgroup sums at once: wave sums (wave_sum), then the waves' sums in wave order.  ALL: every thread
+// computes the totals (same order, same bits) after one barrier, and s_red must not be written again before a
+// later barrier; otherwise the totals are valid on thread 0 and a second barrier frees s_red.  s_red holds
+// 3 * BS / 64 doubles.
 template <int BS, bool ALL = false>
 __device__ void block_sum3(double& a, double& b, double& c, double* s_red) {
-    for (int o = 32; o > 0; o >>= 1) {
-        a += __shfl_xor(a, o);
-        b += __shfl_xor(b, o);
-        c += __shfl_xor(c, o);
-    }
+    a = wave_sum(a);
+    b = wave_sum(b);
+    c = wave_sum(c);
     const int w = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) {
         s_red[3 * w] = a;
@@ -1420,17 +1447,18 @@ __global__ __launch_bounds__(VBS) void k_pcg_init_final(const dfea_desc* __restr
 }
 
 // the state of local node kk of a chunk for the PCG update: x_k, r_k, p_{k-1}, s_{k-1}, w_k (w of a shared
-// node summed from the previous kernel's partials in slot order), the preconditioner (d: the Jacobi diagonal,
-// or the inverse block's diagonal with its off-diagonal entry o), fixed bits, and whether this chunk writes
-// the node (owned nodes; a shared node's first-slot chunk; halo plans: nodes past nown are owned by another
-// chunk and only read)
+// node is the slot-order sum of the previous kernel's partials: node_state loads the first SUM_UNROLL of them
+// into pv, node_w adds them), the preconditioner (d: the Jacobi diagonal, or the inverse block's diagonal with
+// its off-diagonal entry o), fixed bits, and whether this chunk writes the node (owned nodes; a shared node's
+// first-slot chunk; halo plans: nodes past nown are owned by another chunk and only read)
 template <typename T>
 struct NodeState {
     V2<T> x, r, p, s, w, d;
+    V2<T> pv[SUM_UNROLL];
     T o;
-    int n;
+    int n, j0, cnt;
     unsigned char b;
-    bool writer;
+    bool writer, sh;
 };
 
 // u = M r: r / diag (point Jacobi) or the inverse nodal block times r (block Jacobi, dgo != nullptr)
@@ -1440,8 +1468,9 @@ __device__ __forceinline__ V2<T> precond_apply(V2<T> r, V2<T> d, T o, bool blk) 
     return V2<T>{r.x / d.x, r.y / d.y};
 }
 
-// the state of local node kk whose node id g, slot range rg and slot sl (shared entries) are already loaded
-template <typename T>
+// the state of local node kk whose node id g, slot range rg and slot sl (shared entries) are already loaded;
+// DEFER: a shared node's partials are left in pv for node_w (else w is summed here)
+template <typename T, bool DEFER>
 __device__ __forceinline__ NodeState<T> node_state(const dfea_desc& D, const ChunkView& ch, int kk, uint32_t g,
                                                    int2 rg, int sl, bool upd, const PcgSet<T>& Rd, const V2<T>* dg,
                                                    const T* dgo, const V2<T>* part_rd) {
@@ -1453,13 +1482,45 @@ __device__ __forceinline__ NodeState<T> node_state(const dfea_desc& D, const Chu
     ns.d = gload2<T>(dg, (uint32_t)ns.n);
     ns.o = dgo ? gload<false>(dgo, (uint32_t)ns.n) : (T)0;
     ns.r = gload2<T>(Rd.r, (uint32_t)ns.n);
+    ns.sh = DEFER && upd && shared;
     if (upd) {
-        ns.w = shared ? sum_slots<T>(part_rd, rg.x, rg.y) : gload2<T>(Rd.w, (uint32_t)ns.n);
+        if (shared && !DEFER) {
+            ns.w = sum_slots<T>(part_rd, rg.x, rg.y);
+        } else if (shared) {
+            ns.j0 = rg.x;
+            ns.cnt = rg.y;
+#pragma unroll
+            for (int q = 0; q < SUM_UNROLL; ++q)
+                if (q < rg.y) ns.pv[q] = part_rd[rg.x + q];
+        } else {
+            ns.w = gload2<T>(Rd.w, (uint32_t)ns.n);
+        }
         ns.x = gload2<T>(Rd.x, (uint32_t)ns.n);
         ns.p = gload2<T>(Rd.p, (uint32_t)ns.n);
         ns.s = gload2<T>(Rd.s, (uint32_t)ns.n);
     }
     return ns;
+}
+
+// w of a shared node: its partials added in slot order, as sum_slots adds them (same operations, same bits);
+// called where w is used, so the loads of an early-loaded state stay in flight across the scalar step
+template <typename T>
+__device__ __forceinline__ void node_w(NodeState<T>& ns, const V2<T>* part_rd) {
+    if (!ns.sh) return;
+    V2<T> acc{0, 0};
+#pragma unroll
+    for (int q = 0; q < SUM_UNROLL; ++q) {
+        if (q < ns.cnt) {
+            acc.x += ns.pv[q].x;
+            acc.y += ns.pv[q].y;
+        }
+    }
+    for (int j = ns.j0 + SUM_UNROLL; j < ns.j0 + ns.cnt; ++j) {
+        const V2<T> pj = part_rd[j];
+        acc.x += pj.x;
+        acc.y += pj.y;
+    }
+    ns.w = acc;
 }
 
 // one PCG iteration (see above); par = launch parity; MINW: minimum waves per SIMD for the register budget.
@@ -1483,7 +1544,9 @@ __global__ __launch_bounds__(BS, MINW) void k_pcg(const dfea_desc* __restrict__ 
     const dfea_desc& D = tab[it.s];
     double* sc = D.sc;
     const double* S = sc + (par ? DFEA_SC_S1 : DFEA_SC_S0);
-    if (S[PS_ST] != DFEA_PCG_RUNNING) return;
+    // the status is tested once the chunk record and the chunk partials are requested (their loads are valid
+    // whatever the status), so a running kernel does not wait for the status before issuing them
+    const double st0 = S[PS_ST];
     const double k = S[PS_K];
     // the step's scalars, requested now (a load behind the reduction's barrier would wait a round trip there)
     const double s_rtol = sc[DFEA_SC_RTOL], s_bb = sc[DFEA_SC_BB], s_maxit = sc[DFEA_SC_MAXIT];
@@ -1509,17 +1572,17 @@ __global__ __launch_bounds__(BS, MINW) void k_pcg(const dfea_desc* __restrict__ 
     constexpr int NRD = 3;                        // chunk partials per thread loaded ahead of the scalar step
 
     // 1a. the previous kernel's chunk partials {r.u, w.u, r.r} (index clamped: extra threads read chunk 0 and
-    // drop it)
+    // drop it; the first kernel, k < 0, reads and drops them)
     const double* rd = pcg_red(D, 1 - par);
     const bool presum = D.n_chunks > PCG_PRESUM_CHUNKS;      // the totals come from k_pcg_reduce
     double pr[NRD][3];
     double tot[3] = {0.0, 0.0, 0.0};
-    if (k >= 0.0 && presum) {
+    if (presum) {
         const double* tt = pcg_tot(D, 1 - par);
         tot[0] = tt[0];
         tot[1] = tt[1];
         tot[2] = tt[2];
-    } else if (k >= 0.0) {
+    } else {
 #pragma unroll
         for (int j = 0; j < NRD; ++j) {
             const int i = t + j * BS;
@@ -1528,6 +1591,7 @@ __global__ __launch_bounds__(BS, MINW) void k_pcg(const dfea_desc* __restrict__ 
             for (int m = 0; m < 3; ++m) pr[j][m] = gload<false>(rd, 3u * ic + m);
         }
     }
+    if (st0 != DFEA_PCG_RUNNING) return;
     // 1b. node ids, and for shared entries the slot range of the node and the chunk's own slot
     uint32_t gid[NG];
     int2 rng[NG];
@@ -1577,7 +1641,7 @@ __global__ __launch_bounds__(BS, MINW) void k_pcg(const dfea_desc* __restrict__ 
 #pragma unroll
         for (int i = 0; i < NGE; ++i) {
             const int kk = t + i * BS;
-            if (kk < ch.nk) nse[i] = node_state<T>(D, ch, kk, gid[i], rng[i], slt[i], k >= 0.0, Rd, dg, dgo, part_rd);
+            if (kk < ch.nk) nse[i] = node_state<T, true>(D, ch, kk, gid[i], rng[i], slt[i], k >= 0.0, Rd, dg, dgo, part_rd);
         }
     }
 
@@ -1663,7 +1727,8 @@ __global__ __launch_bounds__(BS, MINW) void k_pcg(const dfea_desc* __restrict__ 
         if (kk >= ch.nk) break;
         NodeState<T> ns;
         if (EARLY && i < NGE) ns = nse[EARLY ? (i < NGE ? i : 0) : 0];
-        else ns = node_state<T>(D, ch, kk, gid[i], rng[i], slt[i], k >= 0.0, Rd, dg, dgo, part_rd);
+        else ns = node_state<T, false>(D, ch, kk, gid[i], rng[i], slt[i], k >= 0.0, Rd, dg, dgo, part_rd);
+        node_w<T>(ns, part_rd);
         const int n = ns.n;
         const unsigned char b = ns.b;
         V2<T> rn = ns.r;

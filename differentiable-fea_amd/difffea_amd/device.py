@@ -21,6 +21,12 @@ from .plan import MeshPlan, chunking_for
 
 F64, F32 = torch.float64, torch.float32
 
+# the scalar-block entries the host reads after a round of PCG kernels (status, k, maxit, iters) and writes
+# before a solve (rtol, status, k, maxit) are contiguous, so each is one copy
+_SC_RTOL, _SC_STATUS = _lib.SC["rtol"], _lib.SC["status"]
+assert (_lib.SC["k"], _lib.SC["maxit"], _lib.SC["iters"]) == (_SC_STATUS + 1, _SC_STATUS + 2, _SC_STATUS + 3)
+assert _SC_STATUS == _SC_RTOL + 1
+
 
 def _stream():
     return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
@@ -183,6 +189,10 @@ class HipSystem:
         self.vec["uacc"] = torch.zeros(nd, dtype=F64, device=dv)
         self.vec["uD"] = torch.zeros(nd, dtype=F64, device=dv) if uD is None else uD.to(dv, F64).contiguous()
         self._graphs = {}
+        # pinned host staging of scalars: [0:4] status reads, [4:8] the PCG set-up write, [8] dot products; the
+        # event marks the set-up copy, so the staging is not rewritten while a copy from it may be pending
+        self._hst = torch.empty(9, dtype=F64, pin_memory=True)
+        self._hst_ev = torch.cuda.Event()
         self.desc = self._make_desc()
         self.table = m.upload(self.desc)
 
@@ -261,6 +271,19 @@ class HipSystem:
                                      _ptr(x), _ptr(y), _stream()), "dfea_dot")
         return self.sc[_lib.SC["norm"]].clone()
 
+    def dot_host(self, x, y=None):
+        """The same dot product read to the host (one copy through pinned memory, one synchronisation)."""
+        m = self.mesh
+        x = x.to(F64).contiguous()
+        y = x if y is None else y.to(F64).contiguous()
+        _lib.check(self.lib.dfea_dot(_ptr(self.table), _ptr(m.vitems), m.n_vblk, _ptr(m.solve_items), 1,
+                                     _ptr(x), _ptr(y), _stream()), "dfea_dot")
+        n = _lib.SC["norm"]
+        h = self._hst[8:9]
+        h.copy_(self.sc[n:n + 1], non_blocking=True)
+        torch.cuda.current_stream(m.device).synchronize()
+        return float(h[0])
+
     def norm(self, x):
         return torch.sqrt(self.dot(x))
 
@@ -317,8 +340,11 @@ class HipSystem:
         m = self.mesh
         if b is not None:
             self.vec["b"].copy_(b)
-        self.sc[_lib.SC["rtol"]] = float(rtol)
-        self.sc[_lib.SC["maxit"]] = float(max_iter)
+        self._hst_ev.synchronize()
+        h = self._hst[4:8]
+        h.copy_(torch.tensor([float(rtol), 0.0, 0.0, float(max_iter)], dtype=F64))   # rtol, status, k, maxit
+        self.sc[_SC_RTOL:_SC_RTOL + 4].copy_(h, non_blocking=True)
+        self._hst_ev.record()
         _lib.check(self.lib.dfea_pcg_init(_ptr(self.table), _ptr(m.vitems), m.n_vblk, _ptr(m.solve_items), 1,
                                           int(neg_res), self.prec_id, _stream()), "dfea_pcg_init")
 
@@ -358,24 +384,49 @@ class HipSystem:
             pass
 
     def pcg_status(self):
-        s = self.sc[[_lib.SC["status"], _lib.SC["iters"], _lib.SC["k"]]].cpu()
-        return int(s[0]), int(s[1]), int(s[2])
+        """(status, iterations, k) of the solve (one copy through pinned memory, one synchronisation)."""
+        h = self._hst[0:4]
+        h.copy_(self.sc[_SC_STATUS:_SC_STATUS + 4], non_blocking=True)    # status, k, maxit, iters
+        torch.cuda.current_stream(self.mesh.device).synchronize()
+        st, k, _, its = h.tolist()
+        return int(st), int(its), int(k)
 
-    def pcg_run(self, max_iter, chunk=16, graph=None):
+    ROUND_GRAPHS = (128, 64, 32, 16)
+
+    def pcg_round(self, n_iter, graph=None):
+        """n_iter PCG kernels (rounded up to even): replays of the graphs of ROUND_GRAPHS kernels (each
+        captured once per system), then direct launches of the remainder."""
+        if not (self.use_graphs if graph is None else graph):
+            self.pcg_iterate(n_iter, False)
+            return
+        n = n_iter + (n_iter & 1)
+        for g in self.ROUND_GRAPHS:
+            while n >= g:
+                self.pcg_iterate(g, True)
+                n -= g
+        if n:
+            self.pcg_iterate(n, False)
+
+    def pcg_run(self, max_iter, chunk=16, graph=None, first=None):
         """Iterate a started PCG until it stops; returns (iterations, status).
 
-        Kernels run in chunks (graph replays) of even length doubling up to 128; solves that finish inside
-        a chunk skip the remaining kernels on the device, and the status block is read once per chunk.  The
-        first kernel forms A M b and the test on the last update is made by the kernel after it, hence + 2."""
-        st, its, k = self.pcg_status()
-        while st == _lib.PCG_RUNNING:
-            self.pcg_iterate(min(chunk, max_iter - k + 2), graph)
+        Kernels run in rounds and the status block is read after each round; kernels of a solve that has
+        stopped skip their work on the device.  The first round has `first` kernels (default `chunk`: an
+        estimate of the iterations saves rounds) and is launched without reading the status before it; later
+        rounds have `chunk` kernels, doubling up to 128.  The first kernel forms A M b and the test on the last
+        update is made by the kernel after it, so k iterations take k + 2 kernels."""
+        k = 0
+        n = chunk if first is None else max(2, int(first))
+        while True:
+            self.pcg_round(min(n, max_iter - k + 2), graph)
             st, its, k = self.pcg_status()
+            if st != _lib.PCG_RUNNING:
+                return its, st
+            n = chunk
             chunk = min(2 * chunk, 128)
-        return its, st
 
-    def pcg_solve(self, b=None, rtol=1e-5, max_iter=500, neg_res=False, chunk=16, graph=None):
+    def pcg_solve(self, b=None, rtol=1e-5, max_iter=500, neg_res=False, chunk=16, graph=None, first=None):
         """Run the fused PCG to completion; returns (x view, iterations, status)."""
         self.pcg_start(b, rtol, max_iter, neg_res)
-        its, st = self.pcg_run(max_iter, chunk, graph)
+        its, st = self.pcg_run(max_iter, chunk, graph, first)
         return self.vec["x"], its, st

@@ -29,6 +29,22 @@ from .device import HipSystem
 MIXED_RTOL_FLOOR = 1e-3
 
 
+def first_round(it, counts, prev_counts):
+    """Kernels of the first PCG round of Newton iteration `it` of a load step: an estimate of the CG
+    iterations + 2.  The Eisenstat-Walker forcing terms repeat from one load step to the next, and so do the
+    iteration counts (100x100 Q4: 4, 11, 18, 68, 97, 125, 194, 250 in the first step; 4, 11, 17, 69, 86, 120,
+    188, 252, 318 in the second), so the estimate is the count of the same iteration of the previous load step
+    (`prev_counts`) with a margin, else 1.5x the previous iteration's count; a short round costs one more
+    status read (a synchronisation, tens of microseconds), a long one a few microseconds per extra kernel."""
+    if it < len(prev_counts):
+        est = 1.1 * prev_counts[it] + 8
+    elif counts:
+        est = 1.5 * counts[-1] + 8
+    else:
+        est = 14
+    return int(est) + 2
+
+
 def pcg(A_fn, b, Minv_fn, rtol=1e-5, max_iter=500):
     """Textbook Jacobi-PCG with x0 = 0 (returns (x, n_iters))."""
     x = torch.zeros_like(b)
@@ -75,21 +91,23 @@ class HipNewton:
         self.ucand = torch.empty(nd, dtype=torch.float64, device=self.device)
         self._alpha = torch.empty(1, dtype=torch.float64, device=self.device)
         self.pcg_chunk = 16
+        self._step_counts = []          # CG iterations of the Newton iterations of the last load step solved
 
     # -- building blocks -------------------------------------------------------------------------------------
     def residual(self, u, lf, out=None):
         return self.sys.residual(u, out=self.sys.vec["R"] if out is None else out, lf=lf)
 
     def norm(self, R):
-        return float(torch.sqrt(self.sys.dot(R)))
+        return math.sqrt(self.sys.dot_host(R))
 
-    def linear_solve(self, rtol, max_iter):
-        """PCG on J du = -R with R in sys.vec['R']; returns (du, iterations)."""
+    def linear_solve(self, rtol, max_iter, first=None):
+        """PCG on J du = -R with R in sys.vec['R']; returns (du, iterations).  first: kernels of the first
+        round of PCG kernels (``first_round``)."""
         s = self.sys
         if s.precision == "mixed":
             rtol = max(rtol, MIXED_RTOL_FLOOR)
         s.pcg_start(rtol=rtol, max_iter=max_iter, neg_res=True)
-        its, _ = s.pcg_run(max_iter, self.pcg_chunk)
+        its, _ = s.pcg_run(max_iter, self.pcg_chunk, first=first)
         return s.vec["x"], its
 
     def trial(self, u, du, alpha, out):
@@ -119,16 +137,19 @@ class HipNewton:
         r0 = self.norm(R)
         rn = r0
         records = []
+        counts, prev = [], self._step_counts
         for it in range(max_newton + 1):
             if rn < tol:
                 if verbose:
                     print(f"  converged in {it} Newton iterations (||R|| = {rn:.3e})")
+                self._step_counts = counts
                 return u, True, records
             if it == max_newton:
                 break
             eta = min(0.5, math.sqrt(rn / (r0 + 1e-14))) if cg_rtol is None else cg_rtol
             self.sys.moduli(u)
-            du, k = self.linear_solve(eta, cg_max_iter)
+            du, k = self.linear_solve(eta, cg_max_iter, first_round(it, counts, prev))
+            counts.append(int(k))
             if self.ucand is u:
                 self.ucand = torch.empty_like(u)
             u, rn, alpha, ok = self.armijo(u, du, rn, lf)
