@@ -676,16 +676,22 @@ __global__ __launch_bounds__(BS, MINW) void k_apply(const dfea_desc* __restrict_
     const T* dN = std::is_same<T, double>::value ? (const T*)D.dN64 : (const T*)D.dN32;
     const T* C = std::is_same<T, double>::value ? (const T*)D.C64 : (const T*)D.C32;
 
-    // 0. element (local node indices).  Partials plans: tail threads of the last chunk read the last element
-    // and drop the result, so every load below is unconditional (no branch splits the loads in flight).
+    // 0. element (local node indices).  Partials plans: threads past the chunk's elements read its last element
+    // and drop the result, so every load below is unconditional (no branch splits the loads in flight).  The
+    // element range comes from the chunk record (tail-balanced plans have chunks of two sizes), except where the
+    // element stream starts before the record arrives (STREAM_FIRST: plans of equal chunks only)
     constexpr int CE = Groups<NEN, BS, CEX>::CE;
     int l[NEN];
     int e;
     bool has_el;
-    if constexpr (BS % CE == 0) {
+    if constexpr (BS % CE == 0 && STREAM_FIRST) {
         e = it.b * CE + t % CE;
         has_el = e < D.n_el;
         if (!has_el) e = D.n_el - 1;
+    } else if constexpr (BS % CE == 0) {
+        const int tc = t % CE;
+        has_el = tc < ch.ne;
+        e = ch.e0 + (has_el ? tc : ch.ne - 1);
     } else {
         has_el = thread_element<NEN>(D, ch, t, e, l);
     }
@@ -1577,39 +1583,52 @@ __global__ __launch_bounds__(BS, MINW) void k_pcg(const dfea_desc* __restrict__ 
     const bool presum = D.n_chunks > PCG_PRESUM_CHUNKS;      // the totals come from k_pcg_reduce
     double pr[NRD][3];
     double tot[3] = {0.0, 0.0, 0.0};
-    if (presum) {
-        const double* tt = pcg_tot(D, 1 - par);
-        tot[0] = tt[0];
-        tot[1] = tt[1];
-        tot[2] = tt[2];
-    } else {
+    auto load_partials = [&]() {
+        if (presum) {
+            const double* tt = pcg_tot(D, 1 - par);
+            tot[0] = tt[0];
+            tot[1] = tt[1];
+            tot[2] = tt[2];
+        } else {
 #pragma unroll
-        for (int j = 0; j < NRD; ++j) {
-            const int i = t + j * BS;
-            const uint32_t ic = (uint32_t)(i < D.n_chunks ? i : 0);
+            for (int j = 0; j < NRD; ++j) {
+                const int i = t + j * BS;
+                const uint32_t ic = (uint32_t)(i < D.n_chunks ? i : 0);
 #pragma unroll
-            for (int m = 0; m < 3; ++m) pr[j][m] = gload<false>(rd, 3u * ic + m);
+                for (int m = 0; m < 3; ++m) pr[j][m] = gload<false>(rd, 3u * ic + m);
+            }
         }
-    }
-    if (st0 != DFEA_PCG_RUNNING) return;
+    };
     // 1b. node ids, and for shared entries the slot range of the node and the chunk's own slot
     uint32_t gid[NG];
     int2 rng[NG];
     int slt[NG];
+    auto load_nodes = [&]() {
 #pragma unroll
-    for (int i = 0; i < NG; ++i) {
-        const int kk = t + i * BS;
-        const int kc = kk < ch.nk ? kk : ch.nk - 1;
-        gid[i] = gload<false>(D.lnode, (uint32_t)(ch.k0 + kc));
-        rng[i] = make_int2(0, 0);
-        slt[i] = 0;
-        if (D.n_part) {
-            const uint32_t ent = (uint32_t)(kc >= ch.nown ? ch.pbase + (kc - ch.nown) : 0);
-            const V2<int> r2 = gload2<int>(reinterpret_cast<const V2<int>*>(prg), ent);
-            rng[i] = make_int2(r2.x, r2.y);
-            slt[i] = gload<false>(D.pslot, ent);
+        for (int i = 0; i < NG; ++i) {
+            const int kk = t + i * BS;
+            const int kc = kk < ch.nk ? kk : ch.nk - 1;
+            gid[i] = gload<false>(D.lnode, (uint32_t)(ch.k0 + kc));
+            rng[i] = make_int2(0, 0);
+            slt[i] = 0;
+            if (D.n_part) {
+                const uint32_t ent = (uint32_t)(kc >= ch.nown ? ch.pbase + (kc - ch.nown) : 0);
+                const V2<int> r2 = gload2<int>(reinterpret_cast<const V2<int>*>(prg), ent);
+                rng[i] = make_int2(r2.x, r2.y);
+                slt[i] = gload<false>(D.pslot, ent);
+            }
         }
+    };
+    // EARLY: the node ids first -- the node states wait for them, and the partials (written by every chunk of
+    // the previous kernel, so fetched from beyond the XCD's L2) would hold them back (loads return in order)
+    if constexpr (EARLY) {
+        load_nodes();
+        load_partials();
+    } else {
+        load_partials();
+        load_nodes();
     }
+    if (st0 != DFEA_PCG_RUNNING) return;
     // 1c. element (partials plans: tail threads of the last chunk read the last element and drop it) and its
     // first PF Gauss points
     int l[NEN];
@@ -1617,9 +1636,9 @@ __global__ __launch_bounds__(BS, MINW) void k_pcg(const dfea_desc* __restrict__ 
     bool has_el;
     InvRegs<NEN, BS, CEX> inv;
     if constexpr (BS % CE == 0) {
-        e = it.b * CE + t % CE;
-        has_el = e < D.n_el;
-        if (!has_el) e = D.n_el - 1;
+        const int tc = t % CE;
+        has_el = tc < ch.ne;
+        e = ch.e0 + (has_el ? tc : ch.ne - 1);
 #pragma unroll
         for (int a = 0; a < NEN; ++a) l[a] = gload<false>(D.lelem, (uint32_t)(a * D.n_el + e));
         inv_load<NEN, BS, CEX>(D, ch, t, inv);
@@ -1863,10 +1882,15 @@ int launch(K kern, int grid, int block, void* stream, const char* name, A... arg
 }
 
 // element kind of a launch: nen, with DFEA_HALO set for halo plans (DFEA_MATLIB selects the instantiations
-// with the whole material library and is not part of the kind)
+// with the whole material library, DFEA_TAIL marks tail-balanced linear partials plans; neither is part of the
+// kind)
 enum class Kind { T3, Q4, Q4H, Q4S, T6, Q8, Q9, Bad };
 Kind kind_of(int k) {
     k &= ~DFEA_MATLIB;
+    if (k & DFEA_TAIL) {
+        k &= ~DFEA_TAIL;
+        if (k != 3 && k != 4) return Kind::Bad;
+    }
     switch (k) {
         case 3: return Kind::T3;
         case 4: return Kind::Q4;
@@ -1892,8 +1916,8 @@ int dispatch_lib(int kind, F f) {
         case Kind::Q8: return f.template run<8, BS_QUAD, LIB>();
         case Kind::Q9: return f.template run<9, BS_QUAD, LIB>();
         default:
-            return fail(DFEA_EINVAL, "unknown element kind (expected 3, 4, 6, 8, 9, 4 | DFEA_HALO or 4 | DFEA_SMALL, "
-                                     "optionally | DFEA_MATLIB)");
+            return fail(DFEA_EINVAL, "unknown element kind (expected 3, 4, 6, 8, 9, 4 | DFEA_HALO, 4 | DFEA_SMALL, "
+                                     "3 | DFEA_TAIL or 4 | DFEA_TAIL, optionally | DFEA_MATLIB)");
     }
 }
 
@@ -1984,6 +2008,7 @@ struct Launch_apply {
     void* st;
     void* part = nullptr;        // shared-node partials buffer (nullptr: the descriptor's part)
     int part_cap = 0x7fffffff;   // its capacity in entries
+    bool tail = false;           // tail-balanced plans: the gather-first product at any launch size
     template <typename T, int NEN, int BS0> int go() {
         constexpr bool SMALL = NEN == 4 && BS0 == BS_QUAD;     // small-chunk quad4 plans (see pcg_iter)
         constexpr int CEX = SMALL ? CHUNK_QUAD : default_ce<NEN>();
@@ -1993,7 +2018,7 @@ struct Launch_apply {
         // linear elements: Gauss points requested one step ahead while the launch fits in the workgroup slots
         // of the (costlier in registers) ahead variant; larger quad4 launches start the element stream before
         // the gather instead
-        const bool ahead = NEN <= 4 && !SMALL && n <= APPLY_AHEAD_ITEMS;
+        const bool ahead = NEN <= 4 && !SMALL && (n <= APPLY_AHEAD_ITEMS || tail);
         V2<T>* pp = static_cast<V2<T>*>(part);
         int rc = ahead ? launch(k_apply<NEN, nq_of<NEN>(), BS, T, 2, 1, NEN <= 4 && !SMALL, false, CEX>, n, BS, st,
                                 "k_apply", t, it, vv, yy, pp, part_cap)
@@ -2143,7 +2168,9 @@ int dfea_apply(const dfea_desc* table, const int32_t* items, int n_items, const 
                int n_fin_items, const void* v, void* y, int nen, int precision, void* stream) {
     if (!table || (precision != DFEA_FP64 && precision != DFEA_FP32))
         return fail(DFEA_EINVAL, "dfea_apply: bad argument");
-    return dispatch(nen, Launch_apply{table, items, n_items, fin_items, n_fin_items, v, y, precision, stream});
+    Launch_apply la{table, items, n_items, fin_items, n_fin_items, v, y, precision, stream};
+    la.tail = (nen & DFEA_TAIL) != 0;
+    return dispatch(nen, la);
 }
 
 int dfea_apply_noalloc(const dfea_desc* table, int n_chunks, int n_fin_blocks, int n_part, const void* v, void* y,
@@ -2165,6 +2192,7 @@ int dfea_apply_noalloc(const dfea_desc* table, int n_chunks, int n_fin_blocks, i
     Launch_apply la{table, nullptr, n_chunks, nullptr, n_fin_blocks, v, y, precision, stream};
     la.part = part;
     la.part_cap = n_part;
+    la.tail = (nen & DFEA_TAIL) != 0;
     return dispatch(nen, la);
 }
 

@@ -19,6 +19,7 @@ MAXCOL = 16
 FIXED_X, FIXED_Y, NODE_MASK = 0x40000000, 0x80000000, 0x3FFFFFFF
 HALO = 0x100          # element-kind flag of halo plans
 SMALL = 0x400         # element-kind flag of small-chunk quad4 plans (64-element chunks)
+TAIL = 0x800          # element-kind flag of tail-balanced linear partials plans (chunks of two sizes)
 MATLIB = 0x200        # element-kind flag: material kernels with the whole material library
 MAT_NP = 8            # dfea_desc.mat_p slots
 NOALLOC_SCRATCH_BYTES = 8 << 20   # DFEA_NOALLOC_SCRATCH_BYTES: static device scratch of dfea_apply_noalloc

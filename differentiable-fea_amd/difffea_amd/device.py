@@ -71,17 +71,26 @@ class HipMesh:
             cell_ij=None if cij is None else cij.detach().cpu().numpy())
         if p.n_el != conn.shape[0] or p.n_nodes != nodes.shape[0]:
             raise ValueError("the plan was built for another mesh")
-        # element kind of every launch: nen, flagged for halo plans and for small-chunk quad4 plans
+        # element kind of every launch: nen, flagged for halo plans, small-chunk quad4 plans and tail-balanced plans
         small = p.mode == "partials" and self.nen == 4 and p.chunk_elems == self.lib.dfea_chunk_elems(4 | _lib.SMALL)
-        self.kind = self.nen | (_lib.HALO if p.mode == "halo" else 0) | (_lib.SMALL if small else 0)
+        tail = p.mode == "partials" and getattr(p, "tail", False)
+        self.kind = self.nen | (_lib.HALO if p.mode == "halo" else 0) | (_lib.SMALL if small else 0) | \
+            (_lib.TAIL if tail else 0)
         inv_off, inv_slot = p.inverse_map() if p.mode == "partials" else (np.zeros(1, np.int32), np.zeros(1, np.uint16))
         if p.validate(self.lib.dfea_chunk_nodes(), (inv_off, inv_slot) if p.mode == "partials" else None) > \
                 self.lib.dfea_chunk_threads(self.kind):
             raise ValueError("chunk larger than the kernel's workgroup")
         E = self.lib.dfea_chunk_elems(self.kind)
-        if p.mode == "partials" and (not np.array_equal(p.chunks[:, 0], np.arange(p.n_chunks) * E) or
-                                     p.chunks[:, 1].max() > E):
-            raise ValueError(f"partials plans must cut the element order into chunks of {E} elements")
+        if p.mode == "partials":
+            ne = p.chunks[:, 1].astype(np.int64)
+            if not np.array_equal(p.chunks[:, 0], np.cumsum(ne) - ne) or ne.max() > E or ne.min() < 1:
+                raise ValueError(f"partials plans must cut the element order into consecutive chunks of at most "
+                                 f"{E} elements")
+            if not tail and not np.array_equal(p.chunks[:, 0], np.arange(p.n_chunks) * E):
+                raise ValueError(f"partials plans other than tail-balanced ones must cut the element order into "
+                                 f"chunks of {E} elements")
+            if tail and self.nen > 4:
+                raise ValueError("tail-balanced plans are for linear elements")
         self.n_el, self.n_nodes = p.n_el, p.n_nodes
         self.n_dof = 2 * p.n_nodes
         dv = device

@@ -5,7 +5,8 @@ Built once per (mesh, Dirichlet mask) on the host with vectorised numpy; uploade
 1. Element order.  Elements are sorted along a Morton (Z) curve of their cell coordinates, then cut into
    chunks of ``chunk_elems`` consecutive elements.  On structured grids the cell coordinates are the exact
    integer cell indices (detected by rank), so chunks are square tiles (16x16 quad4, 8x8 quad9); otherwise
-   element centroids are quantised onto a power-of-two grid.
+   element centroids are quantised onto a power-of-two grid.  Tail-balanced plans (linear elements, see
+   ``tail_chunk_sizes``) cut the elements past whole rounds of 256 chunks into chunks of half the size.
 2. Colours.  Elements are coloured so that two elements of one colour never share a node: the parity of
    the cell indices on structured grids (4 colours), else Jones-Plassmann on the node-sharing graph.  Inside
    a chunk the elements are sorted by colour, so each colour is a contiguous range of threads.
@@ -151,6 +152,44 @@ def chunking_for(nen, n_el, chunking="auto"):
     return (_lib.load().dfea_chunk_elems(4 | _lib.SMALL), _lib.SMALL) if small else (None, 0)
 
 
+# Tail-balanced plans.  A product or PCG launch of n chunk workgroups puts workgroup b on XCD b mod 8 and, on
+# the GPU's 256 CUs, next to workgroups b +- 256 (MI355X: with 625 chunks CU j holds workgroups j, j + 256 and,
+# for j < 113, j + 512).  When the chunks fill k whole rounds of 256 and part of one more, the CUs that carry an
+# extra chunk set the kernel time (400 x 400 quad4: 113 CUs with three chunks, 143 with two).  A tail plan keeps
+# k rounds of full chunks and cuts the remaining elements into chunks of half the size, one for each of up to
+# 256 CUs, so the busiest CU carries k + 1/2 chunks' worth instead of k + 1.
+CUS = 256
+NXCD = 8
+
+
+def tail_chunk_sizes(n_el, E, cus=CUS, nxcd=NXCD):
+    """Element counts of the chunks of a tail-balanced plan of n_el elements in chunks of up to E, in chunk
+    order; None where the plain plan (chunks of E) is kept: fewer than one or more than two whole rounds of
+    `cus` chunks (beyond that the workgroups no longer all fit on the GPU at once), or a remainder of more
+    than `cus` half chunks.  Chunk c of a launch of n workgroups runs on workgroup b = nxcd * pos + x where
+    c = x * (n // nxcd) + min(x, n % nxcd) + pos (the kernels' XCD remap); the chunks whose workgroups come
+    after the whole rounds (pos >= rounds * cus / nxcd) are the half chunks, and the last of them in element
+    order takes what remains."""
+    rounds = n_el // (cus * E)
+    rest = n_el - rounds * cus * E
+    half = E // 2
+    if rounds < 1 or rounds > 2 or rest == 0 or rest > cus * half or half < 1 or (rounds * cus) % nxcd:
+        return None
+    n_half = -(-rest // half)
+    n = rounds * cus + n_half
+    base, r = divmod(n, nxcd)
+    full_pos = rounds * cus // nxcd
+    sizes = []
+    for x in range(nxcd):
+        nx = base + (1 if x < r else 0)
+        sizes += [E] * min(nx, full_pos) + [half] * max(0, nx - full_pos)
+    sizes = np.array(sizes, np.int64)
+    halves = np.flatnonzero(sizes != E)
+    sizes[halves[-1]] = rest - (n_half - 1) * half
+    assert len(halves) == n_half and sizes.sum() == n_el
+    return sizes
+
+
 def _require(ok, what):
     if not ok:
         raise ValueError(f"invalid mesh plan: {what}")
@@ -159,7 +198,10 @@ def _require(ok, what):
 class MeshPlan:
     """Integer tables of one mesh + Dirichlet mask (numpy arrays, see module docstring)."""
 
-    def __init__(self, nodes, conn, fixed, chunk_elems=None, max_nodes=None, cell_ij=None, mode="auto"):
+    def __init__(self, nodes, conn, fixed, chunk_elems=None, max_nodes=None, cell_ij=None, mode="auto",
+                 balance=True):
+        """balance: tail-balanced chunk sizes where ``tail_chunk_sizes`` gives them (partials plans of linear
+        elements in their default chunk size); False keeps chunks of chunk_elems everywhere."""
         nodes = np.asarray(nodes, np.float64)
         conn = np.asarray(conn, np.int64)
         fixed = np.asarray(fixed, bool).reshape(-1, 2)
@@ -197,13 +239,24 @@ class MeshPlan:
             colour = jones_plassmann(conn)
         if colour.max() >= _lib.MAXCOL:
             raise ValueError(f"mesh needs {colour.max() + 1} colours, more than {_lib.MAXCOL}")
-        n_chunks = (n_el + E - 1) // E
-        chunk_of_pos = np.arange(n_el) // E
-        # within a chunk: sort by colour, keep Morton order inside a colour
+        sizes = tail_chunk_sizes(n_el, E, CUS, NXCD) if (balance and mode == "partials" and nen <= 4 and
+                                             E == lib.dfea_chunk_elems(nen)) else None
+        self.tail = sizes is not None
+        if sizes is None:
+            n_chunks = (n_el + E - 1) // E
+            sizes = np.minimum(E, n_el - np.arange(n_chunks) * E)
+            chunk_of_pos = np.arange(n_el) // E                    # chunk of each Morton position
+        else:
+            # full chunks take the Morton order's whole tiles in chunk order, the half chunks the rest
+            n_chunks = len(sizes)
+            full = np.flatnonzero(sizes == E)
+            halves = np.flatnonzero(sizes != E)
+            chunk_of_pos = np.r_[np.repeat(full, E), np.repeat(halves, sizes[halves])]
+        # within a chunk: sort by colour, keep Morton order inside a colour (internal order = chunk order)
         perm = order[np.lexsort((np.arange(n_el), colour[order], chunk_of_pos))]
         self.perm = perm.astype(np.int32)                          # internal e -> original element
         ce = conn[perm]                                            # (n_el, nen), internal order
-        ch_e = chunk_of_pos                                        # chunk of internal element
+        ch_e = np.repeat(np.arange(n_chunks), sizes)               # chunk of internal element
         col_sorted = colour[perm]
 
         bits = fixed[:, 0].astype(np.uint64) * np.uint64(_lib.FIXED_X) + \
@@ -221,8 +274,8 @@ class MeshPlan:
 
         # colour ranges per chunk
         chunks = np.zeros((n_chunks, _lib.CHUNK_INTS), np.int32)
-        e0 = np.arange(n_chunks) * E
-        ne = np.minimum(E, n_el - e0)
+        ne = sizes
+        e0 = np.cumsum(ne) - ne
         chunks[:, 0] = e0
         chunks[:, 1] = ne
         ncol = np.zeros(n_chunks, np.int32)

@@ -11,7 +11,8 @@
  *     buffers; a descriptor table in device memory describes a batch of independent solves.
  *   - Element kind (`nen` arguments): the nodes per element, 3 (tri3, 1 Gauss point), 4 (quad4, 2x2),
  *     6 (tri6, 6-point degree-4 rule), 8 (quad8, 3x3) or 9 (quad9, 3x3), with nq of the descriptors equal to
- *     that rule's size; 4 | DFEA_HALO for quad4 halo plans; | DFEA_MATLIB when a descriptor's material is
+ *     that rule's size; 4 | DFEA_HALO for quad4 halo plans, 4 | DFEA_SMALL for small-chunk quad4 plans,
+ *     3 | DFEA_TAIL or 4 | DFEA_TAIL for tail-balanced plans; | DFEA_MATLIB when a descriptor's material is
  *     not neo-Hookean.
  *   - Work items: an int32 pair {solve index, block index} per workgroup; for a batch, the concatenation
  *     over solves.  Any item table may be NULL for one solve: solve 0 (the table's first descriptor),
@@ -46,6 +47,10 @@ extern "C" {
 #define DFEA_HALO 0x100        /* element-kind flag: nen | DFEA_HALO selects the halo-plan kernels */
 #define DFEA_SMALL 0x400       /* element-kind flag: 4 | DFEA_SMALL selects the quad4 kernels of small-chunk plans
                                   (64-element chunks; the product and PCG give each element 4 threads) */
+#define DFEA_TAIL 0x800        /* element-kind flag: 3 | DFEA_TAIL or 4 | DFEA_TAIL for tail-balanced partials plans
+                                  (chunks of dfea_chunk_elems elements and, past whole rounds of 256 chunks,
+                                  chunks of half that size): the product and PCG read each chunk's element
+                                  range from its record */
 #define DFEA_MATLIB 0x200      /* element-kind flag: the descriptors may use any DFEA_MAT_* material; without it
                                   the material kernels compile only the neo-Hookean closed forms (fewer
                                   registers) and give NaN for a descriptor of another material */

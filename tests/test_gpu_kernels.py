@@ -293,3 +293,63 @@ def test_small_and_large_chunk_plans_agree_with_oracle(precision):
         out[ch] = (y.double().cpu(), k)
     assert rel(out["small"][0], out["large"][0]) < tol
     assert abs(out["small"][1] - out["large"][1]) <= max(2, 0.02 * out["large"][1])
+
+
+@pytest.mark.parametrize("et,n", [("quad4", 400), ("tri3", 283)])
+def test_tail_balanced_and_plain_plans_agree_with_oracle(et, n):
+    """Tail-balanced plans (whole rounds of 256-element chunks, the rest in half chunks) against the plain plan
+    and the oracle: the product in FP64 and FP32, a PCG solve, and a batched product of two solves (more work
+    items than the issue-ahead variant's slots: the gather-first kernel runs for tail plans at any launch size)."""
+    import ctypes
+    import numpy as np
+    from difffea_amd import _lib
+    from difffea_amd.device import HipMesh, HipSystem, _ptr, _stream
+    from difffea_amd.materials import material_spec
+    from difffea_amd.plan import MeshPlan
+    p = fem.precompute_geometry(unit_square(et, n))
+    u = stretch_state(p, 1.0 / n, seed=0)
+    v = torch.randn(u.shape[0], dtype=torch.float64, generator=torch.Generator().manual_seed(1))
+    th = torch.ones(p["conn"].shape[0], dtype=torch.float64)
+    ref = fem.make_A_operator_qp(u, th, p)(v)
+    A_ref = fem.make_A_operator_qp(u, th, p)
+    b = v * p["m"]
+    out = {}
+    for balance in (True, False):
+        plan = MeshPlan(p["nodes"].numpy(), p["conn"].numpy(), p["fixed"].numpy(), balance=balance,
+                        cell_ij=p["cell_ij"].numpy() if "cell_ij" in p else None)
+        assert plan.tail == balance
+        m = HipMesh(p, DEV, plan=plan)
+        assert bool(m.kind & _lib.TAIL) == balance
+        for precision, tol in (("fp64", 1e-11), ("mixed", 1e-5)):
+            s = HipSystem(m, th, material_spec(p), p["f"], precision)
+            s.moduli(u.to(DEV))
+            y = s.apply(v.to(DEV).to(s.vdt))
+            assert rel(y, ref) < tol
+            out[(balance, precision)] = y.double().cpu()
+        s = HipSystem(m, th, material_spec(p), p["f"], "fp64")
+        s.moduli(u.to(DEV))
+        x, k, st = s.pcg_solve(b.to(DEV), rtol=1e-8, max_iter=5000)
+        assert st == _lib.PCG_CONVERGED
+        assert float(torch.norm(b - A_ref(x.cpu()))) <= 1.01e-8 * float(torch.norm(b))
+        out[(balance, "pcg")] = (x.cpu(), k)
+        if balance:
+            # two solves of the tail plan in one launch (2 x 738 or 2 x 626 work items) give the single-solve bits
+            s2 = HipSystem(m, th, material_spec(p), p["f"], "fp64")
+            s2.moduli(u.to(DEV))
+            w = torch.randn(u.shape[0], dtype=torch.float64, generator=torch.Generator().manual_seed(2)).to(DEV)
+            y_alone = [s.apply(v.to(DEV)), s2.apply(w)]
+            table = torch.cat([m.upload(s.desc), m.upload(s2.desc)])
+            nc, nf = m.plan.n_chunks, m.n_finblk
+            items = torch.tensor(np.r_[np.c_[np.zeros(nc), np.arange(nc)], np.c_[np.ones(nc), np.arange(nc)]]
+                                 .astype(np.int32).reshape(-1), device=DEV)
+            fin = torch.tensor(np.r_[np.c_[np.zeros(nf), np.arange(nf)], np.c_[np.ones(nf), np.arange(nf)]]
+                               .astype(np.int32).reshape(-1), device=DEV)
+            for sys_, vec in ((s, v.to(DEV)), (s2, w)):
+                sys_.vec["p"].copy_(vec)
+            assert 2 * nc > 768
+            _lib.check(m.lib.dfea_apply(_ptr(table), _ptr(items), 2 * nc, _ptr(fin), 2 * nf, None, None, m.kind,
+                                        _lib.FP64, _stream()), "dfea_apply")
+            assert torch.equal(s.vec["w"], y_alone[0]) and torch.equal(s2.vec["w"], y_alone[1])
+    assert rel(out[(True, "fp64")], out[(False, "fp64")]) < 1e-12
+    (xa, ka), (xb, kb) = out[(True, "pcg")], out[(False, "pcg")]
+    assert abs(ka - kb) <= max(2, 0.02 * kb) and rel(xa, xb) < 1e-6

@@ -7,7 +7,7 @@ import torch
 from helpers import cantilever, state, unit_square
 from oracle import fem
 from difffea_amd import _lib
-from difffea_amd.plan import MeshPlan, jones_plassmann
+from difffea_amd.plan import CUS, NXCD, MeshPlan, jones_plassmann, tail_chunk_sizes
 from difffea_amd.problem import build_problem
 
 
@@ -176,16 +176,76 @@ def test_plan_tables_and_emulated_assembly_match_oracle(name, mode):
 
 
 def test_structured_meshes_get_square_tiles_and_four_colours():
-    plan = _plan(unit_square("quad4", 400))
+    plan = _plan(unit_square("quad4", 400), balance=False)
     st = plan.stats()
     assert st["mode"] == "partials" and st["n_chunks"] == 625 and st["max_chunk_nodes"] == 289
-    assert st["max_colours"] == 4 and st["max_threads"] == 256
+    assert st["max_colours"] == 4 and st["max_threads"] == 256 and not plan.tail
+    # tail-balanced (the default here): 512 16x16 tiles (17x17 nodes) and 226 16x8 half tiles (17x9 nodes)
+    tp = _plan(unit_square("quad4", 400))
+    ne, nk = tp.chunks[:, 1], tp.chunks[:, 3]
+    assert tp.tail and tp.n_chunks == 738 and (ne == 256).sum() == 512 and (ne == 128).sum() == 226
+    assert set(nk[ne == 256]) == {289} and set(nk[ne == 128]) == {153}
+    assert tp.stats()["max_colours"] == 4 and np.array_equal(tp.chunks[:, 0], np.cumsum(ne) - ne)
     halo = _plan(unit_square("quad4", 400), mode="halo").stats()
     # 16x16 core tiles + 33 halo elements; 16x16 owned nodes + the halo's nodes (18x18)
     assert halo["mode"] == "halo" and halo["n_chunks"] == 625 and halo["max_threads"] == 289
     assert halo["max_chunk_nodes"] == 324 and halo["n_shared"] == 0
     plan9 = _plan(unit_square("quad9", 200))
     assert plan9.stats()["mode"] == "partials" and plan9.stats()["max_chunk_nodes"] == 289
+
+
+def _xcd_remap(b, n):
+    # the kernels' workgroup -> chunk map (workgroup b runs on XCD b mod 8; each XCD gets a contiguous range)
+    x, pos = b % NXCD, b // NXCD
+    base, rem = divmod(n, NXCD)
+    return x * base + min(x, rem) + pos
+
+
+@pytest.mark.parametrize("n_el", [160000, 144400, 90000, 80089, 160178])
+def test_tail_plans_put_the_half_chunks_on_the_workgroups_past_the_whole_rounds(n_el):
+    E = 256
+    sizes = tail_chunk_sizes(n_el, E)
+    n = len(sizes)
+    rounds = n_el // (CUS * E)
+    assert sizes.sum() == n_el and (sizes == E).sum() == rounds * CUS and (sizes[sizes != E] <= E // 2).all()
+    assert n - rounds * CUS <= CUS                            # at most one half chunk per CU
+    chunk = np.array([_xcd_remap(b, n) for b in range(n)])
+    assert np.array_equal(np.sort(chunk), np.arange(n))     # the remap is a permutation
+    # workgroup b shares CU b mod 256 with b +- 256: the whole rounds carry full chunks, the rest half chunks
+    assert (sizes[chunk[:rounds * CUS]] == E).all() and (sizes[chunk[rounds * CUS:]] < E).all()
+
+
+@pytest.mark.parametrize("et", ["quad4", "tri3"])
+def test_tail_plan_emulated_assembly_matches_oracle(et, monkeypatch):
+    # a GPU of 8 CUs: a 50x50 mesh (2,500 quad4 / 5,000 tri3 elements) then fills whole rounds of 256-element
+    # chunks plus half chunks, so the emulation covers a tail plan at a size the oracle evaluates quickly
+    import difffea_amd.plan as plan_mod
+    monkeypatch.setattr(plan_mod, "CUS", 8)
+    p = fem.precompute_geometry(unit_square(et, 50 if et == "quad4" else 35))
+    plan = _plan(p)
+    assert plan.tail and len(set(plan.chunks[:, 1])) > 1
+    plan.validate(None, plan.inverse_map())
+    th = torch.ones(p["conn"].shape[0], dtype=torch.float64)
+    u, v = state(p, 4, scale=1e-3)
+    C, dN = fem.qp_setup(u, th, p)
+    dots = []
+    y = _emulate_apply(plan, C.numpy()[plan.perm], dN.numpy()[plan.perm], v.numpy(), dots)
+    ref = fem.make_A_operator_qp(u, th, p, C=C)(v).numpy()
+    assert np.abs(y - ref).max() < 1e-12 * np.abs(ref).max()
+    vAv = float(v.numpy() @ ref)
+    assert abs(sum(dots) - vAv) < 1e-12 * float(np.abs(v.numpy() * ref).sum())
+
+
+def test_tail_plans_only_where_they_balance():
+    assert tail_chunk_sizes(65536, 256) is None            # exactly one round
+    assert tail_chunk_sizes(50000, 256) is None            # under one round: one chunk per CU or fewer
+    assert tail_chunk_sizes(122500, 256) is None           # 478 chunks: remainder past half a round
+    assert tail_chunk_sizes(566 * 566, 256) is None        # more than three rounds
+    p = unit_square("quad4", 300)                          # 90,000 elements: 1 round + 96 half chunks
+    a, b = _plan(p), _plan(p, balance=False)
+    assert a.tail and not b.tail and a.n_chunks == 448 and b.n_chunks == 352
+    # the same elements, a different cut
+    assert np.array_equal(np.sort(a.perm), np.sort(b.perm))
 
 
 def test_jones_plassmann_is_a_valid_colouring():
