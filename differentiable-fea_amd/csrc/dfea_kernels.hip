@@ -456,6 +456,14 @@ __device__ __forceinline__ V2<T> gload2(const V2<T>* base, uint32_t i) {
     const vec2 w = *((gptr<vec2>)(gptr<V2<T>>)base + i);
     return V2<T>{w.x, w.y};
 }
+// entry k of the local-node tables (lnode, pslot, prange) of the chunk of work item b in a partials plan: the
+// tables hold each chunk at the fixed stride lstride, so the index needs only the chunk index (no chunk record);
+// k is clamped into the chunk's padded range (padding entries are valid, and dropped by the callers)
+__device__ __forceinline__ uint32_t lentry(const dfea_desc& D, int b, int k) {
+    const int ls = D.lstride;
+    return (uint32_t)(b * ls + (k < ls ? k : ls - 1));
+}
+
 template <bool NT, typename T>
 __device__ __forceinline__ T stream_load(const T* p) {
     if (NT) return __builtin_nontemporal_load(p);
@@ -514,10 +522,13 @@ struct InvRegs {
     int of[NOF];
 };
 
-template <int NEN, int BS, int CEX = default_ce<NEN>()>
-__device__ __forceinline__ void inv_load(const dfea_desc& D, const ChunkView& ch, int t, InvRegs<NEN, BS, CEX>& R) {
+// FIXED: the offsets are addressed by the fixed stride of the local-node tables (no wait for the chunk record)
+template <int NEN, int BS, int CEX = default_ce<NEN>(), bool FIXED = false>
+__device__ __forceinline__ void inv_load(const dfea_desc& D, const ChunkView& ch, int blk, int t,
+                                         InvRegs<NEN, BS, CEX>& R) {
     const int n = ch.ne * NEN;
     const uint32_t base = (uint32_t)(ch.e0 * NEN);
+    const int ls = D.lstride;
 #pragma unroll
     for (int i = 0; i < InvRegs<NEN, BS, CEX>::NSL; ++i) {
         const int j = t + i * BS;
@@ -526,7 +537,9 @@ __device__ __forceinline__ void inv_load(const dfea_desc& D, const ChunkView& ch
 #pragma unroll
     for (int i = 0; i < InvRegs<NEN, BS, CEX>::NOF; ++i) {
         const int k = t + i * BS;
-        R.of[i] = gload<false>(D.inv_off, (uint32_t)(ch.k0 + (k <= ch.nk ? k : ch.nk))) - (int)base;
+        // offsets k <= nk of the chunk (entries past nk hold the chunk's end)
+        const uint32_t j = FIXED ? (uint32_t)(blk * ls + (k <= ls ? k : ls)) : (uint32_t)(ch.k0 + (k <= ch.nk ? k : ch.nk));
+        R.of[i] = gload<false>(D.inv_off, j) - (int)base;
     }
 }
 
@@ -728,7 +741,7 @@ __global__ __launch_bounds__(BS, MINW) void k_apply(const dfea_desc* __restrict_
     if constexpr (INV) {
 #pragma unroll
         for (int a = 0; a < NEN; ++a) l[a] = gload<false>(D.lelem, (uint32_t)(a * D.n_el + e));
-        inv_load<NEN, BS, CEX>(D, ch, t, inv);
+        inv_load<NEN, BS, CEX>(D, ch, it.b, t, inv);
     }
     __builtin_amdgcn_sched_barrier(0);
     if constexpr (!(INV && STREAM_FIRST)) prefetch_gauss_points<NEN, NQ, BS, PF, T, true, CEX>(dN, C, ne, e, t, d, c);
@@ -1392,7 +1405,8 @@ __device__ __forceinline__ PcgSet<T> pcg_set(const dfea_desc& D, int q) {
     return v;
 }
 
-// PCG start: x = 0, r = b, p = s = 0 (parity-0 buffers); partials of b.b
+// PCG start: x = 0, r = b, p = s = 0 (parity-0 buffers) and r = b in the parity-1 buffer (the first kernel, of
+// parity 0, reads r_0 there like every kernel reads the buffers of the other parity); partials of b.b
 template <typename T>
 __global__ __launch_bounds__(VBS) void k_pcg_init(const dfea_desc* __restrict__ tab, const int* __restrict__ items,
                                                  int neg_res) {
@@ -1402,6 +1416,7 @@ __global__ __launch_bounds__(VBS) void k_pcg_init(const dfea_desc* __restrict__ 
     const int n = ndof(D);
     T* x = static_cast<T*>(D.vec[DFEA_X]);
     T* r = static_cast<T*>(D.vec[DFEA_R]);
+    T* r2 = static_cast<T*>(D.vec[DFEA_R2]);
     T* p = static_cast<T*>(D.vec[DFEA_P]);
     T* sv = static_cast<T*>(D.vec[DFEA_S]);
     const T* bT = static_cast<const T*>(D.vec[DFEA_B]);
@@ -1415,6 +1430,7 @@ __global__ __launch_bounds__(VBS) void k_pcg_init(const dfea_desc* __restrict__ 
         const T b = neg_res ? (T)(-res[i]) : bT[i];
         x[i] = 0;
         r[i] = b;
+        r2[i] = b;
         p[i] = 0;
         sv[i] = 0;
         bb += (double)b * (double)b;
@@ -1489,7 +1505,9 @@ __device__ __forceinline__ NodeState<T> node_state(const dfea_desc& D, const Chu
     ns.o = dgo ? gload<false>(dgo, (uint32_t)ns.n) : (T)0;
     ns.r = gload2<T>(Rd.r, (uint32_t)ns.n);
     ns.sh = DEFER && upd && shared;
-    if (upd) {
+    // DEFER (early-loaded states): requested whatever k is (the buffers of the other parity always exist; the
+    // first kernel, k < 0, drops x, p, s, w), so the loads need not wait for the iteration counter
+    if (DEFER || upd) {
         if (shared && !DEFER) {
             ns.w = sum_slots<T>(part_rd, rg.x, rg.y);
         } else if (shared) {
@@ -1567,7 +1585,10 @@ __global__ __launch_bounds__(BS, MINW) void k_pcg(const dfea_desc* __restrict__ 
                                                                                        : (const T*)D.diag32);
     const bool blk = D.precond == DFEA_PRECOND_BLOCK;
     const T* dgo = blk ? reinterpret_cast<const T*>(dg + D.n_nodes) : nullptr;
-    const int qr = k < 0.0 ? par : 1 - par;     // state buffers read (x_k, r_k, p_{k-1}, s_{k-1}, w_k)
+    // state buffers read (x_k, r_k, p_{k-1}, s_{k-1}, w_k; the first kernel, k < 0, only r_0, which dfea_pcg_init
+    // writes to both parities): EARLY launches take parity 1 - par for every k, so the buffer addresses do not
+    // wait for the iteration counter; larger launches (measured faster so) keep parity par for k < 0
+    const int qr = EARLY || k >= 0.0 ? 1 - par : par;
     const PcgSet<T> Rd = pcg_set<T>(D, qr);
     const PcgSet<T> Wr = pcg_set<T>(D, par);   // state buffers written
     const V2<T>* part_rd = reinterpret_cast<const V2<T>*>(D.part) + (size_t)(1 - par) * D.n_part;
@@ -1608,11 +1629,14 @@ __global__ __launch_bounds__(BS, MINW) void k_pcg(const dfea_desc* __restrict__ 
         for (int i = 0; i < NG; ++i) {
             const int kk = t + i * BS;
             const int kc = kk < ch.nk ? kk : ch.nk - 1;
-            gid[i] = gload<false>(D.lnode, (uint32_t)(ch.k0 + kc));
+            gid[i] = gload<false>(D.lnode, INV ? lentry(D, it.b, kk) : (uint32_t)(ch.k0 + kc));
             rng[i] = make_int2(0, 0);
             slt[i] = 0;
             if (D.n_part) {
-                const uint32_t ent = (uint32_t)(kc >= ch.nown ? ch.pbase + (kc - ch.nown) : 0);
+                // EARLY (small launches, latency-bound): by the fixed stride, without waiting for the chunk record;
+                // else from the record, so the entries of owned nodes share one cache line (less traffic)
+                const uint32_t ent = INV && EARLY ? lentry(D, it.b, kk)
+                                                  : (uint32_t)(kc >= ch.nown ? ch.pbase + (kc - ch.nown) : 0);
                 const V2<int> r2 = gload2<int>(reinterpret_cast<const V2<int>*>(prg), ent);
                 rng[i] = make_int2(r2.x, r2.y);
                 slt[i] = gload<false>(D.pslot, ent);
@@ -1641,7 +1665,7 @@ __global__ __launch_bounds__(BS, MINW) void k_pcg(const dfea_desc* __restrict__ 
         e = ch.e0 + (has_el ? tc : ch.ne - 1);
 #pragma unroll
         for (int a = 0; a < NEN; ++a) l[a] = gload<false>(D.lelem, (uint32_t)(a * D.n_el + e));
-        inv_load<NEN, BS, CEX>(D, ch, t, inv);
+        inv_load<NEN, BS, CEX, true>(D, ch, it.b, t, inv);
     } else {
         has_el = thread_element<NEN>(D, ch, t, e, l);
     }
@@ -2109,7 +2133,7 @@ int dfea_desc_offset(int field) {
         (int)offsetof(dfea_desc, nen), (int)offsetof(dfea_desc, nq), (int)offsetof(dfea_desc, precision),
         (int)offsetof(dfea_desc, n_finblk), (int)offsetof(dfea_desc, n_vblk), (int)offsetof(dfea_desc, n_halo),
         (int)offsetof(dfea_desc, n_cols), (int)offsetof(dfea_desc, precond), (int)offsetof(dfea_desc, material),
-        (int)offsetof(dfea_desc, mat_scal)};
+        (int)offsetof(dfea_desc, mat_scal), (int)offsetof(dfea_desc, lstride)};
     if (field < 0 || field >= (int)(sizeof off / sizeof off[0])) return -1;
     return off[field];
 }

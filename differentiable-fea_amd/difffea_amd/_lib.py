@@ -50,7 +50,7 @@ class Desc(ctypes.Structure):
         ("nen", ctypes.c_int32), ("nq", ctypes.c_int32), ("precision", ctypes.c_int32),
         ("n_finblk", ctypes.c_int32), ("n_vblk", ctypes.c_int32), ("n_halo", ctypes.c_int32),
         ("n_cols", ctypes.c_int32), ("precond", ctypes.c_int32), ("material", ctypes.c_int32),
-        ("mat_scal", ctypes.c_int32),
+        ("mat_scal", ctypes.c_int32), ("lstride", ctypes.c_int32),
     ]
 
 

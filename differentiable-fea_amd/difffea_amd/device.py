@@ -94,15 +94,19 @@ class HipMesh:
         self.n_el, self.n_nodes = p.n_el, p.n_nodes
         self.n_dof = 2 * p.n_nodes
         dv = device
-        self.t_chunks = _dev_int32(p.chunks, dv)
-        self.t_lnode = _dev_int32(p.lnode, dv)
+        # local-node tables at the fixed stride the kernels address them by (partials plans)
+        dt = p.device_tables(inv_off if p.mode == "partials" else None)
+        self.lstride = dt["lstride"]
+        self.n_lnodes = len(dt["lnode"])
+        self.t_chunks = _dev_int32(dt["chunks"], dv)
+        self.t_lnode = _dev_int32(dt["lnode"], dv)
         self.t_lelem = torch.from_numpy(p.lelem.view(np.int16)).to(dv)
         self.t_sh_node = _dev_int32(p.sh_node, dv)
         self.t_sh_off = _dev_int32(p.sh_off, dv)
-        self.t_pslot = _dev_int32(p.pslot if len(p.pslot) else np.zeros(1, np.int32), dv)
+        self.t_pslot = _dev_int32(dt["pslot"] if len(dt["pslot"]) else np.zeros(1, np.int32), dv)
         self.t_hel = _dev_int32(p.hel if len(p.hel) else np.zeros(1, np.int32), dv)
-        self.t_prange = _dev_int32(p.prange if p.n_part else np.zeros((1, 2), np.int32), dv)
-        self.t_inv_off = _dev_int32(inv_off, dv)
+        self.t_prange = _dev_int32(dt["prange"] if len(dt["pslot"]) else np.zeros((1, 2), np.int32), dv)
+        self.t_inv_off = _dev_int32(dt["inv_off"] if p.mode == "partials" else inv_off, dv)
         self.t_inv_slot = torch.from_numpy(inv_slot.view(np.int16)).to(dv)
         self.t_hlelem = torch.from_numpy(np.ascontiguousarray(p.hlelem).view(np.int16).reshape(-1) if p.hlelem.size
                                          else np.zeros(1, np.int16)).to(dv)
@@ -144,7 +148,8 @@ class HipMesh:
         d.conn, d.perm, d.nodes = self.t_conn.data_ptr(), self.t_perm.data_ptr(), self.t_nodes.data_ptr()
         d.dNdxi, d.gw, d.nfix = self.t_dNdxi.data_ptr(), self.t_gw.data_ptr(), self.t_nfix.data_ptr()
         d.dN64, d.dN32, d.w = self.dN64.data_ptr(), self.dN32.data_ptr(), self.w.data_ptr()
-        d.n_chunks, d.n_lnodes, d.n_shared, d.n_part = p.n_chunks, p.n_lnodes, p.n_shared, p.n_part
+        d.n_chunks, d.n_lnodes, d.n_shared, d.n_part = p.n_chunks, self.n_lnodes, p.n_shared, p.n_part
+        d.lstride = self.lstride
         d.n_el, d.n_nodes, d.nen, d.nq = p.n_el, p.n_nodes, self.nen, self.nq
         d.n_finblk, d.n_vblk = self.n_finblk, self.n_vblk
         return d

@@ -462,6 +462,47 @@ class MeshPlan:
         inv_off = np.searchsorted(entry[order], np.arange(self.n_lnodes + 1)).astype(np.int32)
         return inv_off, inv_slot
 
+    def device_tables(self, inv_off=None):
+        """The local-node tables as the kernels read them: dict(chunks, lnode, pslot, prange, inv_off, lstride).
+
+        Partials plans are laid out at a fixed stride: local-node entry k of chunk c is c * lstride + k
+        (lstride = the largest chunk node count rounded up to 32), for the node table, the partial slot and
+        slot range of shared entries, and the inverse-map offsets, so a workgroup addresses its node tables
+        from its chunk index alone, without waiting for its chunk record.  The chunk records point into that
+        layout (k0 = c * lstride, pbase = c * lstride + nown), so kernels that read them see the same entries.
+        Padding entries hold the chunk's last node (node table), slot 0 and range {0, 0} (owned and padding
+        entries), and the chunk's end offset (inverse map).  Halo plans keep the compact tables (lstride 0).
+        inv_off: this plan's ``inverse_map()[0]`` (partials plans)."""
+        if self.mode != "partials":
+            return dict(chunks=self.chunks, lnode=self.lnode, pslot=self.pslot, prange=self.prange,
+                        inv_off=inv_off, lstride=0)
+        ch = self.chunks
+        nc = self.n_chunks
+        k0, nk, nown = ch[:, 2].astype(np.int64), ch[:, 3].astype(np.int64), ch[:, 4].astype(np.int64)
+        ls = int(-(-int(nk.max()) // 32) * 32)
+        _require(nc * ls < (1 << 31), "padded node tables beyond 2^31 entries")
+        k = np.arange(ls)[None, :]
+        src = k0[:, None] + np.minimum(k, nk[:, None] - 1)        # compact entry of every padded entry
+        lnode = self.lnode[src.reshape(-1)]
+        live_sh = (k >= nown[:, None]) & (k < nk[:, None])         # shared entries of each chunk
+        pslot = np.zeros((nc, ls), np.int32)
+        prange = np.zeros((nc, ls, 2), np.int32)
+        if self.n_part:
+            ent = (ch[:, 5].astype(np.int64)[:, None] + (k - nown[:, None]))[live_sh]
+            pslot[live_sh] = self.pslot[ent]
+            prange[live_sh] = self.prange[ent]
+        dch = ch.copy()
+        dch[:, 2] = np.arange(nc) * ls
+        dch[:, 5] = np.arange(nc) * ls + nown
+        out = dict(chunks=dch, lnode=lnode, pslot=pslot.reshape(-1), prange=prange.reshape(-1, 2), lstride=ls)
+        if inv_off is not None:
+            inv_off = np.asarray(inv_off, np.int64)
+            _require(len(inv_off) == self.n_lnodes + 1, "inverse map size")
+            end = np.r_[k0[1:], self.n_lnodes]                    # compact entry after each chunk's last node
+            inv = inv_off[np.minimum(k0[:, None] + k, end[:, None])]   # entries past nk: the chunk's end offset
+            out["inv_off"] = np.r_[inv.reshape(-1), inv_off[-1]].astype(np.int32)
+        return out
+
     @property
     def prange(self):
         """[n_part][2] int32: first slot and number of slots of the shared node of every partials entry

@@ -109,16 +109,17 @@ typedef struct dfea_chunk {
 /* Everything the kernels need about one solve (device pointers). */
 typedef struct dfea_desc {
     const dfea_chunk* chunks;   /* [n_chunks] */
-    const uint32_t* lnode;      /* [n_lnodes] node id | DFEA_FIXED_X | DFEA_FIXED_Y */
+    const uint32_t* lnode;      /* [n_lnodes] node id | DFEA_FIXED_X | DFEA_FIXED_Y (see lstride) */
     const uint16_t* lelem;      /* [nen][n_el] local index (within the chunk) of element node a */
     const uint32_t* sh_node;    /* [n_shared] node id | fixed bits */
     const int32_t* sh_off;      /* [n_shared+1] */
-    const int32_t* pslot;       /* [n_part] partial slot of each (chunk, shared node) entry, chunk order;
+    const int32_t* pslot;       /* partial slot of each (chunk, shared node) entry: entry pbase + (k - nown) of a
+                                   chunk's shared local node k ([n_lnodes] at the lstride layout, else [n_part]);
                                    the partials of shared node i are slots sh_off[i]..sh_off[i+1]-1 */
     const int32_t* hel;         /* [n_halo] halo mode: internal element id of each halo entry */
     const uint16_t* hlelem;     /* [nen][n_halo] halo mode: local node index of halo entry h, node a */
-    const int32_t* prange;      /* [n_part][2] {first slot, number of slots} of the shared node of each (chunk,
-                                   shared node) entry, chunk order (entry tables load without the node id) */
+    const int32_t* prange;      /* {first slot, number of slots} of the shared node of each (chunk, shared node)
+                                   entry, indexed like pslot (entry tables load without the node id) */
     const int32_t* inv_off;     /* [n_lnodes + 1] partials plans: the element vectors of local node entry i are the
                                    chunk-local slots inv_slot[inv_off[i] .. inv_off[i+1]) */
     const uint16_t* inv_slot;   /* [n_el * nen] slot e_local * nen + a, sorted by slot per node entry (element =
@@ -153,6 +154,11 @@ typedef struct dfea_desc {
     int32_t precond;           /* DFEA_PRECOND_JACOBI or DFEA_PRECOND_BLOCK */
     int32_t material;          /* DFEA_MAT_* */
     int32_t mat_scal;          /* bit k: mat_p[k] is theta-scaled */
+    int32_t lstride;           /* partials plans: the local-node tables (lnode, pslot, prange, inv_off) hold chunk c's
+                                  entries at c * lstride + k, k < lstride (k >= nk: padding -- the chunk's last node,
+                                  slot 0 / range {0, 0}, the chunk's end offset), so the product and PCG kernels
+                                  address them from the chunk index alone; the chunk records agree (k0 = c * lstride,
+                                  pbase = c * lstride + nown).  0 for halo plans (compact tables) */
 } dfea_desc;
 
 /* ---- library ---------------------------------------------------------------------------------------- */

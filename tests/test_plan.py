@@ -1,5 +1,7 @@
 """Mesh plans, checked on the CPU: table consistency, colouring, and a numpy emulation of the chunked,
 colour-ordered assembly the kernels perform, compared with the oracle's scatter-add product."""
+import copy
+
 import numpy as np
 import pytest
 import torch
@@ -57,8 +59,11 @@ def _emulate_apply(plan, C, dN, v, dots=None):
     vv = v.reshape(N, 2)
     shmap = plan.shmap
     inv = plan.inverse_map() if plan.mode == "partials" else None
+    ls = getattr(plan, "lstride", 0)
     for c in range(plan.n_chunks):
         e0, ne, k0, nk, nown, pbase, ncol, h0, nh = plan.chunks[c, :9]
+        if ls:      # fixed-stride tables: the kernels address entry k of chunk c as c * lstride + k
+            assert k0 == c * ls and pbase == c * ls + nown and nk <= ls
         ln = plan.lnode[k0:k0 + nk]
         nodes = (ln & _lib.NODE_MASK).astype(np.int64)
         bits = ln >> 30
@@ -122,6 +127,44 @@ def _emulate_apply(plan, C, dN, v, dots=None):
     return y.reshape(-1)
 
 
+def _device_view(plan):
+    """The plan with the local-node tables the kernels read (``MeshPlan.device_tables``): fixed-stride node,
+    slot and inverse-map tables and the chunk records pointing into them."""
+    inv_off, inv_slot = plan.inverse_map()
+    dt = plan.device_tables(inv_off)
+    v = copy.copy(plan)
+    v.chunks, v.lnode, v.pslot, v.lstride = dt["chunks"], dt["lnode"], dt["pslot"], dt["lstride"]
+    v.n_lnodes = len(dt["lnode"])
+    v.inverse_map = lambda: (dt["inv_off"], inv_slot)
+    return v, dt
+
+
+def _check_device_tables(plan):
+    """Entry by entry: the fixed-stride tables hold the compact tables' entries at c * lstride + k and the
+    documented padding elsewhere."""
+    inv_off, _ = plan.inverse_map()
+    dt = plan.device_tables(inv_off)
+    ls = dt["lstride"]
+    assert ls % 32 == 0 and plan.chunks[:, 3].max() <= ls <= _lib.load().dfea_chunk_nodes()
+    assert len(dt["lnode"]) == len(dt["pslot"]) == len(dt["prange"]) == plan.n_chunks * ls
+    assert len(dt["inv_off"]) == plan.n_chunks * ls + 1 and dt["inv_off"][-1] == plan.n_el * plan.nen
+    pr = plan.prange if plan.n_part else None
+    for c, rec in enumerate(plan.chunks):
+        e0, ne, k0, nk, nown, pbase = (int(x) for x in rec[:6])
+        b = c * ls
+        assert np.array_equal(dt["chunks"][c, 2:6:3], [b, b + nown]) and np.array_equal(
+            np.delete(dt["chunks"][c], [2, 5]), np.delete(rec, [2, 5]))
+        assert np.array_equal(dt["lnode"][b:b + nk], plan.lnode[k0:k0 + nk])
+        assert (dt["lnode"][b + nk:b + ls] == plan.lnode[k0 + nk - 1]).all()
+        assert (dt["pslot"][b:b + nown] == 0).all() and (dt["pslot"][b + nk:b + ls] == 0).all()
+        assert (dt["prange"][b:b + nown] == 0).all() and (dt["prange"][b + nk:b + ls] == 0).all()
+        if plan.n_part:
+            assert np.array_equal(dt["pslot"][b + nown:b + nk], plan.pslot[pbase:pbase + nk - nown])
+            assert np.array_equal(dt["prange"][b + nown:b + nk], pr[pbase:pbase + nk - nown])
+        assert np.array_equal(dt["inv_off"][b:b + nk + 1], inv_off[k0:k0 + nk + 1])
+        assert (dt["inv_off"][b + nk:b + ls + 1] == (e0 + ne) * plan.nen).all()
+
+
 @pytest.mark.parametrize("mode", ["halo", "partials", "small"])
 @pytest.mark.parametrize("name", list(MESHES))
 def test_plan_tables_and_emulated_assembly_match_oracle(name, mode):
@@ -154,6 +197,13 @@ def test_plan_tables_and_emulated_assembly_match_oracle(name, mode):
     y = _emulate_apply(plan, C.numpy()[plan.perm], dN.numpy()[plan.perm], v.numpy(), dots)
     ref = fem.make_A_operator_qp(u, th, p, C=C)(v).numpy()
     assert np.abs(y - ref).max() < 1e-12 * np.abs(ref).max()
+    if plan.mode == "partials":
+        # the same assembly through the fixed-stride tables the kernels read gives the same bits
+        _check_device_tables(plan)
+        dplan, _ = _device_view(plan)
+        ddots = []
+        yd = _emulate_apply(dplan, C.numpy()[plan.perm], dN.numpy()[plan.perm], v.numpy(), ddots)
+        assert np.array_equal(yd, y) and ddots == dots
     # the PCG's p.Ap from chunk partials alone (shared nodes by linearity) equals v.A(v)
     vAv = float(v.numpy() @ ref)
     assert abs(sum(dots) - vAv) < 1e-12 * float(np.abs(v.numpy() * ref).sum())
@@ -232,6 +282,13 @@ def test_tail_plan_emulated_assembly_matches_oracle(et, monkeypatch):
     y = _emulate_apply(plan, C.numpy()[plan.perm], dN.numpy()[plan.perm], v.numpy(), dots)
     ref = fem.make_A_operator_qp(u, th, p, C=C)(v).numpy()
     assert np.abs(y - ref).max() < 1e-12 * np.abs(ref).max()
+    if plan.mode == "partials":
+        # the same assembly through the fixed-stride tables the kernels read gives the same bits
+        _check_device_tables(plan)
+        dplan, _ = _device_view(plan)
+        ddots = []
+        yd = _emulate_apply(dplan, C.numpy()[plan.perm], dN.numpy()[plan.perm], v.numpy(), ddots)
+        assert np.array_equal(yd, y) and ddots == dots
     vAv = float(v.numpy() @ ref)
     assert abs(sum(dots) - vAv) < 1e-12 * float(np.abs(v.numpy() * ref).sum())
 
