@@ -122,13 +122,13 @@ def time_pcg_iteration(p, u, v, theta, precision, iters=400, chunk=50):
     s.moduli(u.to(dev))
     b = (v * p["m"]).to(dev)
     s.pcg_start(b, rtol=0.0, max_iter=iters + 2 * chunk + 1)
-    s.pcg_iterate(chunk)                       # warm-up (graph capture + first replay)
+    s.pcg_iterate(chunk, True)                 # warm-up (graph capture + first replay)
     torch.cuda.synchronize()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     n = (iters // chunk) * chunk
     e0.record()
     for _ in range(n // chunk):
-        s.pcg_iterate(chunk)
+        s.pcg_iterate(chunk, True)
     e1.record()
     torch.cuda.synchronize()
     st, _, k = s.pcg_status()

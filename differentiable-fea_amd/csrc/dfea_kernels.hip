@@ -1439,8 +1439,10 @@ __global__ __launch_bounds__(VBS) void k_pcg_init(const dfea_desc* __restrict__ 
     if (threadIdx.x == 0) vec_red(D)[it.b] = bb;
 }
 
-// ||b||^2 and the scalar block: both parity sets start the solve (k = -1: the first kernel computes w_0)
-__global__ __launch_bounds__(VBS) void k_pcg_init_final(const dfea_desc* __restrict__ tab, const int* __restrict__ items) {
+// ||b||^2 and the scalar block: both parity sets start the solve (k = -1: the first kernel computes w_0);
+// set: the tolerance and the cap come from the arguments (else they are already in the scalar block)
+__global__ __launch_bounds__(VBS) void k_pcg_init_final(const dfea_desc* __restrict__ tab, const int* __restrict__ items,
+                                                       int set, double rtol, double maxit) {
     __shared__ double s_red[16];
     const Item it = load_item(items);
     const dfea_desc& D = tab[it.s];
@@ -1449,6 +1451,10 @@ __global__ __launch_bounds__(VBS) void k_pcg_init_final(const dfea_desc* __restr
         double* sc = D.sc;
         // nothing to do when the right-hand side is zero: x = 0 after 0 iterations
         const double st = sqrt(bb) == 0.0 ? (double)DFEA_PCG_CONVERGED : (double)DFEA_PCG_RUNNING;
+        if (set) {
+            sc[DFEA_SC_RTOL] = rtol;
+            sc[DFEA_SC_MAXIT] = maxit;
+        }
         sc[DFEA_SC_BB] = bb;
         sc[DFEA_SC_RR] = bb;
         sc[DFEA_SC_RZ] = 0.0;
@@ -2246,7 +2252,18 @@ int dfea_pcg_init(const dfea_desc* table, const int32_t* vitems, int n_vitems, c
                  ? launch(k_pcg_init<double>, n_vitems, VBS, stream, "k_pcg_init", table, vitems, neg_res)
                  : launch(k_pcg_init<float>, n_vitems, VBS, stream, "k_pcg_init", table, vitems, neg_res);
     if (rc) return rc;
-    return launch(k_pcg_init_final, n_solves, VBS, stream, "k_pcg_init_final", table, solve_items);
+    return launch(k_pcg_init_final, n_solves, VBS, stream, "k_pcg_init_final", table, solve_items, 0, 0.0, 0.0);
+}
+
+int dfea_pcg_start(const dfea_desc* table, int n_vitems, double rtol, int max_iter, int neg_res, int precision,
+                   void* stream) {
+    if (!table || n_vitems <= 0 || max_iter < 0) return fail(DFEA_EINVAL, "dfea_pcg_start: bad argument");
+    const int* none = nullptr;
+    int rc = precision == DFEA_FP64
+                 ? launch(k_pcg_init<double>, n_vitems, VBS, stream, "k_pcg_init", table, none, neg_res)
+                 : launch(k_pcg_init<float>, n_vitems, VBS, stream, "k_pcg_init", table, none, neg_res);
+    if (rc) return rc;
+    return launch(k_pcg_init_final, 1, VBS, stream, "k_pcg_init_final", table, none, 1, rtol, (double)max_iter);
 }
 
 int dfea_pcg_iterate(const dfea_desc* table, const int32_t* items, int n_items, int n_iter, int nen, int precision,

@@ -21,11 +21,10 @@ from .plan import MeshPlan, chunking_for
 
 F64, F32 = torch.float64, torch.float32
 
-# the scalar-block entries the host reads after a round of PCG kernels (status, k, maxit, iters) and writes
-# before a solve (rtol, status, k, maxit) are contiguous, so each is one copy
-_SC_RTOL, _SC_STATUS = _lib.SC["rtol"], _lib.SC["status"]
+# the scalar-block entries the host reads after a round of PCG kernels (status, k, maxit, iters) are contiguous,
+# so they are one copy
+_SC_STATUS = _lib.SC["status"]
 assert (_lib.SC["k"], _lib.SC["maxit"], _lib.SC["iters"]) == (_SC_STATUS + 1, _SC_STATUS + 2, _SC_STATUS + 3)
-assert _SC_STATUS == _SC_RTOL + 1
 
 
 def _stream():
@@ -203,10 +202,10 @@ class HipSystem:
         self.vec["uacc"] = torch.zeros(nd, dtype=F64, device=dv)
         self.vec["uD"] = torch.zeros(nd, dtype=F64, device=dv) if uD is None else uD.to(dv, F64).contiguous()
         self._graphs = {}
-        # pinned host staging of scalars: [0:4] status reads, [4:8] the PCG set-up write, [8] dot products; the
-        # event marks the set-up copy, so the staging is not rewritten while a copy from it may be pending
+        # pinned host staging of scalars read back: [0:4] PCG status, [8] dot products; the event marks the
+        # copy of a dot product
         self._hst = torch.empty(9, dtype=F64, pin_memory=True)
-        self._hst_ev = torch.cuda.Event()
+        self._dot_ev = torch.cuda.Event()
         self.desc = self._make_desc()
         self.table = m.upload(self.desc)
 
@@ -242,7 +241,8 @@ class HipSystem:
     def moduli(self, u):
         self._ensure_moduli_buffers()
         m = self.mesh
-        u = u.to(F64).contiguous()
+        if u.dtype != F64 or not u.is_contiguous():
+            u = u.to(F64).contiguous()
         _lib.check(self.lib.dfea_moduli(_ptr(self.table), _ptr(m.items), m.plan.n_chunks, _ptr(m.fin_items),
                                         m.n_finblk, _ptr(u), self.kind, self.prec_id, _stream()), "dfea_moduli")
 
@@ -259,7 +259,8 @@ class HipSystem:
 
     def residual(self, u, out=None, lf=1.0):
         m = self.mesh
-        u = u.to(F64).contiguous()
+        if u.dtype != F64 or not u.is_contiguous():
+            u = u.to(F64).contiguous()
         out = torch.empty_like(u) if out is None else out
         _lib.check(self.lib.dfea_residual(_ptr(self.table), _ptr(m.items), m.plan.n_chunks, _ptr(m.fin_items),
                                           m.n_finblk, _ptr(u), _ptr(out), float(lf), self.kind, _stream()),
@@ -287,16 +288,30 @@ class HipSystem:
 
     def dot_host(self, x, y=None):
         """The same dot product read to the host (one copy through pinned memory, one synchronisation)."""
+        return self.dot_start(x, y)()
+
+    def dot_start(self, x, y=None):
+        """Enqueue the dot product and its copy to pinned host memory; returns a function that waits for that
+        copy alone (work enqueued after this call keeps running) and gives the value."""
         m = self.mesh
-        x = x.to(F64).contiguous()
-        y = x if y is None else y.to(F64).contiguous()
+        if x.dtype != F64 or not x.is_contiguous():
+            x = x.to(F64).contiguous()
+        if y is None:
+            y = x
+        elif y.dtype != F64 or not y.is_contiguous():
+            y = y.to(F64).contiguous()
         _lib.check(self.lib.dfea_dot(_ptr(self.table), _ptr(m.vitems), m.n_vblk, _ptr(m.solve_items), 1,
                                      _ptr(x), _ptr(y), _stream()), "dfea_dot")
         n = _lib.SC["norm"]
         h = self._hst[8:9]
         h.copy_(self.sc[n:n + 1], non_blocking=True)
-        torch.cuda.current_stream(m.device).synchronize()
-        return float(h[0])
+        ev = self._dot_ev
+        ev.record()
+
+        def wait():
+            ev.synchronize()
+            return float(h[0])
+        return wait
 
     def norm(self, x):
         return torch.sqrt(self.dot(x))
@@ -354,15 +369,14 @@ class HipSystem:
         m = self.mesh
         if b is not None:
             self.vec["b"].copy_(b)
-        self._hst_ev.synchronize()
-        h = self._hst[4:8]
-        h.copy_(torch.tensor([float(rtol), 0.0, 0.0, float(max_iter)], dtype=F64))   # rtol, status, k, maxit
-        self.sc[_SC_RTOL:_SC_RTOL + 4].copy_(h, non_blocking=True)
-        self._hst_ev.record()
-        _lib.check(self.lib.dfea_pcg_init(_ptr(self.table), _ptr(m.vitems), m.n_vblk, _ptr(m.solve_items), 1,
-                                          int(neg_res), self.prec_id, _stream()), "dfea_pcg_init")
+        # one launch pair; the tolerance and the cap travel as kernel arguments
+        _lib.check(self.lib.dfea_pcg_start(_ptr(self.table), m.n_vblk, float(rtol), int(max_iter), int(neg_res),
+                                           self.prec_id, _stream()), "dfea_pcg_start")
 
-    use_graphs = True
+    # solve rounds as direct launches (HIP graphs on request): a graph launch costs 35-60 us of host time before its
+    # first kernel starts, while direct launches cost ~3 us each and keep ahead of kernels of >= ~7 us; measured on
+    # MI355X, the 100x100 Q4 solve (2,953 PCG kernels in 26 Newton steps) 27.7 ms with graphs, 26.7 ms direct
+    use_graphs = False
 
     def _graph(self, n_iter):
         """Executable HIP graph of n_iter fused PCG iterations (captured once per chunk length)."""
@@ -406,6 +420,7 @@ class HipSystem:
         return int(st), int(its), int(k)
 
     ROUND_GRAPHS = (128, 64, 32, 16)
+    LEAD_DIRECT = 2          # graph rounds: kernels launched directly at the start of a solve (even: pairs)
 
     def pcg_round(self, n_iter, graph=None):
         """n_iter PCG kernels (rounded up to even): replays of the graphs of ROUND_GRAPHS kernels (each
@@ -431,6 +446,12 @@ class HipSystem:
         update is made by the kernel after it, so k iterations take k + 2 kernels."""
         k = 0
         n = chunk if first is None else max(2, int(first))
+        n = min(n, max_iter + 2)
+        if self.LEAD_DIRECT and n > self.LEAD_DIRECT and (self.use_graphs if graph is None else graph):
+            # the first kernels of the solve as direct launches: the GPU starts on them while the host submits the
+            # graphs of the rest of the round (a graph launch takes tens of microseconds of host time)
+            self.pcg_iterate(self.LEAD_DIRECT, False)
+            n -= self.LEAD_DIRECT
         while True:
             self.pcg_round(min(n, max_iter - k + 2), graph)
             st, its, k = self.pcg_status()

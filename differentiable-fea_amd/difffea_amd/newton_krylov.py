@@ -13,6 +13,7 @@
   refinement loop, so the converged residual meets the same FP64 tolerance.  The inner relative tolerance
   is floored at ``MIXED_RTOL_FLOOR`` (FP32 CG stagnates near eps32 times the condition number).
 """
+import ctypes
 import math
 import time
 
@@ -27,6 +28,7 @@ from .device import HipSystem
 # 41.5 / 41.5 / 40.6 / 39.2 ms, O-grid 64x256 Q9 0.634 / 0.630 / 0.592 / 0.578 s, 256x256 Q4 adjoint sensitivity
 # 0.307 / 0.296 / 0.290 / 0.279 s; Newton still meets the FP64 tolerance (it is the outer refinement)
 MIXED_RTOL_FLOOR = 1e-3
+_N_ALPHAS = 16          # line-search step lengths 2^0 .. 2^-15 kept on the device
 
 
 def first_round(it, counts, prev_counts):
@@ -90,6 +92,9 @@ class HipNewton:
         nd = self.mesh.n_dof
         self.ucand = torch.empty(nd, dtype=torch.float64, device=self.device)
         self._alpha = torch.empty(1, dtype=torch.float64, device=self.device)
+        # the line search's step lengths 2^-j on the device (a trial reads its alpha there, no fill per trial)
+        self._alphas = torch.tensor([2.0 ** -j for j in range(_N_ALPHAS)], dtype=torch.float64, device=self.device)
+        self._moduli_at = None          # the state the system's moduli were last computed at (see armijo)
         self.pcg_chunk = 16
         self._step_counts = []          # CG iterations of the Newton iterations of the last load step solved
 
@@ -111,11 +116,15 @@ class HipNewton:
         return s.vec["x"], its
 
     def trial(self, u, du, alpha, out):
-        self._alpha.fill_(alpha)
+        j = -math.log2(alpha) if alpha > 0.0 else -1.0
+        if j == int(j) and 0 <= j < _N_ALPHAS:
+            a = ctypes.c_void_p(self._alphas.data_ptr() + 8 * int(j))
+        else:
+            self._alpha.fill_(alpha)
+            a = ctypes_ptr(self._alpha)
         m = self.mesh
         _lib.check(m.lib.dfea_axpy(ctypes_ptr(self.sys.table), ctypes_ptr(m.vitems), m.n_vblk, 1, ctypes_ptr(u),
-                                   ctypes_ptr(du), None, ctypes_ptr(self._alpha), ctypes_ptr(out),
-                                   self.sys.prec_id, stream()), "dfea_axpy")
+                                   ctypes_ptr(du), None, a, ctypes_ptr(out), self.sys.prec_id, stream()), "dfea_axpy")
         return out
 
     # -- reference state machine ---------------------------------------------------------------------------
@@ -124,7 +133,13 @@ class HipNewton:
         while alpha >= min_alpha:
             uc = self.trial(u, du, alpha, self.ucand)
             R = self.residual(uc, lf)
-            rc = self.norm(R)
+            pending = self.sys.dot_start(R)
+            # the next Newton iteration's moduli at the trial state, enqueued before the norm is read: when the
+            # trial is accepted (the usual case) the GPU computes them while the host waits for the norm and
+            # decides; otherwise they are recomputed where needed (same kernel, same state, same bits)
+            self.sys.moduli(uc)
+            self._moduli_at = uc
+            rc = math.sqrt(pending())
             if rc < (1.0 - c1 * alpha) * rn:          # False for NaN (inverted element)
                 self.ucand = u
                 return uc, rc, alpha, True
@@ -132,6 +147,7 @@ class HipNewton:
         return u, rn, alpha, False
 
     def at_load(self, u0, lf, tol, max_newton, step_id, cg_rtol, cg_max_iter, callback, verbose):
+        self._moduli_at = None
         u = self.sys.lift(u0, self.uD, lf)
         R = self.residual(u, lf)
         r0 = self.norm(R)
@@ -147,7 +163,9 @@ class HipNewton:
             if it == max_newton:
                 break
             eta = min(0.5, math.sqrt(rn / (r0 + 1e-14))) if cg_rtol is None else cg_rtol
-            self.sys.moduli(u)
+            if self._moduli_at is not u:                # else computed after the line search that accepted u
+                self.sys.moduli(u)
+            self._moduli_at = None
             du, k = self.linear_solve(eta, cg_max_iter, first_round(it, counts, prev))
             counts.append(int(k))
             if self.ucand is u:

@@ -259,6 +259,12 @@ int dfea_axpy(const dfea_desc* table, const int32_t* items, int n_items, int mod
  * (converged, 0 iterations, x = 0).  Replaces `newton_krylov.py:51-60`. */
 int dfea_pcg_init(const dfea_desc* table, const int32_t* vitems, int n_vitems, const int32_t* solve_items,
                   int n_solves, int neg_res, int precision, void* stream);
+/* The same start for one solve (the table's first descriptor, NULL item tables over n_vitems vector blocks),
+ * with the relative tolerance and the cap passed by value: the set-up kernel writes them into the scalar block,
+ * so the caller copies nothing to the device first.  Replaces `newton_krylov.py:51-60` in a single solve's
+ * Newton loop (`newton_krylov.py:147-148`). */
+int dfea_pcg_start(const dfea_desc* table, int n_vitems, double rtol, int max_iter, int neg_res, int precision,
+                   void* stream);
 /* PCG kernels for every running solve of the batch, one kernel per iteration (per chunk: the scalar step
  * -- convergence of the last residual, the iteration cap, curvature, alpha and beta -- from the previous
  * kernel's partials, evaluated identically by every workgroup; the update of the chunk's nodes; the next
