@@ -55,6 +55,8 @@ constexpr int nq_of() { return NEN == 3 ? 1 : NEN == 4 ? 4 : NEN == 6 ? 6 : 9; }
 // PCG launches of at most this many workgroups (one per CU or fewer: latency-bound) load the first node's
 // state before the scalar step
 constexpr int PCG_EARLY_ITEMS = 256;
+// PCG launches whose element data (moduli and gradients) exceed this many bytes load it with nontemporal hints
+constexpr double PCG_NT_BYTES = 240.0 * (1 << 20);
 
 // product launches of at most this many workgroups request the Gauss points one step ahead (linear elements):
 // that variant holds 3 workgroups of 256 threads per CU, the other 4
@@ -1563,7 +1565,7 @@ __device__ __forceinline__ void node_w(NodeState<T>& ns, const V2<T>* part_rd) {
 // than the GPU has CUs.  The scalars the step reads (tolerance, cap, the previous iteration's gamma and
 // alpha) are requested at the start, before the first barrier.
 template <int NEN, int NQ, int BS, typename T, int PF = 2, int MINW = 1, bool EARLY = false,
-          int CEX = default_ce<NEN>()>
+          int CEX = default_ce<NEN>(), bool NT = false>
 __global__ __launch_bounds__(BS, MINW) void k_pcg(const dfea_desc* __restrict__ tab, const int* __restrict__ items, int par) {
     static_assert(3 * (BS / 64) <= 16, "block_sum3 needs 3 slots per wave in ChunkLDS::red");
     // partials plans (chunks of exactly CE elements) sum nodes through the inverse map, halo plans by colour
@@ -1679,7 +1681,7 @@ __global__ __launch_bounds__(BS, MINW) void k_pcg(const dfea_desc* __restrict__ 
     T d[QG][NEN][2];
     T c[QG][10];
     __builtin_amdgcn_sched_barrier(0);
-    prefetch_gauss_points<NEN, NQ, BS, PF, T, false, CEX>(dN, C, ne, e, t, d, c);
+    prefetch_gauss_points<NEN, NQ, BS, PF, T, NT, CEX>(dN, C, ne, e, t, d, c);
     __builtin_amdgcn_sched_barrier(0);
 
     // 1d. the node states (independent of the scalars)
@@ -1816,7 +1818,7 @@ __global__ __launch_bounds__(BS, MINW) void k_pcg(const dfea_desc* __restrict__ 
     __syncthreads();
 
     // 4. w_{k+1} = A u_{k+1}
-    element_product<NEN, NQ, BS, PF, T, false, false, CEX>(ch, t, has_el, l, dN, C, ne, e, d, c, L, GL);
+    element_product<NEN, NQ, BS, PF, T, NT, false, CEX>(ch, t, has_el, l, dN, C, ne, e, d, c, L, GL);
 
     double d_part = 0.0;
 #pragma unroll
@@ -1966,14 +1968,19 @@ int pcg_iter(const dfea_desc* table, const int32_t* items, int n_items, int n_it
     constexpr int CEX = SMALL ? CHUNK_QUAD : default_ce<NEN>();
     constexpr int BSP = quadratic<NEN>() ? bs_groups<T>() : SMALL ? BS_Q4S : BS;
     const bool early = n_items <= PCG_EARLY_ITEMS;
+    // element data beyond what the Infinity Cache holds next to the state vectors: streamed with nontemporal loads
+    const bool nt = (double)n_items * CEX * nq_of<NEN>() * (10 + 2 * NEN) * sizeof(T) > PCG_NT_BYTES;
     for (int i = 0; i < n_iter + (n_iter & 1); ++i) {
         // small launches (one workgroup per CU or fewer): every node state and, for linear elements, every Gauss
         // point requested before the scalar step
         constexpr int PF_EARLY = NEN <= 4 && BSP % CHUNK_LIN == 0 ? nq_of<NEN>() : 2;
+        constexpr int MW = NEN <= 4 && !SMALL ? 3 : 1;
         const int rc = early ? launch(k_pcg<NEN, nq_of<NEN>(), BSP, T, PF_EARLY, 1, true, CEX>, n_items, BSP, stream,
                                       "k_pcg", table, items, i & 1)
-                             : launch(k_pcg<NEN, nq_of<NEN>(), BSP, T, 2, (NEN <= 4 && !SMALL ? 3 : 1), false, CEX>,
-                                      n_items, BSP, stream, "k_pcg", table, items, i & 1);
+                     : nt ? launch(k_pcg<NEN, nq_of<NEN>(), BSP, T, 2, MW, false, CEX, true>, n_items, BSP, stream,
+                                   "k_pcg", table, items, i & 1)
+                          : launch(k_pcg<NEN, nq_of<NEN>(), BSP, T, 2, MW, false, CEX>, n_items, BSP, stream,
+                                   "k_pcg", table, items, i & 1);
         if (rc) return rc;
         if (n_items > PCG_PRESUM_CHUNKS) {      // a solve of the launch may have more chunks: sum its partials
             const int grid = n_items < PCG_REDUCE_GRID ? n_items : PCG_REDUCE_GRID;

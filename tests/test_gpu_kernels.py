@@ -210,6 +210,31 @@ def test_apply_properties_at_the_largest_served_size():
     assert rel(A32(a.float()), Aa) < 1e-5
 
 
+@pytest.mark.parametrize("et,n,precision", [("quad4", 700, "fp64"), ("quad9", 512, "mixed")])
+def test_pcg_with_streamed_element_data_matches_textbook_pcg(et, n, precision):
+    """Launches whose element data exceed the Infinity Cache (700x700 quad4 FP64: 282 MB of moduli and gradients;
+    side-512 quad9 FP32: 264 MB) run the PCG kernel with nontemporal element loads.  No oracle at this size: after
+    exactly 60 iterations (the cap) the fused kernel's iterate equals the textbook PCG built from the hip product,
+    the system's Jacobi diagonal and torch vector operations -- an independent arrangement of the same method."""
+    from difffea_amd.newton_krylov import pcg
+    from difffea_amd.operators import make_system
+    p = unit_square(et, n)
+    u = stretch_state(p, 1.0 / n, seed=5)
+    th = torch.ones(p["conn"].shape[0], dtype=torch.float64)
+    s = make_system(u.to(DEV), th.to(DEV), p, precision)
+    m = s.mesh
+    elem_bytes = m.n_el * m.nq * (10 + 2 * m.nen) * (8 if precision == "fp64" else 4)
+    assert elem_bytes > 240 * (1 << 20)
+    s.moduli(u.to(DEV))
+    b = (torch.randn(u.shape[0], dtype=torch.float64, generator=torch.Generator().manual_seed(6)) * p["m"]).to(DEV)
+    x, k, st = s.pcg_solve(b, rtol=1e-14, max_iter=60)
+    assert (st, k) == (3, 60)
+    bb = b.to(s.vdt)
+    x_ref, k_ref = pcg(lambda w: s.apply(w), bb, s.precondition, rtol=1e-14, max_iter=60)
+    assert k_ref == 60
+    assert rel(x, x_ref) < (1e-10 if precision == "fp64" else 1e-3)
+
+
 @pytest.mark.parametrize("et,n", [("quad4", 37), ("quad9", 19), ("tri6", 13)])
 def test_precompute_geometry_matches_oracle(et, n):
     """dN_dX and w_detJ of the geometry kernel, read back in the original element order, against the
