@@ -44,6 +44,10 @@ constexpr int BS_Q4S = 256;       // product and PCG workgroups of small-chunk q
 // 64x256 O-grid)
 template <typename T>
 constexpr int bs_groups() { return std::is_same<T, double>::value ? 128 : 192; }
+// the PCG workgroup of quadratic-element meshes of at most PCG_EARLY_ITEMS chunks (DFEA_WIDE: one workgroup per CU
+// or fewer): more Gauss-point groups per element, so more waves share each CU's work
+template <typename T>
+constexpr int bs_groups_wide() { return std::is_same<T, double>::value ? 192 : 256; }
 
 // element types by node count: tri3 (3), quad4 (4), tri6 (6), quad8 (8), quad9 (9)
 template <int NEN>
@@ -1914,11 +1918,15 @@ int launch(K kern, int grid, int block, void* stream, const char* name, A... arg
 }
 
 // element kind of a launch: nen, with DFEA_HALO set for halo plans (DFEA_MATLIB selects the instantiations
-// with the whole material library, DFEA_TAIL marks tail-balanced linear partials plans; neither is part of the
-// kind)
+// with the whole material library, DFEA_TAIL marks tail-balanced linear partials plans, DFEA_WIDE the PCG
+// workgroups of small quadratic meshes; none of them is part of the kind)
 enum class Kind { T3, Q4, Q4H, Q4S, T6, Q8, Q9, Bad };
 Kind kind_of(int k) {
     k &= ~DFEA_MATLIB;
+    if (k & DFEA_WIDE) {
+        k &= ~DFEA_WIDE;
+        if (k != 6 && k != 8 && k != 9) return Kind::Bad;
+    }
     if (k & DFEA_TAIL) {
         k &= ~DFEA_TAIL;
         if (k != 3 && k != 4) return Kind::Bad;
@@ -1960,13 +1968,15 @@ int dispatch(int kind, F f) {
 
 // PCG kernels in pairs of parities 0, 1 (n_iter rounded up to even).  Quadratic-element chunks (64 elements,
 // up to 289 nodes) run with bs_groups threads: Gauss-point groups per element, node updates on all lanes.
-template <int NEN, int BS, typename T>
+// WIDE (DFEA_WIDE meshes of quadratic elements): workgroups of bs_groups_wide threads, whatever the launch size,
+// so a solve's arithmetic does not depend on the batch it runs in
+template <int NEN, int BS, typename T, bool WIDE = false>
 int pcg_iter(const dfea_desc* table, const int32_t* items, int n_items, int n_iter, void* stream) {
     // small-chunk quad4 plans (dispatched with the quadratic elements' chunk workgroup): 64-element chunks,
     // 4 threads per element (one Gauss point each)
     constexpr bool SMALL = NEN == 4 && BS == BS_QUAD;
     constexpr int CEX = SMALL ? CHUNK_QUAD : default_ce<NEN>();
-    constexpr int BSP = quadratic<NEN>() ? bs_groups<T>() : SMALL ? BS_Q4S : BS;
+    constexpr int BSP = quadratic<NEN>() ? (WIDE ? bs_groups_wide<T>() : bs_groups<T>()) : SMALL ? BS_Q4S : BS;
     const bool early = n_items <= PCG_EARLY_ITEMS;
     // element data beyond what the Infinity Cache holds next to the state vectors: streamed with nontemporal loads
     const bool nt = (double)n_items * CEX * nq_of<NEN>() * (10 + 2 * NEN) * sizeof(T) > PCG_NT_BYTES;
@@ -2071,7 +2081,12 @@ struct Launch_apply {
 
 struct Launch_pcg_iterate {
     const dfea_desc* t; const int32_t* it; int n; int k; int prec; void* st;
+    bool wide = false;
     template <int NEN, int BS, bool LIB> int run() {
+        if constexpr (quadratic<NEN>()) {
+            if (wide) return prec == DFEA_FP64 ? pcg_iter<NEN, BS, double, true>(t, it, n, k, st)
+                                               : pcg_iter<NEN, BS, float, true>(t, it, n, k, st);
+        }
         return prec == DFEA_FP64 ? pcg_iter<NEN, BS, double>(t, it, n, k, st)
                                  : pcg_iter<NEN, BS, float>(t, it, n, k, st);
     }
@@ -2277,7 +2292,9 @@ int dfea_pcg_iterate(const dfea_desc* table, const int32_t* items, int n_items, 
                      void* stream) {
     if (!table || n_iter < 0 || (precision != DFEA_FP64 && precision != DFEA_FP32))
         return fail(DFEA_EINVAL, "dfea_pcg_iterate: bad argument");
-    return dispatch(nen, Launch_pcg_iterate{table, items, n_items, n_iter, precision, stream});
+    Launch_pcg_iterate lp{table, items, n_items, n_iter, precision, stream};
+    lp.wide = (nen & DFEA_WIDE) != 0;
+    return dispatch(nen, lp);
 }
 
 int dfea_pcg_graph_create(const dfea_desc* table, const int32_t* items, int n_items, int n_iter, int nen,

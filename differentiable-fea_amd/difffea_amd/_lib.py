@@ -21,6 +21,8 @@ HALO = 0x100          # element-kind flag of halo plans
 SMALL = 0x400         # element-kind flag of small-chunk quad4 plans (64-element chunks)
 TAIL = 0x800          # element-kind flag of tail-balanced linear partials plans (chunks of two sizes)
 MATLIB = 0x200        # element-kind flag: material kernels with the whole material library
+WIDE = 0x1000         # element-kind flag: PCG workgroups of small quadratic-element meshes (more groups)
+PCG_WIDE_CHUNKS = 256  # quadratic meshes of at most this many chunks get DFEA_WIDE
 MAT_NP = 8            # dfea_desc.mat_p slots
 NOALLOC_SCRATCH_BYTES = 8 << 20   # DFEA_NOALLOC_SCRATCH_BYTES: static device scratch of dfea_apply_noalloc
 VEC = dict(x=0, r=1, w=2, p=3, s=4, b=5, u=6, R=7, uacc=8, uD=9, x2=10, r2=11, w2=12, p2=13, s2=14)

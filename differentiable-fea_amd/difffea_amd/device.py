@@ -73,8 +73,11 @@ class HipMesh:
         # element kind of every launch: nen, flagged for halo plans, small-chunk quad4 plans and tail-balanced plans
         small = p.mode == "partials" and self.nen == 4 and p.chunk_elems == self.lib.dfea_chunk_elems(4 | _lib.SMALL)
         tail = p.mode == "partials" and getattr(p, "tail", False)
+        # quadratic meshes of at most PCG_WIDE_CHUNKS chunks: PCG workgroups with more Gauss-point groups (a property
+        # of the mesh, not of the launch, so a solve gives the same bits alone and in a batch)
+        wide = p.mode == "partials" and self.nen > 4 and p.n_chunks <= _lib.PCG_WIDE_CHUNKS
         self.kind = self.nen | (_lib.HALO if p.mode == "halo" else 0) | (_lib.SMALL if small else 0) | \
-            (_lib.TAIL if tail else 0)
+            (_lib.TAIL if tail else 0) | (_lib.WIDE if wide else 0)
         inv_off, inv_slot = p.inverse_map() if p.mode == "partials" else (np.zeros(1, np.int32), np.zeros(1, np.uint16))
         if p.validate(self.lib.dfea_chunk_nodes(), (inv_off, inv_slot) if p.mode == "partials" else None) > \
                 self.lib.dfea_chunk_threads(self.kind):

@@ -12,8 +12,8 @@
  *   - Element kind (`nen` arguments): the nodes per element, 3 (tri3, 1 Gauss point), 4 (quad4, 2x2),
  *     6 (tri6, 6-point degree-4 rule), 8 (quad8, 3x3) or 9 (quad9, 3x3), with nq of the descriptors equal to
  *     that rule's size; 4 | DFEA_HALO for quad4 halo plans, 4 | DFEA_SMALL for small-chunk quad4 plans,
- *     3 | DFEA_TAIL or 4 | DFEA_TAIL for tail-balanced plans; | DFEA_MATLIB when a descriptor's material is
- *     not neo-Hookean.
+ *     3 | DFEA_TAIL or 4 | DFEA_TAIL for tail-balanced plans, 6, 8 or 9 | DFEA_WIDE for small quadratic-element
+ *     meshes (PCG workgroups); | DFEA_MATLIB when a descriptor's material is not neo-Hookean.
  *   - Work items: an int32 pair {solve index, block index} per workgroup; for a batch, the concatenation
  *     over solves.  Any item table may be NULL for one solve: solve 0 (the table's first descriptor),
  *     blocks 0 .. n_items-1, which saves the workgroups one dependent load.
@@ -51,6 +51,10 @@ extern "C" {
                                   (chunks of dfea_chunk_elems elements and, past whole rounds of 256 chunks,
                                   chunks of half that size): the product and PCG read each chunk's element
                                   range from its record */
+#define DFEA_WIDE 0x1000       /* element-kind flag: 6, 8 or 9 | DFEA_WIDE for quadratic-element meshes of at most 256
+                                  chunks (one workgroup per CU or fewer): the PCG kernel gives each element more
+                                  Gauss-point groups (192 threads per chunk in FP64, 256 in FP32, instead of 128 and
+                                  192), whatever the size of the launch the mesh's solve runs in */
 #define DFEA_MATLIB 0x200      /* element-kind flag: the descriptors may use any DFEA_MAT_* material; without it
                                   the material kernels compile only the neo-Hookean closed forms (fewer
                                   registers) and give NaN for a descriptor of another material */

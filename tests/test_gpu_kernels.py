@@ -210,6 +210,37 @@ def test_apply_properties_at_the_largest_served_size():
     assert rel(A32(a.float()), Aa) < 1e-5
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("et,n", [("quad9", 14), ("quad8", 16), ("tri6", 11)])
+def test_wide_pcg_workgroups_agree_with_oracle_and_default_groups(et, n, precision):
+    """Quadratic meshes of at most 256 chunks carry DFEA_WIDE: their PCG kernel gives each element more
+    Gauss-point groups (192 / 256 threads per chunk).  Against the oracle's PCG and against the same mesh with
+    the flag cleared (the default 128 / 192 threads): same iteration count within one, solutions within the
+    solve tolerance."""
+    from difffea_amd import _lib
+    from difffea_amd.device import HipMesh, HipSystem
+    from difffea_amd.materials import material_spec
+    p, u, v, th = _setup(et, n, seed=13, nonuniform=True, amp=0.01)
+    b = v * p["m"]
+    x_ref, k_ref = fem.pcg(fem.make_A_operator_qp(u, th, p), b, fem.make_jacobi_preconditioner(u, th, p),
+                           rtol=1e-10, max_iter=3000)
+    out = {}
+    for wide in (True, False):
+        m = HipMesh(p, DEV)
+        assert m.kind & _lib.WIDE
+        if not wide:
+            m.kind &= ~_lib.WIDE
+        s = HipSystem(m, th, material_spec(p), p["f"], precision)
+        s.moduli(u.to(DEV))
+        x, k, st = s.pcg_solve(b.to(DEV).to(s.vdt), rtol=1e-10 if precision == "fp64" else 1e-6, max_iter=3000)
+        assert st == _lib.PCG_CONVERGED
+        if precision == "fp64":
+            assert abs(k - k_ref) <= 1
+        assert rel(x, x_ref) < (1e-8 if precision == "fp64" else 1e-4)
+        out[wide] = (x.double().cpu(), k)
+    assert rel(out[True][0], out[False][0]) < (1e-9 if precision == "fp64" else 1e-4)
+
+
 @pytest.mark.parametrize("et,n,precision", [("quad4", 700, "fp64"), ("quad9", 512, "mixed")])
 def test_pcg_with_streamed_element_data_matches_textbook_pcg(et, n, precision):
     """Launches whose element data exceed the Infinity Cache (700x700 quad4 FP64: 282 MB of moduli and gradients;
