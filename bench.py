@@ -576,7 +576,8 @@ def main():
         "pcg_iteration_fp64": pcg64, "pcg_iteration_mixed": pcg32, "postprocess_q4_400": post,
         "newton_step_kernels_q4_400": steps,
         "element_products_fp64": elems,
-        "cpu_baseline": None if cpu is None else {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")},
+        "cpu_baseline": None if cpu is None else {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample",
+                                                                      "cpu_model", "os_cpu_count")},
         "solves": solves,
     }
     print(json.dumps(out))
