@@ -675,10 +675,12 @@ __device__ __forceinline__ void element_product(const ChunkView& ch, int t, bool
 
 // PF: Gauss points whose data is loaded before the gather barrier; MINW: minimum waves per SIMD the
 // register allocation must allow (1 = no constraint); STREAM_FIRST (partials plans): those PF Gauss points are
-// requested before the gather rather than behind it
+// requested before the gather rather than behind it; DIRECT (one solve): the chunk records come as the first
+// argument (preloaded in registers), so the record load does not wait for the descriptor load
 template <int NEN, int NQ, int BS, typename T, int PF = 2, int MINW = 1, bool AHEAD = false,
-          bool STREAM_FIRST = false, int CEX = default_ce<NEN>()>
-__global__ __launch_bounds__(BS, MINW) void k_apply(const dfea_desc* __restrict__ tab, const int* __restrict__ items,
+          bool STREAM_FIRST = false, int CEX = default_ce<NEN>(), bool DIRECT = false>
+__global__ __launch_bounds__(BS, MINW) void k_apply(const dfea_chunk* __restrict__ chunks_arg,
+                                                   const dfea_desc* __restrict__ tab, const int* __restrict__ items,
                                                    const T* __restrict__ vin, T* __restrict__ vout,
                                                    V2<T>* __restrict__ part_arg, int part_cap) {
     // partials plans (chunks of exactly CE elements) sum nodes through the inverse map, halo plans by colour
@@ -687,7 +689,7 @@ __global__ __launch_bounds__(BS, MINW) void k_apply(const dfea_desc* __restrict_
     __shared__ GroupLDS<NEN, BS, T, CEX> GL;
     const Item it = load_item(items);
     const dfea_desc& D = tab[it.s];
-    const ChunkView ch = chunk_view(D.chunks + it.b);
+    const ChunkView ch = chunk_view((DIRECT ? chunks_arg : D.chunks) + it.b);
     const int t = threadIdx.x;
     const T* v = vin ? vin : static_cast<const T*>(D.vec[DFEA_P]);
     T* y = vout ? vout : static_cast<T*>(D.vec[DFEA_W]);
@@ -833,6 +835,28 @@ __global__ __launch_bounds__(FBS) void k_apply_fin(const dfea_desc* __restrict__
         if (j1 > part_cap) return;
         const V2<T> acc = sum_slots<T>(part, j0, j1 - j0);
         const uint32_t g = D.sh_node[i];
+        const int n = (int)(g & DFEA_NODE_MASK);
+        const V2<T> raw = reinterpret_cast<const V2<T>*>(v)[n];
+        V2<T> out;
+        out.x = (g & DFEA_FIXED_X) ? pivot(acc.x, raw.x) : acc.x;
+        out.y = (g & DFEA_FIXED_Y) ? pivot(acc.y, raw.y) : acc.y;
+        reinterpret_cast<V2<T>*>(y)[n] = out;
+    }
+}
+
+// the same for one solve, with every pointer an argument (kernel arguments arrive preloaded in SGPRs): the node
+// tables' loads start at once instead of after the descriptor load; part holds at least sh_off[n_shared] entries
+template <typename T>
+__global__ __launch_bounds__(FBS) void k_apply_fin_direct(const int* __restrict__ sh_off,
+                                                         const uint32_t* __restrict__ sh_node,
+                                                         const V2<T>* __restrict__ part, const T* __restrict__ v,
+                                                         T* __restrict__ y, int n_shared) {
+    const int i = xcd_remap(blockIdx.x, gridDim.x) * FBS + threadIdx.x;
+    if (i < n_shared) {
+        const int j0 = sh_off[i];
+        const int j1 = sh_off[i + 1];
+        const V2<T> acc = sum_slots<T>(part, j0, j1 - j0);
+        const uint32_t g = sh_node[i];
         const int n = (int)(g & DFEA_NODE_MASK);
         const V2<T> raw = reinterpret_cast<const V2<T>*>(v)[n];
         V2<T> out;
@@ -2056,6 +2080,7 @@ struct Launch_apply {
     void* part = nullptr;        // shared-node partials buffer (nullptr: the descriptor's part)
     int part_cap = 0x7fffffff;   // its capacity in entries
     bool tail = false;           // tail-balanced plans: the gather-first product at any launch size
+    const dfea_desc* host = nullptr;   // one solve: the host copy of its descriptor (finish pointers as arguments)
     template <typename T, int NEN, int BS0> int go() {
         constexpr bool SMALL = NEN == 4 && BS0 == BS_QUAD;     // small-chunk quad4 plans (see pcg_iter)
         constexpr int CEX = SMALL ? CHUNK_QUAD : default_ce<NEN>();
@@ -2067,11 +2092,20 @@ struct Launch_apply {
         // the gather instead
         const bool ahead = NEN <= 4 && !SMALL && (n <= APPLY_AHEAD_ITEMS || tail);
         V2<T>* pp = static_cast<V2<T>*>(part);
-        int rc = ahead ? launch(k_apply<NEN, nq_of<NEN>(), BS, T, 2, 1, NEN <= 4 && !SMALL, false, CEX>, n, BS, st,
-                                "k_apply", t, it, vv, yy, pp, part_cap)
-                       : launch(k_apply<NEN, nq_of<NEN>(), BS, T, 2, 1, false, NEN == 4 && !SMALL, CEX>, n, BS, st,
-                                "k_apply", t, it, vv, yy, pp, part_cap);
+        constexpr bool AH = NEN <= 4 && !SMALL, SF = NEN == 4 && !SMALL;
+        const dfea_chunk* hc = host ? host->chunks : nullptr;
+        int rc = host ? (ahead ? launch(k_apply<NEN, nq_of<NEN>(), BS, T, 2, 1, AH, false, CEX, true>, n, BS, st,
+                                        "k_apply", hc, t, it, vv, yy, pp, part_cap)
+                               : launch(k_apply<NEN, nq_of<NEN>(), BS, T, 2, 1, false, SF, CEX, true>, n, BS, st,
+                                        "k_apply", hc, t, it, vv, yy, pp, part_cap))
+                      : (ahead ? launch(k_apply<NEN, nq_of<NEN>(), BS, T, 2, 1, AH, false, CEX>, n, BS, st,
+                                        "k_apply", hc, t, it, vv, yy, pp, part_cap)
+                               : launch(k_apply<NEN, nq_of<NEN>(), BS, T, 2, 1, false, SF, CEX>, n, BS, st,
+                                        "k_apply", hc, t, it, vv, yy, pp, part_cap));
         if (rc) return rc;
+        if (host)
+            return launch(k_apply_fin_direct<T>, nf, FBS, st, "k_apply_fin", host->sh_off, host->sh_node,
+                          (const V2<T>*)(pp ? (const void*)pp : (const void*)host->part), vv, yy, host->n_shared);
         return launch(k_apply_fin<T>, nf, FBS, st, "k_apply_fin", t, fi, vv, yy, (const V2<T>*)pp, part_cap);
     }
     template <int NEN, int BS, bool LIB> int run() {
@@ -2222,6 +2256,17 @@ int dfea_apply(const dfea_desc* table, const int32_t* items, int n_items, const 
         return fail(DFEA_EINVAL, "dfea_apply: bad argument");
     Launch_apply la{table, items, n_items, fin_items, n_fin_items, v, y, precision, stream};
     la.tail = (nen & DFEA_TAIL) != 0;
+    return dispatch(nen, la);
+}
+
+int dfea_apply_host(const dfea_desc* host_desc, const dfea_desc* table, int n_chunks, int n_fin_blocks, const void* v,
+                    void* y, int nen, int precision, void* stream) {
+    if (!host_desc || !table || !v || !y || n_chunks < 0 || (precision != DFEA_FP64 && precision != DFEA_FP32) ||
+        n_fin_blocks != (host_desc->n_shared + FBS - 1) / FBS || n_chunks != host_desc->n_chunks)
+        return fail(DFEA_EINVAL, "dfea_apply_host: bad argument");
+    Launch_apply la{table, nullptr, n_chunks, nullptr, n_fin_blocks, v, y, precision, stream};
+    la.tail = (nen & DFEA_TAIL) != 0;
+    la.host = host_desc;
     return dispatch(nen, la);
 }
 

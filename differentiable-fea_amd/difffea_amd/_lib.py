@@ -81,6 +81,7 @@ _SIGNATURES = {
     "dfea_axpy": (_I, [_P, _P, _I, _I, _P, _P, _P, _P, _P, _I, _P]),
     "dfea_pcg_init": (_I, [_P, _P, _I, _P, _I, _I, _I, _P]),
     "dfea_pcg_start": (_I, [_P, _I, _D, _I, _I, _I, _P]),
+    "dfea_apply_host": (_I, [_P, _P, _I, _I, _P, _P, _I, _I, _P]),
     "dfea_pcg_iterate": (_I, [_P, _P, _I, _I, _I, _I, _P]),
     "dfea_adjoint_grad": (_I, [_P, _P, _I, _P, _P, _P, _I, _P]),
     "dfea_pcg_graph_create": (_I, [_P, _P, _I, _I, _I, _I, ctypes.POINTER(_P)]),

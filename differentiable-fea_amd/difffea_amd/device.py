@@ -275,9 +275,9 @@ class HipSystem:
         if v.dtype != self.vdt or not v.is_contiguous():
             v = v.to(self.vdt).contiguous()
         out = torch.empty_like(v) if out is None else out
-        _lib.check(self.lib.dfea_apply(_ptr(self.table), _ptr(m.items), m.plan.n_chunks, _ptr(m.fin_items),
-                                       m.n_finblk, _ptr(v), _ptr(out), self.kind, self.prec_id, _stream()),
-                   "dfea_apply")
+        # one solve: the host copy of the descriptor lets the shared-node finish take its pointers as arguments
+        _lib.check(self.lib.dfea_apply_host(ctypes.byref(self.desc), _ptr(self.table), m.plan.n_chunks, m.n_finblk,
+                                            _ptr(v), _ptr(out), self.kind, self.prec_id, _stream()), "dfea_apply_host")
         return out
 
     def dot(self, x, y=None):

@@ -224,6 +224,14 @@ int dfea_fields(const dfea_desc* table, const int32_t* items, int n_items, const
 int dfea_apply(const dfea_desc* table, const int32_t* items, int n_items, const int32_t* fin_items,
                int n_fin_items, const void* v, void* y, int nen, int precision, void* stream);
 
+/* The product of one solve (the table's first descriptor) given also the caller's host copy of that descriptor:
+ * the shared-node finish takes its node tables and partials as kernel arguments (they arrive preloaded in
+ * registers), so its loads do not wait for the descriptor.  host_desc must equal the table's first descriptor;
+ * v and y are required.  Same result, bit for bit, as dfea_apply.  Replaces the closures of
+ * `make_A_operator_qp` / `_qp_ew` (`src/difffea/operators.py:345-412`) for a single solve. */
+int dfea_apply_host(const dfea_desc* host_desc, const dfea_desc* table, int n_chunks, int n_fin_blocks, const void* v,
+                    void* y, int nen, int precision, void* stream);
+
 /* The same product for ONE solve without any device allocation (no counterpart in the reference: it is for
  * callers that cannot allocate -- code under HIP graph capture, tight serving loops, harnesses that forbid
  * hipMalloc).  The shared-node partials that dfea_apply keeps in the descriptor's part buffer go instead to

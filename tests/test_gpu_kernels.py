@@ -210,6 +210,33 @@ def test_apply_properties_at_the_largest_served_size():
     assert rel(A32(a.float()), Aa) < 1e-5
 
 
+@pytest.mark.parametrize("et,n,mode", [("quad4", 400, "auto"), ("quad9", 64, "auto"), ("tri3", 90, "auto"),
+                                       ("quad4", 37, "halo")])
+def test_apply_host_equals_apply_bitwise(et, n, mode):
+    """dfea_apply_host (the single-solve product whose shared-node finish takes its pointers as kernel arguments)
+    gives the same bits as dfea_apply, in both precisions, on partials (tail-balanced included) and halo plans."""
+    import ctypes
+    from difffea_amd import _lib
+    from difffea_amd.device import HipMesh, HipSystem
+    from difffea_amd.materials import material_spec
+    p = unit_square(et, n)
+    u = stretch_state(p, 1.0 / n, seed=9)
+    th = torch.ones(p["conn"].shape[0], dtype=torch.float64)
+    m = HipMesh(p, DEV, plan_mode=mode)
+    assert m.plan.mode == ("halo" if mode == "halo" else "partials")
+    for precision in ("fp64", "mixed"):
+        s = HipSystem(m, th, material_spec(p), p["f"], precision)
+        s.moduli(u.to(DEV))
+        v = torch.randn(u.shape[0], dtype=torch.float64, generator=torch.Generator().manual_seed(2)).to(DEV).to(s.vdt)
+        y1, y2 = torch.empty_like(v), torch.full_like(v, float("nan"))
+        _lib.check(s.lib.dfea_apply(ctypes.c_void_p(s.table.data_ptr()), None, m.plan.n_chunks, None, m.n_finblk,
+                                    ctypes.c_void_p(v.data_ptr()), ctypes.c_void_p(y1.data_ptr()), s.kind, s.prec_id,
+                                    ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)), "dfea_apply")
+        s.apply(v, y2)
+        torch.cuda.synchronize()
+        assert torch.equal(y1, y2)
+
+
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 @pytest.mark.parametrize("et,n", [("quad9", 14), ("quad8", 16), ("tri6", 11)])
 def test_wide_pcg_workgroups_agree_with_oracle_and_default_groups(et, n, precision):
