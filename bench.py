@@ -395,13 +395,15 @@ def served_comparison(n_requests, side=(32, 512), seed=5, preconditioner="jacobi
                     p99_latency_s=round(float(np.percentile(lat, 99)), 4), **kw)
 
     # plain FP64 solver, one request after another (mesh upload + geometry and the solve timed)
-    lat, plain_u = [], {}
+    lat, plain_u, plain_cg, plain_not_run = [], {}, 0, 0
     t0 = time.perf_counter()
     for r in reqs:
         t1 = time.perf_counter()
         p, th, plan = prepared[r.rid]
         solver = HipNewton(p, th.to(dev), mesh=HipMesh(p, dev, plan=plan), preconditioner=preconditioner)
         u, info = solver.solve(r.n_load_steps, 1e-8 * reference_residual_norm(solver.sys, p, dev), cg_max_iter=20000)
+        plain_cg += info["total_cg"]
+        plain_not_run += info["pcg_not_run"]
         if r.adjoint:
             lam, _ = solve_adjoint(solver, u, p["f"].to(dev), rtol=1e-8, max_iter=20000)
             compute_implicit_vjp(solver, u, lam)
@@ -413,7 +415,7 @@ def served_comparison(n_requests, side=(32, 512), seed=5, preconditioner="jacobi
             print(f"# plain FP64 solver: {len(lat)} of {len(reqs)} requests, {time.perf_counter() - t0:.1f} s",
                   file=sys.stderr, flush=True)
     plain = summary(lat, time.perf_counter() - t0, len(reqs), latency_from="start of the request's solve",
-                    mode="HipNewton, one request after another")
+                    mode="HipNewton, one request after another", n_cg=plain_cg, n_pcg_not_run=plain_not_run)
     service = list(lat)
 
     def serve(precision, realtime, rate=None):
@@ -424,6 +426,7 @@ def served_comparison(n_requests, side=(32, 512), seed=5, preconditioner="jacobi
                                                          flush=True) if d % 16 == 0 else None)
         summ["n_newton"] = int(sum(x["total_newton"] for x in recs))
         summ["n_cg"] = int(sum(x["total_cg"] for x in recs))
+        summ["n_pcg_not_run"] = int(sum(x["pcg_not_run"] for x in recs))
         return {x["rid"]: x for x in recs}, summ
 
     r64, sat64 = serve("fp64", False)
@@ -437,7 +440,7 @@ def served_comparison(n_requests, side=(32, 512), seed=5, preconditioner="jacobi
     arrivals = [r.arrival for r in make_stream(n_requests, rate=rate, seed=seed, side=side)]
     q = fifo_latencies(arrivals, service)
     keys = ("wall_s", "solves_per_s", "p50_latency_s", "p99_latency_s", "latency_from", "converged", "n_newton",
-            "n_cg", "peak_admitted")
+            "n_cg", "n_pcg_not_run", "peak_admitted")
     return {
         "workload": f"{len(reqs)} requests, sides log-uniform in [{side[0]}, {side[1]}] (Q4 and Q9), 70 % Q4, "
                     f"20 % with the compliance gradient, seed {seed}, 1e-8 relative residual",

@@ -47,6 +47,20 @@ def first_round(it, counts, prev_counts):
     return int(est) + 2
 
 
+def nan_pcg_result(x, max_iter):
+    """The PCG's result for a right-hand side with a NaN entry, written into x without running the kernels.
+
+    A NaN residual comes from an inverted element at the Newton state (log J with det F <= 0,
+    `constitutive_relations.py:46`).  In the reference's PCG (`newton_krylov.py:28-88`) r.z, the curvature and
+    the residual norm are then NaN, so the zero right-hand side exit, the curvature exit and the convergence
+    test are all False, and x + alpha p with alpha = NaN is NaN in every entry: it returns that x after
+    max_iter iterations (x = 0 when max_iter is 0).  The fused kernels reach the same x and count (NaN
+    propagates through every product and sum); the line search then rejects every NaN trial and the load step
+    is cut back.  Returns (x, max_iter)."""
+    x.fill_(float("nan") if max_iter > 0 else 0.0)
+    return x, max_iter
+
+
 def pcg(A_fn, b, Minv_fn, rtol=1e-5, max_iter=500):
     """Textbook Jacobi-PCG with x0 = 0 (returns (x, n_iters))."""
     x = torch.zeros_like(b)
@@ -95,6 +109,8 @@ class HipNewton:
         # the line search's step lengths 2^-j on the device (a trial reads its alpha there, no fill per trial)
         self._alphas = torch.tensor([2.0 ** -j for j in range(_N_ALPHAS)], dtype=torch.float64, device=self.device)
         self._moduli_at = None          # the state the system's moduli were last computed at (see armijo)
+        self._rhs_nan = False           # the current Newton residual has a NaN entry (see nan_pcg_result)
+        self.pcg_not_run = 0            # PCG solves whose result nan_pcg_result wrote
         self.pcg_chunk = 16
         self._step_counts = []          # CG iterations of the Newton iterations of the last load step solved
 
@@ -109,6 +125,8 @@ class HipNewton:
         """PCG on J du = -R with R in sys.vec['R']; returns (du, iterations).  first: kernels of the first
         round of PCG kernels (``first_round``)."""
         s = self.sys
+        if self._rhs_nan:
+            return nan_pcg_result(s.vec["x"], max_iter)
         if s.precision == "mixed":
             rtol = max(rtol, MIXED_RTOL_FLOOR)
         s.pcg_start(rtol=rtol, max_iter=max_iter, neg_res=True)
@@ -163,7 +181,10 @@ class HipNewton:
             if it == max_newton:
                 break
             eta = min(0.5, math.sqrt(rn / (r0 + 1e-14))) if cg_rtol is None else cg_rtol
-            if self._moduli_at is not u:                # else computed after the line search that accepted u
+            self._rhs_nan = math.isnan(rn)
+            if self._rhs_nan:
+                self.pcg_not_run += 1                   # no tangent needed: the PCG's result is known
+            elif self._moduli_at is not u:              # else computed after the line search that accepted u
                 self.sys.moduli(u)
             self._moduli_at = None
             du, k = self.linear_solve(eta, cg_max_iter, first_round(it, counts, prev))
@@ -186,7 +207,8 @@ class HipNewton:
         n = self.mesh.n_dof
         u = torch.zeros(n, dtype=torch.float64, device=self.device)
         info = {"converged": False, "load_factor": 0.0, "history": [], "total_newton": 0, "total_cg": 0,
-                "n_cutbacks": 0, "snapshots": [(0.0, u.clone())], "precision": self.sys.precision}
+                "n_cutbacks": 0, "snapshots": [(0.0, u.clone())], "precision": self.sys.precision,
+                "pcg_not_run": 0}       # Newton iterations with a NaN residual: PCG result written, not computed
         d_nom = 1.0 / n_load_steps
         d = d_nom
         lf = 0.0
@@ -200,6 +222,7 @@ class HipNewton:
             info["history"] += recs
             info["total_newton"] += len(recs)
             info["total_cg"] += sum(r["cg_iters"] for r in recs)
+            info["pcg_not_run"] = self.pcg_not_run
             if ok:
                 u, lf = u_try.clone(), lf_try
                 info["snapshots"].append((lf, u.clone()))

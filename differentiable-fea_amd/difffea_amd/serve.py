@@ -33,7 +33,7 @@ from .materials import material_spec
 from .plan import MeshPlan, chunking_for
 from .problem import build_problem
 from .adjoint import MAX_REFINE
-from .newton_krylov import MIXED_RTOL_FLOOR
+from .newton_krylov import MIXED_RTOL_FLOOR, nan_pcg_result
 
 HBM_SHARE = 0.8          # share of the free device memory the server admits requests into by default
 
@@ -186,14 +186,15 @@ def solve_ops(job, tol, max_newton=25, max_cutbacks=8, cg_max_iter=500, adjoint_
 
     yields ("lift", lf) -> None, ("resnorm", lf) -> ||R||, ("moduli",) -> None,
     ("pcg", rtol, max_iter, neg_res) -> iterations, ("trial", alpha) -> None, ("accept",) -> None,
-    ("restore",) -> None, ("adjoint",) -> iterations (PCG on the adjoint right-hand side or, in mixed precision,
+    ("restore",) -> None, ("nan_pcg", max_iter) -> iterations (a NaN residual: the PCG's known result,
+    ``nan_pcg_result``), ("adjoint",) -> iterations (PCG on the adjoint right-hand side or, in mixed precision,
     on the refinement residual), ("adjoint_residual",) -> ||m f - A lam|| / ||m f|| (FP64 tangent),
     ("adjoint_grad",) -> None.  Returns the reference's ``info`` entries (`newton_krylov.py:226-264`: converged,
     load_factor, history of per-iteration records, total_newton, total_cg, n_cutbacks) plus the last
     residual norm.
     """
     info = {"total_newton": 0, "total_cg": 0, "n_cutbacks": 0, "converged": False, "residual": float("nan"),
-            "load_factor": 0.0, "history": []}
+            "load_factor": 0.0, "history": [], "pcg_not_run": 0}
     mixed = job.sys.precision == "mixed"
     d_nom = 1.0 / job.req.n_load_steps
     d, lf, in_row, step = d_nom, 0.0, 0, 0
@@ -215,8 +216,12 @@ def solve_ops(job, tol, max_newton=25, max_cutbacks=8, cg_max_iter=500, adjoint_
             eta = min(0.5, math.sqrt(rn / (r0 + 1e-14)))
             if mixed:
                 eta = max(eta, MIXED_RTOL_FLOOR)
-            yield ("moduli",)
-            k = yield ("pcg", eta, cg_max_iter, True)
+            if math.isnan(rn):
+                k = yield ("nan_pcg", cg_max_iter)
+                info["pcg_not_run"] += 1
+            else:
+                yield ("moduli",)
+                k = yield ("pcg", eta, cg_max_iter, True)
             info["total_newton"] += 1
             info["total_cg"] += k
             alpha, accepted = 1.0, False
@@ -450,6 +455,8 @@ class Server:
         # restore: u_acc = saved converged state (rare, per solve)
         for j in by.get("restore", []):
             j.sys.vec["uacc"].copy_(j.saved)
+        for j in by.get("nan_pcg", []):
+            j.nan_iters = nan_pcg_result(j.sys.vec["x"], j.op[1])[1]
         # residual + norm (lf per solve in sc[LF])
         js = by.get("resnorm", [])
         if js:
@@ -555,6 +562,8 @@ class Server:
                     continue
                 j.pcg_running = False
                 val = int(sc[j.slot, _lib.SC["iters"]])
+            elif kind == "nan_pcg":
+                val = j.nan_iters
             elif kind == "adjoint_residual":
                 val = j.adj_rel
             elif kind == "resnorm":

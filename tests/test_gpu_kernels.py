@@ -342,7 +342,7 @@ def test_inverted_elements_give_the_reference_nans(prec):
                        rtol=0.5, max_iter=30)
     x, k, st = s.pcg_solve((-Rref).to(DEV).to(s.vdt), rtol=0.5, max_iter=30)
     assert (st, k, k_ref) == (3, 30, 30)
-    assert torch.isnan(x).any()
+    assert torch.isnan(x).all()           # the result `newton_krylov.nan_pcg_result` writes without a run
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])

@@ -60,6 +60,41 @@ def test_solve_matches_reference_golden(name):
     check_trajectory(history_array(info), z)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_nan_residual_pcg_result_equals_the_kernels(precision):
+    """Where a Newton residual is NaN (inverted elements after the lift) the PCG's result is written without
+    running it (``nan_pcg_result``: every entry NaN, max_iter iterations).  Running the fused PCG kernels on
+    that right-hand side instead must give the same trajectory, the same cut-backs and the same bits."""
+    from difffea_amd.device import HipMesh
+    from difffea_amd.newton_krylov import HipNewton
+
+    class RunPcg(HipNewton):
+        def residual(self, u, lf, out=None):
+            self.at_u = u                    # the last residual evaluated is the Newton state's
+            return super().residual(u, lf, out)
+
+        def linear_solve(self, rtol, max_iter, first=None):
+            if self._rhs_nan:
+                self._rhs_nan = False
+                self.sys.moduli(self.at_u)
+                x, k = super().linear_solve(rtol, max_iter, first)
+                assert torch.isnan(x).all() and k == max_iter
+                return x, k
+            return super().linear_solve(rtol, max_iter, first)
+
+    z, spec = load_golden("g2_q4_nan_cutback_4")
+    p = golden_problem(spec)
+    th = torch.ones(p["conn"].shape[0], dtype=torch.float64, device=DEV)
+    out = []
+    for cls in (HipNewton, RunPcg):
+        s = cls(p, th, precision=precision, mesh=HipMesh(p, DEV))
+        out.append(s.solve(int(z["n_load_steps"]), float(z["tol"]), max_newton=golden_max_newton(z)))
+    (u0, i0), (u1, i1) = out
+    assert i0["pcg_not_run"] >= 1 and i0["n_cutbacks"] >= 1 and i0["converged"]
+    assert np.array_equal(history_array(i0), history_array(i1), equal_nan=True)
+    assert torch.equal(u0, u1)
+
+
 @pytest.mark.parametrize("et", ["quad4", "quad9"])
 def test_mixed_precision_solve_reaches_fp64_tolerance(et):
     p = fem.precompute_geometry(cantilever(et, h=0.25, load=(0.0, -20.0)))
