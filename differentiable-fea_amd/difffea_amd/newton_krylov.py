@@ -11,7 +11,7 @@
 * ``precision="mixed"``: moduli, A(v) and the inner CG vectors in FP32 (dot products accumulated in FP64),
   while the Newton residual, the state u and the line search stay FP64 -- Newton itself is the outer
   refinement loop, so the converged residual meets the same FP64 tolerance.  The inner relative tolerance
-  is floored at ``MIXED_RTOL_FLOOR`` (FP32 CG stagnates near eps32 times the condition number).
+  is floored at ``MIXED_RTOL_FLOOR``.
 """
 import ctypes
 import math
@@ -23,11 +23,16 @@ from . import _lib
 from .operators import PRECONDITIONERS, PRECISIONS, TANGENT_OPERATORS, get_mesh, _check_material
 from .device import HipSystem
 
-# FP32 CG stagnates near eps32 * cond(A) relative residual (about 1e-4 .. 1e-3 on the meshes here): inner solves
-# are not asked for less.  Measured on MI355X, mixed solves with floors 1e-6 / 1e-5 / 1e-4 / 1e-3: 100x100 Q4
-# 41.5 / 41.5 / 40.6 / 39.2 ms, O-grid 64x256 Q9 0.634 / 0.630 / 0.592 / 0.578 s, 256x256 Q4 adjoint sensitivity
-# 0.307 / 0.296 / 0.290 / 0.279 s; Newton still meets the FP64 tolerance (it is the outer refinement)
-MIXED_RTOL_FLOOR = 1e-3
+# Inner tolerance floor of the mixed-precision Newton steps.  On the same state and right-hand side the FP32 PCG
+# needs the FP64 iteration count (100x100 Q4 and 200x200 Q9 Newton states, rtol 1e-3 and 1e-4: equal within a few
+# iterations), but the rounding of its FP32 recurrences leaves a rough component in the next Newton residual that
+# tight inner solves then chase (100x100 Q4, second load step: 383 and 422 iterations at rtol 1e-3 where the FP64
+# path needs 252 and 316 at rtol 1e-3 / 3e-5; the same solves with FP64 vectors and FP32 moduli need 252 and 352).
+# Looser inner solves cost a Newton iteration more and save the tail.  Measured on MI355X, floors 1e-3 / 3e-3 /
+# 1e-2 / 3e-2: 100x100 Q4 28.1 / 27.0 / 25.6 / 27.0 ms, O-grid 64x256 Q9 0.591 / 0.567 / 0.507 / 0.597 s, 200x200
+# Q9 stretch 0.478 / 0.443 / 0.428 / 0.411 s; the 64-request stream one request after another 26.2 s (1e-3) and
+# 25.1 s (1e-2).  Newton still meets the FP64 tolerance (it is the outer refinement).
+MIXED_RTOL_FLOOR = 1e-2
 _N_ALPHAS = 16          # line-search step lengths 2^0 .. 2^-15 kept on the device
 
 
