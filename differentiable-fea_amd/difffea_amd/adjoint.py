@@ -17,14 +17,10 @@ import torch
 from torch.func import grad
 
 from .device import HipSystem
-from .newton_krylov import HipNewton
+from .newton_krylov import HipNewton, MIXED_RTOL_FLOOR
 
 
 MAX_REFINE = 12       # refinement steps of the mixed-precision adjoint, at most
-# relative tolerance of each FP32 correction solve, at least: every refinement step reduces the error by about
-# this factor, so looser corrections need more FP64 residuals (256x256 Q4 adjoint: 6,483 inner iterations and
-# 0.058 s with 1e-3, 8,087 and 0.077 s with 1e-2)
-REFINE_RTOL_FLOOR = 1e-3
 
 
 def solve_adjoint(solver, u_star, dL_du, rtol=1e-6, max_iter=500):
@@ -57,7 +53,7 @@ def refine(s, s64, b, rtol, max_iter):
     its = 0
     for _ in range(MAX_REFINE):
         s.vec["b"].copy_(r.to(s.vdt))
-        s.pcg_start(rtol=max(rtol, REFINE_RTOL_FLOOR), max_iter=max_iter, neg_res=False)
+        s.pcg_start(rtol=max(rtol, MIXED_RTOL_FLOOR), max_iter=max_iter, neg_res=False)
         k, _ = s.pcg_run(max_iter)
         its += k
         lam += s.vec["x"].to(torch.float64)

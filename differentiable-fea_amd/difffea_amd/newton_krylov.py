@@ -31,7 +31,10 @@ from .device import HipSystem
 # Looser inner solves cost a Newton iteration more and save the tail.  Measured on MI355X, floors 1e-3 / 3e-3 /
 # 1e-2 / 3e-2: 100x100 Q4 28.1 / 27.0 / 25.6 / 27.0 ms, O-grid 64x256 Q9 0.591 / 0.567 / 0.507 / 0.597 s, 200x200
 # Q9 stretch 0.478 / 0.443 / 0.428 / 0.411 s; the 64-request stream one request after another 26.2 s (1e-3) and
-# 25.1 s (1e-2).  Newton still meets the FP64 tolerance (it is the outer refinement).
+# 25.1 s (1e-2).  Newton still meets the FP64 tolerance (it is the outer refinement).  The mixed adjoint's
+# refinement corrections use the same floor: one reduces the FP64 residual by about 1e-2 whatever its own
+# tolerance (256x256 Q4 adjoint, floors 1e-4 / 1e-3 / 3e-3 / 1e-2: 4 / 5 / 5 / 6 corrections, 0.067 / 0.073 /
+# 0.060 / 0.060 s).
 MIXED_RTOL_FLOOR = 1e-2
 _N_ALPHAS = 16          # line-search step lengths 2^0 .. 2^-15 kept on the device
 

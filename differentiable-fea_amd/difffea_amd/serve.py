@@ -32,7 +32,7 @@ from .elements import FAMILY, NEN, ORDER, default_nq, element_type_of
 from .materials import material_spec
 from .plan import MeshPlan, chunking_for
 from .problem import build_problem
-from .adjoint import MAX_REFINE, REFINE_RTOL_FLOOR
+from .adjoint import MAX_REFINE
 from .newton_krylov import MIXED_RTOL_FLOOR, nan_pcg_result
 
 HBM_SHARE = 0.8          # share of the free device memory the server admits requests into by default
@@ -487,7 +487,7 @@ class Server:
         for j in new:
             j.pcg_rounds = 0
         if new:
-            adj_rtol = max(self.adjoint_rtol, REFINE_RTOL_FLOOR) if self.precision == "mixed" else self.adjoint_rtol
+            adj_rtol = max(self.adjoint_rtol, MIXED_RTOL_FLOOR) if self.precision == "mixed" else self.adjoint_rtol
             slots = [j.slot for j in new]
             self._set_sc(slots, "rtol", [float(j.op[1]) if j.op[0] == "pcg" else adj_rtol for j in new])
             self._set_sc(slots, "maxit", [float(j.op[2]) if j.op[0] == "pcg" else float(self.cg_max_iter)
