@@ -18,7 +18,7 @@ How each function is pinned (tests/test_oracle_*.py):
                               C = mu d_ik d_jl + lam G_ij G_kl + (mu - lam lnJ) G_il G_kj, linear-elastic
                               tensor at F = I, objectivity, stress-free reference state
   * global_residual        -- rigid-body invariance, R_int = grad(total energy), finite differences
-  * qp / qp_ew / jvp A(v)  -- mutual agreement < 1e-11, symmetry, central finite differences of R
+  * qp / qp_ew / jvp / linearize A(v) -- mutual agreement < 1e-11, symmetry, finite differences of R
   * jacobi diagonal        -- diagonal of the dense operator
   * pcg                    -- dense solve
   * newton_krylov_solve    -- equilibrium, reactions balance the applied load, golden trajectory (G2)
@@ -29,7 +29,7 @@ How each function is pinned (tests/test_oracle_*.py):
                               and = the fixed-dof sum of grad(total energy)
 """
 import torch
-from torch.func import grad, hessian, jvp, vjp, vmap
+from torch.func import grad, hessian, jvp, linearize, vjp, vmap
 
 DT = torch.float64
 
@@ -280,6 +280,18 @@ def make_A_operator(u, theta, params):
     return A
 
 
+def make_A_operator_linearize(u, theta, params):
+    """'linearize' (operators.py:299-313): the residual is linearised once at u (torch.func.linearize keeps the
+    primal evaluation), and each product pushes m v through the stored linearisation; the same operator as
+    'jvp'."""
+    m = params["m"]
+    _, lin = linearize(lambda w: global_residual(w, theta, params), u)
+
+    def A(v):
+        return m * lin(m * v) + (1.0 - m) * v
+    return A
+
+
 def qp_setup(u, theta, params):
     """C[e,q,2i+j,2k+l] = w_detJ[e,q] * d2psi/dF_ij dF_kl (hessian by AD), and dN_dX."""
     conn = params["conn"]
@@ -338,7 +350,7 @@ def make_A_operator_qp_ew(u, theta, params):
     return A
 
 
-ORACLE_TANGENTS = ("jvp", "qp", "qp_ew")
+ORACLE_TANGENTS = ("jvp", "linearize", "qp", "qp_ew")
 
 
 def make_tangent_operator(kind, u, theta, params):
@@ -348,6 +360,8 @@ def make_tangent_operator(kind, u, theta, params):
         return make_A_operator_qp_ew(u, theta, params)
     if kind == "jvp":
         return make_A_operator(u, theta, params)
+    if kind == "linearize":
+        return make_A_operator_linearize(u, theta, params)
     raise ValueError(f"Unknown tangent operator '{kind}'. Choose one of {ORACLE_TANGENTS}.")
 
 

@@ -33,6 +33,7 @@ def test_apply_fp64_matches_oracle(et, n):
     y = A(v.to(DEV))
     assert rel(y, ref_qp) < 1e-11
     assert rel(y, ref_jvp) < 1e-11
+    assert rel(y, fem.make_A_operator_linearize(u, th, p)(v)) < 1e-11
 
 
 @pytest.mark.parametrize("et,n", CASES)

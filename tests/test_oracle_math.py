@@ -210,7 +210,7 @@ def test_tangent_variants_agree_and_match_finite_differences(et):
     th = 0.5 + torch.rand(p["conn"].shape[0], dtype=DTYPE, generator=torch.Generator().manual_seed(4))
     u, v = state(p, 0)
     ref = fem.make_A_operator(u, th, p)(v)
-    for k in ("qp", "qp_ew"):
+    for k in ("qp", "qp_ew", "linearize"):
         y = fem.make_tangent_operator(k, u, th, p)(v)
         assert float((y - ref).abs().max() / ref.abs().max()) < 1e-12
     AT = fem.make_AT_operator(u, th, p)(v)
