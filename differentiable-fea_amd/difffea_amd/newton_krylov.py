@@ -25,9 +25,11 @@ from .device import HipSystem
 
 # Inner tolerance floor of the mixed-precision Newton steps.  On the same state and right-hand side the FP32 PCG
 # needs the FP64 iteration count (100x100 Q4 and 200x200 Q9 Newton states, rtol 1e-3 and 1e-4: equal within a few
-# iterations), but the rounding of its FP32 recurrences leaves a rough component in the next Newton residual that
-# tight inner solves then chase (100x100 Q4, second load step: 383 and 422 iterations at rtol 1e-3 where the FP64
-# path needs 252 and 316 at rtol 1e-3 / 3e-5; the same solves with FP64 vectors and FP32 moduli need 252 and 352).
+# iterations), but the FP32 rounding of its iterates leaves a true residual against the FP64 tangent that no
+# inner tolerance removes (smooth right-hand side: about 9e-3 at 100x100 Q4, 8e-2 at 256x256, 0.43 at 512x512),
+# so each Newton step is accurate to that floor and tight inner solves only chase it (100x100 Q4, second load
+# step: 383 and 422 iterations at rtol 1e-3 where the FP64 path needs 252 and 316 at rtol 1e-3 / 3e-5; the same
+# solves with FP64 vectors and FP32 moduli need 252 and 352).
 # Looser inner solves cost a Newton iteration more and save the tail.  Measured on MI355X, floors 1e-3 / 3e-3 /
 # 1e-2 / 3e-2: 100x100 Q4 28.1 / 27.0 / 25.6 / 27.0 ms, O-grid 64x256 Q9 0.591 / 0.567 / 0.507 / 0.597 s, 200x200
 # Q9 stretch 0.478 / 0.443 / 0.428 / 0.411 s; the 64-request stream one request after another 26.2 s (1e-3) and
