@@ -1,7 +1,8 @@
 """The material library in the HIP kernels (hand-derived closed forms of `materials.py:65-162` of the
 reference) against the oracle's automatic differentiation of the same energies: residual, moduli through
 A(v), Jacobi diagonal, block-Jacobi blocks, post-processing fields and the adjoint gradient within 1e-11
-relative (FP64); complete solves within 1e-10; the neo-Hookean-only kernels refuse other materials with NaN."""
+relative (FP64); complete solves within 1e-10; the neo-Hookean-only kernels refuse other materials with NaN;
+the fibre material's compliance gradient against central differences of complete solves (1e-6)."""
 import pytest
 import torch
 
@@ -100,3 +101,28 @@ def test_neo_hookean_only_kernels_refuse_other_materials():
     assert bool(torch.isnan(out[free]).all())
     good = s.residual(u.to(DEV), lf=1.0)
     assert bool(torch.isfinite(good).all())
+
+
+def test_adjoint_sensitivity_with_fibre_material_matches_finite_differences():
+    """The reference's fibre-material sensitivity check (`tests/test_materials.py:101-114`) through the HIP path:
+    compliance gradient from the adjoint solve and the gradient kernel against central differences of complete
+    HIP solves (step 1e-6) within 1e-6 relative."""
+    from difffea_amd.adjoint import compute_sensitivity
+    from difffea_amd.newton_krylov import solve_newton_krylov
+    p = cantilever("quad4", h=1.0, load=(60.0, 0.0), material="fiber_reinforced")
+    n_el = p["conn"].shape[0]
+    theta = (1.0 + 0.2 * torch.sin(torch.arange(n_el, dtype=torch.float64))).to(DEV)
+    f = p["f"].to(DEV)
+
+    def loss(u):
+        return torch.dot(f, u)
+
+    _, g = compute_sensitivity(p, theta, loss, n_load_steps=2, tol=1e-12)
+    eps, g_fd = 1e-6, torch.zeros(n_el, dtype=torch.float64)
+    for i in range(n_el):
+        tp, tm = theta.clone(), theta.clone()
+        tp[i] += eps
+        tm[i] -= eps
+        g_fd[i] = (float(loss(solve_newton_krylov(p, tp, 2, 1e-12, verbose=False))) -
+                   float(loss(solve_newton_krylov(p, tm, 2, 1e-12, verbose=False)))) / (2 * eps)
+    assert float(torch.norm(g.cpu() - g_fd) / torch.norm(g_fd)) < 1e-6
