@@ -362,14 +362,16 @@ def solve_config4(precision):
     return g, res
 
 
-def served_comparison(n_requests, side=(32, 512), seed=5, preconditioner="jacobi"):
+def served_comparison(n_requests, side=(32, 512), seed=5, preconditioner="jacobi",
+                      legs=("plain", "fp64", "mixed", "poisson")):
     """The served request stream (config-5 distribution: sides log-uniform in ``side`` for Q4 and Q9 alike,
     70 % Q4, mu ~ U(0.2, 0.6), lambda/mu ~ U(1, 3), stretch ~ U(1 %, 15 %) in 3 load steps, 20 % with the
-    compliance gradient; seed 5), every request to 1e-8 relative residual, in four runs on the same requests:
+    compliance gradient; seed 5), every request to 1e-8 relative residual, in four runs on the same requests
+    (``legs`` picks some of them; the Poisson run needs the plain one):
 
     * plain: the FP64 GPU solver (``HipNewton``, the single-solve API) one request after another;
     * fp64 / mixed: the continuous-batching server, all requests queued at t = 0 (capacity);
-    * mixed_poisson: the mixed server with Poisson arrivals at 0.8x the plain solver's capacity (latency from
+    * poisson: the mixed server with Poisson arrivals at 0.8x the plain solver's capacity (latency from
       arrival), beside the plain solver behind a first-in-first-out queue with the same arrivals (latency from
       its measured service times).
 
@@ -382,6 +384,8 @@ def served_comparison(n_requests, side=(32, 512), seed=5, preconditioner="jacobi
     from difffea_amd.device import HipMesh
     from difffea_amd.newton_krylov import HipNewton
     from difffea_amd.serve import Server, fifo_latencies, make_stream, reference_residual_norm
+    if "poisson" in legs and "plain" not in legs:
+        raise ValueError("the Poisson run's arrival rate comes from the plain run")
     dev = torch.device("cuda")
     reqs = make_stream(n_requests, rate=1e9, seed=seed, side=side)
     t0 = time.perf_counter()
@@ -394,76 +398,94 @@ def served_comparison(n_requests, side=(32, 512), seed=5, preconditioner="jacobi
                     p50_latency_s=round(float(np.percentile(lat, 50)), 4),
                     p99_latency_s=round(float(np.percentile(lat, 99)), 4), **kw)
 
-    # plain FP64 solver, one request after another (mesh upload + geometry and the solve timed)
-    lat, plain_u, plain_cg, plain_not_run = [], {}, 0, 0
-    t0 = time.perf_counter()
-    for r in reqs:
-        t1 = time.perf_counter()
-        p, th, plan = prepared[r.rid]
-        solver = HipNewton(p, th.to(dev), mesh=HipMesh(p, dev, plan=plan), preconditioner=preconditioner)
-        u, info = solver.solve(r.n_load_steps, 1e-8 * reference_residual_norm(solver.sys, p, dev), cg_max_iter=20000)
-        plain_cg += info["total_cg"]
-        plain_not_run += info["pcg_not_run"]
-        if r.adjoint:
-            lam, _ = solve_adjoint(solver, u, p["f"].to(dev), rtol=1e-8, max_iter=20000)
-            compute_implicit_vjp(solver, u, lam)
-        torch.cuda.synchronize()
-        lat.append(time.perf_counter() - t1)
-        plain_u[r.rid] = u.cpu()
-        del solver
-        if len(lat) % 16 == 0:
-            print(f"# plain FP64 solver: {len(lat)} of {len(reqs)} requests, {time.perf_counter() - t0:.1f} s",
-                  file=sys.stderr, flush=True)
-    plain = summary(lat, time.perf_counter() - t0, len(reqs), latency_from="start of the request's solve",
-                    mode="HipNewton, one request after another", n_cg=plain_cg, n_pcg_not_run=plain_not_run)
-    service = list(lat)
+    def progress(label):
+        # every 16th result, and every one of the last 16 (the largest requests finish last)
+        return lambda d, t: print(f"# {label}: {d} of {t}", file=sys.stderr, flush=True) \
+            if d % 16 == 0 or d > t - 16 else None
 
-    def serve(precision, realtime, rate=None):
-        rs = make_stream(n_requests, rate=rate or 1e9, seed=seed, side=side)
-        srv = Server(precision, max_slots=64, preconditioner=preconditioner)
-        recs, summ = srv.run(rs, realtime=realtime, keep_solution=True, prepared=prepared,
-                             progress=lambda d, t: print(f"# served {precision}: {d} of {t}", file=sys.stderr,
-                                                         flush=True) if d % 16 == 0 else None)
-        summ["n_newton"] = int(sum(x["total_newton"] for x in recs))
-        summ["n_cg"] = int(sum(x["total_cg"] for x in recs))
-        summ["n_pcg_not_run"] = int(sum(x["pcg_not_run"] for x in recs))
-        return {x["rid"]: x for x in recs}, summ
-
-    r64, sat64 = serve("fp64", False)
-    r32, sat32 = serve("mixed", False)
-    err_u = max(float((r32[i]["u"] - r64[i]["u"]).abs().max() / r64[i]["u"].abs().max()) for i in r64)
-    err_g = [float((r32[i]["grad"] - r64[i]["grad"]).abs().max() / r64[i]["grad"].abs().max())
-             for i in r64 if "grad" in r64[i]]
-    plain_vs_fp64 = max(float((plain_u[i] - r64[i]["u"]).abs().max() / r64[i]["u"].abs().max()) for i in r64)
-    rate = 0.8 * plain["solves_per_s"]
-    _, poi32 = serve("mixed", True, rate)
-    arrivals = [r.arrival for r in make_stream(n_requests, rate=rate, seed=seed, side=side)]
-    q = fifo_latencies(arrivals, service)
-    keys = ("wall_s", "solves_per_s", "p50_latency_s", "p99_latency_s", "latency_from", "converged", "n_newton",
-            "n_cg", "n_pcg_not_run", "peak_admitted")
-    return {
+    out = {
         "workload": f"{len(reqs)} requests, sides log-uniform in [{side[0]}, {side[1]}] (Q4 and Q9), 70 % Q4, "
                     f"20 % with the compliance gradient, seed {seed}, 1e-8 relative residual",
         "n_q9": int(sum(r.element_type == "quad9" for r in reqs)), "max_dofs": int(max(
             prepared[r.rid][0]["nodes"].shape[0] for r in reqs) * 2),
         "host_setup_s": round(setup_s, 2), "host_setup_s_per_request": round(setup_s / len(reqs), 4),
-        "plain_fp64": plain,
-        "served_fp64": {k: sat64[k] for k in keys if k in sat64},
-        "served_mixed": {k: sat32[k] for k in keys if k in sat32},
-        "mixed_vs_fp64_served": round(sat32["solves_per_s"] / sat64["solves_per_s"], 3),
-        "fp64_served_vs_plain": round(sat64["solves_per_s"] / plain["solves_per_s"], 3),
-        "mixed_served_vs_plain": round(sat32["solves_per_s"] / plain["solves_per_s"], 3),
-        "mixed_vs_plain_incl_host_setup": round(sat32["solves_per_s"] * (plain["wall_s"] + setup_s)
-                                                / len(reqs), 3),
-        "mixed_vs_fp64_max_rel_error_u": err_u,
-        "mixed_vs_fp64_max_rel_error_grad": max(err_g) if err_g else None,
-        "plain_vs_served_fp64_max_rel_error_u": plain_vs_fp64,
-        "poisson_rate_per_s": round(rate, 3),
-        "served_mixed_poisson": {k: poi32[k] for k in keys if k in poi32},
-        "plain_fp64_fifo_poisson": dict(p50_latency_s=round(float(np.percentile(q, 50)), 4),
-                                        p99_latency_s=round(float(np.percentile(q, 99)), 4),
-                                        latency_from="arrival (service times of the plain run)"),
     }
+    plain = service = plain_u = None
+    if "plain" in legs:
+        # plain FP64 solver, one request after another (mesh upload + geometry and the solve timed)
+        lat, plain_u, plain_cg, plain_not_run = [], {}, 0, 0
+        report = progress("plain FP64 solver")
+        t0 = time.perf_counter()
+        for r in reqs:
+            t1 = time.perf_counter()
+            p, th, plan = prepared[r.rid]
+            solver = HipNewton(p, th.to(dev), mesh=HipMesh(p, dev, plan=plan), preconditioner=preconditioner)
+            u, info = solver.solve(r.n_load_steps, 1e-8 * reference_residual_norm(solver.sys, p, dev),
+                                   cg_max_iter=20000)
+            plain_cg += info["total_cg"]
+            plain_not_run += info["pcg_not_run"]
+            if r.adjoint:
+                lam, _ = solve_adjoint(solver, u, p["f"].to(dev), rtol=1e-8, max_iter=20000)
+                compute_implicit_vjp(solver, u, lam)
+            torch.cuda.synchronize()
+            lat.append(time.perf_counter() - t1)
+            plain_u[r.rid] = u.cpu()
+            del solver
+            report(len(lat), len(reqs))
+        plain = summary(lat, time.perf_counter() - t0, len(reqs), latency_from="start of the request's solve",
+                        mode="HipNewton, one request after another", n_cg=plain_cg, n_pcg_not_run=plain_not_run)
+        service = list(lat)
+        out["plain_fp64"] = plain
+
+    def serve(precision, realtime, rate=None):
+        rs = make_stream(n_requests, rate=rate or 1e9, seed=seed, side=side)
+        srv = Server(precision, max_slots=64, preconditioner=preconditioner)
+        label = f"served {precision}" + (" (Poisson arrivals)" if realtime else "")
+        recs, summ = srv.run(rs, realtime=realtime, keep_solution=True, prepared=prepared, progress=progress(label))
+        summ["n_newton"] = int(sum(x["total_newton"] for x in recs))
+        summ["n_cg"] = int(sum(x["total_cg"] for x in recs))
+        summ["n_pcg_not_run"] = int(sum(x["pcg_not_run"] for x in recs))
+        return {x["rid"]: x for x in recs}, summ
+
+    keys = ("wall_s", "solves_per_s", "p50_latency_s", "p99_latency_s", "latency_from", "converged", "n_newton",
+            "n_cg", "n_pcg_not_run", "peak_admitted")
+    r64 = r32 = sat64 = sat32 = None
+    if "fp64" in legs:
+        r64, sat64 = serve("fp64", False)
+        out["served_fp64"] = {k: sat64[k] for k in keys if k in sat64}
+    if "mixed" in legs:
+        r32, sat32 = serve("mixed", False)
+        out["served_mixed"] = {k: sat32[k] for k in keys if k in sat32}
+    if sat64 and sat32:
+        out["mixed_vs_fp64_served"] = round(sat32["solves_per_s"] / sat64["solves_per_s"], 3)
+    if plain and sat64:
+        out["fp64_served_vs_plain"] = round(sat64["solves_per_s"] / plain["solves_per_s"], 3)
+    if plain and sat32:
+        out["mixed_served_vs_plain"] = round(sat32["solves_per_s"] / plain["solves_per_s"], 3)
+        out["mixed_vs_plain_incl_host_setup"] = round(sat32["solves_per_s"] * (plain["wall_s"] + setup_s)
+                                                      / len(reqs), 3)
+    ref_u = r64 if r64 else ({i: {"u": u} for i, u in plain_u.items()} if plain_u else None)
+    if r32 and ref_u:
+        out["mixed_vs_fp64_max_rel_error_u"] = max(float((r32[i]["u"] - ref_u[i]["u"]).abs().max()
+                                                         / ref_u[i]["u"].abs().max()) for i in ref_u)
+    if r32 and r64:
+        err_g = [float((r32[i]["grad"] - r64[i]["grad"]).abs().max() / r64[i]["grad"].abs().max())
+                 for i in r64 if "grad" in r64[i]]
+        out["mixed_vs_fp64_max_rel_error_grad"] = max(err_g) if err_g else None
+    if plain_u and r64:
+        out["plain_vs_served_fp64_max_rel_error_u"] = max(float((plain_u[i] - r64[i]["u"]).abs().max()
+                                                                / r64[i]["u"].abs().max()) for i in r64)
+    if "poisson" in legs:
+        rate = 0.8 * plain["solves_per_s"]
+        _, poi32 = serve("mixed", True, rate)
+        arrivals = [r.arrival for r in make_stream(n_requests, rate=rate, seed=seed, side=side)]
+        q = fifo_latencies(arrivals, service)
+        out["poisson_rate_per_s"] = round(rate, 3)
+        out["served_mixed_poisson"] = {k: poi32[k] for k in keys if k in poi32}
+        out["plain_fp64_fifo_poisson"] = dict(p50_latency_s=round(float(np.percentile(q, 50)), 4),
+                                              p99_latency_s=round(float(np.percentile(q, 99)), 4),
+                                              latency_from="arrival (service times of the plain run)")
+    return out
 
 
 def main():
@@ -479,6 +501,8 @@ def main():
     ap.add_argument("--serve-side", type=int, nargs=2, default=(32, 512),
                     help="side range of the served meshes (the configured workload: 32 512)")
     ap.add_argument("--served-only", action="store_true", help="only the served-stream comparison, one JSON line")
+    ap.add_argument("--served-legs", default="plain,fp64,mixed,poisson",
+                    help="runs of the served comparison (comma-separated subset of plain, fp64, mixed, poisson)")
     ap.add_argument("--config4-only", action="store_true",
                     help="only the 256x256 Q4 adjoint sensitivity (FP64 and mixed), one JSON line; for profiling it")
     ap.add_argument("--config1-only", action="store_true",
@@ -498,7 +522,8 @@ def main():
         print(json.dumps({prec: solve_config4(prec)[1] for prec in ("fp64", "mixed")}))
         return
     if args.served_only:
-        print(json.dumps(served_comparison(args.serve_requests, tuple(args.serve_side))))
+        print(json.dumps(served_comparison(args.serve_requests, tuple(args.serve_side),
+                                           legs=tuple(args.served_legs.split(",")))))
         return
 
     p, u, v, th = problem("quad4", 400)
