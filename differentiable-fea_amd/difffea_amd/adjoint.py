@@ -27,7 +27,7 @@ def solve_adjoint(solver, u_star, dL_du, rtol=1e-6, max_iter=500):
     """PCG on A lam = m * dL/du at u* (A symmetric = A^T); returns (lam, iterations).
 
     FP64: one PCG to ``rtol``.  Mixed precision: iterative refinement to the same FP64 relative residual --
-    the correction solves A d = r run in FP32 (moduli, product, vectors; FP64 dot products) to the FP32 floor,
+    the correction solves A d = r run in FP32 (moduli, product, vectors; FP64 dot products) to MIXED_RTOL_FLOOR,
     lam accumulates in FP64, and r = b - A lam comes from an FP64 tangent at u* (its own moduli), so the
     adjoint meets ``rtol`` as the FP64 solve does."""
     s = solver.sys
