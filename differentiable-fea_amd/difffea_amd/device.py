@@ -27,7 +27,15 @@ _SC_STATUS = _lib.SC["status"]
 assert (_lib.SC["k"], _lib.SC["maxit"], _lib.SC["iters"]) == (_SC_STATUS + 1, _SC_STATUS + 2, _SC_STATUS + 3)
 
 
+# torch's current stream of the current device as a raw handle: the launch path asks for it on every call, and
+# the stream object of torch.cuda.current_stream() costs about 3 us of host time against 0.1 us for the handle
+_RAW_STREAM = getattr(torch._C, "_cuda_getCurrentRawStream", None)
+_CUR_DEVICE = getattr(torch._C, "_cuda_getDevice", None)
+
+
 def _stream():
+    if _RAW_STREAM is not None and _CUR_DEVICE is not None:
+        return ctypes.c_void_p(_RAW_STREAM(_CUR_DEVICE()))
     return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
 
 

@@ -21,7 +21,7 @@ import torch
 
 from . import _lib
 from .operators import PRECONDITIONERS, PRECISIONS, TANGENT_OPERATORS, get_mesh, _check_material
-from .device import HipSystem
+from .device import HipSystem, _stream
 
 # Inner tolerance floor of the mixed-precision Newton steps.  On the same state and right-hand side the FP32 PCG
 # needs the FP64 iteration count (100x100 Q4 and 200x200 Q9 Newton states, rtol 1e-3 and 1e-4: equal within a few
@@ -258,8 +258,7 @@ def ctypes_ptr(t):
 
 
 def stream():
-    import ctypes
-    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    return _stream()
 
 
 @torch.no_grad()
