@@ -442,13 +442,13 @@ def served_comparison(n_requests, side=(32, 512), seed=5, preconditioner="jacobi
         srv = Server(precision, max_slots=64, preconditioner=preconditioner)
         label = f"served {precision}" + (" (Poisson arrivals)" if realtime else "")
         recs, summ = srv.run(rs, realtime=realtime, keep_solution=True, prepared=prepared, progress=progress(label))
-        summ["n_newton"] = int(sum(x["total_newton"] for x in recs))
-        summ["n_cg"] = int(sum(x["total_cg"] for x in recs))
-        summ["n_pcg_not_run"] = int(sum(x["pcg_not_run"] for x in recs))
+        summ["n_newton"] = int(sum(x.get("total_newton", 0) for x in recs))
+        summ["n_cg"] = int(sum(x.get("total_cg", 0) for x in recs))
+        summ["n_pcg_not_run"] = int(sum(x.get("pcg_not_run", 0) for x in recs))
         return {x["rid"]: x for x in recs}, summ
 
-    keys = ("wall_s", "solves_per_s", "p50_latency_s", "p99_latency_s", "latency_from", "converged", "n_newton",
-            "n_cg", "n_pcg_not_run", "peak_admitted")
+    keys = ("wall_s", "solves_per_s", "p50_latency_s", "p99_latency_s", "latency_from", "converged", "errors",
+            "n_newton", "n_cg", "n_pcg_not_run", "peak_admitted")
     r64 = r32 = sat64 = sat32 = None
     if "fp64" in legs:
         r64, sat64 = serve("fp64", False)

@@ -14,7 +14,9 @@ Per-solve reductions use only the solve's own blocks, and no kernel's arithmetic
 solve's result is bitwise the same alone and in any mix.
 
 Latency = completion time - arrival time; the report holds solves/s, p50 / p99 latency and per-request
-Newton / CG counts and final residuals.
+Newton / CG counts and final residuals.  A request that cannot be set up or admitted (invalid parameters, a
+footprint beyond the admission budget, device memory exhausted) gets a record with ``error`` and takes no slot
+or memory; the other requests run as if it had not come.
 """
 import ctypes
 import heapq
@@ -367,6 +369,13 @@ class Server:
 
     def _admit(self, req, prepared=None):
         slot = self.free.pop()
+        try:
+            return self._admit_to(slot, req, prepared)
+        except BaseException:
+            self.free.append(slot)             # nothing of this request was left on the device
+            raise
+
+    def _admit_to(self, slot, req, prepared):
         params, theta, plan = prepared if prepared is not None else self.prepare(req)
         mesh = HipMesh(params, self.device, plan=plan)
         sys_ = HipSystem(mesh, theta, material_spec(params), params["f"], self.precision, uD=params["u_D"],
@@ -607,13 +616,36 @@ class Server:
                 prepare_arrived()
                 while i < len(pending) and (not realtime or pending[i].arrival <= now) and self.free:
                     r = pending[i]
-                    fb = request_footprint(r, self.precision, self.preconditioner)
-                    if self.jobs and self.used_bytes + fb > self.hbm_budget:
+                    try:
+                        fb = request_footprint(r, self.precision, self.preconditioner)
+                    except Exception as exc:      # a request the set-up cannot size: its own error record
+                        fb, err = 0, exc
+                    else:
+                        err = None if fb <= self.hbm_budget else MemoryError(
+                            f"the request needs {fb} bytes of device memory, more than the admission budget "
+                            f"{self.hbm_budget}")
+                    if err is None and self.jobs and self.used_bytes + fb > self.hbm_budget:
                         break
                     fut = futures.get(i)
-                    if self.jobs and (fut is None or not fut.done()):
+                    if err is None and self.jobs and (fut is None or not fut.done()):
                         break                      # admitted once its host set-up is done; the GPU keeps working
-                    job = self._admit(r, futures.pop(i).result() if fut is not None else None)
+                    if err is None:
+                        try:
+                            job = self._admit(r, futures.pop(i).result() if fut is not None else None)
+                        except Exception as exc:  # invalid request, device memory exhausted, ...
+                            err = exc
+                    if err is not None:
+                        # the request fails alone: an error record, no slot or memory kept, the others go on
+                        futures.pop(i, None)
+                        t_now = time.perf_counter()
+                        records.append(dict(rid=r.rid, element_type=r.element_type, n=r.n, adjoint=r.adjoint,
+                                            converged=False, error=f"{type(err).__name__}: {err}",
+                                            latency=t_now - (t0 + r.arrival if realtime else t_now), queue=0.0))
+                        i += 1
+                        if progress is not None:
+                            progress(len(records), len(pending))
+                        prepare_arrived()
+                        continue
                     job.t_arrival = t0 + r.arrival if realtime else job.t_admit
                     peak_jobs = max(peak_jobs, len(self.jobs))
                     peak_bytes = max(peak_bytes, self.used_bytes)
@@ -639,12 +671,13 @@ class Server:
         finally:
             pool.shutdown(wait=True)
         wall = time.perf_counter() - t0
-        lat = np.array([r["latency"] for r in records])
+        solved = [r for r in records if "error" not in r]
+        lat = np.array([r["latency"] for r in solved]) if solved else np.zeros(1)
         summary = dict(precision=self.precision, preconditioner=self.preconditioner, n_requests=len(records),
                        wall_s=wall,
                        latency_from="arrival" if realtime else "admission",
-                       solves_per_s=len(records) / wall, p50_latency_s=float(np.percentile(lat, 50)),
+                       solves_per_s=len(solved) / wall, p50_latency_s=float(np.percentile(lat, 50)),
                        p99_latency_s=float(np.percentile(lat, 99)),
-                       converged=int(sum(r["converged"] for r in records)),
+                       converged=int(sum(r["converged"] for r in records)), errors=len(records) - len(solved),
                        hbm_budget=self.hbm_budget, peak_admitted=peak_jobs, peak_admitted_bytes=peak_bytes)
         return records, summary

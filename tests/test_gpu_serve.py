@@ -147,3 +147,28 @@ def test_hbm_footprint_admission_binds_and_keeps_results():
     roomy = {r["rid"]: r for r in roomy}
     for r in tight:
         assert torch.equal(r["u"], roomy[r["rid"]]["u"])
+
+
+def test_failed_requests_get_error_records_and_leave_the_others_unchanged():
+    """Requests that cannot be served -- an unknown element type, an unknown material in a caller's problem, a
+    footprint beyond the whole admission budget -- each get an error record, take no slot or memory, and every
+    other request gives the same bits as in a stream without them."""
+    from difffea_amd.serve import Request, Server, request_footprint, request_problem
+    reqs = _stream()
+    big = max(request_footprint(r, "fp64") for r in reqs)
+    p, th = request_problem(reqs[1])
+    bad = [Request(100, 0.0, "quad7", 8, 0.4, 0.8, 0.05),
+           Request(101, 0.0, "quad4", 0, 0.0, 0.0, 0.0, params=dict(p, material="rubber"), theta=th),
+           Request(102, 0.0, "quad4", 400, 0.4, 0.8, 0.05)]
+    assert request_footprint(bad[2], "fp64") > 2.2 * big
+    srv = Server("fp64", max_slots=16, hbm_budget=int(2.2 * big))
+    mixed, summ = srv.run(reqs[:5] + bad + reqs[5:], realtime=False, keep_solution=True)
+    clean, _ = Server("fp64", max_slots=16, hbm_budget=int(2.2 * big)).run(reqs, realtime=False, keep_solution=True)
+    got = {r["rid"]: r for r in mixed}
+    assert summ["errors"] == 3 and summ["converged"] == len(reqs) and len(mixed) == len(reqs) + 3
+    assert got[100]["error"].startswith("KeyError") and got[101]["error"].startswith("ValueError")
+    assert got[102]["error"].startswith("MemoryError")
+    assert not any(got[i]["converged"] for i in (100, 101, 102))
+    assert srv.used_bytes == 0 and len(srv.free) == 16 and not srv.jobs
+    for r in clean:
+        assert torch.equal(got[r["rid"]]["u"], r["u"])
