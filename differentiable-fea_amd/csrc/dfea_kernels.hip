@@ -1627,8 +1627,12 @@ __global__ __launch_bounds__(BS, MINW) void k_pcg(const dfea_desc* __restrict__ 
     const int qr = EARLY || k >= 0.0 ? 1 - par : par;
     const PcgSet<T> Rd = pcg_set<T>(D, qr);
     const PcgSet<T> Wr = pcg_set<T>(D, par);   // state buffers written
-    const V2<T>* part_rd = reinterpret_cast<const V2<T>*>(D.part) + (size_t)(1 - par) * D.n_part;
-    V2<T>* part_wr = reinterpret_cast<V2<T>*>(D.part) + (size_t)par * D.n_part;
+    // the shared-node partials of parity q: FP64 pairs q n_part ..; FP32 solves keep theirs in the buffer's second
+    // half (float2 offset (2 + q) n_part), clear of the FP64 pairs the residual kernels write from its start, so
+    // a residual of the same solve may run between two rounds of PCG kernels
+    constexpr size_t PB = sizeof(T) == 4 ? 2 : 0;
+    const V2<T>* part_rd = reinterpret_cast<const V2<T>*>(D.part) + (PB + 1 - par) * (size_t)D.n_part;
+    V2<T>* part_wr = reinterpret_cast<V2<T>*>(D.part) + (PB + par) * (size_t)D.n_part;
     const int2* prg = reinterpret_cast<const int2*>(D.prange);
     constexpr int CE = Groups<NEN, BS, CEX>::CE;
     constexpr int NG = (MAXLN + BS - 1) / BS;     // local nodes per thread, at most

@@ -217,6 +217,7 @@ class HipSystem:
         # copy of a dot product
         self._hst = torch.empty(9, dtype=F64, pin_memory=True)
         self._dot_ev = torch.cuda.Event()
+        self._st_ev = torch.cuda.Event()
         self.desc = self._make_desc()
         self.table = m.upload(self.desc)
 
@@ -424,11 +425,21 @@ class HipSystem:
 
     def pcg_status(self):
         """(status, iterations, k) of the solve (one copy through pinned memory, one synchronisation)."""
+        return self.pcg_status_start()()
+
+    def pcg_status_start(self):
+        """Enqueue the copy of the status block to pinned host memory; returns a function that waits for that
+        copy alone (work enqueued after this call keeps running) and gives (status, iterations, k)."""
         h = self._hst[0:4]
         h.copy_(self.sc[_SC_STATUS:_SC_STATUS + 4], non_blocking=True)    # status, k, maxit, iters
-        torch.cuda.current_stream(self.mesh.device).synchronize()
-        st, k, _, its = h.tolist()
-        return int(st), int(its), int(k)
+        ev = self._st_ev
+        ev.record()
+
+        def wait():
+            ev.synchronize()
+            st, k, _, its = h.tolist()
+            return int(st), int(its), int(k)
+        return wait
 
     ROUND_GRAPHS = (128, 64, 32, 16)
     LEAD_DIRECT = 2          # graph rounds: kernels launched directly at the start of a solve (even: pairs)
@@ -447,14 +458,16 @@ class HipSystem:
         if n:
             self.pcg_iterate(n, False)
 
-    def pcg_run(self, max_iter, chunk=16, graph=None, first=None):
+    def pcg_run(self, max_iter, chunk=16, graph=None, first=None, after_round=None):
         """Iterate a started PCG until it stops; returns (iterations, status).
 
         Kernels run in rounds and the status block is read after each round; kernels of a solve that has
         stopped skip their work on the device.  The first round has `first` kernels (default `chunk`: an
         estimate of the iterations saves rounds) and is launched without reading the status before it; later
         rounds have `chunk` kernels, doubling up to 128.  The first kernel forms A M b and the test on the last
-        update is made by the kernel after it, so k iterations take k + 2 kernels."""
+        update is made by the kernel after it, so k iterations take k + 2 kernels.  ``after_round``: a function
+        called after each round's status copy is enqueued and before the host waits for it -- work it enqueues
+        (that must not change the PCG's inputs) runs on the GPU meanwhile, on the round's answer so far."""
         k = 0
         n = chunk if first is None else max(2, int(first))
         n = min(n, max_iter + 2)
@@ -465,7 +478,10 @@ class HipSystem:
             n -= self.LEAD_DIRECT
         while True:
             self.pcg_round(min(n, max_iter - k + 2), graph)
-            st, its, k = self.pcg_status()
+            status = self.pcg_status_start()
+            if after_round is not None:
+                after_round()
+            st, its, k = status()
             if st != _lib.PCG_RUNNING:
                 return its, st
             n = chunk

@@ -120,6 +120,7 @@ class HipNewton:
         self._alphas = torch.tensor([2.0 ** -j for j in range(_N_ALPHAS)], dtype=torch.float64, device=self.device)
         self._moduli_at = None          # the state the system's moduli were last computed at (see armijo)
         self._rhs_nan = False           # the current Newton residual has a NaN entry (see nan_pcg_result)
+        self._spec = None               # the line search's first trial, enqueued behind the PCG (linear_solve)
         self.pcg_not_run = 0            # PCG solves whose result nan_pcg_result wrote
         self.pcg_chunk = 16
         self._step_counts = []          # CG iterations of the Newton iterations of the last load step solved
@@ -131,16 +132,28 @@ class HipNewton:
     def norm(self, R):
         return math.sqrt(self.sys.dot_host(R))
 
-    def linear_solve(self, rtol, max_iter, first=None):
+    def linear_solve(self, rtol, max_iter, first=None, trial_at=None):
         """PCG on J du = -R with R in sys.vec['R']; returns (du, iterations).  first: kernels of the first
-        round of PCG kernels (``first_round``)."""
+        round of PCG kernels (``first_round``).  trial_at = (u, lf): behind every PCG round the line search's
+        first trial u + du and its residual norm are enqueued on the round's answer (``_spec``), so the GPU
+        goes on with them while the host reads the PCG status; the last round's are the line search's."""
         s = self.sys
+        self._spec = None
         if self._rhs_nan:
             return nan_pcg_result(s.vec["x"], max_iter)
         if s.precision == "mixed":
             rtol = max(rtol, MIXED_RTOL_FLOOR)
         s.pcg_start(rtol=rtol, max_iter=max_iter, neg_res=True)
-        its, _ = s.pcg_run(max_iter, self.pcg_chunk, first=first)
+        after = None
+        if trial_at is not None:
+            u, lf = trial_at
+
+            def after():
+                # the trial and the residual write ucand and vec['R'], which the running PCG does not read
+                # (its right-hand side was copied at the start)
+                uc = self.trial(u, s.vec["x"], 1.0, self.ucand)
+                self._spec = (uc, s.dot_start(self.residual(uc, lf)))
+        its, _ = s.pcg_run(max_iter, self.pcg_chunk, first=first, after_round=after)
         return s.vec["x"], its
 
     def trial(self, u, du, alpha, out):
@@ -156,12 +169,17 @@ class HipNewton:
         return out
 
     # -- reference state machine ---------------------------------------------------------------------------
-    def armijo(self, u, du, rn, lf, c1=1e-4, min_alpha=2.0 ** -5):
+    def armijo(self, u, du, rn, lf, c1=1e-4, min_alpha=2.0 ** -5, spec=None):
+        """The reference's backtracking line search; spec: the first trial (alpha = 1) already enqueued, as
+        (candidate, pending norm) from ``linear_solve``."""
         alpha = 1.0
         while alpha >= min_alpha:
-            uc = self.trial(u, du, alpha, self.ucand)
-            R = self.residual(uc, lf)
-            pending = self.sys.dot_start(R)
+            if alpha == 1.0 and spec is not None:
+                uc, pending = spec
+            else:
+                uc = self.trial(u, du, alpha, self.ucand)
+                R = self.residual(uc, lf)
+                pending = self.sys.dot_start(R)
             # the next Newton iteration's moduli at the trial state, enqueued before the norm is read: when the
             # trial is accepted (the usual case) the GPU computes them while the host waits for the norm and
             # decides; otherwise they are recomputed where needed (same kernel, same state, same bits)
@@ -197,11 +215,11 @@ class HipNewton:
             elif self._moduli_at is not u:              # else computed after the line search that accepted u
                 self.sys.moduli(u)
             self._moduli_at = None
-            du, k = self.linear_solve(eta, cg_max_iter, first_round(it, counts, prev))
-            counts.append(int(k))
             if self.ucand is u:
                 self.ucand = torch.empty_like(u)
-            u, rn, alpha, ok = self.armijo(u, du, rn, lf)
+            du, k = self.linear_solve(eta, cg_max_iter, first_round(it, counts, prev), trial_at=(u, lf))
+            counts.append(int(k))
+            u, rn, alpha, ok = self.armijo(u, du, rn, lf, spec=self._spec)
             rec = dict(step=step_id, load_factor=lf, iteration=it, residual=float(rn), cg_iters=int(k), alpha=alpha)
             records.append(rec)
             if callback is not None:

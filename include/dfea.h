@@ -144,7 +144,11 @@ typedef struct dfea_desc {
                                    per node, then m01 per node */
     float* diag32;
     double* f;                  /* [2N] external force at load factor 1 */
-    double* part;               /* [n_part] pairs (double2 or float2) */
+    double* part;               /* shared-node partials, 4 n_part doubles: the product and the Newton-step kernels
+                                   use n_part pairs of their precision from the start (block-Jacobi moduli also the
+                                   second half); the PCG's two parities are FP64 pairs [q n_part, (q + 1) n_part)
+                                   or FP32 pairs [(2 + q) n_part, (3 + q) n_part), so an FP64 residual of the same
+                                   solve may run between two rounds of PCG kernels */
     double* red;                /* reduction partials, dfea_red_size() doubles */
     double* sc;                 /* [DFEA_NSC] */
     void* vec[DFEA_NVEC];       /* PCG / Newton vectors; x,r,z,p,Ap,b in the solve's precision; u (evaluation

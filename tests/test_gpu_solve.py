@@ -73,14 +73,14 @@ def test_nan_residual_pcg_result_equals_the_kernels(precision):
             self.at_u = u                    # the last residual evaluated is the Newton state's
             return super().residual(u, lf, out)
 
-        def linear_solve(self, rtol, max_iter, first=None):
+        def linear_solve(self, rtol, max_iter, first=None, trial_at=None):
             if self._rhs_nan:
                 self._rhs_nan = False
                 self.sys.moduli(self.at_u)
-                x, k = super().linear_solve(rtol, max_iter, first)
+                x, k = super().linear_solve(rtol, max_iter, first, trial_at)
                 assert torch.isnan(x).all() and k == max_iter
                 return x, k
-            return super().linear_solve(rtol, max_iter, first)
+            return super().linear_solve(rtol, max_iter, first, trial_at)
 
     z, spec = load_golden("g2_q4_nan_cutback_4")
     p = golden_problem(spec)
@@ -92,6 +92,35 @@ def test_nan_residual_pcg_result_equals_the_kernels(precision):
     (u0, i0), (u1, i1) = out
     assert i0["pcg_not_run"] >= 1 and i0["n_cutbacks"] >= 1 and i0["converged"]
     assert np.array_equal(history_array(i0), history_array(i1), equal_nan=True)
+    assert torch.equal(u0, u1)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("et,n", [("quad4", 30), ("quad9", 12)])
+def test_trial_behind_pcg_rounds_changes_no_bit(et, n, precision):
+    """The line search's first trial and its residual are enqueued behind every round of PCG kernels while the
+    host reads the status; those of rounds that were not the last run while the PCG goes on and must not
+    disturb it (its shared-node partials, its right-hand side): short rounds (many of them) give the same
+    trajectory and bits as the solve without the early trial."""
+    from difffea_amd.device import HipMesh
+    from difffea_amd.newton_krylov import HipNewton
+    from difffea_amd.problem import build_problem
+
+    class NoEarlyTrial(HipNewton):
+        def linear_solve(self, rtol, max_iter, first=None, trial_at=None):
+            return super().linear_solve(rtol, max_iter, first, None)
+
+    p = build_problem(dict(element_type=et, nx=n, ny=n, E=1.0, nu=0.3, load_mode="displacement",
+                           load_value=(0.1, None)))
+    th = torch.ones(p["conn"].shape[0], dtype=torch.float64, device=DEV)
+    out = []
+    for cls in (HipNewton, NoEarlyTrial):
+        s = cls(p, th, precision=precision, mesh=HipMesh(p, DEV))
+        s.pcg_chunk = 2
+        out.append(s.solve(3, 1e-9, cg_max_iter=2000))
+    (u0, i0), (u1, i1) = out
+    assert i0["converged"] and i0["total_cg"] > 20
+    assert np.array_equal(history_array(i0), history_array(i1))
     assert torch.equal(u0, u1)
 
 
