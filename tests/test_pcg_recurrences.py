@@ -41,7 +41,7 @@ def cg_gear(A, b, dinv, rtol, max_iter):
             return x, max_iter, MAXITER
         beta = gam / gam_prev if k >= 1 else 0.0
         den = delta - beta * gam / alpha_prev if k >= 1 else delta
-        if not den > 0.0:
+        if den <= 0.0:          # as the kernel and the reference: a NaN curvature runs on
             return (dinv * r if k == 0 else x), k, CURVATURE
         alpha = gam / den
         u = dinv * r
@@ -105,3 +105,26 @@ def test_negative_curvature_first_and_later_iteration():
     x, k, st = cg_gear(np.diag(d), b2, np.ones(5), 1e-12, 50)
     assert st == CURVATURE and k == k_ref >= 1
     assert np.abs(x - x_ref).max() <= 1e-12 * np.abs(x_ref).max()
+
+
+@pytest.mark.parametrize("max_iter", [0, 1, 7])
+def test_nan_right_hand_side_result_written_without_a_run(max_iter):
+    """A Newton residual with a NaN entry (an inverted element) gets the PCG's answer without a launch
+    (`newton_krylov.nan_pcg_result`): the oracle's textbook PCG and the kernel sequence return exactly that --
+    every entry NaN after max_iter iterations (zeros after none), the cap as the iteration count."""
+    from difffea_amd.newton_krylov import nan_pcg_result
+    p = fem.precompute_geometry(unit_square("quad4", 4))
+    th = torch.ones(p["conn"].shape[0], dtype=torch.float64)
+    u, _ = state(p, 0)
+    A = dense(fem.make_A_operator(u, th, p), u.numel())
+    b = torch.randn(u.numel(), dtype=torch.float64, generator=torch.Generator().manual_seed(2))
+    b[5] = float("nan")
+    dinv = 1.0 / torch.diagonal(A)
+    x_ref, k_ref = fem.pcg(lambda v: A @ v, b, lambda r: dinv * r, rtol=1e-8, max_iter=max_iter)
+    x_k, k_k, st = cg_gear(A.numpy(), b.numpy(), dinv.numpy(), 1e-8, max_iter)
+    x, k = nan_pcg_result(torch.empty_like(b), max_iter)
+    assert k == k_ref == k_k == max_iter
+    assert torch.equal(torch.isnan(x), torch.isnan(x_ref)) and torch.isnan(x).all() == (max_iter > 0)
+    assert np.array_equal(np.isnan(x_k), torch.isnan(x).numpy())
+    if max_iter == 0:
+        assert torch.equal(x, x_ref)
