@@ -384,6 +384,9 @@ def served_comparison(n_requests, side=(32, 512), seed=5, preconditioner="jacobi
     from difffea_amd.device import HipMesh
     from difffea_amd.newton_krylov import HipNewton
     from difffea_amd.serve import Server, fifo_latencies, make_stream, reference_residual_norm
+    unknown = set(legs) - {"plain", "fp64", "mixed", "poisson"}
+    if unknown:
+        raise ValueError(f"unknown runs of the served comparison: {sorted(unknown)}")
     if "poisson" in legs and "plain" not in legs:
         raise ValueError("the Poisson run's arrival rate comes from the plain run")
     dev = torch.device("cuda")
