@@ -363,11 +363,12 @@ def solve_config4(precision):
 
 
 def served_comparison(n_requests, side=(32, 512), seed=5, preconditioner="jacobi",
-                      legs=("plain", "fp64", "mixed", "poisson")):
+                      legs=("plain", "fp64", "mixed", "poisson"), part=None):
     """The served request stream (config-5 distribution: sides log-uniform in ``side`` for Q4 and Q9 alike,
     70 % Q4, mu ~ U(0.2, 0.6), lambda/mu ~ U(1, 3), stretch ~ U(1 %, 15 %) in 3 load steps, 20 % with the
     compliance gradient; seed 5), every request to 1e-8 relative residual, in four runs on the same requests
-    (``legs`` picks some of them; the Poisson run needs the plain one):
+    (``legs`` picks some of them; the Poisson run needs the plain one; ``part`` = (start, stop) serves that slice
+    of the stream, arrivals shifted to start at 0, so a stream too long for one run is measured in slices):
 
     * plain: the FP64 GPU solver (``HipNewton``, the single-solve API) one request after another;
     * fp64 / mixed: the continuous-batching server, all requests queued at t = 0 (capacity);
@@ -389,8 +390,17 @@ def served_comparison(n_requests, side=(32, 512), seed=5, preconditioner="jacobi
         raise ValueError(f"unknown runs of the served comparison: {sorted(unknown)}")
     if "poisson" in legs and "plain" not in legs:
         raise ValueError("the Poisson run's arrival rate comes from the plain run")
+    import dataclasses
     dev = torch.device("cuda")
-    reqs = make_stream(n_requests, rate=1e9, seed=seed, side=side)
+
+    def stream(rate):
+        rs = make_stream(n_requests, rate=rate, seed=seed, side=side)
+        if part is None:
+            return rs
+        rs = rs[part[0]:part[1]]
+        return [dataclasses.replace(r, arrival=r.arrival - rs[0].arrival) for r in rs]
+
+    reqs = stream(1e9)
     t0 = time.perf_counter()
     prepared = {r.rid: Server.prepare(r) for r in reqs}
     setup_s = time.perf_counter() - t0
@@ -407,7 +417,8 @@ def served_comparison(n_requests, side=(32, 512), seed=5, preconditioner="jacobi
             if d % 16 == 0 or d > t - 16 else None
 
     out = {
-        "workload": f"{len(reqs)} requests, sides log-uniform in [{side[0]}, {side[1]}] (Q4 and Q9), 70 % Q4, "
+        "workload": f"{len(reqs)} requests" + ("" if part is None else f" ({part[0]}..{part[1] - 1} of {n_requests})") +
+                    f", sides log-uniform in [{side[0]}, {side[1]}] (Q4 and Q9), 70 % Q4, "
                     f"20 % with the compliance gradient, seed {seed}, 1e-8 relative residual",
         "n_q9": int(sum(r.element_type == "quad9" for r in reqs)), "max_dofs": int(max(
             prepared[r.rid][0]["nodes"].shape[0] for r in reqs) * 2),
@@ -441,7 +452,7 @@ def served_comparison(n_requests, side=(32, 512), seed=5, preconditioner="jacobi
         out["plain_fp64"] = plain
 
     def serve(precision, realtime, rate=None):
-        rs = make_stream(n_requests, rate=rate or 1e9, seed=seed, side=side)
+        rs = stream(rate or 1e9)
         srv = Server(precision, max_slots=64, preconditioner=preconditioner)
         label = f"served {precision}" + (" (Poisson arrivals)" if realtime else "")
         recs, summ = srv.run(rs, realtime=realtime, keep_solution=True, prepared=prepared, progress=progress(label))
@@ -481,7 +492,7 @@ def served_comparison(n_requests, side=(32, 512), seed=5, preconditioner="jacobi
     if "poisson" in legs:
         rate = 0.8 * plain["solves_per_s"]
         _, poi32 = serve("mixed", True, rate)
-        arrivals = [r.arrival for r in make_stream(n_requests, rate=rate, seed=seed, side=side)]
+        arrivals = [r.arrival for r in stream(rate)]
         q = fifo_latencies(arrivals, service)
         out["poisson_rate_per_s"] = round(rate, 3)
         out["served_mixed_poisson"] = {k: poi32[k] for k in keys if k in poi32}
@@ -506,6 +517,8 @@ def main():
     ap.add_argument("--served-only", action="store_true", help="only the served-stream comparison, one JSON line")
     ap.add_argument("--served-legs", default="plain,fp64,mixed,poisson",
                     help="runs of the served comparison (comma-separated subset of plain, fp64, mixed, poisson)")
+    ap.add_argument("--serve-slice", default=None,
+                    help="start:stop -- serve only that slice of the --serve-requests stream (long streams in parts)")
     ap.add_argument("--config4-only", action="store_true",
                     help="only the 256x256 Q4 adjoint sensitivity (FP64 and mixed), one JSON line; for profiling it")
     ap.add_argument("--config1-only", action="store_true",
@@ -525,8 +538,9 @@ def main():
         print(json.dumps({prec: solve_config4(prec)[1] for prec in ("fp64", "mixed")}))
         return
     if args.served_only:
+        part = None if args.serve_slice is None else tuple(int(x) for x in args.serve_slice.split(":"))
         print(json.dumps(served_comparison(args.serve_requests, tuple(args.serve_side),
-                                           legs=tuple(args.served_legs.split(",")))))
+                                           legs=tuple(args.served_legs.split(",")), part=part)))
         return
 
     p, u, v, th = problem("quad4", 400)
