@@ -290,7 +290,10 @@ int dfea_pcg_start(const dfea_desc* table, int n_vitems, double rtol, int max_it
  * Kernels run in pairs of parities 0, 1: n_iter is rounded up to even, and dfea_pcg_init may only be
  * called between two calls.  The first kernel after dfea_pcg_init forms A M b; each further kernel is one
  * iteration, whose test on the residual it produced is made by the next kernel.  Callers launch while
- * sc[DFEA_SC_STATUS] reads DFEA_PCG_RUNNING; finished solves skip their work.  items: chunk items. */
+ * sc[DFEA_SC_STATUS] reads DFEA_PCG_RUNNING; finished solves skip their work.  items: chunk items.
+ * Between two calls the solve's residual, dot products and axpy operations may run on its descriptor (they
+ * touch neither the PCG vectors, nor the partials the next kernel reads, nor the scalars it uses); the
+ * moduli may not (the PCG reads them every iteration). */
 int dfea_pcg_iterate(const dfea_desc* table, const int32_t* items, int n_items, int n_iter, int nen, int precision,
                      void* stream);
 
