@@ -291,9 +291,9 @@ int dfea_pcg_start(const dfea_desc* table, int n_vitems, double rtol, int max_it
  * called between two calls.  The first kernel after dfea_pcg_init forms A M b; each further kernel is one
  * iteration, whose test on the residual it produced is made by the next kernel.  Callers launch while
  * sc[DFEA_SC_STATUS] reads DFEA_PCG_RUNNING; finished solves skip their work.  items: chunk items.
- * Between two calls the solve's residual, dot products and axpy operations may run on its descriptor (they
- * touch neither the PCG vectors, nor the partials the next kernel reads, nor the scalars it uses); the
- * moduli may not (the PCG reads them every iteration). */
+ * Between two calls the solve's residual, dot products and axpy operations may run on its descriptor, with
+ * outputs other than the PCG vectors (they touch neither the partials the next kernel reads nor the scalars
+ * it uses); the moduli may not (the PCG reads them every iteration). */
 int dfea_pcg_iterate(const dfea_desc* table, const int32_t* items, int n_items, int n_iter, int nen, int precision,
                      void* stream);
 
