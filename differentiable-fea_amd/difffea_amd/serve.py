@@ -385,7 +385,6 @@ class Server:
             torch.frombuffer(bytearray(bytes(sys_.desc)), dtype=torch.uint8))
         job = Job(req, slot, mesh, sys_, params, theta)
         job.bytes = request_footprint(req, self.precision, self.preconditioner)
-        self.used_bytes += job.bytes
         pl = mesh.plan
         job.items = {"chunk": self._items(slot, pl.n_chunks), "fin": self._items(slot, mesh.n_finblk),
                      "vec": self._items(slot, mesh.n_vblk), "solve": self._items(slot, 1)}
@@ -401,6 +400,7 @@ class Server:
                             cg_max_iter=self.cg_max_iter, adjoint_rtol=self.adjoint_rtol)
         job.op = next(job.gen)
         job.t_admit = time.perf_counter()
+        self.used_bytes += job.bytes           # counted once nothing of the admission can fail any more
         self.jobs[slot] = job
         return job
 
